@@ -1,0 +1,239 @@
+/*
+ * libmemi - MI355X (gfx950) kernels for ManifoldEM's per-projection-direction hot path.
+ *
+ * C ABI: plain pointers and sizes; every array argument is DEVICE memory unless its name ends
+ * in _host.  Every entry point enqueues work on `stream` (a hipStream_t passed as void*;
+ * NULL = the default stream), returns 0 on success or a negative MEMI_E* code, and never
+ * throws.  memi_last_error() gives the text of the last failure on the calling host thread.
+ * Entry points keep no mutable global state: two calls on different streams do not interact
+ * beyond the buffers they are given.
+ *
+ * Each declaration names the reference function (file:line in evanseitz/ManifoldEM_Python,
+ * modules/) whose arithmetic it replaces.  The Python drop-in modules in
+ * manifoldem_python_amd/ keep the reference call signatures and call these.
+ *
+ * Layout conventions
+ *   image   : float32 [n][N][N], row-major, the reference's in-memory orientation
+ *             (the SPIDER memmap transposed, getDistanceCTF_local_Conj9combinedS2.py:256-258)
+ *   spectrum: complex64 [n][N][Nh] (interleaved re, im), Nh = N/2 + 1, the rfft2 layout;
+ *             the reference's full fft2 spectra are Hermitian (real images, even CTF), so the
+ *             half spectrum with column weights (1 for kx = 0 and kx = N/2, 2 otherwise)
+ *             reproduces every full-spectrum sum.
+ *   Kh      : N * Nh half-spectrum length; Khp = Kh rounded up to a multiple of 64.
+ *   operand : the packed distance operand, one row per image, four segments of Khp each:
+ *             [ w|X|^2 | C^2 | sqrt(w) Re(CX) | sqrt(w) Im(CX) ] (zero padded);
+ *             float32 [rows][4*Khp] (precision FP32) or a bf16 hi/lo pair (SPLIT_BF16).
+ *   PD batch: a memi_pd_desc array; PD p owns operand/image rows row0 .. row0+n-1 and the
+ *             n*n block of every per-PD matrix starting at element d_off.
+ */
+#ifndef MEMI_H
+#define MEMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MEMI_OK 0
+#define MEMI_EINVAL (-1)      /* bad argument (shape, size, null pointer) */
+#define MEMI_ELAUNCH (-2)     /* a kernel launch or HIP runtime call failed */
+#define MEMI_ENOTSUP (-3)     /* unsupported option */
+
+#define MEMI_PREC_FP32 0       /* plain FP32 MFMA Gram (v_mfma_f32_16x16x4_f32) */
+#define MEMI_PREC_SPLIT_BF16 1 /* split-BF16: hi*hi + hi*lo + lo*hi, FP32 accumulate */
+
+#define MEMI_FILTER_BUTTER 1   /* sqrt(1 / (1 + (Q/Qc)^(2*order)))          */
+#define MEMI_FILTER_GAUSS 2    /* exp(-(log 2 / 2) (Q/Qc)^2)                  */
+#define MEMI_FILTER_BSPLINE 4  /* divide by the cubic B-spline transfer fn    */
+
+typedef struct memi_pd_desc {
+  int64_t row0;      /* in : first image / operand row of the PD                         */
+  int64_t d_off;     /* out: element offset of the PD's n*n block in per-PD matrices     */
+  int64_t mask_off;  /* out: word offset of the PD's kNN bitmask (n rows of ceil(n/32))   */
+  int64_t list_off;  /* out: element offset of the PD's k*n kNN lists                    */
+  int32_t n;         /* in : particles in the PD                                         */
+  int32_t k;         /* in : nearest neighbours kept (k >= n: full kernel)               */
+} memi_pd_desc;
+
+/* Sizes and table offsets of a batch plan (memi_plan). */
+typedef struct memi_plan_info {
+  int64_t n_pd;
+  int64_t n_items;     /* Gram work items: (PD, tile i, tile j, segment, chunk)          */
+  int64_t n_utiles;    /* 128x128 tiles of the upper triangles                          */
+  int64_t n_ftiles;    /* 128x128 tiles of the full matrices                            */
+  int64_t n_rows;      /* sum of n                                                      */
+  int64_t off_pds, off_items, off_utiles, off_ftiles, off_rows;  /* byte offsets         */
+  int64_t plan_bytes;  /* size of the plan buffer                                       */
+  int64_t d_elems;     /* sum n^2: per-PD matrices (D, L)                               */
+  int64_t mask_words;  /* sum n*ceil(n/32): kNN bitmasks                                */
+  int64_t list_elems;  /* sum min(k,n)*n: kNN lists                                     */
+  int64_t ws_floats;   /* Gram partial-sum workspace (float32)                          */
+  int64_t flag_cap;    /* sum n(n-1)/2: capacity of the recompute list (int32 triplets) */
+  int32_t chunks;      /* K chunks per operand segment                                  */
+  int32_t max_n;
+} memi_plan_info;
+
+typedef struct memi_ctf_params {
+  double pix_size;   /* A per pixel           (p.pix_size)    */
+  double cs_mm;      /* spherical aberration  (p.Cs)          */
+  double ekv;        /* voltage kV            (p.EkV)         */
+  double ampc;       /* amplitude contrast    (p.AmpContrast) */
+  double gauss_env;  /* envelope half width; +inf = none (p.gaussEnv) */
+} memi_ctf_params;
+
+int memi_version(void);
+const char* memi_last_error(void);
+
+/* ---------------- Stage 1: image preparation (getDistanceCTF_local_Conj9combinedS2) ------ */
+
+/* Load + normalise, getDistanceCTF_local_Conj9combinedS2.py:246-283 (annularMask.py:19-32):
+ * out[i] = t.T (flipud'ed when conj[i] != 0) of stack[src[i]], normalised by the mean and
+ * ddof-0 std of t*(1-msk) taken over all N*N pixels, msk the disk r < N/2 centred on
+ * (N/2-1, N/2).  stack: float32 [n_stack][N][N] in SPIDER order. */
+int memi_load_normalize(const float* stack, int64_t n_stack, const int64_t* src,
+                        const int32_t* conj, int64_t n, int N, float* out, void* stream);
+
+/* Spectral multiply on rfft2 spectra, getDistanceCTF_local_Conj9combinedS2.py:285-293
+ * (create_grid/create_filter :139-167) and the cubic-spline prefilter of rotatefill.py:24
+ * (scipy ndimage.rotate order 3): kind is an OR of MEMI_FILTER_* flags. */
+int memi_spectrum_scale(void* spec, int64_t n, int N, int kind, double Qc, int order,
+                        void* stream);
+
+/* img[i] *= disk mask (annularMask.op(0, N/2, N, N)) when disk != 0, and *= msk2 [N][N]
+ * when msk2 != NULL; getDistanceCTF_local_Conj9combinedS2.py:326 and :344. */
+int memi_mask_multiply(float* img, int64_t n, int N, int disk, const float* msk2,
+                       void* stream);
+
+/* In-plane rotation, rotatefill.py:21-28: order-3 B-spline interpolation of the
+ * periodically extended coefficient image coef[i] (prefiltered with MEMI_FILTER_BSPLINE),
+ * rotated by angle_deg[i] about ((N-1)/2, (N-1)/2) with scipy.ndimage.rotate's axes.
+ * Equals the reference's 3x3-tile rotation up to the tile's ~0.27^(0.79 N) boundary term. */
+int memi_spline_rotate(const float* coef, const double* angle_deg, int64_t n, int N,
+                       float* out, void* stream);
+
+/* CTF + packed distance operand, ctemh_cryoFrank.py:24-44 and
+ * getDistanceCTF_local_Conj9combinedS2.py:336-344,392-396: from fy (rfft2 of the aligned
+ * images) and per-image defocus df (A), writes ctf_half float32 [n][Kh] (C on the half
+ * spectrum) and operand rows (op_hi only for FP32; op_hi + op_lo bf16 for SPLIT_BF16),
+ * starting at operand row `row0`. */
+int memi_ctf_pack(const void* fy, const double* df, int64_t n, int N,
+                  const memi_ctf_params* ctf, int precision, int64_t Khp, int64_t row0,
+                  void* op_hi, void* op_lo, float* ctf_half, void* stream);
+
+/* Phase flip, getDistanceCTF_local_Conj9combinedS2.py:346: out = sign(C) * fy. */
+int memi_phase_flip(const void* fy, const float* ctf_half, int64_t n, int N, void* out,
+                    void* stream);
+
+/* Wiener average numerator, getDistanceCTF_local_Conj9combinedS2.py:352-366,422-430:
+ * out[k] = sum_i spec_i[k] * C_i[k] / -(sum_i C_i[k]^2 + 1/snr) over the half spectrum. */
+int memi_wiener_average(const void* spec, const float* ctf_half, int64_t n, int N, double snr,
+                        void* out, void* stream);
+
+/* Per-pixel image moments over the image axis (getDistanceCTF...:348,367,400):
+ * sum[p] = sum_i img[i][p], sumsq[p] = sum_i img[i][p]^2 (float64). */
+int memi_image_moments(const float* img, int64_t n, int64_t npix, double* sum, double* sumsq,
+                       void* stream);
+
+/* ---------------- Stage 1: distance Gram (getDistanceCTF...:391-397) --------------------- */
+
+/* Batch plan (host only).  From pds_in (row0, n, k per PD; rows packed contiguously, i.e.
+ * row0 of PD p = sum of n over PDs before p) compute the table sizes into
+ * *info; when plan_host is non-NULL (capacity plan_cap bytes >= info->plan_bytes) also write
+ * the tables: PD descriptors with their offsets filled in, Gram work items over the 128x128
+ * upper-triangle tiles of every PD x 4 operand segments x `chunks`, the upper tiles, the full
+ * tiles and the row list.  The caller copies the plan to device memory once and passes it to
+ * every batched entry point below (plan_dev). */
+int memi_plan(const memi_pd_desc* pds_in_host, int npd, int chunks, void* plan_host,
+              int64_t plan_cap, memi_plan_info* info);
+
+/* CTF-weighted distances D_ij = sum_k |C_j X_ik - C_i X_jk|^2 for every PD of the plan,
+ * getDistanceCTF_local_Conj9combinedS2.py:391-397 (dead direct form :106-122).  MFMA Gram of
+ * the packed operand into per-segment partials (ws, info->ws_floats floats), float64
+ * reduction D = P - 2R with P = S + S^T the |X|^2 C^2 terms and R the cross term, exact zero
+ * diagonal, and float32 direct-form recomputation of entries whose cancellation ratio
+ * (P + 2|R|)/D exceeds `kappa` (flags: 3*info->flag_cap int32; *flag_count: int32 on the
+ * device, zeroed here, gets the number flagged).  D: float32 [info->d_elems].  fy/ctf_half:
+ * the spectra and CTFs that produced the operand (rows indexed like the operand). */
+int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const void* op_hi,
+                      const void* op_lo, int64_t Khp, int precision, double kappa,
+                      const void* fy, const float* ctf_half, int N, float* D, float* ws,
+                      int32_t* flags, int32_t* flag_count, void* stream);
+
+/* ---------------- Stage 2: diffusion map (DMembeddingII / fergusonE / slaplacianonFly) ---- */
+
+/* kNN selection, DMembeddingII.py:43-57: per row i of every PD, D_ii := -inf and the k
+ * smallest entries of row i are kept (ties broken by lower column index).  Writes the
+ * membership bitmask (bit j of row i at word mask_off + i*ceil(n/32) + j/32).  When idx_out /
+ * val_out are non-NULL, also the sorted lists in the reference layout of yInd1 / yVal1
+ * (k x n, column i = point i, first entry self with value 0) at list_off. */
+int memi_knn_select(const void* plan_dev, const memi_plan_info* info, const float* D,
+                    uint32_t* mask, int32_t* idx_out, float* val_out, void* stream);
+
+/* Ferguson bandwidth sweep, fergusonE.py:25-43 with the graph of DMembeddingII.py:97-140:
+ * for each PD and each logEps[e], sums over the kernel's entries x = v/(2 eps), v = 0 on the
+ * zero entries of the kNN lists (D < 1e-6, self included) and v = D_ij on the symmetrised
+ * kNN entries, of exp(-x) where x < thr = max(-log(0.01 * mean exp(-v/(2 max eps))), 10).
+ * out: float64 [npd][n_eps][3] = {sum exp(-x), sum x exp(-x), #terms} followed by
+ * [npd][2] = {#entries, sum exp(-v/(2 max eps))} (the threshold sums); zeroed here. */
+int memi_ferguson_sweep(const void* plan_dev, const memi_plan_info* info, const float* D,
+                        const uint32_t* mask, const double* logEps, int n_eps, double* out,
+                        void* stream);
+
+/* Markov normalisation, slaplacianonFly.py:42-81 with alpha = 1: W = exp(-v/sigma^2) on the
+ * kernel's entries, d = column sums, W/(d_i d_j), d' = sqrt(column sums), L = |L + L^T| / 2.
+ * sigma: float64 [npd]; L: float64 [info->d_elems]; deg: float64 scratch [2 * n_rows]. */
+int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, const float* D,
+                          const uint32_t* mask, const double* sigma, double* L, double* deg,
+                          void* stream);
+
+/* ---------------- Stage 3: NLSA (NLSA.py / get_wiener.py / svdRF.py / L2_distance.py) ---- */
+
+/* Reordered distances and delay embedding, psiAnalysisParS2.py:89-90 and NLSA.py:31-34:
+ * ConD[i][j] = sum_{c < con_order} D[ord[i+c]][ord[j+c]], summed in increasing c (float64),
+ * D float32 [n][n], ConD float64 [m][m], m = n - con_order. */
+int memi_nlsa_condist(const float* D, const int64_t* ord, int n, int con_order, double* ConD,
+                      void* stream);
+
+/* Wiener-filtered supervector spectra, NLSA.py:50-86 and get_wiener.py:9-21 by linearity:
+ * out[ii][c][k] = sum_{i<m} mu_psi[i][c] * G[s+i][k] / W_i[k], s = con_order-1-ii,
+ * W_i = sum_{j=i+1}^{i+con_order} C_j^2 + 1/snr, G_j = spec_j * C_j (images and CTFs already
+ * in psi-sorted order, half spectrum).  out: complex64 [con_order][ell][Kh]. */
+int memi_nlsa_supervectors(const void* spec, const float* ctf_half, int n, int N,
+                           int con_order, const double* mu_psi, int ell, double snr, void* out,
+                           void* stream);
+
+/* Tall-skinny products for svdRF.py:9-35: G = A^T A (A float64 [M][c], G [c][c]). */
+int memi_tall_gram(const double* A, int64_t M, int c, double* G, void* stream);
+/* Y = A * B (A float64 [M][c], B [c][d] given on the host, Y [M][d]). */
+int memi_tall_apply(const double* A, int64_t M, int c, const double* B_host, int d, double* Y,
+                    void* stream);
+
+/* Rank-2 reconstruction, NLSA.py:96-138 in factored form: IMGT[p][j] =
+ * (sum_q Ubar[p][q] H[q][j]) * inv_sd[j], Ubar float64 [npix][q_dim], H float64
+ * [q_dim][ncols], inv_sd [ncols], IMGT float64 [npix][ncols]. */
+int memi_nlsa_imgt(const double* Ubar, const double* H, const double* inv_sd, int64_t npix,
+                   int q_dim, int ncols, double* IMGT, void* stream);
+
+/* Column-centre U2 (float64 [npix][q_dim]) in place and return its Gram Ubar^T Ubar
+ * (float64 [q_dim][q_dim]); NLSA.py:112-138 + L2_distance.py:26-41 in factored form. */
+int memi_center_gram(double* U2, int64_t npix, int q_dim, double* G, void* stream);
+
+/* ---------------- Differentiation of the fused distance -> kernel op --------------------- */
+
+/* Forward: for each eps_e, S[e][i] = sum_j exp(-D_ij / (2 eps_e)) and
+ * T[e][i] = sum_j exp(-D_ij/(2 eps_e)) * D_ij (float64), D float32 [n][n]
+ * (the kernel of fergusonE.py:40). */
+int memi_kernel_rowsums(const float* D, int n, const double* eps, int n_eps, double* S,
+                        double* T, void* stream);
+
+/* Backward helper: Wt[i][j] = sum_e g[e] * (-1/(2 eps_e)) * (K_ij/S[e][i] + K_ji/S[e][j]),
+ * K recomputed from D (float32 [n][n]); Wt float32 [n][n]. */
+int memi_kernel_grad_weights(const float* D, int n, const double* eps, int n_eps,
+                             const double* S, const double* g, float* Wt, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MEMI_H */

@@ -1,0 +1,269 @@
+"""ctypes binding of libmemi.so (the HIP kernels) for torch device tensors.
+
+Every wrapper takes torch tensors that must already live on the GPU, enqueues on torch's
+current stream and raises ``RuntimeError`` with the library's message on a non-zero status.
+There is no CPU fallback: if the library is missing or a tensor is on the CPU the call fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmemi.so")
+
+PREC_FP32 = 0
+PREC_SPLIT_BF16 = 1
+FILTER_BUTTER = 1
+FILTER_GAUSS = 2
+FILTER_BSPLINE = 4
+
+c_i64 = ctypes.c_int64
+c_i32 = ctypes.c_int32
+c_int = ctypes.c_int
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+
+
+class PdDesc(ctypes.Structure):
+    _fields_ = [("row0", c_i64), ("d_off", c_i64), ("mask_off", c_i64), ("list_off", c_i64),
+                ("n", c_i32), ("k", c_i32)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("n_pd", c_i64), ("n_items", c_i64), ("n_utiles", c_i64), ("n_ftiles", c_i64),
+                ("n_rows", c_i64), ("off_pds", c_i64), ("off_items", c_i64),
+                ("off_utiles", c_i64), ("off_ftiles", c_i64), ("off_rows", c_i64),
+                ("plan_bytes", c_i64), ("d_elems", c_i64), ("mask_words", c_i64),
+                ("list_elems", c_i64), ("ws_floats", c_i64), ("flag_cap", c_i64),
+                ("chunks", c_i32), ("max_n", c_i32)]
+
+
+class CtfParams(ctypes.Structure):
+    _fields_ = [("pix_size", c_dbl), ("cs_mm", c_dbl), ("ekv", c_dbl), ("ampc", c_dbl),
+                ("gauss_env", c_dbl)]
+
+
+# name -> argument types (all return int status)
+_SIGS = {
+    "memi_load_normalize": [c_vp, c_i64, c_vp, c_vp, c_i64, c_int, c_vp, c_vp],
+    "memi_spectrum_scale": [c_vp, c_i64, c_int, c_int, c_dbl, c_int, c_vp],
+    "memi_mask_multiply": [c_vp, c_i64, c_int, c_int, c_vp, c_vp],
+    "memi_spline_rotate": [c_vp, c_vp, c_i64, c_int, c_vp, c_vp],
+    "memi_ctf_pack": [c_vp, c_vp, c_i64, c_int, ctypes.POINTER(CtfParams), c_int, c_i64, c_i64,
+                      c_vp, c_vp, c_vp, c_vp],
+    "memi_phase_flip": [c_vp, c_vp, c_i64, c_int, c_vp, c_vp],
+    "memi_wiener_average": [c_vp, c_vp, c_i64, c_int, c_dbl, c_vp, c_vp],
+    "memi_image_moments": [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp],
+    "memi_plan": [c_vp, c_int, c_int, c_vp, c_i64, ctypes.POINTER(PlanInfo)],
+    "memi_ctf_distance": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_i64, c_int, c_dbl, c_vp,
+                          c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "memi_knn_select": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp],
+    "memi_ferguson_sweep": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
+    "memi_markov_normalize": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "memi_nlsa_condist": [c_vp, c_vp, c_int, c_int, c_vp, c_vp],
+    "memi_nlsa_supervectors": [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_vp, c_vp],
+    "memi_tall_gram": [c_vp, c_i64, c_int, c_vp, c_vp],
+    "memi_tall_apply": [c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp],
+    "memi_nlsa_imgt": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp],
+    "memi_center_gram": [c_vp, c_i64, c_int, c_vp, c_vp],
+    "memi_kernel_rowsums": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp],
+    "memi_kernel_grad_weights": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
+}
+EXPORTS = ["memi_version", "memi_last_error"] + list(_SIGS)
+
+_lib = None
+
+
+def load():
+    """Load libmemi.so once (no GPU needed); raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python -m manifoldem_python_amd.build` (hipcc, gfx950)")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.memi_version.restype = c_int
+        lib.memi_last_error.restype = ctypes.c_char_p
+        for name, args in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = c_int
+        _lib = lib
+    return _lib
+
+
+def _check(status, name):
+    if status != 0:
+        msg = load().memi_last_error().decode(errors="replace")
+        raise RuntimeError(f"{name} failed ({status}): {msg}")
+
+
+def _p(t):
+    """Device pointer of a CUDA tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise RuntimeError("libmemi needs GPU tensors; got "
+                           f"{type(t).__name__} on {getattr(t, 'device', 'host')}")
+    if not t.is_contiguous():
+        raise RuntimeError("libmemi needs contiguous tensors")
+    return c_vp(t.data_ptr())
+
+
+def _stream():
+    return c_vp(torch.cuda.current_stream().cuda_stream)
+
+
+def call(name, *args):
+    _check(getattr(load(), name)(*args), name)
+
+
+# ------------------------------------------------------------------------- Stage 1 ---------
+def load_normalize(stack, src, conj, N, out):
+    if src.numel() and (int(src.min()) < 0 or int(src.max()) >= stack.shape[0]):
+        raise RuntimeError("load_normalize: image index outside the stack")
+    if tuple(stack.shape[1:]) != (N, N) or tuple(out.shape) != (src.shape[0], N, N):
+        raise RuntimeError("load_normalize: stack/out shapes do not match N")
+    call("memi_load_normalize", _p(stack), stack.shape[0], _p(src), _p(conj), src.shape[0], N,
+         _p(out), _stream())
+
+
+def spectrum_scale(spec, N, kind, Qc=1.0, order=1):
+    call("memi_spectrum_scale", _p(spec), spec.shape[0], N, kind, float(Qc), int(order), _stream())
+
+
+def mask_multiply(img, N, disk=True, msk2=None):
+    call("memi_mask_multiply", _p(img), img.shape[0], N, int(bool(disk)), _p(msk2), _stream())
+
+
+def spline_rotate(coef, angle_deg, N, out):
+    call("memi_spline_rotate", _p(coef), _p(angle_deg), coef.shape[0], N, _p(out), _stream())
+
+
+def ctf_params(pix_size, cs, ekv, ampc, gauss_env):
+    return CtfParams(float(pix_size), float(cs), float(ekv), float(ampc), float(gauss_env))
+
+
+def ctf_pack(fy, df, N, ctf, precision, Khp, row0, op_hi, op_lo, ctf_half):
+    call("memi_ctf_pack", _p(fy), _p(df), fy.shape[0], N, ctypes.byref(ctf), precision, Khp, row0,
+         _p(op_hi), _p(op_lo), _p(ctf_half), _stream())
+
+
+def phase_flip(fy, ctf_half, N, out):
+    call("memi_phase_flip", _p(fy), _p(ctf_half), fy.shape[0], N, _p(out), _stream())
+
+
+def wiener_average(spec, ctf_half, N, snr, out):
+    call("memi_wiener_average", _p(spec), _p(ctf_half), spec.shape[0], N, float(snr), _p(out),
+         _stream())
+
+
+def image_moments(img, s, s2):
+    n = img.shape[0]
+    call("memi_image_moments", _p(img), n, img.numel() // max(n, 1), _p(s), _p(s2), _stream())
+
+
+# ----------------------------------------------------------------------------- plans --------
+class Plan:
+    """Host-built batch plan (memi_plan) uploaded once to the device."""
+
+    def __init__(self, rows0, ns, ks, chunks=1, device="cuda"):
+        npd = len(ns)
+        descs = (PdDesc * max(npd, 1))()
+        for p in range(npd):
+            descs[p].row0 = int(rows0[p])
+            descs[p].n = int(ns[p])
+            descs[p].k = int(ks[p])
+        info = PlanInfo()
+        call("memi_plan", ctypes.cast(descs, c_vp), npd, chunks, None, 0, ctypes.byref(info))
+        host = np.zeros(max(info.plan_bytes, 16), dtype=np.uint8)
+        call("memi_plan", ctypes.cast(descs, c_vp), npd, chunks,
+             c_vp(host.ctypes.data), host.nbytes, ctypes.byref(info))
+        self.info = info
+        self.host = host
+        self.dev = torch.from_numpy(host).to(device)
+        pds = np.frombuffer(host[info.off_pds:info.off_pds + 40 * npd].tobytes(),
+                            dtype=np.dtype([("row0", "<i8"), ("d_off", "<i8"),
+                                            ("mask_off", "<i8"), ("list_off", "<i8"),
+                                            ("n", "<i4"), ("k", "<i4")]))
+        self.pds = pds
+        self.ns = np.asarray(ns, dtype=np.int64)
+        self.ks = np.asarray(ks, dtype=np.int64)
+        self.rows0 = np.asarray(rows0, dtype=np.int64)
+        self.chunks = chunks
+
+    @property
+    def npd(self):
+        return int(self.info.n_pd)
+
+    def block(self, flat, p):
+        """View of PD p's n x n block of a per-PD matrix buffer."""
+        n = int(self.pds["n"][p])
+        off = int(self.pds["d_off"][p])
+        return flat[off:off + n * n].view(n, n)
+
+
+def ctf_distance(plan, op_hi, op_lo, Khp, precision, kappa, fy, ctf_half, N, D, ws, flags,
+                 flag_count):
+    call("memi_ctf_distance", _p(plan.dev), ctypes.byref(plan.info), _p(op_hi), _p(op_lo), Khp,
+         precision, float(kappa), _p(fy), _p(ctf_half), N, _p(D), _p(ws), _p(flags),
+         _p(flag_count), _stream())
+
+
+# ------------------------------------------------------------------------- Stage 2 ---------
+def knn_select(plan, D, mask, idx_out=None, val_out=None):
+    call("memi_knn_select", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask), _p(idx_out),
+         _p(val_out), _stream())
+
+
+def ferguson_sweep(plan, D, mask, logEps, out):
+    call("memi_ferguson_sweep", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask),
+         _p(logEps), logEps.shape[0], _p(out), _stream())
+
+
+def markov_normalize(plan, D, mask, sigma, L, deg):
+    call("memi_markov_normalize", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask),
+         _p(sigma), _p(L), _p(deg), _stream())
+
+
+# ------------------------------------------------------------------------- Stage 3 ---------
+def nlsa_condist(D, ord_, n, con_order, ConD):
+    call("memi_nlsa_condist", _p(D), _p(ord_), n, con_order, _p(ConD), _stream())
+
+
+def nlsa_supervectors(spec, ctf_half, n, N, con_order, mu_psi, snr, out):
+    call("memi_nlsa_supervectors", _p(spec), _p(ctf_half), n, N, con_order, _p(mu_psi),
+         mu_psi.shape[1], float(snr), _p(out), _stream())
+
+
+def tall_gram(A, G):
+    call("memi_tall_gram", _p(A), A.shape[0], A.shape[1], _p(G), _stream())
+
+
+def tall_apply(A, B_host, Y):
+    B = np.ascontiguousarray(B_host, dtype=np.float64)
+    call("memi_tall_apply", _p(A), A.shape[0], A.shape[1], c_vp(B.ctypes.data), B.shape[1],
+         _p(Y), _stream())
+
+
+def nlsa_imgt(Ubar, H, inv_sd, IMGT):
+    call("memi_nlsa_imgt", _p(Ubar), _p(H), _p(inv_sd), Ubar.shape[0], Ubar.shape[1], H.shape[1],
+         _p(IMGT), _stream())
+
+
+def center_gram(U2, G):
+    call("memi_center_gram", _p(U2), U2.shape[0], U2.shape[1], _p(G), _stream())
+
+
+# ------------------------------------------------------------------ differentiation ---------
+def kernel_rowsums(D, eps, S, T):
+    call("memi_kernel_rowsums", _p(D), D.shape[0], _p(eps), eps.shape[0], _p(S), _p(T), _stream())
+
+
+def kernel_grad_weights(D, eps, S, g, Wt):
+    call("memi_kernel_grad_weights", _p(D), D.shape[0], _p(eps), eps.shape[0], _p(S), _p(g),
+         _p(Wt), _stream())

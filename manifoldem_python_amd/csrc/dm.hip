@@ -1,0 +1,450 @@
+// Stage 2 diffusion-map kernels: kNN selection, Ferguson bandwidth sweep, Markov
+// normalisation.  Reference: DMembeddingII.py:43-140, fergusonE.py:25-43,
+// slaplacianonFly.py:42-81.
+//
+// The reference builds a COO list from the kNN lists (zero entries first, then the
+// symmetrised non-zero entries) and then works on that list.  Here the list is implicit:
+// entry (i, j) of a PD belongs to the kernel
+//   * as a zero entry (value 0, weight exp(0) = 1) when j is on i's kNN list and
+//     D_ij < 1e-6 (always true for i == j: the self entry is reset to 0,
+//     DMembeddingII.py:53,115);
+//   * as a non-zero entry with value D_ij when i != j, D_ij >= 1e-6 and j is on i's list or
+//     i is on j's list (construct_matrix0, DMembeddingII.py:73-78, gives d^2 either way);
+//   * not at all otherwise.
+// M[i][j] = [j on i's kNN list] is the bitmask written by k_knn_select.
+#include "common.h"
+
+namespace memi {
+
+constexpr int TB = 128;  // tile side of the full-matrix tiles of the plan
+
+__device__ inline uint32_t float_key(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ inline bool mbit(const uint32_t* __restrict__ mask, int64_t off, int W, int i, int j) {
+  return (mask[off + (int64_t)i * W + (j >> 5)] >> (j & 31)) & 1u;
+}
+
+// Value of entry (i, j) in the kernel, or -1 when it is not an entry.  dij = D_ij = D_ji
+// (D is exactly symmetric), mij = [j on i's list], mji = [i on j's list].
+__device__ inline double entry_from(float dij, bool mij, bool mji, bool diag) {
+  if (diag) return 0.0;
+  if (!((double)dij >= 1e-6)) return mij ? 0.0 : -1.0;
+  return (mij || mji) ? (double)dij : -1.0;
+}
+
+__device__ inline double entry_value(const float* __restrict__ D, const uint32_t* __restrict__ mask,
+                                     const memi_pd_desc& P, int W, bool full, int i, int j) {
+  const float dij = D[P.d_off + (int64_t)i * P.n + j];
+  const bool mij = full || mbit(mask, P.mask_off, W, i, j);
+  const bool mji = full || mbit(mask, P.mask_off, W, j, i);
+  return entry_from(dij, mij, mji, i == j);
+}
+
+// inclusive block scan (256 threads) of an int; returns exclusive prefix, total in *tot
+__device__ inline int block_excl_scan(int v, int* sh, int* tot) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  int base = 0, all = 0;
+  for (int w = 0; w < 4; ++w) {
+    if (w < wid) base += sh[w];
+    all += sh[w];
+  }
+  __syncthreads();
+  *tot = all;
+  return base + x - v;
+}
+
+// ----------------------------------------------------------------------------- kNN ------
+// One block per row.  Dynamic LDS: keys[max_n] (u32) + member flags[max_n] (u8) and, for the
+// optional sorted lists, pairs[pow2 >= k] (u64).
+__global__ __launch_bounds__(256) void k_knn_select(const int32_t* __restrict__ rows,
+                                                    const memi_pd_desc* __restrict__ pds,
+                                                    const float* __restrict__ D,
+                                                    uint32_t* __restrict__ mask,
+                                                    int32_t* __restrict__ idx_out,
+                                                    float* __restrict__ val_out, int max_n,
+                                                    int sort_cap) {
+  extern __shared__ __attribute__((aligned(16))) char dyn[];
+  uint32_t* keys = (uint32_t*)dyn;
+  unsigned char* mem = (unsigned char*)(keys + max_n);
+  unsigned long long* pairs =
+      (unsigned long long*)(dyn + (((size_t)max_n * 5 + 15) & ~(size_t)15));
+  __shared__ int hist[256];
+  __shared__ int scan_sh[8];
+  __shared__ int sel_bin, sel_rem;
+
+  const int p = rows[2 * blockIdx.x], i = rows[2 * blockIdx.x + 1];
+  const memi_pd_desc P = pds[p];
+  const int n = P.n;
+  const int k = min(P.k, n);
+  const int W = (n + 31) / 32;
+  const float* Drow = D + P.d_off + (int64_t)i * n;
+  for (int j = threadIdx.x; j < n; j += blockDim.x)
+    keys[j] = (j == i) ? float_key(-INFINITY) : float_key(Drow[j]);
+  __syncthreads();
+
+  uint32_t T = 0xFFFFFFFFu;
+  int need = n;
+  if (k < n) {
+    uint32_t prefix = 0, pm = 0;
+    int rem = k;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+      hist[threadIdx.x] = 0;
+      __syncthreads();
+      for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const uint32_t key = keys[j];
+        if ((key & pm) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int cum = 0, b = 0;
+        for (; b < 256; ++b) {
+          if (cum + hist[b] >= rem) break;
+          cum += hist[b];
+        }
+        sel_bin = b;
+        sel_rem = rem - cum;
+      }
+      __syncthreads();
+      prefix |= (uint32_t)sel_bin << shift;
+      pm |= 0xFFu << shift;
+      rem = sel_rem;
+      __syncthreads();
+    }
+    T = prefix;
+    need = rem;  // elements equal to T to keep, lowest column first
+  }
+  // member flags: key < T, or key == T with tie rank < need (index order)
+  const int chunk = (n + blockDim.x - 1) / blockDim.x;
+  const int j0 = min(n, (int)threadIdx.x * chunk), j1 = min(n, j0 + chunk);
+  int eq = 0;
+  for (int j = j0; j < j1; ++j) eq += (keys[j] == T);
+  int tot;
+  int rank = block_excl_scan(eq, scan_sh, &tot);
+  for (int j = j0; j < j1; ++j) {
+    const uint32_t key = keys[j];
+    unsigned char m = 0;
+    if (k >= n || key < T) m = 1;
+    else if (key == T) m = (rank++ < need) ? 1 : 0;
+    mem[j] = m;
+  }
+  __syncthreads();
+  uint32_t* mrow = mask + P.mask_off + (int64_t)i * W;
+  for (int w = threadIdx.x; w < W; w += blockDim.x) {
+    uint32_t bits = 0;
+    for (int b = 0; b < 32; ++b) {
+      const int j = w * 32 + b;
+      if (j < n && mem[j]) bits |= 1u << b;
+    }
+    mrow[w] = bits;
+  }
+  if (idx_out == nullptr && val_out == nullptr) return;
+  // sorted lists: compact members, bitonic sort on (key, index)
+  int cnt = 0;
+  for (int j = j0; j < j1; ++j) cnt += mem[j];
+  int pos = block_excl_scan(cnt, scan_sh, &tot);
+  for (int j = j0; j < j1; ++j)
+    if (mem[j]) pairs[pos++] = ((unsigned long long)keys[j] << 32) | (unsigned)j;
+  int L = 1;
+  while (L < tot) L <<= 1;
+  for (int e = tot + threadIdx.x; e < L; e += blockDim.x) pairs[e] = ~0ULL;
+  __syncthreads();
+  for (int size = 2; size <= L; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int e = threadIdx.x; e < L; e += blockDim.x) {
+        const int o = e ^ stride;
+        if (o > e) {
+          const bool up = ((e & size) == 0);
+          const unsigned long long a = pairs[e], b = pairs[o];
+          if ((a > b) == up) {
+            pairs[e] = b;
+            pairs[o] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int r = threadIdx.x; r < k; r += blockDim.x) {
+    const int j = (int)(pairs[r] & 0xFFFFFFFFu);
+    if (idx_out) idx_out[P.list_off + (int64_t)r * n + i] = j;
+    if (val_out) val_out[P.list_off + (int64_t)r * n + i] = (r == 0) ? 0.f : Drow[j];
+  }
+}
+
+// ------------------------------------------------------------------------ Ferguson ------
+// thr pass: per PD, count of entries and sum exp(-v / (2 max eps))  -> acc[pd][2]
+__global__ __launch_bounds__(256) void k_ferguson_thr(const int32_t* __restrict__ ftiles,
+                                                      const memi_pd_desc* __restrict__ pds,
+                                                      const float* __restrict__ D,
+                                                      const uint32_t* __restrict__ mask,
+                                                      const double* __restrict__ logEps,
+                                                      int n_eps, double* __restrict__ acc) {
+  __shared__ double red[8];
+  __shared__ double mx_sh[4];
+  // fergusonE.py:27: d = D2 / (2 max(eps))
+  double mx = -INFINITY;
+  for (int e = threadIdx.x; e < n_eps; e += blockDim.x) mx = fmax(mx, logEps[e]);
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) mx_sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  mx = fmax(fmax(mx_sh[0], mx_sh[1]), fmax(mx_sh[2], mx_sh[3]));
+  const double cmin = 1.0 / (2.0 * exp(mx));
+  const int32_t* f = ftiles + 4 * (int64_t)blockIdx.x;
+  const memi_pd_desc P = pds[f[0]];
+  const int W = (P.n + 31) / 32;
+  const bool full = P.k >= P.n;
+  double s = 0.0, c = 0.0;
+  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
+    const int i = f[1] * TB + e / TB, j = f[2] * TB + e % TB;
+    if (i >= P.n || j >= P.n) continue;
+    const double v = entry_value(D, mask, P, W, full, i, j);
+    if (v < 0) continue;
+    s += exp(-v * cmin);
+    c += 1.0;
+  }
+  s = block_sum(s, red);
+  c = block_sum(c, red);
+  if (threadIdx.x == 0) {
+    atomicAdd(&acc[2 * f[0]], c);
+    atomicAdd(&acc[2 * f[0] + 1], s);
+  }
+}
+
+// sweep: block = (256 eps values) x (one full tile); entries staged through LDS
+__global__ __launch_bounds__(256) void k_ferguson_sweep(const int32_t* __restrict__ ftiles,
+                                                        const memi_pd_desc* __restrict__ pds,
+                                                        const float* __restrict__ D,
+                                                        const uint32_t* __restrict__ mask,
+                                                        const double* __restrict__ logEps,
+                                                        int n_eps,
+                                                        const double* __restrict__ thr_acc,
+                                                        double* __restrict__ out) {
+  constexpr int SUB = 2048;
+  __shared__ float vals[SUB];
+  __shared__ int scan_sh[8];
+  const int32_t* f = ftiles + 4 * (int64_t)blockIdx.y;
+  const int pd = f[0];
+  const memi_pd_desc P = pds[pd];
+  const int W = (P.n + 31) / 32;
+  const bool full = P.k >= P.n;
+  const int e_idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = e_idx < n_eps;
+  // fergusonE.py:25-31: thr = max(-log(0.01 * ss / len), 10)
+  const double thr = fmax(-log(0.01 * thr_acc[2 * pd + 1] / thr_acc[2 * pd]), 10.0);
+  const double c = active ? 1.0 / (2.0 * exp(logEps[e_idx])) : 0.0;
+  const double L2E = 1.4426950408889634;
+  double S = 0.0, SX = 0.0, CN = 0.0;
+  for (int base = 0; base < TB * TB; base += SUB) {
+    // stage this thread's 8 entries, compact the kernel entries
+    float v8[SUB / 256];
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < SUB / 256; ++q) {
+      const int e = base + threadIdx.x * (SUB / 256) + q;
+      const int i = f[1] * TB + e / TB, j = f[2] * TB + e % TB;
+      double v = -1.0;
+      if (i < P.n && j < P.n) v = entry_value(D, mask, P, W, full, i, j);
+      v8[q] = (float)v;
+      cnt += (v >= 0);
+    }
+    int tot;
+    int pos = block_excl_scan(cnt, scan_sh, &tot);
+#pragma unroll
+    for (int q = 0; q < SUB / 256; ++q)
+      if (v8[q] >= 0.f) vals[pos++] = v8[q];
+    __syncthreads();
+    if (active) {
+      for (int t = 0; t < tot; ++t) {
+        const double x = (double)vals[t] * c;
+        if (x < thr) {
+          // exp(-x) = 2^-(y) with y = x log2(e) split into integer and fraction in float64
+          const double y = x * L2E;
+          const double fl = floor(y);
+          const float r = exp2f(-(float)(y - fl));
+          const double ex = ldexp((double)r, -(int)fl);
+          S += ex;
+          SX += x * ex;
+          CN += 1.0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (active) {
+    double* o = out + ((int64_t)pd * n_eps + e_idx) * 3;
+    atomicAdd(o + 0, S);
+    atomicAdd(o + 1, SX);
+    atomicAdd(o + 2, CN);
+  }
+}
+
+// ------------------------------------------------------------------ normalisation ------
+__device__ inline double kernel_weight(double v, double inv_s2) {
+  return v < 0 ? 0.0 : (v == 0.0 ? 1.0 : exp(-v * inv_s2));
+}
+
+// pass 0: deg1[j] += sum_i W_ij ; pass 1: deg2[j] += sum_i W_ij / deg1[i]
+template <int PASS>
+__global__ __launch_bounds__(256) void k_markov_colsum(const int32_t* __restrict__ ftiles,
+                                                       const memi_pd_desc* __restrict__ pds,
+                                                       const float* __restrict__ D,
+                                                       const uint32_t* __restrict__ mask,
+                                                       const double* __restrict__ sigma,
+                                                       double* __restrict__ deg,
+                                                       int64_t n_rows) {
+  const int32_t* f = ftiles + 4 * (int64_t)blockIdx.x;
+  const int pd = f[0];
+  const memi_pd_desc P = pds[pd];
+  const int W = (P.n + 31) / 32;
+  const bool full = P.k >= P.n;
+  const double inv_s2 = 1.0 / (sigma[pd] * sigma[pd]);
+  const int cl = threadIdx.x & (TB - 1), half = threadIdx.x >> 7;
+  const int j = f[2] * TB + cl;
+  if (j >= P.n) return;
+  double s = 0.0;
+  for (int r = half; r < TB; r += 2) {
+    const int i = f[1] * TB + r;
+    if (i >= P.n) break;
+    const double w = kernel_weight(entry_value(D, mask, P, W, full, i, j), inv_s2);
+    s += (PASS == 0) ? w : w / deg[P.row0 + i];
+  }
+  atomicAdd(&deg[(PASS == 0 ? 0 : n_rows) + P.row0 + j], s);
+}
+
+__global__ __launch_bounds__(256) void k_markov_write(const int32_t* __restrict__ ftiles,
+                                                      const memi_pd_desc* __restrict__ pds,
+                                                      const float* __restrict__ D,
+                                                      const uint32_t* __restrict__ mask,
+                                                      const double* __restrict__ sigma,
+                                                      const double* __restrict__ deg,
+                                                      int64_t n_rows, double* __restrict__ L) {
+  const int32_t* f = ftiles + 4 * (int64_t)blockIdx.x;
+  const int pd = f[0];
+  const memi_pd_desc P = pds[pd];
+  const int W = (P.n + 31) / 32;
+  const bool full = P.k >= P.n;
+  const double inv_s2 = 1.0 / (sigma[pd] * sigma[pd]);
+  const int cl = threadIdx.x & (TB - 1), half = threadIdx.x >> 7;
+  const int j = f[2] * TB + cl;
+  if (j >= P.n) return;
+  const double dj = deg[P.row0 + j];
+  const double dpj = sqrt(deg[n_rows + P.row0 + j] / dj);
+  for (int r = half; r < TB; r += 2) {
+    const int i = f[1] * TB + r;
+    if (i >= P.n) break;
+    const double di = deg[P.row0 + i];
+    const double dpi = sqrt(deg[n_rows + P.row0 + i] / di);
+    const float dij = D[P.d_off + (int64_t)i * P.n + j];
+    const bool mij = full || mbit(mask, P.mask_off, W, i, j);
+    const bool mji = full || mbit(mask, P.mask_off, W, j, i);
+    const double wij = kernel_weight(entry_from(dij, mij, mji, i == j), inv_s2);
+    const double wji = kernel_weight(entry_from(dij, mji, mij, i == j), inv_s2);
+    // slaplacianonFly.py:63-78: W/(d_i d_j), then /(d'_i d'_j), then |l + l^T| / 2
+    const double lij = wij / (di * dj) / (dpi * dpj);
+    const double lji = wji / (dj * di) / (dpj * dpi);
+    L[P.d_off + (int64_t)i * P.n + j] = fabs(lij + lji) / 2.0;
+  }
+}
+
+__global__ void k_zero_f64(double* p, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = 0.0;
+}
+
+}  // namespace memi
+
+using namespace memi;
+
+namespace {
+struct PlanView {
+  const memi_pd_desc* pds;
+  const int32_t* ftiles;
+  const int32_t* rows;
+};
+PlanView view(const void* plan, const memi_plan_info* info) {
+  const char* b = (const char*)plan;
+  return PlanView{(const memi_pd_desc*)(b + info->off_pds),
+                  (const int32_t*)(b + info->off_ftiles), (const int32_t*)(b + info->off_rows)};
+}
+}  // namespace
+
+extern "C" {
+
+int memi_knn_select(const void* plan_dev, const memi_plan_info* info, const float* D,
+                    uint32_t* mask, int32_t* idx_out, float* val_out, void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && D && mask, "memi_knn_select: null pointer");
+  if (info->n_rows == 0) return MEMI_OK;
+  const int max_n = info->max_n;
+  const bool lists = idx_out || val_out;
+  int L = 1;
+  while (L < max_n) L <<= 1;
+  size_t shm = (((size_t)max_n * 5 + 15) & ~(size_t)15) + (lists ? (size_t)L * 8 : 0);
+  MEMI_CHECK_ARG(shm <= 160 * 1024 - 2048, "memi_knn_select: n=%d too large for %s", max_n,
+                 lists ? "sorted lists (n <= 8192)" : "the row buffer");
+  PlanView v = view(plan_dev, info);
+  hipLaunchKernelGGL(k_knn_select, dim3((unsigned)info->n_rows), dim3(256), shm,
+                     (hipStream_t)stream, v.rows, v.pds, D, mask, idx_out, val_out, max_n, L);
+  MEMI_CHECK_LAUNCH("k_knn_select");
+  return MEMI_OK;
+}
+
+int memi_ferguson_sweep(const void* plan_dev, const memi_plan_info* info, const float* D,
+                        const uint32_t* mask, const double* logEps, int n_eps, double* out,
+                        void* stream) {
+  // out holds npd*n_eps*3 results followed by 2*npd threshold sums (count, sum exp)
+  MEMI_CHECK_ARG(plan_dev && info && D && mask && logEps && out && n_eps > 0,
+                 "memi_ferguson_sweep: bad arguments");
+  if (info->n_pd == 0) return MEMI_OK;
+  hipStream_t s = (hipStream_t)stream;
+  PlanView v = view(plan_dev, info);
+  // scratch for the threshold sums lives after the outputs: out[npd*n_eps*3 + 2*npd]
+  double* thr_acc = out + info->n_pd * (int64_t)n_eps * 3;
+  const int64_t nz = info->n_pd * ((int64_t)n_eps * 3 + 2);
+  hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(nz)), dim3(256), 0, s, out, nz);
+  MEMI_CHECK_LAUNCH("k_zero_f64");
+  hipLaunchKernelGGL(k_ferguson_thr, dim3((unsigned)info->n_ftiles), dim3(256), 0, s, v.ftiles,
+                     v.pds, D, mask, logEps, n_eps, thr_acc);
+  MEMI_CHECK_LAUNCH("k_ferguson_thr");
+  dim3 grid((unsigned)((n_eps + 255) / 256), (unsigned)info->n_ftiles);
+  hipLaunchKernelGGL(k_ferguson_sweep, grid, dim3(256), 0, s, v.ftiles, v.pds, D, mask, logEps,
+                     n_eps, thr_acc, out);
+  MEMI_CHECK_LAUNCH("k_ferguson_sweep");
+  return MEMI_OK;
+}
+
+int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, const float* D,
+                          const uint32_t* mask, const double* sigma, double* L, double* deg,
+                          void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && D && mask && sigma && L && deg,
+                 "memi_markov_normalize: null pointer");
+  if (info->n_pd == 0) return MEMI_OK;
+  hipStream_t s = (hipStream_t)stream;
+  PlanView v = view(plan_dev, info);
+  const int64_t nr = info->n_rows;
+  hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(2 * nr)), dim3(256), 0, s, deg, 2 * nr);
+  MEMI_CHECK_LAUNCH("k_zero_f64");
+  const dim3 g((unsigned)info->n_ftiles);
+  hipLaunchKernelGGL(k_markov_colsum<0>, g, dim3(256), 0, s, v.ftiles, v.pds, D, mask, sigma,
+                     deg, nr);
+  MEMI_CHECK_LAUNCH("k_markov_colsum<0>");
+  hipLaunchKernelGGL(k_markov_colsum<1>, g, dim3(256), 0, s, v.ftiles, v.pds, D, mask, sigma,
+                     deg, nr);
+  MEMI_CHECK_LAUNCH("k_markov_colsum<1>");
+  hipLaunchKernelGGL(k_markov_write, g, dim3(256), 0, s, v.ftiles, v.pds, D, mask, sigma, deg,
+                     nr, L);
+  MEMI_CHECK_LAUNCH("k_markov_write");
+  return MEMI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,359 @@
+// Stage 1 image preparation kernels (the elementwise work between batched FFTs).
+// Reference: getDistanceCTF_local_Conj9combinedS2.py:244-367 and its leaves
+// annularMask.py:19-32, rotatefill.py:21-28, ctemh_cryoFrank.py:24-44.
+#include "common.h"
+
+namespace memi {
+
+// annularMask.op(0, N/2, N, N): r^2 < (N/2)^2 about row N/2-1, column N/2.
+__device__ inline bool in_disk(int r, int c, int N) {
+  const double xd = r - 0.5 * N + 1.0;
+  const double yd = c - 0.5 * N;
+  return xd * xd + yd * yd < 0.25 * (double)N * (double)N;
+}
+
+// One block per image: background statistics then a tiled transpose into the output.
+__global__ __launch_bounds__(256) void k_load_normalize(const float* __restrict__ stack,
+                                                        const int64_t* __restrict__ src,
+                                                        const int32_t* __restrict__ conj, int N,
+                                                        float* __restrict__ out) {
+  __shared__ double red[8];
+  __shared__ float tile[32][33];
+  const int64_t img = blockIdx.x;
+  const float* raw = stack + src[img] * (int64_t)N * N;
+  const bool cj = conj != nullptr && conj[img] != 0;
+  const int NN = N * N;
+  // raw[a][b] is t[b][a] (t = raw.T); with flipud, t'[r][c] = raw[c][N-1-r].
+  double s = 0.0;
+  for (int p = threadIdx.x; p < NN; p += blockDim.x) {
+    const int a = p / N, b = p % N;
+    const int r = cj ? N - 1 - b : b, c = a;
+    if (!in_disk(r, c, N)) s += (double)raw[p];
+  }
+  const double mean = block_sum(s, red) / NN;
+  double v = 0.0;
+  for (int p = threadIdx.x; p < NN; p += blockDim.x) {
+    const int a = p / N, b = p % N;
+    const int r = cj ? N - 1 - b : b, c = a;
+    const double bg = in_disk(r, c, N) ? 0.0 : (double)raw[p];
+    v += (bg - mean) * (bg - mean);
+  }
+  const double sd = sqrt(block_sum(v, red) / NN);
+  float* o = out + img * (int64_t)NN;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int a0 = 0; a0 < N; a0 += 32) {
+    for (int b0 = 0; b0 < N; b0 += 32) {
+      for (int y = ty; y < 32; y += 8) {
+        const int a = a0 + y, b = b0 + tx;
+        tile[y][tx] = (a < N && b < N) ? raw[(int64_t)a * N + b] : 0.f;
+      }
+      __syncthreads();
+      // output row r from raw column b, output column c = raw row a
+      for (int y = ty; y < 32; y += 8) {
+        const int b = b0 + y, a = a0 + tx;
+        if (a < N && b < N) {
+          const int r = cj ? N - 1 - b : b;
+          o[(int64_t)r * N + a] = (float)(((double)tile[tx][y] - mean) / sd);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// cubic B-spline transfer function (4 + 2 cos w) / 6 at FFT index i of n
+__device__ inline double bspline_hat(int i, int n) {
+  return (4.0 + 2.0 * cospi(2.0 * (double)i / (double)n)) / 6.0;
+}
+
+__global__ void k_spectrum_scale(float2* __restrict__ spec, int64_t total, int N, int kind,
+                                 double Qc, int order) {
+  const int Nh = N / 2 + 1;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int col = (int)(e % Nh);
+    const int row = (int)((e / Nh) % N);
+    double f = 1.0;
+    if (kind & (MEMI_FILTER_BUTTER | MEMI_FILTER_GAUSS)) {
+      const double q = radial_q(row, col, N) / Qc;
+      if (kind & MEMI_FILTER_BUTTER) f *= sqrt(1.0 / (1.0 + pow(q, 2.0 * order)));
+      if (kind & MEMI_FILTER_GAUSS) f *= exp(-(log(2.0) / 2.0) * q * q);
+    }
+    if (kind & MEMI_FILTER_BSPLINE) f /= bspline_hat(row, N) * bspline_hat(col, N);
+    float2 v = spec[e];
+    v.x = (float)(v.x * f);
+    v.y = (float)(v.y * f);
+    spec[e] = v;
+  }
+}
+
+__global__ void k_mask_multiply(float* __restrict__ img, int64_t total, int N, int disk,
+                                const float* __restrict__ msk2) {
+  const int NN = N * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int p = (int)(e % NN);
+    float m = 1.f;
+    if (disk && !in_disk(p / N, p % N, N)) m = 0.f;
+    if (msk2) m *= msk2[p];
+    img[e] *= m;
+  }
+}
+
+// rotatefill: output (r, c) samples the periodic cubic spline at
+//   src_r = h + cos*(r-h) + sin*(c-h),  src_c = h - sin*(r-h) + cos*(c-h),  h = (N-1)/2,
+// which is scipy.ndimage.rotate's affine map rot_matrix = [[cos, sin], [-sin, cos]] about
+// the centre of the 3N tile, restricted to the central copy.
+__device__ inline void bspline_w(float t, float w[4]) {
+  const float u = 1.f - t;
+  const float t2 = t * t, t3 = t2 * t;
+  w[0] = u * u * u * (1.f / 6.f);
+  w[1] = 2.f / 3.f - t2 + 0.5f * t3;
+  w[2] = 2.f / 3.f - u * u + 0.5f * u * u * u;
+  w[3] = t3 * (1.f / 6.f);
+}
+
+__global__ __launch_bounds__(256) void k_spline_rotate(const float* __restrict__ coef,
+                                                       const double* __restrict__ angle_deg,
+                                                       int N, int64_t n, float* __restrict__ out) {
+  const int NN = N * N;
+  const int64_t total = n * NN;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t img = e / NN;
+    const int p = (int)(e % NN);
+    const int r = p / N, c = p % N;
+    double sn, cs;
+    sincospi(angle_deg[img] / 180.0, &sn, &cs);
+    const double h = 0.5 * (N - 1);
+    const double dr = r - h, dc = c - h;
+    const double sr = h + cs * dr + sn * dc;
+    const double sc = h - sn * dr + cs * dc;
+    const double fr = floor(sr), fc = floor(sc);
+    float wr[4], wc[4];
+    bspline_w((float)(sr - fr), wr);
+    bspline_w((float)(sc - fc), wc);
+    const float* cf = coef + img * NN;
+    int ir = ((int)fr - 1) % N;
+    if (ir < 0) ir += N;
+    int ic0 = ((int)fc - 1) % N;
+    if (ic0 < 0) ic0 += N;
+    float acc = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const float* rowp = cf + ir * N;
+      int ic = ic0;
+      float racc = 0.f;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        racc += wc[b] * rowp[ic];
+        ic = (ic + 1 == N) ? 0 : ic + 1;
+      }
+      acc += wr[a] * racc;
+      ir = (ir + 1 == N) ? 0 : ir + 1;
+    }
+    out[e] = acc;
+  }
+}
+
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__global__ void k_ctf_pack(const float2* __restrict__ fy, const double* __restrict__ df,
+                           int64_t n, int N, memi_ctf_params cp, int prec, int64_t Khp,
+                           int64_t row0, void* __restrict__ op_hi, void* __restrict__ op_lo,
+                           float* __restrict__ ctf_half) {
+  const int Nh = N / 2 + 1;
+  const int64_t Kh = (int64_t)N * Nh;
+  const int64_t total = n * Khp;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / Khp;
+    const int64_t k = e % Khp;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (k < Kh) {
+      const int row = (int)(k / Nh), col = (int)(k % Nh);
+      const double kk = radial_q(row, col, N) / (2.0 * cp.pix_size);
+      const double C = ctf_value(kk, df[i], cp.cs_mm, cp.ekv, cp.ampc, cp.gauss_env);
+      ctf_half[i * Kh + k] = (float)C;
+      const float2 X = fy[i * Kh + k];
+      const double w = half_weight(col, N);
+      const double sw = sqrt(w);
+      v[0] = w * ((double)X.x * X.x + (double)X.y * X.y);
+      v[1] = C * C;
+      v[2] = sw * C * X.x;
+      v[3] = sw * C * X.y;
+    }
+    const int64_t base = (row0 + i) * 4 * Khp + k;
+    if (prec == MEMI_PREC_FP32) {
+      float* o = (float*)op_hi;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) o[base + s * Khp] = (float)v[s];
+    } else {
+      uint16_t* oh = (uint16_t*)op_hi;
+      uint16_t* ol = (uint16_t*)op_lo;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const float f = (float)v[s];
+        const uint16_t h = bf16_rne(f);
+        oh[base + s * Khp] = h;
+        ol[base + s * Khp] = bf16_rne((float)((double)f - (double)bf16_to_f(h)));
+      }
+    }
+  }
+}
+
+__global__ void k_phase_flip(const float2* __restrict__ fy, const float* __restrict__ ctf,
+                             int64_t total, float2* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const float c = ctf[e];
+    const float s = (c > 0.f) ? 1.f : ((c < 0.f) ? -1.f : 0.f);
+    float2 v = fy[e];
+    v.x *= s;
+    v.y *= s;
+    out[e] = v;
+  }
+}
+
+// thread per half-spectrum bin: loop over images (coalesced across bins)
+__global__ void k_wiener_average(const float2* __restrict__ spec, const float* __restrict__ ctf,
+                                 int64_t n, int64_t Kh, double snr, float2* __restrict__ out) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < Kh;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    double nr = 0.0, ni = 0.0, den = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double c = ctf[i * Kh + k];
+      const float2 x = spec[i * Kh + k];
+      nr += x.x * c;
+      ni += x.y * c;
+      den += c * c;
+    }
+    const double w = -(den + 1.0 / snr);
+    out[k] = make_float2((float)(nr / w), (float)(ni / w));
+  }
+}
+
+__global__ void k_image_moments(const float* __restrict__ img, int64_t n, int64_t npix,
+                                double* __restrict__ sum, double* __restrict__ sumsq) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0, s2 = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double v = img[i * npix + p];
+      s += v;
+      s2 += v * v;
+    }
+    sum[p] = s;
+    sumsq[p] = s2;
+  }
+}
+
+}  // namespace memi
+
+using namespace memi;
+
+extern "C" {
+
+int memi_load_normalize(const float* stack, int64_t n_stack, const int64_t* src,
+                        const int32_t* conj, int64_t n, int N, float* out, void* stream) {
+  MEMI_CHECK_ARG(stack && src && out, "memi_load_normalize: null pointer");
+  MEMI_CHECK_ARG(n >= 0 && N > 0 && n_stack >= 0, "memi_load_normalize: bad sizes");
+  if (n == 0) return MEMI_OK;
+  hipLaunchKernelGGL(k_load_normalize, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream,
+                     stack, src, conj, N, out);
+  MEMI_CHECK_LAUNCH("k_load_normalize");
+  return MEMI_OK;
+}
+
+int memi_spectrum_scale(void* spec, int64_t n, int N, int kind, double Qc, int order,
+                        void* stream) {
+  MEMI_CHECK_ARG(spec && n >= 0 && N > 0, "memi_spectrum_scale: bad arguments");
+  MEMI_CHECK_ARG((kind & ~(MEMI_FILTER_BUTTER | MEMI_FILTER_GAUSS | MEMI_FILTER_BSPLINE)) == 0,
+                 "memi_spectrum_scale: unknown filter kind %d", kind);
+  MEMI_CHECK_ARG(!(kind & (MEMI_FILTER_BUTTER | MEMI_FILTER_GAUSS)) || Qc > 0,
+                 "memi_spectrum_scale: Qc must be positive");
+  const int64_t total = n * (int64_t)N * (N / 2 + 1);
+  if (total == 0) return MEMI_OK;
+  hipLaunchKernelGGL(k_spectrum_scale, dim3(grid_for(total)), dim3(256), 0,
+                     (hipStream_t)stream, (float2*)spec, total, N, kind, Qc, order);
+  MEMI_CHECK_LAUNCH("k_spectrum_scale");
+  return MEMI_OK;
+}
+
+int memi_mask_multiply(float* img, int64_t n, int N, int disk, const float* msk2, void* stream) {
+  MEMI_CHECK_ARG(img && n >= 0 && N > 0, "memi_mask_multiply: bad arguments");
+  const int64_t total = n * (int64_t)N * N;
+  if (total == 0 || (!disk && !msk2)) return MEMI_OK;
+  hipLaunchKernelGGL(k_mask_multiply, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream,
+                     img, total, N, disk, msk2);
+  MEMI_CHECK_LAUNCH("k_mask_multiply");
+  return MEMI_OK;
+}
+
+int memi_spline_rotate(const float* coef, const double* angle_deg, int64_t n, int N, float* out,
+                       void* stream) {
+  MEMI_CHECK_ARG(coef && angle_deg && out && n >= 0 && N > 0, "memi_spline_rotate: bad args");
+  MEMI_CHECK_ARG(coef != out, "memi_spline_rotate: in-place rotation is not supported");
+  const int64_t total = n * (int64_t)N * N;
+  if (total == 0) return MEMI_OK;
+  hipLaunchKernelGGL(k_spline_rotate, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream,
+                     coef, angle_deg, N, n, out);
+  MEMI_CHECK_LAUNCH("k_spline_rotate");
+  return MEMI_OK;
+}
+
+int memi_ctf_pack(const void* fy, const double* df, int64_t n, int N, const memi_ctf_params* ctf,
+                  int precision, int64_t Khp, int64_t row0, void* op_hi, void* op_lo,
+                  float* ctf_half, void* stream) {
+  MEMI_CHECK_ARG(fy && df && ctf && op_hi && ctf_half, "memi_ctf_pack: null pointer");
+  MEMI_CHECK_ARG(precision == MEMI_PREC_FP32 || (precision == MEMI_PREC_SPLIT_BF16 && op_lo),
+                 "memi_ctf_pack: precision %d needs its operand buffers", precision);
+  const int64_t Kh = (int64_t)N * (N / 2 + 1);
+  MEMI_CHECK_ARG(Khp >= Kh && Khp % 64 == 0, "memi_ctf_pack: Khp=%lld must be >= Kh=%lld and a "
+                 "multiple of 64", (long long)Khp, (long long)Kh);
+  MEMI_CHECK_ARG(ctf->pix_size > 0, "memi_ctf_pack: pixel size must be positive");
+  const int64_t total = n * Khp;
+  if (total == 0) return MEMI_OK;
+  hipLaunchKernelGGL(k_ctf_pack, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)fy, df, n, N, *ctf, precision, Khp, row0, op_hi, op_lo,
+                     ctf_half);
+  MEMI_CHECK_LAUNCH("k_ctf_pack");
+  return MEMI_OK;
+}
+
+int memi_phase_flip(const void* fy, const float* ctf_half, int64_t n, int N, void* out,
+                    void* stream) {
+  MEMI_CHECK_ARG(fy && ctf_half && out, "memi_phase_flip: null pointer");
+  const int64_t total = n * (int64_t)N * (N / 2 + 1);
+  if (total == 0) return MEMI_OK;
+  hipLaunchKernelGGL(k_phase_flip, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)fy, ctf_half, total, (float2*)out);
+  MEMI_CHECK_LAUNCH("k_phase_flip");
+  return MEMI_OK;
+}
+
+int memi_wiener_average(const void* spec, const float* ctf_half, int64_t n, int N, double snr,
+                        void* out, void* stream) {
+  MEMI_CHECK_ARG(spec && ctf_half && out && snr > 0, "memi_wiener_average: bad arguments");
+  const int64_t Kh = (int64_t)N * (N / 2 + 1);
+  hipLaunchKernelGGL(k_wiener_average, dim3(grid_for(Kh, 64)), dim3(64), 0, (hipStream_t)stream,
+                     (const float2*)spec, ctf_half, n, Kh, snr, (float2*)out);
+  MEMI_CHECK_LAUNCH("k_wiener_average");
+  return MEMI_OK;
+}
+
+int memi_image_moments(const float* img, int64_t n, int64_t npix, double* sum, double* sumsq,
+                       void* stream) {
+  MEMI_CHECK_ARG(img && sum && sumsq && npix > 0, "memi_image_moments: bad arguments");
+  hipLaunchKernelGGL(k_image_moments, dim3(grid_for(npix, 64)), dim3(64), 0,
+                     (hipStream_t)stream, img, n, npix, sum, sumsq);
+  MEMI_CHECK_LAUNCH("k_image_moments");
+  return MEMI_OK;
+}
+
+}  // extern "C"

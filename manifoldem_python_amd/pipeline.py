@@ -1,0 +1,329 @@
+"""GPU orchestration of the per-PD hot path on one MI355X.
+
+Stage 1 (``getDistanceCTF_local_Conj9combinedS2``): image preparation as batched rocFFT
+transforms (``torch.fft``) with the elementwise work between them in HIP kernels, then the MFMA
+distance Gram.  Stage 2 (``DMembeddingII``): kNN selection, Ferguson sweep and Markov
+normalisation in HIP kernels, the tanh fit on the host (scipy, as the reference), and a float64
+symmetric eigensolve on the GPU.
+
+Host-side work kept from the reference: per-image in-plane angles and the per-PD reference
+angle (``get_psi``, ``calc_avg_pd``, ``q2Spider``), the 4-parameter tanh fit of the Ferguson curve
+and the final psi/mu arithmetic on the (n x nEigs+1) eigenvectors.
+"""
+from __future__ import annotations
+
+import math
+import warnings
+
+import numpy as np
+import torch
+from scipy import optimize
+from scipy.optimize import OptimizeWarning, curve_fit
+
+from . import _native as nat
+
+warnings.simplefilter("ignore", OptimizeWarning)
+
+LOG_EPS = np.arange(-150, 150.2, 0.2)        # DMembeddingII.py:146 (1501 points)
+KAPPA = {nat.PREC_FP32: 1.0e4, nat.PREC_SPLIT_BF16: 8.0}
+CHUNKS = {nat.PREC_FP32: 4, nat.PREC_SPLIT_BF16: 1}
+
+
+def round_up(x, m):
+    return (x + m - 1) // m * m
+
+
+def half_len(N):
+    return N * (N // 2 + 1)
+
+
+def khp_of(N):
+    return round_up(half_len(N), 64)
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("manifoldem_python_amd needs a ROCm GPU (torch.cuda is unavailable); "
+                           "there is no CPU path")
+    return torch.device("cuda")
+
+
+# ------------------------------------------------------------------ host geometry ---------
+def calc_avg_pd(q):
+    """``getDistanceCTF_local_Conj9combinedS2.py:169-185``."""
+    nS = q.shape[1]
+    return 2 * np.vstack((q[1] * q[3] - q[0] * q[2],
+                          q[0] * q[1] + q[2] * q[3],
+                          q[0] ** 2 + q[3] ** 2 - np.ones((1, nS)) / 2.0))
+
+
+def get_psi(q, PD):
+    """``getDistanceCTF...:187-202`` for all images: Psi = 2 arctan(s/c), NaN -> 0 (:319-320)."""
+    s = -(1 + PD[2]) * q[3] - PD[0] * q[1] - PD[1] * q[2]
+    c = (1 + PD[2]) * q[0] + PD[1] * q[1] - PD[0] * q[2]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        Psi = 2 * np.arctan(s / c)
+    Psi = np.where(np.isnan(Psi), 0.0, Psi)
+    return Psi, s, c
+
+
+def _qmult(q, s):
+    q0, qv, s0, sv = q[0], q[1:4], s[0], s[1:4]
+    c = np.array([qv[1] * sv[2] - qv[2] * sv[1], qv[2] * sv[0] - qv[0] * sv[2],
+                  qv[0] * sv[1] - qv[1] * sv[0]])
+    return np.concatenate(([q0 * s0 - np.dot(qv, sv)], q0 * sv + s0 * qv + c))
+
+
+def psi_ang(PD):
+    """``getDistanceCTF...:206-213`` with ``q2Spider.py:19-60``: ZYZ angles by the same TRF
+    least-squares fit from x0 = 0 (ftol 1e-12); returns psi in degrees, mod 360."""
+    Qr = np.array([1 + PD[2], PD[1], -PD[0], 0.0])
+    q = Qr / np.sqrt(np.sum(Qr ** 2))
+
+    def dev1(a):
+        q1 = np.array([np.cos(a[0] / 2.0), 0.0, 0.0, -np.sin(a[0] / 2.0)])
+        q2 = np.array([np.cos(a[1] / 2.0), 0.0, -np.sin(a[1] / 2.0), 0.0])
+        q3 = np.array([np.cos(a[2] / 2.0), 0.0, 0.0, -np.sin(a[2] / 2.0)])
+        return q - _qmult(q3, _qmult(q2, q1))
+
+    res = optimize.least_squares(dev1, np.array([0, 0, 0]), bounds=(-np.inf, np.inf), ftol=1e-12)
+    return np.mod(res.x[2], 2 * np.pi) * (180 / np.pi)
+
+
+def pd_geometry(q):
+    """PD axis, per-image (Psi, Nom, Dnom) and psi_p of one PD."""
+    PDs = calc_avg_pd(q)
+    PD = np.sum(PDs, 1)
+    PD = PD / np.linalg.norm(PD)
+    Psi, s, c = get_psi(q, PD)
+    return dict(PDs=PDs, PD=PD, Psi=Psi, Nom=s, Dnom=c, psi_p=psi_ang(PD))
+
+
+# ----------------------------------------------------------------------- Stage 1 ----------
+class Operand:
+    """Packed distance operand for `rows` images of side N in one precision."""
+
+    def __init__(self, rows, N, precision, dev):
+        self.N = N
+        self.Khp = khp_of(N)
+        self.precision = precision
+        shape = (rows, 4 * self.Khp)
+        if precision == nat.PREC_FP32:
+            self.hi = torch.empty(shape, dtype=torch.float32, device=dev)
+            self.lo = None
+        else:
+            self.hi = torch.empty(shape, dtype=torch.int16, device=dev)
+            self.lo = torch.empty(shape, dtype=torch.int16, device=dev)
+
+
+def filter_kind(filterPar):
+    t = filterPar.get("type", "Butter")
+    if t == "Butter":
+        return nat.FILTER_BUTTER
+    if t == "Gauss":
+        return nat.FILTER_GAUSS
+    raise ValueError("%s filter is unsupported" % t)
+
+
+def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, operand=None,
+                   row0=0, msk2=None, fy_out=None, ctf_out=None, want_images=True):
+    """a1-a6 for one set of images (all device tensors): load + normalise, Butterworth,
+    disk mask, two cubic-spline rotations, fy = rfft2(img * msk2), CTF and operand rows.
+
+    Returns a dict of device tensors: imgAll (n,N,N) f32, fy (n,N,Nh) c64, ctf (n,Kh) f32.
+    """
+    n = int(src.shape[0])
+    dev = stack.device
+    y = torch.empty((n, N, N), dtype=torch.float32, device=dev)
+    nat.load_normalize(stack, src, conj, N, y)
+    Y = torch.fft.rfft2(y)
+    nat.spectrum_scale(Y, N, filter_kind(filterPar), filterPar["Qc"], filterPar["N"])
+    y = torch.fft.irfft2(Y, s=(N, N)).contiguous()
+    nat.mask_multiply(y, N, disk=True)
+    for ang in (rot1_deg, rot2_deg):
+        Y = torch.fft.rfft2(y)
+        nat.spectrum_scale(Y, N, nat.FILTER_BSPLINE)
+        coef = torch.fft.irfft2(Y, s=(N, N)).contiguous()
+        nat.spline_rotate(coef, ang, N, y)
+    del Y, coef
+    img = y
+    if msk2 is not None:
+        img = y.clone()
+        nat.mask_multiply(img, N, disk=False, msk2=msk2)
+    fy = torch.fft.rfft2(img)
+    if fy_out is not None:
+        fy_out.copy_(fy.reshape(fy_out.shape))
+        fy = fy_out.view(n, N, N // 2 + 1)
+    ctf_half = ctf_out if ctf_out is not None else torch.empty((n, half_len(N)),
+                                                              dtype=torch.float32, device=dev)
+    if operand is not None:
+        nat.ctf_pack(fy, df, N, ctf, operand.precision, operand.Khp, row0, operand.hi,
+                     operand.lo if operand.lo is not None else operand.hi, ctf_half)
+    else:
+        tmp = Operand(n, N, nat.PREC_FP32, dev)
+        nat.ctf_pack(fy, df, N, ctf, nat.PREC_FP32, tmp.Khp, 0, tmp.hi, tmp.hi, ctf_half)
+    return dict(imgAll=y if want_images else None, fy=fy, ctf=ctf_half)
+
+
+def averages(imgAll, fy, ctf_half, N, msk2=None, snr=5.0):
+    """a6-a7 outputs of one PD: imgAllFlip, imgAvgFlip, imgAllIntensity, imgAvg (Wiener)."""
+    n = imgAll.shape[0]
+    dev = imgAll.device
+    flip = torch.empty_like(fy)
+    nat.phase_flip(fy, ctf_half, N, flip)
+    imgAllFlip = torch.fft.irfft2(flip, s=(N, N)).contiguous()
+    s = torch.empty(N * N, dtype=torch.float64, device=dev)
+    s2 = torch.empty_like(s)
+    nat.image_moments(imgAllFlip, s, s2)
+    m2 = 1.0 if msk2 is None else msk2.double().reshape(-1)
+    imgAvgFlip = (s / n * m2).view(N, N)
+    imgAllIntensity = (s2 / n).view(N, N)
+    spec = fy if msk2 is None else torch.fft.rfft2(imgAll)
+    wnum = torch.empty((N, N // 2 + 1), dtype=torch.complex64, device=dev)
+    nat.wiener_average(spec.contiguous(), ctf_half, N, snr, wnum)
+    imgAvg = torch.fft.irfft2(wnum, s=(N, N)).double().reshape(-1) * m2 / n
+    return dict(imgAllFlip=imgAllFlip, imgAvgFlip=imgAvgFlip, imgAllIntensity=imgAllIntensity,
+                imgAvg=imgAvg.view(N, N))
+
+
+def full_ctf(ctf_half, N):
+    """Half-spectrum CTF (n, N*Nh) -> full FFT layout (n, N*N); C depends on |k| only."""
+    n = ctf_half.shape[0]
+    Nh = N // 2 + 1
+    h = ctf_half.view(n, N, Nh)
+    cols = torch.arange(N, device=h.device)
+    src = torch.where(cols < Nh, cols, N - cols)
+    return h[:, :, src].reshape(n, N * N)
+
+
+class Distances:
+    """Distance matrices of a batch of PDs (flat float32 buffer, PD blocks per the plan)."""
+
+    def __init__(self, plan, D, flag_count):
+        self.plan = plan
+        self.D = D
+        self.flag_count = flag_count
+
+    def block(self, p):
+        return self.plan.block(self.D, p)
+
+
+def ctf_distances(plan, operand, fy, ctf_half, N, kappa=None, ws=None):
+    """a8 for every PD of the plan: returns Distances (float32 D on the device)."""
+    dev = operand.hi.device
+    info = plan.info
+    kappa = KAPPA[operand.precision] if kappa is None else kappa
+    D = torch.empty(info.d_elems, dtype=torch.float32, device=dev)
+    if ws is None or ws.numel() < info.ws_floats:
+        ws = torch.empty(max(info.ws_floats, 1), dtype=torch.float32, device=dev)
+    flags = torch.empty(3 * info.flag_cap, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    nat.ctf_distance(plan, operand.hi, operand.lo if operand.lo is not None else operand.hi,
+                     operand.Khp, operand.precision, kappa, fy, ctf_half, N, D, ws, flags, cnt)
+    return Distances(plan, D, cnt)
+
+
+# ----------------------------------------------------------------------- Stage 2 ----------
+def tanh_model(xx, aa0, aa1, aa2, aa3):
+    """``fergusonE.py:21-23``."""
+    return aa3 + aa2 * np.tanh(aa0 * xx + aa1)
+
+
+def ferguson_fit(logEps, logSumWij, a0, rand=None):
+    """``fergusonE.py:45-57``: curve_fit restarted from rand(4,1) - 0.5 while the summed
+    parameter standard errors exceed 100.  ``rand`` defaults to numpy's global generator, as
+    in the reference."""
+    rand = np.random.rand if rand is None else rand
+    resnorm = np.inf
+    while resnorm > 100:
+        popt, pcov = curve_fit(tanh_model, logEps, logSumWij, p0=np.ravel(a0))
+        resnorm = np.sum(np.sqrt(np.fabs(np.diag(pcov))))
+        a0 = 1 * (rand(4, 1) - 0.5)
+        residuals = logSumWij - tanh_model(logEps, *popt)
+        ss_res = np.sum(residuals ** 2)
+        ss_tot = np.sum((logSumWij - np.mean(logSumWij)) ** 2)
+        R_squared = 1 - ss_res / ss_tot
+    return popt, resnorm, R_squared
+
+
+class DMResult:
+    __slots__ = ("lamb", "psi", "sigma", "mu", "logEps", "logSumWij", "popt", "R_squared",
+                 "slope", "V")
+
+    def as_tuple(self):
+        return (self.lamb, self.psi, self.sigma, self.mu, self.logEps, self.logSumWij, self.popt,
+                self.R_squared)
+
+
+def top_eigs_lm(L, k):
+    """``sembeddingonFly.py:27-35`` semantics on a dense float64 GPU matrix: the k
+    eigenpairs of largest magnitude, sorted by descending eigenvalue."""
+    lam, V = torch.linalg.eigh(L)
+    sel = torch.argsort(-lam.abs(), stable=True)[:k]
+    lam, V = lam[sel], V[:, sel]
+    order = torch.argsort(lam, descending=True, stable=True)
+    return lam[order], V[:, order]
+
+
+def knn_mask(plan, D):
+    """a10: kNN membership bitmask of every PD (int32 words on the device)."""
+    mask = torch.empty(max(plan.info.mask_words, 1), dtype=torch.int32, device=D.device)
+    nat.knn_select(plan, D, mask)
+    return mask
+
+
+def ferguson_sums(plan, D, mask, logEps):
+    """a12 sums: (npd, n_eps) arrays of sum W, d log(sum W)/d log eps and #terms."""
+    npd = plan.npd
+    le = torch.as_tensor(np.asarray(logEps, dtype=np.float64), device=D.device)
+    n_eps = le.shape[0]
+    fer = torch.empty(npd * n_eps * 3 + 2 * npd, dtype=torch.float64, device=D.device)
+    nat.ferguson_sweep(plan, D, mask, le, fer)
+    fer = fer.cpu().numpy()[:npd * n_eps * 3].reshape(npd, n_eps, 3)
+    S = fer[:, :, 0]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        slope = fer[:, :, 1] / S
+    return S, slope, fer[:, :, 2]
+
+
+def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, mask=None):
+    """a10-a16 for every PD of the plan.  D: flat float32 device buffer.  Returns a list of
+    DMResult (numpy outputs)."""
+    dev = D.device
+    info = plan.info
+    npd = plan.npd
+    if mask is None:
+        mask = knn_mask(plan, D)
+    S, slope, _ = ferguson_sums(plan, D, mask, logEps)
+    rand = np.random.rand if rand is None else rand
+    results = []
+    sig = np.zeros(npd)
+    for p in range(npd):
+        r = DMResult()
+        a0 = (rand(4, 1) - 0.5) if a0s is None else a0s[p]
+        with np.errstate(divide="ignore"):
+            r.logSumWij = np.log(S[p])
+        r.slope = slope[p]
+        r.logEps = np.asarray(logEps)
+        r.popt, _, r.R_squared = ferguson_fit(r.logEps, r.logSumWij, a0, rand)
+        r.sigma = tune * np.sqrt(2 * np.exp(-r.popt[1] / r.popt[0]))
+        sig[p] = r.sigma
+        results.append(r)
+    sigma = torch.as_tensor(sig, device=dev)
+    L = torch.empty(max(info.d_elems, 1), dtype=torch.float64, device=dev)
+    deg = torch.empty(2 * info.n_rows, dtype=torch.float64, device=dev)
+    nat.markov_normalize(plan, D, mask, sigma, L, deg)
+    for p in range(npd):
+        n = int(plan.ns[p])
+        nEigs = min(num_eigs, n - 3)
+        lam, V = top_eigs_lm(plan.block(L, p), nEigs + 1)
+        lam = lam.cpu().numpy()
+        V = V.cpu().numpy()
+        r = results[p]
+        r.lamb = lam
+        psi = np.zeros((n, nEigs))
+        psi[:, :V.shape[1] - 1] = V[:, 1:] / V[:, 0:1]
+        r.psi = psi
+        r.mu = V[:, 0] ** 2
+        r.V = V
+    return results

@@ -1,0 +1,112 @@
+"""GPU parity of Stage 2 (kNN, Ferguson sweep, Markov normalisation, eigensolve)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _distance_matrix(n, seed):
+    """A realistic symmetric distance matrix: squared distances of noisy points on a curve."""
+    rng = np.random.default_rng(seed)
+    t = np.sort(rng.uniform(0, 1, n))
+    X = np.stack([np.cos(3 * t), np.sin(3 * t), t], axis=1) + 0.05 * rng.normal(size=(n, 3))
+    D = ((X[:, None, :] - X[None, :, :]) ** 2).sum(-1) * 1.0e4
+    return D.astype(np.float32).astype(np.float64)   # representable in float32 exactly
+
+
+def _plan(n, k):
+    from manifoldem_python_amd import _native as nat
+    return nat.Plan([0], [n], [k], chunks=1)
+
+
+@pytest.mark.parametrize("n,k", [(200, 40), (333, 150), (129, 129)])
+def test_knn_lists_match_sort(cuda, n, k):
+    import torch
+    from manifoldem_python_amd import _native as nat
+    from oracle import dm
+    D = _distance_matrix(n, 3)
+    yInd1, yVal1 = dm.initialize(n, k, D.copy())
+    plan = _plan(n, k)
+    Dg = torch.as_tensor(D.astype(np.float32).ravel(), device=cuda)
+    mask = torch.empty(plan.info.mask_words, dtype=torch.int32, device=cuda)
+    idx = torch.empty(plan.info.list_elems, dtype=torch.int32, device=cuda)
+    val = torch.empty(plan.info.list_elems, dtype=torch.float32, device=cuda)
+    nat.knn_select(plan, Dg, mask, idx, val)
+    gi = idx.cpu().numpy().reshape(k, n)
+    gv = val.cpu().numpy().reshape(k, n)
+    ri = yInd1.reshape(n, k).T
+    rv = yVal1.reshape(n, k).T
+    np.testing.assert_array_equal(gi, ri)
+    np.testing.assert_allclose(gv, rv, rtol=1e-7)
+    # bitmask rows agree with the lists
+    W = (n + 31) // 32
+    m = mask.cpu().numpy().view(np.uint32).reshape(n, W)
+    bits = np.unpackbits(m.view(np.uint8).reshape(n, W, 4)[:, :, ::-1], axis=2,
+                         bitorder="big").reshape(n, W * 32)
+    # big-endian unpack of little-endian words: rebuild bit j of word w
+    member = np.zeros((n, n), bool)
+    for i in range(n):
+        for w in range(W):
+            word = int(m[i, w])
+            for b in range(32):
+                j = 32 * w + b
+                if j < n and (word >> b) & 1:
+                    member[i, j] = True
+    ref = np.zeros((n, n), bool)
+    for i in range(n):
+        ref[i, ri[:, i]] = True
+    np.testing.assert_array_equal(member, ref)
+    del bits
+
+
+@pytest.mark.parametrize("n,k,n_eps", [(200, 200, 101), (300, 60, 1501)])
+def test_dm_embed_matches_oracle(cuda, n, k, n_eps):
+    import torch
+    from manifoldem_python_amd import pipeline as pl
+    from oracle import dm
+    D = _distance_matrix(n, 11)
+    logEps = np.linspace(-150, 150, n_eps) if n_eps != 1501 else dm.LOG_EPS
+    rng_o = np.random.default_rng(7)
+    ref = dm.op(D, k, 3, 60000, num_eigs=8, rng=rng_o, logEps=logEps)
+    rng_g = np.random.default_rng(7)
+    plan = _plan(n, k)
+    Dg = torch.as_tensor(D.astype(np.float32).ravel(), device=cuda)
+    res = pl.dm_embed(plan, Dg, tune=3, num_eigs=8, logEps=logEps,
+                      rand=lambda *sh: rng_g.random(sh))[0]
+    lamb, psi, sigma, mu, _, lsw, popt, R2 = ref
+    assert np.max(np.abs(res.logSumWij - lsw)) < 1e-6
+    assert abs(res.sigma - sigma) / sigma < 1e-5
+    assert np.max(np.abs(res.lamb - lamb)) < 1e-6
+    # eigenvectors up to sign: compare psi via the subspace of the leading vectors
+    Vr = np.column_stack([np.sqrt(mu)] + [psi[:, j] * np.sqrt(mu) for j in range(psi.shape[1])])
+    Vg = res.V
+    for j in range(Vg.shape[1]):
+        c = abs(np.dot(Vr[:, j] / np.linalg.norm(Vr[:, j]), Vg[:, j]))
+        assert c > 1 - 1e-6, (j, c)
+    np.testing.assert_allclose(res.mu, mu, rtol=1e-5, atol=1e-12)
+
+
+def test_ferguson_slope_matches_finite_differences(cuda):
+    import torch
+    from manifoldem_python_amd import pipeline as pl
+    n = 250
+    D = _distance_matrix(n, 5)
+    # keep every x = D/(2 eps) below the threshold 10 so log S is smooth in eps
+    logEps = np.linspace(np.log(D.max() / 20) + 0.5, 25, 301)
+    plan = _plan(n, n)
+    Dg = torch.as_tensor(D.astype(np.float32).ravel(), device=cuda)
+    S, slope, _ = pl.ferguson_sums(plan, Dg, pl.knn_mask(plan, Dg), logEps)
+    # float64 finite differences of log S(eps) on the same kernel entries
+    Df = D.astype(np.float32).astype(np.float64)
+    v = Df[~np.eye(n, dtype=bool)]
+    v = v[v >= 1e-6]
+    nz = n  # diagonal zero entries
+
+    def logS(le):
+        x = v / (2 * np.exp(le))
+        return np.log(nz + np.sum(np.exp(-x[x < 10])))
+
+    h = 1e-4
+    fd = np.array([(logS(le + h) - logS(le - h)) / (2 * h) for le in logEps])
+    assert np.max(np.abs(slope[0] - fd)) < 1e-6
+    assert np.max(np.abs(np.log(S[0]) - np.array([logS(le) for le in logEps]))) < 1e-6
