@@ -1,0 +1,265 @@
+#!/usr/bin/env python
+"""Throughput of the per-PD hot path on one MI355X.
+
+A step is one pass of the hot path over the workload already resident in HBM: image
+preparation (load/normalise, Butterworth, two spline rotations, CTF, FFTs), the CTF-weighted
+distance Gram, and the diffusion-map embedding (kNN, 1501-point Ferguson sweep, tanh fit,
+Markov normalisation, float64 eigensolve).  Prints ONE JSON line.
+
+Workload (default, --workload c2): one PD of n=2000 synthetic 128x128 particles, defocus
+U(0.5, 3.5) um, kNN k=200, 10 eigenvectors.  --workload c3: 500 PDs, lognormal sizes
+(median 400, sigma 0.6, clipped to [100, 3000]), 96x96, k=min(n,150), 8 eigenvectors.
+
+value = image-pair distances per second over the whole step (split-BF16 Gram);
+fp32 = the same with the plain FP32 MFMA Gram.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+FILTER = dict(type="Butter", Qc=0.5, N=8)
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 path")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_workload(name, dev):
+    import torch
+    from manifoldem_python_amd import synth, batch as B, _native as nat
+    ctf = nat.ctf_params(synth.PIX, synth.CS, synth.EKV, synth.AMPC, np.inf)
+    if name == "c2":
+        n, N = 2000, 128
+        s = synth.make_pd(synth.seeds(3)[1], n, N, df_range_um=(0.5, 3.5), device=dev,
+                          as_torch=True)
+        pds = [dict(src=np.arange(n), conj=np.zeros(n, np.int32), q=s.q, df=s.df, k=200)]
+        stack = s.raw
+        meta = dict(workload="c2_single_large_pd", n_pd=1, particles=n, image=N, knn=200,
+                    n_eigs=10, n_eps=1501)
+        num_eigs = 10
+    else:
+        N = 96
+        sizes = synth.lognormal_sizes(synth.seeds(4)[2], 500)
+        seeds = np.random.SeedSequence(synth.SEED + 3).spawn(len(sizes))
+        stacks, pds, off = [], [], 0
+        for p, n in enumerate(sizes):
+            rng = np.random.default_rng(seeds[p])
+            axis = synth.random_axis(rng)
+            s = synth.make_pd(seeds[p], int(n), N, axis=axis, device=dev, as_torch=True)
+            stacks.append(s.raw)
+            pds.append(dict(src=np.arange(n) + off, conj=np.zeros(n, np.int32), q=s.q, df=s.df,
+                            k=min(int(n), 150)))
+            off += int(n)
+        stack = torch.cat(stacks)
+        meta = dict(workload="c3_batched_pds", n_pd=len(sizes), particles=int(sizes.sum()),
+                    median_particles=int(np.median(sizes)), image=N, knn="min(n,150)",
+                    n_eigs=8, n_eps=1501)
+        num_eigs = 8
+    batch = B.make_batch(stack, pds, N, ctf, FILTER, dev)
+    return batch, meta, num_eigs
+
+
+def run_precision(batch, precision, steps, warmup, num_eigs, events):
+    """Warm up, then time exactly `steps` full steps; returns timings and the last results."""
+    import torch
+    from manifoldem_python_amd import batch as B
+    wsp = B.Workspace(batch, precision)
+    rng = np.random.default_rng(0)
+    rand = lambda *sh: rng.random(sh)
+    for _ in range(warmup):
+        dist = B.distances(batch, wsp)
+        B.embed(wsp, dist, num_eigs=num_eigs, rand=rand)
+    torch.cuda.synchronize()
+    pairs = [events.new_pair() for _ in range(steps)]
+    t_dist = 0.0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        a = time.perf_counter()
+        dist = B.distances(batch, wsp, events=pairs[k])
+        torch.cuda.synchronize()
+        t_dist += time.perf_counter() - a
+        res = B.embed(wsp, dist, num_eigs=num_eigs, rand=rand)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    gram_ms = [events.elapsed_ms(p) for p in pairs]
+    return dict(t=t, t_dist=t_dist, gram_ms=float(np.mean(gram_ms)), wsp=wsp, dist=dist,
+                res=res, flags=int(dist.flag_count.item()))
+
+
+def fp64_error(batch, wsp, dist):
+    """Per-entry relative error of D against a float64 Gram of the same spectra/CTFs."""
+    import torch
+    N = batch.N
+    Nh = N // 2 + 1
+    w = torch.full((Nh,), 2.0, dtype=torch.float64, device=wsp.fy.device)
+    w[0] = 1.0
+    if N % 2 == 0:
+        w[-1] = 1.0
+    errs = []
+    for p in range(wsp.plan.npd if wsp.plan.npd <= 8 else 8):
+        r0, n = int(wsp.plan.rows0[p]), int(wsp.plan.ns[p])
+        X = wsp.fy[r0:r0 + n].to(torch.complex128)
+        C = wsp.ctf[r0:r0 + n].to(torch.float64).view(n, N, Nh)
+        A = (X.abs() ** 2 * w).reshape(n, -1)
+        Bm = (C ** 2).reshape(n, -1)
+        Y = (C * X * w.sqrt()).reshape(n, -1)
+        P = A @ Bm.T
+        D64 = P + P.T - 2 * (Y @ Y.conj().T).real
+        Dg = dist.block(p).double()
+        off = ~torch.eye(n, dtype=torch.bool, device=Dg.device)
+        errs.append(((Dg - D64).abs()[off] / D64.abs()[off]).cpu().numpy())
+    e = np.concatenate(errs)
+    return dict(max=float(e.max()), p99=float(np.percentile(e, 99)), median=float(np.median(e)))
+
+
+def cpu_baseline(meta, batch):
+    """Float64 oracle (the numpy/scipy restatement of the reference) timed on a bounded sample
+    on this host, extrapolated to the workload with the stated cost model."""
+    import torch
+    from oracle import distance as od, dm as odm
+    from threadpoolctl import threadpool_info
+    N = batch.N
+    ns = batch.ns
+    n_img = 24
+    stack = batch.stack[:n_img].cpu().numpy()
+    q = batch.geometry[0]
+    p = dict(nPix=N, pix_size=1.5, Cs=2.7, EkV=300.0, gaussEnv=np.inf, AmpContrast=0.1)
+    df = batch.df[:n_img].cpu().numpy()
+    ind = np.arange(n_img)
+    qq = np.tile(np.array([[1.0], [0], [0], [0]]), (1, n_img))
+    t0 = time.perf_counter()
+    od.op([ind, qq, df, None, 0], FILTER, stack, 2 * n_img, dict(avgOnly=True), p)
+    t_img = (time.perf_counter() - t0) / n_img
+    # Gram: float64 np.dot pair at n_g images, cost ~ n^2
+    n_g = 1000
+    rng = np.random.default_rng(0)
+    fy = (rng.normal(size=(n_g, N * N)) + 1j * rng.normal(size=(n_g, N * N)))
+    C = rng.normal(size=(n_g, N * N))
+    t0 = time.perf_counter()
+    od.gram_distance(fy, C)
+    t_gram_pair = (time.perf_counter() - t0) / (n_g * n_g)
+    del fy, C
+    # Ferguson: a 20-value sample of the eps sweep on a kernel-sized value list, scaled to 1501
+    n_ref = int(ns.max())
+    k_ref = int(batch.ks.max())
+    nnz = int(2 * n_ref * k_ref)
+    vals = rng.uniform(1e6, 2e6, nnz)
+    le = np.linspace(-150, 150, 20)
+    t0 = time.perf_counter()
+    odm.ferguson_sums(np.sqrt(vals), le)
+    t_fer_per_eps_value = (time.perf_counter() - t0) / (20 * nnz)
+    # eigsh on a sparse normalised kernel of the reference size
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import eigsh
+    A = sp.random(n_ref, n_ref, density=min(1.0, 2 * k_ref / n_ref), random_state=1)
+    A = (A + A.T) / 2
+    t0 = time.perf_counter()
+    eigsh(A, k=11, maxiter=300)
+    t_eig_ref = time.perf_counter() - t0
+    t_total = 0.0
+    for n, k in zip(ns, batch.ks):
+        nnz_p = 2 * n * min(k, n)
+        t_total += (n * t_img + n * n * t_gram_pair + 1501 * nnz_p * t_fer_per_eps_value +
+                    t_eig_ref * (n / n_ref) ** 2)
+    pairs = float(np.sum(ns * (ns - 1) / 2))
+    threads = sum(x.get("num_threads", 0) for x in threadpool_info()) or 1
+    sample = (f"oracle timed on host: per-image prep ({n_img} images, single thread), float64 "
+              f"Gram {n_g}^2 pairs (BLAS {threads} threads), Ferguson sweep 20 of 1501 eps on "
+              f"{nnz} values, eigsh k=11 on n={n_ref}; extrapolated per PD as "
+              f"n*t_img + n^2*t_pair + 1501*nnz*t_eps + t_eig*(n/{n_ref})^2")
+    return dict(value=pairs / t_total, unit="pairs/s", cores=int(threads), kind="port",
+                sample=sample, seconds_per_step=t_total)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); PDs are "
+              "independent, so a multi-GPU run would partition the PD list by cost with no "
+              "collective (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    dev = torch.device("cuda")
+    nat.load()
+    log("building workload", args.workload)
+    batch, meta, num_eigs = build_workload(args.workload, dev)
+    torch.cuda.synchronize()
+    ev = nat.HipEvents()
+    log("timing split-BF16")
+    bf = run_precision(batch, nat.PREC_SPLIT_BF16, args.steps, args.warmup, num_eigs, ev)
+    ns = batch.ns
+    pairs = float(np.sum(ns * (ns - 1) / 2))
+    Kh = pl.half_len(batch.N)
+    flops = 8.0 * Kh * pairs                       # algorithmic: 8 K_h per unordered pair
+    n_items = bf["wsp"].plan.info.n_items
+    exec_flops = 3 * n_items * 2.0 * 128 * 128 * bf["wsp"].operand.Khp  # hi*hi + hi*lo + lo*hi
+    gram_s = bf["gram_ms"] / 1e3
+    achieved = flops / gram_s / 1e12
+    out = {
+        "metric": "image-pair distances/s & PDs/s, split-BF16 vs FP32 at <=1e-5 rel err vs FP64",
+        "value": pairs * args.steps / bf["t"],
+        "unit": "pairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * bf["t"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3 (split) Gram, fp32 prep, fp64 eigensolve",
+        "data": "synthetic",
+        "config": dict(meta, precision="split-bf16"),
+        "pds_per_s": batch.ns.size * args.steps / bf["t"],
+        "distance_ms_per_step": 1e3 * bf["t_dist"] / args.steps,
+        "gram_kernel_ms": bf["gram_ms"],
+        "kappa_flagged": bf["flags"],
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                     "kernel": "k_gram_tiles<split-bf16>",
+                     "mfma_util": exec_flops / gram_s / 1e12 / PEAK_BF16_TFLOPS},
+    }
+    if not args.no_check:
+        out["d_rel_err_vs_fp64"] = fp64_error(batch, bf["wsp"], bf["dist"])
+    del bf
+    if not args.no_fp32:
+        log("timing FP32")
+        f32 = run_precision(batch, nat.PREC_FP32, args.steps, args.warmup, num_eigs, ev)
+        a32 = flops / (f32["gram_ms"] / 1e3) / 1e12
+        out["fp32"] = {"value": pairs * args.steps / f32["t"],
+                       "ms_per_step": 1e3 * f32["t"] / args.steps,
+                       "pds_per_s": batch.ns.size * args.steps / f32["t"],
+                       "gram_kernel_ms": f32["gram_ms"],
+                       "roofline": {"bound": "mfma", "achieved": a32, "peak": PEAK_FP32_TFLOPS,
+                                    "unit": "TFLOP/s", "frac": a32 / PEAK_FP32_TFLOPS}}
+        if not args.no_check:
+            out["fp32"]["d_rel_err_vs_fp64"] = fp64_error(batch, f32["wsp"], f32["dist"])
+        out["speedup_vs_fp32"] = out["value"] / out["fp32"]["value"]
+        del f32
+    ev.close()
+    if not args.no_cpu:
+        log("timing CPU baseline sample")
+        out["cpu_baseline"] = cpu_baseline(meta, batch)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -154,11 +154,14 @@ int memi_plan(const memi_pd_desc* pds_in_host, int npd, int chunks, void* plan_h
  * diagonal, and float32 direct-form recomputation of entries whose cancellation ratio
  * (P + 2|R|)/D exceeds `kappa` (flags: 3*info->flag_cap int32; *flag_count: int32 on the
  * device, zeroed here, gets the number flagged).  D: float32 [info->d_elems].  fy/ctf_half:
- * the spectra and CTFs that produced the operand (rows indexed like the operand). */
+ * the spectra and CTFs that produced the operand (rows indexed like the operand).
+ * ev_start / ev_stop: optional hipEvent_t recorded on `stream` right before and after the
+ * MFMA Gram kernel (NULL: not recorded), for timing that kernel alone. */
 int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const void* op_hi,
                       const void* op_lo, int64_t Khp, int precision, double kappa,
                       const void* fy, const float* ctf_half, int N, float* D, float* ws,
-                      int32_t* flags, int32_t* flag_count, void* stream);
+                      int32_t* flags, int32_t* flag_count, void* ev_start, void* ev_stop,
+                      void* stream);
 
 /* ---------------- Stage 2: diffusion map (DMembeddingII / fergusonE / slaplacianonFly) ---- */
 
@@ -186,6 +189,15 @@ int memi_ferguson_sweep(const void* plan_dev, const memi_plan_info* info, const 
 int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, const float* D,
                           const uint32_t* mask, const double* sigma, double* L, double* deg,
                           void* stream);
+
+/* One Lanczos step (full reorthogonalisation, CGS2) on every active PD's L, for the
+ * largest-magnitude eigenpairs that sembeddingonFly.py:27 asks ARPACK eigsh for.
+ * V: float64 bases, column-major per PD at m*row0 (column 0 = unit start vector, set by the
+ * caller); W: float64 [n_rows] scratch; ab: float64 [npd][m][2] gets {alpha_j, beta_j};
+ * active: int32 [npd] (0 = skip).  Step j reads column j and writes column j+1. */
+int memi_lanczos_step(const void* plan_dev, const memi_plan_info* info, const double* L,
+                      double* V, int m, int j, const int32_t* active, double* W, double* ab,
+                      void* stream);
 
 /* ---------------- Stage 3: NLSA (NLSA.py / get_wiener.py / svdRF.py / L2_distance.py) ---- */
 

@@ -60,10 +60,12 @@ _SIGS = {
     "memi_image_moments": [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp],
     "memi_plan": [c_vp, c_int, c_int, c_vp, c_i64, ctypes.POINTER(PlanInfo)],
     "memi_ctf_distance": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_i64, c_int, c_dbl, c_vp,
-                          c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
+                          c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_knn_select": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_ferguson_sweep": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_markov_normalize": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "memi_lanczos_step": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_int, c_int, c_vp, c_vp,
+                          c_vp, c_vp],
     "memi_nlsa_condist": [c_vp, c_vp, c_int, c_int, c_vp, c_vp],
     "memi_nlsa_supervectors": [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_vp, c_vp],
     "memi_tall_gram": [c_vp, c_i64, c_int, c_vp, c_vp],
@@ -208,10 +210,44 @@ class Plan:
 
 
 def ctf_distance(plan, op_hi, op_lo, Khp, precision, kappa, fy, ctf_half, N, D, ws, flags,
-                 flag_count):
+                 flag_count, events=None):
+    ev0, ev1 = (None, None) if events is None else (c_vp(events[0]), c_vp(events[1]))
     call("memi_ctf_distance", _p(plan.dev), ctypes.byref(plan.info), _p(op_hi), _p(op_lo), Khp,
          precision, float(kappa), _p(fy), _p(ctf_half), N, _p(D), _p(ws), _p(flags),
-         _p(flag_count), _stream())
+         _p(flag_count), ev0, ev1, _stream())
+
+
+class HipEvents:
+    """Raw hipEvent_t pairs (libamdhip64) for timing one kernel inside a library call on the
+    stream it runs on."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(c_vp)]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), c_vp, c_vp]
+        self.hip.hipEventSynchronize.argtypes = [c_vp]
+        self.hip.hipEventDestroy.argtypes = [c_vp]
+        self.pairs = []
+
+    def new_pair(self):
+        a, b = c_vp(), c_vp()
+        if self.hip.hipEventCreate(ctypes.byref(a)) or self.hip.hipEventCreate(ctypes.byref(b)):
+            raise RuntimeError("hipEventCreate failed")
+        self.pairs.append((a.value, b.value))
+        return self.pairs[-1]
+
+    def elapsed_ms(self, pair):
+        ms = ctypes.c_float()
+        self.hip.hipEventSynchronize(c_vp(pair[1]))
+        if self.hip.hipEventElapsedTime(ctypes.byref(ms), c_vp(pair[0]), c_vp(pair[1])):
+            raise RuntimeError("hipEventElapsedTime failed")
+        return float(ms.value)
+
+    def close(self):
+        for a, b in self.pairs:
+            self.hip.hipEventDestroy(c_vp(a))
+            self.hip.hipEventDestroy(c_vp(b))
+        self.pairs = []
 
 
 # ------------------------------------------------------------------------- Stage 2 ---------
@@ -228,6 +264,11 @@ def ferguson_sweep(plan, D, mask, logEps, out):
 def markov_normalize(plan, D, mask, sigma, L, deg):
     call("memi_markov_normalize", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask),
          _p(sigma), _p(L), _p(deg), _stream())
+
+
+def lanczos_step(plan, L, V, m, j, active, W, ab):
+    call("memi_lanczos_step", _p(plan.dev), ctypes.byref(plan.info), _p(L), _p(V), int(m), int(j),
+         _p(active), _p(W), _p(ab), _stream())
 
 
 # ------------------------------------------------------------------------- Stage 3 ---------

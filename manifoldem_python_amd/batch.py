@@ -1,0 +1,103 @@
+"""Batched per-PD pipeline: many projection directions in one set of launches.
+
+Replaces the reference's per-PD process pool (``GetDistancesS2.py:53-124``,
+``manifoldAnalysis.py:57-116``): all images of all PDs are prepared in one chunked pass, all
+PD distance matrices come from one grouped MFMA launch over a (PD, tile, segment) work list,
+and the kNN / Ferguson / normalisation kernels each run once over the whole batch.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import pipeline as pl
+
+
+@dataclass
+class Batch:
+    """Device-resident inputs of a batch of PDs (images packed PD after PD)."""
+    N: int
+    stack: torch.Tensor        # (n_img, N, N) float32 SPIDER-order images
+    src: torch.Tensor          # (rows,) int64 stack index per particle
+    conj: torch.Tensor         # (rows,) int32 conjugate flag per particle
+    rot1: torch.Tensor         # (rows,) float64 first rotation (deg) = -(180/pi) Psi
+    rot2: torch.Tensor         # (rows,) float64 second rotation (deg) = -psi_p of its PD
+    df: torch.Tensor           # (rows,) float64 defocus (A)
+    ns: np.ndarray             # particles per PD
+    ks: np.ndarray             # kNN per PD
+    geometry: list             # per-PD host geometry (PD, Psi, Nom, Dnom, psi_p)
+    ctf: object                # nat.CtfParams
+    filterPar: dict
+
+    @property
+    def rows0(self):
+        return np.concatenate(([0], np.cumsum(self.ns)[:-1])).astype(np.int64)
+
+    @property
+    def rows(self):
+        return int(np.sum(self.ns))
+
+
+def make_batch(stack, pds, N, ctf, filterPar, device):
+    """pds: list of dicts {src, conj, q, df, k} (numpy) for stack images on `device`."""
+    src, conj, rot1, rot2, df, ns, ks, geo = [], [], [], [], [], [], [], []
+    for d in pds:
+        g = pl.pd_geometry(d["q"])
+        n = len(d["src"])
+        src.append(np.asarray(d["src"], np.int64))
+        conj.append(np.asarray(d["conj"], np.int32))
+        rot1.append(-(180 / math.pi) * g["Psi"])
+        rot2.append(np.full(n, -g["psi_p"]))
+        df.append(np.asarray(d["df"], np.float64))
+        ns.append(n)
+        ks.append(min(int(d.get("k", n)), n))
+        geo.append(g)
+    cat = lambda xs, dt: torch.as_tensor(np.concatenate(xs).astype(dt), device=device)
+    return Batch(N=N, stack=stack, src=cat(src, np.int64), conj=cat(conj, np.int32),
+                 rot1=cat(rot1, np.float64), rot2=cat(rot2, np.float64), df=cat(df, np.float64),
+                 ns=np.asarray(ns), ks=np.asarray(ks), geometry=geo, ctf=ctf,
+                 filterPar=filterPar)
+
+
+class Workspace:
+    """Buffers reused across steps of the same batch and precision."""
+
+    def __init__(self, batch, precision, chunk_rows=16384):
+        dev = batch.stack.device
+        self.precision = precision
+        self.plan = nat.Plan(batch.rows0, batch.ns, batch.ks, chunks=pl.CHUNKS[precision],
+                             device=dev)
+        self.operand = pl.Operand(batch.rows, batch.N, precision, dev)
+        Nh = batch.N // 2 + 1
+        self.fy = torch.empty((batch.rows, batch.N, Nh), dtype=torch.complex64, device=dev)
+        self.ctf = torch.empty((batch.rows, batch.N * Nh), dtype=torch.float32, device=dev)
+        self.ws = torch.empty(max(self.plan.info.ws_floats, 1), dtype=torch.float32, device=dev)
+        self.chunk_rows = chunk_rows
+
+
+def prepare(batch, wsp):
+    """a1-a6 for every particle of the batch, in chunks of rows (bounded scratch)."""
+    R = batch.rows
+    for r0 in range(0, R, wsp.chunk_rows):
+        r1 = min(R, r0 + wsp.chunk_rows)
+        sl = slice(r0, r1)
+        pl.prepare_images(batch.stack, batch.src[sl], batch.conj[sl], batch.rot1[sl],
+                          batch.rot2[sl], batch.df[sl], batch.N, batch.filterPar, batch.ctf,
+                          operand=wsp.operand, row0=r0, fy_out=wsp.fy[sl], ctf_out=wsp.ctf[sl],
+                          want_images=False)
+
+
+def distances(batch, wsp, events=None):
+    """a1-a8: prepared operand then the grouped Gram; returns pl.Distances."""
+    prepare(batch, wsp)
+    return pl.ctf_distances(wsp.plan, wsp.operand, wsp.fy, wsp.ctf, batch.N, ws=wsp.ws,
+                            events=events)
+
+
+def embed(wsp, dist, tune=3, num_eigs=15, logEps=pl.LOG_EPS, rand=None):
+    """a10-a16 for every PD of the batch."""
+    return pl.dm_embed(wsp.plan, dist.D, tune, num_eigs, logEps, rand=rand)

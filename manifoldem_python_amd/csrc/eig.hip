@@ -1,0 +1,118 @@
+// Lanczos eigensolver steps for the normalised diffusion kernels of a PD batch (float64).
+//
+// Reference: sembeddingonFly.py:17-39 calls ARPACK eigsh(l, k=nEigs+1, which='LM'), an
+// implicitly restarted Lanczos method.  Here every PD of a batch runs Lanczos with full
+// reorthogonalisation in lockstep: one step is a grouped matvec w = L v_j over all PDs
+// (k_lanczos_matvec, many workgroups per PD) and one workgroup per PD that orthogonalises w
+// against the whole basis twice (classical Gram-Schmidt, CGS2), records alpha_j, beta_j and
+// appends v_{j+1} = w / beta_j.  The host diagonalises the small tridiagonal T and forms the
+// Ritz vectors V S.
+//
+// Layout: L float64, PD p's n*n block at plan d_off; V float64 column-major per PD, PD p at
+// element p_vo = m * row0(p) (row0 = packed row offset), column c of length n at p_vo + c*n;
+// W float64 [n_rows]; ab float64 [npd][m][2] = {alpha_j, beta_j}; active int32 [npd].
+#include "common.h"
+
+namespace memi {
+
+__global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restrict__ rows,
+                                                        int64_t n_rows,
+                                                        const memi_pd_desc* __restrict__ pds,
+                                                        const double* __restrict__ L,
+                                                        const double* __restrict__ V, int m,
+                                                        int j, const int32_t* __restrict__ active,
+                                                        double* __restrict__ W) {
+  const int lane = threadIdx.x & 63;
+  const int64_t gr = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // one wave per row
+  if (gr >= n_rows) return;
+  const int pd = rows[2 * gr], i = rows[2 * gr + 1];
+  if (!active[pd]) return;
+  const memi_pd_desc P = pds[pd];
+  const int n = P.n;
+  if (j >= min(n, m)) return;
+  const double* Lr = L + P.d_off + (int64_t)i * n;
+  const double* v = V + (int64_t)m * P.row0 + (int64_t)j * n;
+  double s = 0.0;
+  for (int c = lane; c < n; c += 64) s += Lr[c] * v[c];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) W[P.row0 + i] = s;
+}
+
+// one block per PD: CGS2 against columns 0..j, then normalise into column j+1
+__global__ __launch_bounds__(256) void k_lanczos_orth(const memi_pd_desc* __restrict__ pds,
+                                                      double* __restrict__ V, int m, int j,
+                                                      const int32_t* __restrict__ active,
+                                                      double* __restrict__ W,
+                                                      double* __restrict__ ab) {
+  extern __shared__ double h[];  // m coefficients
+  __shared__ double red[8];
+  const int pd = blockIdx.x;
+  if (!active[pd]) return;
+  const memi_pd_desc P = pds[pd];
+  const int n = P.n;
+  if (j >= min(n, m)) return;
+  double* Vp = V + (int64_t)m * P.row0;
+  double* w = W + P.row0;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double alpha = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    // h = V[:, 0..j]^T w : one wave per column
+    for (int c = wid; c <= j; c += 4) {
+      const double* vc = Vp + (int64_t)c * n;
+      double s = 0.0;
+      for (int r = lane; r < n; r += 64) s += vc[r] * w[r];
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+      if (lane == 0) h[c] = s;
+    }
+    __syncthreads();
+    alpha += h[j];
+    // w -= V h
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      double acc = 0.0;
+      for (int c = 0; c <= j; ++c) acc += Vp[(int64_t)c * n + r] * h[c];
+      w[r] -= acc;
+    }
+    __syncthreads();
+  }
+  double ss = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) ss += w[r] * w[r];
+  const double beta = sqrt(block_sum(ss, red));
+  if (threadIdx.x == 0) {
+    ab[((int64_t)pd * m + j) * 2 + 0] = alpha;
+    ab[((int64_t)pd * m + j) * 2 + 1] = beta;
+  }
+  if (j + 1 < m) {
+    const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
+    double* vn = Vp + (int64_t)(j + 1) * n;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) vn[r] = w[r] * inv;
+  }
+}
+
+}  // namespace memi
+
+using namespace memi;
+
+extern "C" {
+
+int memi_lanczos_step(const void* plan_dev, const memi_plan_info* info, const double* L,
+                      double* V, int m, int j, const int32_t* active, double* W, double* ab,
+                      void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && L && V && active && W && ab, "memi_lanczos_step: null");
+  MEMI_CHECK_ARG(m >= 2 && m <= 4096 && j >= 0 && j < m, "memi_lanczos_step: bad m=%d j=%d", m,
+                 j);
+  if (info->n_pd == 0) return MEMI_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const char* plan = (const char*)plan_dev;
+  const memi_pd_desc* pds = (const memi_pd_desc*)(plan + info->off_pds);
+  const int32_t* rows = (const int32_t*)(plan + info->off_rows);
+  const int64_t waves = info->n_rows;
+  hipLaunchKernelGGL(k_lanczos_matvec, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, rows,
+                     info->n_rows, pds, L, V, m, j, active, W);
+  MEMI_CHECK_LAUNCH("k_lanczos_matvec");
+  hipLaunchKernelGGL(k_lanczos_orth, dim3((unsigned)info->n_pd), dim3(256),
+                     (size_t)m * sizeof(double), s, pds, V, m, j, active, W, ab);
+  MEMI_CHECK_LAUNCH("k_lanczos_orth");
+  return MEMI_OK;
+}
+
+}  // extern "C"

@@ -396,7 +396,8 @@ int memi_plan(const memi_pd_desc* pds_in, int npd, int chunks, void* plan_host,
 int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const void* op_hi,
                       const void* op_lo, int64_t Khp, int precision, double kappa,
                       const void* fy, const float* ctf_half, int N, float* D, float* ws,
-                      int32_t* flags, int32_t* flag_count, void* stream) {
+                      int32_t* flags, int32_t* flag_count, void* ev_start, void* ev_stop,
+                      void* stream) {
   MEMI_CHECK_ARG(plan_dev && info && op_hi && fy && ctf_half && D && ws && flags && flag_count,
                  "memi_ctf_distance: null pointer");
   MEMI_CHECK_ARG(precision == MEMI_PREC_FP32 || (precision == MEMI_PREC_SPLIT_BF16 && op_lo),
@@ -420,6 +421,10 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
   MEMI_CHECK_ARG(grid < (int64_t)1 << 31, "memi_ctf_distance: too many work items");
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, flag_count, (int64_t)1);
   MEMI_CHECK_LAUNCH("k_zero_i32");
+  if (ev_start && hipEventRecord((hipEvent_t)ev_start, s) != hipSuccess) {
+    set_error("memi_ctf_distance: hipEventRecord failed");
+    return MEMI_ELAUNCH;
+  }
   if (precision == MEMI_PREC_FP32) {
     hipLaunchKernelGGL(k_gram_tiles<MEMI_PREC_FP32>, dim3((unsigned)grid), dim3(256), 0, s,
                        (const char*)op_hi, (const char*)nullptr, Khp, kc, items, info->n_items,
@@ -430,6 +435,10 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
                        info->n_items, pds, ws);
   }
   MEMI_CHECK_LAUNCH("k_gram_tiles");
+  if (ev_stop && hipEventRecord((hipEvent_t)ev_stop, s) != hipSuccess) {
+    set_error("memi_ctf_distance: hipEventRecord failed");
+    return MEMI_ELAUNCH;
+  }
   hipLaunchKernelGGL(k_gram_finalize, dim3((unsigned)info->n_utiles), dim3(256), 0, s, ut, pds,
                      ws, chunks, kappa, D, flags, info->flag_cap, flag_count);
   MEMI_CHECK_LAUNCH("k_gram_finalize");

@@ -66,24 +66,30 @@ __device__ inline double bspline_hat(int i, int n) {
   return (4.0 + 2.0 * cospi(2.0 * (double)i / (double)n)) / 6.0;
 }
 
-__global__ void k_spectrum_scale(float2* __restrict__ spec, int64_t total, int N, int kind,
-                                 double Qc, int order) {
+// thread per half-spectrum bin (factor evaluated once), images looped in chunks
+__global__ void k_spectrum_scale(float2* __restrict__ spec, int64_t n, int N, int kind,
+                                 double Qc, int order, int img_chunk) {
   const int Nh = N / 2 + 1;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int col = (int)(e % Nh);
-    const int row = (int)((e / Nh) % N);
-    double f = 1.0;
-    if (kind & (MEMI_FILTER_BUTTER | MEMI_FILTER_GAUSS)) {
-      const double q = radial_q(row, col, N) / Qc;
-      if (kind & MEMI_FILTER_BUTTER) f *= sqrt(1.0 / (1.0 + pow(q, 2.0 * order)));
-      if (kind & MEMI_FILTER_GAUSS) f *= exp(-(log(2.0) / 2.0) * q * q);
-    }
-    if (kind & MEMI_FILTER_BSPLINE) f /= bspline_hat(row, N) * bspline_hat(col, N);
-    float2 v = spec[e];
-    v.x = (float)(v.x * f);
-    v.y = (float)(v.y * f);
-    spec[e] = v;
+  const int64_t Kh = (int64_t)N * Nh;
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= Kh) return;
+  const int col = (int)(k % Nh);
+  const int row = (int)(k / Nh);
+  double f = 1.0;
+  if (kind & (MEMI_FILTER_BUTTER | MEMI_FILTER_GAUSS)) {
+    const double q = radial_q(row, col, N) / Qc;
+    if (kind & MEMI_FILTER_BUTTER) f *= sqrt(1.0 / (1.0 + pow(q, 2.0 * order)));
+    if (kind & MEMI_FILTER_GAUSS) f *= exp(-(log(2.0) / 2.0) * q * q);
+  }
+  if (kind & MEMI_FILTER_BSPLINE) f /= bspline_hat(row, N) * bspline_hat(col, N);
+  const float ff = (float)f;
+  const int64_t i0 = (int64_t)blockIdx.y * img_chunk;
+  const int64_t i1 = min(n, i0 + img_chunk);
+  for (int64_t i = i0; i < i1; ++i) {
+    float2 v = spec[i * Kh + k];
+    v.x *= ff;
+    v.y *= ff;
+    spec[i * Kh + k] = v;
   }
 }
 
@@ -115,17 +121,16 @@ __device__ inline void bspline_w(float t, float w[4]) {
 
 __global__ __launch_bounds__(256) void k_spline_rotate(const float* __restrict__ coef,
                                                        const double* __restrict__ angle_deg,
-                                                       int N, int64_t n, float* __restrict__ out) {
+                                                       int N, float* __restrict__ out) {
   const int NN = N * N;
-  const int64_t total = n * NN;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t img = e / NN;
-    const int p = (int)(e % NN);
+  const int64_t img = blockIdx.x;
+  double sn, cs;
+  sincospi(angle_deg[img] / 180.0, &sn, &cs);
+  const double h = 0.5 * (N - 1);
+  const float* cf = coef + img * NN;
+  float* o = out + img * NN;
+  for (int p = threadIdx.x; p < NN; p += blockDim.x) {
     const int r = p / N, c = p % N;
-    double sn, cs;
-    sincospi(angle_deg[img] / 180.0, &sn, &cs);
-    const double h = 0.5 * (N - 1);
     const double dr = r - h, dc = c - h;
     const double sr = h + cs * dr + sn * dc;
     const double sc = h - sn * dr + cs * dc;
@@ -133,26 +138,24 @@ __global__ __launch_bounds__(256) void k_spline_rotate(const float* __restrict__
     float wr[4], wc[4];
     bspline_w((float)(sr - fr), wr);
     bspline_w((float)(sc - fc), wc);
-    const float* cf = coef + img * NN;
     int ir = ((int)fr - 1) % N;
     if (ir < 0) ir += N;
     int ic0 = ((int)fc - 1) % N;
     if (ic0 < 0) ic0 += N;
+    int ics[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) ics[b] = (ic0 + b) % N;
     float acc = 0.f;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const float* rowp = cf + ir * N;
-      int ic = ic0;
       float racc = 0.f;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        racc += wc[b] * rowp[ic];
-        ic = (ic + 1 == N) ? 0 : ic + 1;
-      }
+      for (int b = 0; b < 4; ++b) racc += wc[b] * rowp[ics[b]];
       acc += wr[a] * racc;
       ir = (ir + 1 == N) ? 0 : ir + 1;
     }
-    out[e] = acc;
+    o[p] = acc;
   }
 }
 
@@ -163,26 +166,45 @@ __device__ inline uint16_t bf16_rne(float f) {
 }
 __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// thread per half-spectrum bin k (grid.y: image chunks).  The CTF phase
+// wr = (w1/2 k^2 - w2) k^2 = a_k - c_k df is formed in float64, reduced to [-pi, pi] in float64,
+// and only then passed to the float32 sincos (|C| error ~1e-7).
 __global__ void k_ctf_pack(const float2* __restrict__ fy, const double* __restrict__ df,
                            int64_t n, int N, memi_ctf_params cp, int prec, int64_t Khp,
                            int64_t row0, void* __restrict__ op_hi, void* __restrict__ op_lo,
-                           float* __restrict__ ctf_half) {
+                           float* __restrict__ ctf_half, int img_chunk) {
   const int Nh = N / 2 + 1;
   const int64_t Kh = (int64_t)N * Nh;
-  const int64_t total = n * Khp;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = e / Khp;
-    const int64_t k = e % Khp;
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= Khp) return;
+  const bool live = k < Kh;
+  const int row = live ? (int)(k / Nh) : 0, col = live ? (int)(k % Nh) : 0;
+  const double kk = live ? radial_q(row, col, N) / (2.0 * cp.pix_size) : 0.0;
+  const double cs = cp.cs_mm * 1.0e7;
+  const double wav = 12.3986 / sqrt((2.0 * 511.0 + cp.ekv) * cp.ekv);
+  const double k2 = kk * kk;
+  const double a_k = 0.5 * (M_PI * cs * wav * wav * wav) * k2 * k2;
+  const double c_k = M_PI * wav * k2;
+  double env = 1.0;
+  if (isfinite(cp.gauss_env)) {
+    const double sigm = cp.gauss_env / sqrt(2.0 * log(2.0));
+    env = exp(-k2 / (2.0 * sigm * sigm));
+  }
+  const double w = live ? half_weight(col, N) : 0.0;
+  const double sw = sqrt(w);
+  const double inv2pi = 0.15915494309189535, twopi = 6.283185307179586;
+  const int64_t i0 = (int64_t)blockIdx.y * img_chunk;
+  const int64_t i1 = min(n, i0 + img_chunk);
+  for (int64_t i = i0; i < i1; ++i) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    if (k < Kh) {
-      const int row = (int)(k / Nh), col = (int)(k % Nh);
-      const double kk = radial_q(row, col, N) / (2.0 * cp.pix_size);
-      const double C = ctf_value(kk, df[i], cp.cs_mm, cp.ekv, cp.ampc, cp.gauss_env);
+    if (live) {
+      const double wr = a_k - c_k * df[i];
+      const double red = wr - twopi * rint(wr * inv2pi);
+      float sf, cf;
+      sincosf((float)red, &sf, &cf);
+      const double C = ((double)sf - cp.ampc * (double)cf) * env;
       ctf_half[i * Kh + k] = (float)C;
       const float2 X = fy[i * Kh + k];
-      const double w = half_weight(col, N);
-      const double sw = sqrt(w);
       v[0] = w * ((double)X.x * X.x + (double)X.y * X.y);
       v[1] = C * C;
       v[2] = sw * C * X.x;
@@ -192,16 +214,16 @@ __global__ void k_ctf_pack(const float2* __restrict__ fy, const double* __restri
     if (prec == MEMI_PREC_FP32) {
       float* o = (float*)op_hi;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) o[base + s * Khp] = (float)v[s];
+      for (int sg = 0; sg < 4; ++sg) o[base + sg * Khp] = (float)v[sg];
     } else {
       uint16_t* oh = (uint16_t*)op_hi;
       uint16_t* ol = (uint16_t*)op_lo;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const float f = (float)v[s];
+      for (int sg = 0; sg < 4; ++sg) {
+        const float f = (float)v[sg];
         const uint16_t h = bf16_rne(f);
-        oh[base + s * Khp] = h;
-        ol[base + s * Khp] = bf16_rne((float)((double)f - (double)bf16_to_f(h)));
+        oh[base + sg * Khp] = h;
+        ol[base + sg * Khp] = bf16_rne((float)((double)f - (double)bf16_to_f(h)));
       }
     }
   }
@@ -277,10 +299,13 @@ int memi_spectrum_scale(void* spec, int64_t n, int N, int kind, double Qc, int o
                  "memi_spectrum_scale: unknown filter kind %d", kind);
   MEMI_CHECK_ARG(!(kind & (MEMI_FILTER_BUTTER | MEMI_FILTER_GAUSS)) || Qc > 0,
                  "memi_spectrum_scale: Qc must be positive");
-  const int64_t total = n * (int64_t)N * (N / 2 + 1);
-  if (total == 0) return MEMI_OK;
-  hipLaunchKernelGGL(k_spectrum_scale, dim3(grid_for(total)), dim3(256), 0,
-                     (hipStream_t)stream, (float2*)spec, total, N, kind, Qc, order);
+  const int64_t Kh = (int64_t)N * (N / 2 + 1);
+  if (n == 0) return MEMI_OK;
+  const int chunk = 16;
+  dim3 g((unsigned)((Kh + 255) / 256), (unsigned)((n + chunk - 1) / chunk));
+  MEMI_CHECK_ARG(g.y < 65536u * 16, "memi_spectrum_scale: too many images");
+  hipLaunchKernelGGL(k_spectrum_scale, g, dim3(256), 0, (hipStream_t)stream, (float2*)spec, n, N,
+                     kind, Qc, order, chunk);
   MEMI_CHECK_LAUNCH("k_spectrum_scale");
   return MEMI_OK;
 }
@@ -299,10 +324,10 @@ int memi_spline_rotate(const float* coef, const double* angle_deg, int64_t n, in
                        void* stream) {
   MEMI_CHECK_ARG(coef && angle_deg && out && n >= 0 && N > 0, "memi_spline_rotate: bad args");
   MEMI_CHECK_ARG(coef != out, "memi_spline_rotate: in-place rotation is not supported");
-  const int64_t total = n * (int64_t)N * N;
-  if (total == 0) return MEMI_OK;
-  hipLaunchKernelGGL(k_spline_rotate, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream,
-                     coef, angle_deg, N, n, out);
+  if (n == 0) return MEMI_OK;
+  MEMI_CHECK_ARG(n < ((int64_t)1 << 31), "memi_spline_rotate: too many images");
+  hipLaunchKernelGGL(k_spline_rotate, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream, coef,
+                     angle_deg, N, out);
   MEMI_CHECK_LAUNCH("k_spline_rotate");
   return MEMI_OK;
 }
@@ -317,11 +342,11 @@ int memi_ctf_pack(const void* fy, const double* df, int64_t n, int N, const memi
   MEMI_CHECK_ARG(Khp >= Kh && Khp % 64 == 0, "memi_ctf_pack: Khp=%lld must be >= Kh=%lld and a "
                  "multiple of 64", (long long)Khp, (long long)Kh);
   MEMI_CHECK_ARG(ctf->pix_size > 0, "memi_ctf_pack: pixel size must be positive");
-  const int64_t total = n * Khp;
-  if (total == 0) return MEMI_OK;
-  hipLaunchKernelGGL(k_ctf_pack, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream,
-                     (const float2*)fy, df, n, N, *ctf, precision, Khp, row0, op_hi, op_lo,
-                     ctf_half);
+  if (n == 0) return MEMI_OK;
+  const int chunk = 8;
+  dim3 g((unsigned)((Khp + 255) / 256), (unsigned)((n + chunk - 1) / chunk));
+  hipLaunchKernelGGL(k_ctf_pack, g, dim3(256), 0, (hipStream_t)stream, (const float2*)fy, df, n, N,
+                     *ctf, precision, Khp, row0, op_hi, op_lo, ctf_half, chunk);
   MEMI_CHECK_LAUNCH("k_ctf_pack");
   return MEMI_OK;
 }

@@ -208,8 +208,9 @@ class Distances:
         return self.plan.block(self.D, p)
 
 
-def ctf_distances(plan, operand, fy, ctf_half, N, kappa=None, ws=None):
-    """a8 for every PD of the plan: returns Distances (float32 D on the device)."""
+def ctf_distances(plan, operand, fy, ctf_half, N, kappa=None, ws=None, events=None):
+    """a8 for every PD of the plan: returns Distances (float32 D on the device).  `events`: an
+    optional (start, stop) hipEvent_t pair recorded around the MFMA Gram kernel."""
     dev = operand.hi.device
     info = plan.info
     kappa = KAPPA[operand.precision] if kappa is None else kappa
@@ -219,7 +220,8 @@ def ctf_distances(plan, operand, fy, ctf_half, N, kappa=None, ws=None):
     flags = torch.empty(3 * info.flag_cap, dtype=torch.int32, device=dev)
     cnt = torch.zeros(1, dtype=torch.int32, device=dev)
     nat.ctf_distance(plan, operand.hi, operand.lo if operand.lo is not None else operand.hi,
-                     operand.Khp, operand.precision, kappa, fy, ctf_half, N, D, ws, flags, cnt)
+                     operand.Khp, operand.precision, kappa, fy, ctf_half, N, D, ws, flags, cnt,
+                     events)
     return Distances(plan, D, cnt)
 
 
@@ -255,6 +257,71 @@ class DMResult:
                 self.R_squared)
 
 
+def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202):
+    """Largest-magnitude eigenpairs of every PD's float64 kernel block of L (sembeddingonFly.py
+    semantics: the k eigenpairs of largest |lambda|, sorted by descending lambda) by batched
+    Lanczos with full reorthogonalisation on the GPU.
+
+    Converged when every selected Ritz pair has residual |beta_J s_J| <= tol * max|theta| or
+    the Krylov space is the whole space.  Returns a list of (lam, V (n x k) device tensor,
+    steps, max residual)."""
+    from scipy.linalg import eigh_tridiagonal
+    dev = L.device
+    npd = plan.npd
+    ns = plan.ns.astype(int)
+    rows0 = plan.rows0
+    n_rows = int(ns.sum())
+    if m is None:
+        m = int(min(ns.max(), 512))
+    m = int(max(2, min(m, ns.max())))
+    V = torch.zeros(m * n_rows, dtype=torch.float64, device=dev)
+    W = torch.empty(n_rows, dtype=torch.float64, device=dev)
+    ab = torch.zeros(npd * m * 2, dtype=torch.float64, device=dev)
+    rng = np.random.default_rng(seed)
+    v0 = np.empty(n_rows)
+    idx = np.empty(n_rows, dtype=np.int64)
+    for p in range(npd):
+        n = ns[p]
+        v = 1.0 + 0.1 * rng.standard_normal(n)
+        v0[rows0[p]:rows0[p] + n] = v / np.linalg.norm(v)
+        idx[rows0[p]:rows0[p] + n] = m * rows0[p] + np.arange(n)
+    V[torch.as_tensor(idx, device=dev)] = torch.as_tensor(v0, device=dev)
+    active_h = np.ones(npd, dtype=np.int32)
+    active = torch.as_tensor(active_h, device=dev)
+    steps = np.minimum(ns, m)
+    out = [None] * npd
+    j = 0
+    while active_h.any():
+        stop = min(m, j + check_every)
+        while j < stop:
+            nat.lanczos_step(plan, L, V, m, j, active, W, ab)
+            j += 1
+        abh = ab.view(npd, m, 2).cpu().numpy()
+        for p in np.nonzero(active_h)[0]:
+            J = int(min(j, steps[p]))
+            alpha, beta = abh[p, :J, 0], abh[p, :J, 1]
+            theta, S = eigh_tridiagonal(alpha, beta[:J - 1])
+            k = ks[p]
+            sel = np.argsort(-np.abs(theta), kind="stable")[:k]
+            res = np.abs(beta[J - 1] * S[J - 1, sel])
+            scale = np.max(np.abs(theta[sel]))
+            done = (J >= ns[p]) or (np.all(res <= tol * scale)) or (J >= m)
+            if done:
+                order = np.argsort(-theta[sel], kind="stable")
+                sel = sel[order]
+                out[p] = (theta[sel], S[:, sel], J, float(np.max(res[order])) if J < ns[p] else 0.0)
+                active_h[p] = 0
+        active.copy_(torch.as_tensor(active_h, device=dev))
+    results = []
+    for p in range(npd):
+        lam, S, J, res = out[p]
+        n = ns[p]
+        Vp = V[m * rows0[p]:m * rows0[p] + J * n].view(J, n)
+        Y = (torch.as_tensor(np.ascontiguousarray(S.T), device=dev) @ Vp).T
+        results.append((lam, Y, J, res))
+    return results
+
+
 def top_eigs_lm(L, k):
     """``sembeddingonFly.py:27-35`` semantics on a dense float64 GPU matrix: the k
     eigenpairs of largest magnitude, sorted by descending eigenvalue."""
@@ -286,9 +353,11 @@ def ferguson_sums(plan, D, mask, logEps):
     return S, slope, fer[:, :, 2]
 
 
-def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, mask=None):
+def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, mask=None,
+             eig="lanczos"):
     """a10-a16 for every PD of the plan.  D: flat float32 device buffer.  Returns a list of
-    DMResult (numpy outputs)."""
+    DMResult (numpy outputs).  eig: "lanczos" (batched GPU Lanczos) or "dense" (rocSOLVER
+    syevd on each PD, for checking)."""
     dev = D.device
     info = plan.info
     npd = plan.npd
@@ -313,11 +382,17 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
     L = torch.empty(max(info.d_elems, 1), dtype=torch.float64, device=dev)
     deg = torch.empty(2 * info.n_rows, dtype=torch.float64, device=dev)
     nat.markov_normalize(plan, D, mask, sigma, L, deg)
+    ks = [min(num_eigs, int(n) - 3) + 1 for n in plan.ns]
+    if eig == "lanczos":
+        eigs = lanczos_eigs(plan, L, ks)
     for p in range(npd):
         n = int(plan.ns[p])
-        nEigs = min(num_eigs, n - 3)
-        lam, V = top_eigs_lm(plan.block(L, p), nEigs + 1)
-        lam = lam.cpu().numpy()
+        nEigs = ks[p] - 1
+        if eig == "lanczos":
+            lam, V, _, _ = eigs[p]
+        else:
+            lam, V = top_eigs_lm(plan.block(L, p), nEigs + 1)
+            lam = lam.cpu().numpy()
         V = V.cpu().numpy()
         r = results[p]
         r.lamb = lam

@@ -110,3 +110,34 @@ def test_ferguson_slope_matches_finite_differences(cuda):
     fd = np.array([(logS(le + h) - logS(le - h)) / (2 * h) for le in logEps])
     assert np.max(np.abs(slope[0] - fd)) < 1e-6
     assert np.max(np.abs(np.log(S[0]) - np.array([logS(le) for le in logEps]))) < 1e-6
+
+
+def test_batched_lanczos_matches_dense(cuda):
+    """Three PDs of ragged sizes in one plan: GPU Lanczos eigenpairs (largest magnitude)
+    against numpy eigh of each normalised kernel block."""
+    import torch
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    ns, ks = [150, 400, 257], [150, 120, 257]
+    D = [_distance_matrix(n, 20 + i) for i, n in enumerate(ns)]
+    rows0 = np.concatenate(([0], np.cumsum(ns)[:-1]))
+    plan = nat.Plan(rows0, ns, ks, chunks=1)
+    flat = torch.as_tensor(np.concatenate([d.astype(np.float32).ravel() for d in D]),
+                           device=cuda)
+    mask = pl.knn_mask(plan, flat)
+    sigma = torch.as_tensor([np.sqrt(np.median(d)) for d in D], device=cuda)
+    L = torch.empty(plan.info.d_elems, dtype=torch.float64, device=cuda)
+    deg = torch.empty(2 * plan.info.n_rows, dtype=torch.float64, device=cuda)
+    nat.markov_normalize(plan, flat, mask, sigma, L, deg)
+    kk = [12, 16, 9]
+    res = pl.lanczos_eigs(plan, L, kk)
+    for p in range(3):
+        A = plan.block(L, p).cpu().numpy()
+        assert np.allclose(A, A.T)
+        w, U = np.linalg.eigh(A)
+        sel = np.argsort(-np.abs(w), kind="stable")[:kk[p]]
+        sel = sel[np.argsort(-w[sel], kind="stable")]
+        lam, Y, J, r = res[p]
+        np.testing.assert_allclose(lam, w[sel], atol=1e-9)
+        Y = Y.cpu().numpy()
+        for c in range(kk[p]):
+            assert abs(abs(np.dot(Y[:, c], U[:, sel[c]])) - 1) < 1e-6, (p, c)
