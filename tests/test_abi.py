@@ -1,0 +1,77 @@
+"""The C-ABI library: loads without a GPU, exports every entry point include/memi.h declares,
+and its host-only plan builder and argument checks behave.  CPU only (no kernel launches)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "memi.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(memi_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from manifoldem_python_amd import _native as nat
+    lib = nat.load()
+    syms = _header_symbols()
+    assert len(syms) >= 24
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(nat.EXPORTS)
+    assert lib.memi_version() >= 10000
+
+
+def test_plan_tables():
+    from manifoldem_python_amd import _native as nat
+    ns, ks = [300, 129, 1], [150, 200, 1]
+    rows0 = np.concatenate(([0], np.cumsum(ns)[:-1]))
+    pl = nat.Plan(rows0, ns, ks, chunks=2, device="cpu")
+    I = pl.info
+    T = [(n + 127) // 128 for n in ns]
+    assert I.n_utiles == sum(t * (t + 1) // 2 for t in T)
+    assert I.n_ftiles == sum(t * t for t in T)
+    assert I.n_items == I.n_utiles * 4 * 2
+    assert I.n_rows == sum(ns)
+    assert I.d_elems == sum(n * n for n in ns)
+    assert I.mask_words == sum(n * ((n + 31) // 32) for n in ns)
+    assert I.list_elems == sum(min(k, n) * n for n, k in zip(ns, ks))
+    assert I.flag_cap == sum(n * (n - 1) // 2 for n in ns)
+    np.testing.assert_array_equal(pl.pds["d_off"], [0, 90000, 90000 + 129 * 129])
+    items = pl.host[I.off_items:I.off_items + I.n_items * 24].view(np.int32).reshape(-1, 6)
+    # every (tile, segment, chunk) slot appears exactly once
+    assert len(set(items[:, 5].tolist())) == I.n_items
+    assert items[:, 5].max() == I.n_utiles * 4 * 2 - 1
+    assert set(items[:, 3].tolist()) == {0, 1, 2, 3}
+    ut = pl.host[I.off_utiles:I.off_utiles + I.n_utiles * 16].view(np.int32).reshape(-1, 4)
+    assert np.all(ut[:, 1] <= ut[:, 2])
+
+
+def test_plan_rejects_bad_input():
+    from manifoldem_python_amd import _native as nat
+    with pytest.raises(RuntimeError, match="packed contiguously"):
+        nat.Plan([0, 5], [10, 10], [5, 5], device="cpu")
+    with pytest.raises(RuntimeError, match="n=0"):
+        nat.Plan([0], [0], [5], device="cpu")
+
+
+def test_wrappers_refuse_host_tensors():
+    import torch
+    from manifoldem_python_amd import _native as nat
+    x = torch.zeros(4, 8, 8)
+    with pytest.raises(RuntimeError, match="GPU tensors"):
+        nat.mask_multiply(x, 8)
+
+
+def test_no_cpu_fallback_in_product_modules():
+    """The product package never imports the oracle."""
+    pkg = os.path.join(ROOT, "manifoldem_python_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith(".py"):
+                src = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in src and "from oracle" not in src, f
