@@ -190,6 +190,20 @@ int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, cons
                           const uint32_t* mask, const double* sigma, double* L, double* deg,
                           void* stream);
 
+/* fergusonE.py:25-43 on a flat value list (the reference's D argument; squared here):
+ * out float64 [n_eps][3] = {sum exp(-x), sum x exp(-x), #terms} for x = D^2/(2 eps) < thr,
+ * followed by 1 slot for the threshold sum; zeroed here. */
+int memi_ferguson_values(const double* vals, int64_t nv, const double* logEps, int n_eps,
+                         double* out, void* stream);
+
+/* slaplacianonFly.py:42-78 on COO triplets (rows, cols, squared distances): out[e] =
+ * W_e / (d_r d_c) / (d'_r d'_c) with W = exp(-val/sigma^2), d = column sums (^alpha),
+ * d' = sqrt(column sums of W/(d d)); duplicates behave as the reference's csc sums.
+ * deg: float64 scratch [2*nS].  The caller forms |l + l^T| / 2. */
+int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, int64_t nnz,
+                    int64_t nS, double sigma, double alpha, double* out, double* deg,
+                    void* stream);
+
 /* One Lanczos step (full reorthogonalisation, CGS2) on every active PD's L, for the
  * largest-magnitude eigenpairs that sembeddingonFly.py:27 asks ARPACK eigsh for.
  * V: float64 bases, column-major per PD at m*row0 (column 0 = unit start vector, set by the
@@ -203,9 +217,10 @@ int memi_lanczos_step(const void* plan_dev, const memi_plan_info* info, const do
 
 /* Reordered distances and delay embedding, psiAnalysisParS2.py:89-90 and NLSA.py:31-34:
  * ConD[i][j] = sum_{c < con_order} D[ord[i+c]][ord[j+c]], summed in increasing c (float64),
- * D float32 [n][n], ConD float64 [m][m], m = n - con_order. */
-int memi_nlsa_condist(const float* D, const int64_t* ord, int n, int con_order, double* ConD,
-                      void* stream);
+ * D [n][n] float32 (d_is_f64 = 0) or float64 (d_is_f64 = 1), ConD float64 [m][m],
+ * m = n - con_order, 1 <= con_order <= 97. */
+int memi_nlsa_condist(const void* D, int d_is_f64, const int64_t* ord, int n, int con_order,
+                      double* ConD, void* stream);
 
 /* Wiener-filtered supervector spectra, NLSA.py:50-86 and get_wiener.py:9-21 by linearity:
  * out[ii][c][k] = sum_{i<m} mu_psi[i][c] * G[s+i][k] / W_i[k], s = con_order-1-ii,

@@ -64,9 +64,11 @@ _SIGS = {
     "memi_knn_select": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_ferguson_sweep": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_markov_normalize": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "memi_ferguson_values": [c_vp, c_i64, c_vp, c_int, c_vp, c_vp],
+    "memi_markov_coo": [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_vp],
     "memi_lanczos_step": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_int, c_int, c_vp, c_vp,
                           c_vp, c_vp],
-    "memi_nlsa_condist": [c_vp, c_vp, c_int, c_int, c_vp, c_vp],
+    "memi_nlsa_condist": [c_vp, c_int, c_vp, c_int, c_int, c_vp, c_vp],
     "memi_nlsa_supervectors": [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_vp, c_vp],
     "memi_tall_gram": [c_vp, c_i64, c_int, c_vp, c_vp],
     "memi_tall_apply": [c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp],
@@ -266,6 +268,21 @@ def markov_normalize(plan, D, mask, sigma, L, deg):
          _p(sigma), _p(L), _p(deg), _stream())
 
 
+def ferguson_values(vals, logEps, out):
+    call("memi_ferguson_values", _p(vals), vals.numel(), _p(logEps), logEps.numel(), _p(out),
+         _stream())
+
+
+def markov_coo(row, col, val, nS, sigma, alpha, out, deg):
+    if not (row.numel() == col.numel() == val.numel() == out.numel()):
+        raise RuntimeError("markov_coo: row/col/val/out lengths differ")
+    if row.numel() and (int(row.min()) < 0 or int(row.max()) >= nS or int(col.min()) < 0
+                        or int(col.max()) >= nS):
+        raise RuntimeError("markov_coo: index outside [0, nS)")
+    call("memi_markov_coo", _p(row), _p(col), _p(val), val.numel(), int(nS), float(sigma),
+         float(alpha), _p(out), _p(deg), _stream())
+
+
 def lanczos_step(plan, L, V, m, j, active, W, ab):
     call("memi_lanczos_step", _p(plan.dev), ctypes.byref(plan.info), _p(L), _p(V), int(m), int(j),
          _p(active), _p(W), _p(ab), _stream())
@@ -273,7 +290,12 @@ def lanczos_step(plan, L, V, m, j, active, W, ab):
 
 # ------------------------------------------------------------------------- Stage 3 ---------
 def nlsa_condist(D, ord_, n, con_order, ConD):
-    call("memi_nlsa_condist", _p(D), _p(ord_), n, con_order, _p(ConD), _stream())
+    if D.dtype not in (torch.float32, torch.float64) or D.numel() != n * n:
+        raise RuntimeError("nlsa_condist: D must be float32/float64 with n*n elements")
+    if ord_.numel() != n or ConD.numel() != (n - con_order) ** 2:
+        raise RuntimeError("nlsa_condist: ord / ConD sizes do not match n and ConOrder")
+    call("memi_nlsa_condist", _p(D), int(D.dtype == torch.float64), _p(ord_), n, con_order,
+         _p(ConD), _stream())
 
 
 def nlsa_supervectors(spec, ctf_half, n, N, con_order, mu_psi, snr, out):

@@ -356,6 +356,91 @@ __global__ __launch_bounds__(256) void k_markov_write(const int32_t* __restrict_
   }
 }
 
+// ------------------------------------------------- list forms (fergusonE / slaplacian) ----
+// fergusonE.op(D, logEps, a0) on a flat list: D2 = D*D; thr pass then the eps sweep.
+__global__ __launch_bounds__(256) void k_ferguson_values_thr(const double* __restrict__ v,
+                                                             int64_t nv,
+                                                             const double* __restrict__ logEps,
+                                                             int n_eps, double* __restrict__ acc) {
+  __shared__ double red[8];
+  double mx = -INFINITY;
+  for (int e = threadIdx.x; e < n_eps; e += blockDim.x) mx = fmax(mx, logEps[e]);
+  __shared__ double mx_sh[4];
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) mx_sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  mx = fmax(fmax(mx_sh[0], mx_sh[1]), fmax(mx_sh[2], mx_sh[3]));
+  const double cmin = 1.0 / (2.0 * exp(mx));
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += exp(-(v[i] * v[i]) * cmin);
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) atomicAdd(acc, s);
+}
+
+__global__ __launch_bounds__(256) void k_ferguson_values(const double* __restrict__ v, int64_t nv,
+                                                         const double* __restrict__ logEps,
+                                                         int n_eps, const double* __restrict__ ss,
+                                                         double* __restrict__ out) {
+  constexpr int CH = 2048;
+  __shared__ double vals[CH];
+  const int e_idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = e_idx < n_eps;
+  const double thr = fmax(-log(0.01 * ss[0] / (double)nv), 10.0);
+  const double c = active ? 1.0 / (2.0 * exp(logEps[e_idx])) : 0.0;
+  const int64_t b0 = (int64_t)blockIdx.y * CH;
+  const int cnt = (int)min((int64_t)CH, nv - b0);
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+    const double d = v[b0 + t];
+    vals[t] = d * d;
+  }
+  __syncthreads();
+  if (!active) return;
+  double S = 0.0, SX = 0.0, CN = 0.0;
+  for (int t = 0; t < cnt; ++t) {
+    const double x = vals[t] * c;
+    if (x < thr) {
+      const double ex = exp(-x);
+      S += ex;
+      SX += x * ex;
+      CN += 1.0;
+    }
+  }
+  atomicAdd(out + e_idx * 3 + 0, S);
+  atomicAdd(out + e_idx * 3 + 1, SX);
+  atomicAdd(out + e_idx * 3 + 2, CN);
+}
+
+// slaplacianonFly.op on COO triplets: W = exp(-val/sigma^2); d = column sums (^alpha);
+// W' = W/(d_r d_c); d' = sqrt(column sums of W'); out = W'/(d'_r d'_c).
+template <int PASS>
+__global__ void k_coo_norm(const int64_t* __restrict__ row, const int64_t* __restrict__ col,
+                           const double* __restrict__ val, int64_t nnz, double inv_s2,
+                           double alpha, double* __restrict__ deg, int64_t nS,
+                           double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const double w = exp(-val[e] * inv_s2);
+    const int64_t r = row[e], c = col[e];
+    if (PASS == 0) {
+      atomicAdd(&deg[c], w);
+    } else {
+      double dr = deg[r], dc = deg[c];
+      if (alpha != 1.0) {
+        dr = pow(dr, alpha);
+        dc = pow(dc, alpha);
+      }
+      const double w1 = w / (dr * dc);
+      if (PASS == 1) {
+        atomicAdd(&deg[nS + c], w1);
+      } else {
+        out[e] = w1 / (sqrt(deg[nS + r]) * sqrt(deg[nS + c]));
+      }
+    }
+  }
+}
+
 __global__ void k_zero_f64(double* p, int64_t n) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x)
@@ -444,6 +529,48 @@ int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, cons
   hipLaunchKernelGGL(k_markov_write, g, dim3(256), 0, s, v.ftiles, v.pds, D, mask, sigma, deg,
                      nr, L);
   MEMI_CHECK_LAUNCH("k_markov_write");
+  return MEMI_OK;
+}
+
+int memi_ferguson_values(const double* vals, int64_t nv, const double* logEps, int n_eps,
+                         double* out, void* stream) {
+  MEMI_CHECK_ARG(vals && logEps && out && nv > 0 && n_eps > 0,
+                 "memi_ferguson_values: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nz = (int64_t)n_eps * 3 + 1;
+  hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(nz)), dim3(256), 0, s, out, nz);
+  MEMI_CHECK_LAUNCH("k_zero_f64");
+  double* ss = out + (int64_t)n_eps * 3;
+  hipLaunchKernelGGL(k_ferguson_values_thr, dim3(grid_for(nv, 256, 1024)), dim3(256), 0, s, vals,
+                     nv, logEps, n_eps, ss);
+  MEMI_CHECK_LAUNCH("k_ferguson_values_thr");
+  dim3 g((unsigned)((n_eps + 255) / 256), (unsigned)((nv + 2047) / 2048));
+  MEMI_CHECK_ARG(g.y < 65536u * 32, "memi_ferguson_values: too many values");
+  hipLaunchKernelGGL(k_ferguson_values, g, dim3(256), 0, s, vals, nv, logEps, n_eps, ss, out);
+  MEMI_CHECK_LAUNCH("k_ferguson_values");
+  return MEMI_OK;
+}
+
+int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, int64_t nnz,
+                    int64_t nS, double sigma, double alpha, double* out, double* deg,
+                    void* stream) {
+  MEMI_CHECK_ARG(row && col && val && out && deg && nnz >= 0 && nS > 0 && sigma > 0,
+                 "memi_markov_coo: bad arguments");
+  if (nnz == 0) return MEMI_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(2 * nS)), dim3(256), 0, s, deg, 2 * nS);
+  MEMI_CHECK_LAUNCH("k_zero_f64");
+  const double inv_s2 = 1.0 / (sigma * sigma);
+  const dim3 g(grid_for(nnz));
+  hipLaunchKernelGGL(k_coo_norm<0>, g, dim3(256), 0, s, row, col, val, nnz, inv_s2, alpha, deg,
+                     nS, out);
+  MEMI_CHECK_LAUNCH("k_coo_norm<0>");
+  hipLaunchKernelGGL(k_coo_norm<1>, g, dim3(256), 0, s, row, col, val, nnz, inv_s2, alpha, deg,
+                     nS, out);
+  MEMI_CHECK_LAUNCH("k_coo_norm<1>");
+  hipLaunchKernelGGL(k_coo_norm<2>, g, dim3(256), 0, s, row, col, val, nnz, inv_s2, alpha, deg,
+                     nS, out);
+  MEMI_CHECK_LAUNCH("k_coo_norm<2>");
   return MEMI_OK;
 }
 

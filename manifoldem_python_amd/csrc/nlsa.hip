@@ -11,7 +11,8 @@ namespace memi {
 constexpr int CT = 32;
 constexpr int CMAX = 97;  // largest delay order the LDS block holds (32 + 96 = 128 rows)
 
-__global__ __launch_bounds__(256) void k_condist(const float* __restrict__ D,
+template <typename T>
+__global__ __launch_bounds__(256) void k_condist(const T* __restrict__ D,
                                                  const int64_t* __restrict__ ord, int n, int C,
                                                  double* __restrict__ ConD) {
   extern __shared__ double blk[];  // (CT+C-1)^2
@@ -272,8 +273,8 @@ using namespace memi;
 
 extern "C" {
 
-int memi_nlsa_condist(const float* D, const int64_t* ord, int n, int con_order, double* ConD,
-                      void* stream) {
+int memi_nlsa_condist(const void* D, int d_is_f64, const int64_t* ord, int n, int con_order,
+                      double* ConD, void* stream) {
   MEMI_CHECK_ARG(D && ord && ConD, "memi_nlsa_condist: null pointer");
   MEMI_CHECK_ARG(con_order >= 1 && con_order <= CMAX && n > con_order,
                  "memi_nlsa_condist: need 1 <= ConOrder <= %d and n > ConOrder (n=%d, C=%d)",
@@ -282,8 +283,12 @@ int memi_nlsa_condist(const float* D, const int64_t* ord, int n, int con_order, 
   const int S = CT + con_order - 1;
   const size_t shm = (size_t)S * S * sizeof(double);
   dim3 g((m + CT - 1) / CT, (m + CT - 1) / CT);
-  hipLaunchKernelGGL(k_condist, g, dim3(256), shm, (hipStream_t)stream, D, ord, n, con_order,
-                     ConD);
+  if (d_is_f64)
+    hipLaunchKernelGGL(k_condist<double>, g, dim3(256), shm, (hipStream_t)stream,
+                       (const double*)D, ord, n, con_order, ConD);
+  else
+    hipLaunchKernelGGL(k_condist<float>, g, dim3(256), shm, (hipStream_t)stream,
+                       (const float*)D, ord, n, con_order, ConD);
   MEMI_CHECK_LAUNCH("k_condist");
   return MEMI_OK;
 }

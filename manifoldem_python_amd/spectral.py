@@ -1,0 +1,130 @@
+"""NLSA along one diffusion coordinate on the GPU (``NLSA.op``, ``NLSA.py:23-158``).
+
+Stages and where they run:
+  a19 delay-embedding distances ConD          HIP (memi_nlsa_condist), float64 sums
+  a20 diffusion map of ConD                   the Stage-2 GPU path (pipeline.dm_embed)
+  a21 Wiener-filtered supervectors A          HIP spectral contraction (memi_nlsa_supervectors)
+                                              + rocFFT inverse transforms, by linearity: one
+                                              inverse FFT per (delay, eigenvector) instead of
+                                              one FFT pair per (delay, image)
+  a22 svdRF                                   HIP tall-skinny A^T A and A V S^-1; 8x8 eigh on host
+  a23 rank-2 reconstruction IMGT + z-score    factored form: IMGT = Ubar H diag(1/sd), Ubar the
+                                              pixel-centred 2*ConOrder columns of U, H the Hankel
+                                              matrix of the rank-2 chronos (HIP GEMM)
+  a24 L2 distances of IMGT frames             from the factored Gram H^T (Ubar^T Ubar) H
+  a20 second diffusion map                    Stage-2 GPU path
+  a25 1-D open-manifold fit                   host, vectorised (fit1d)
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import fit1d
+from . import pipeline as pl
+
+
+class NLSAResult:
+    __slots__ = ("IMGT", "Topo_mean", "psirec", "psiC1", "sdiag", "VX", "mu", "tau", "a", "b",
+                 "IMGT_dev", "Drecon2", "dm1", "dm2", "A", "U")
+
+    def as_tuple(self):
+        return (self.IMGT, self.Topo_mean, self.psirec, self.psiC1, self.sdiag, self.VX, self.mu,
+                self.tau)
+
+
+def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tune=3,
+         num_eigs=15, msk2=None, logEps=pl.LOG_EPS, rand=None, snr=5.0, want_imgt=True,
+         keep=False):
+    """One psi of psiAnalysisParS2 / NLSA.op.
+
+    D: (n, n) float32/float64 device distances of the PD (posPath order); ord_d: int64 device
+    sort order along psi (PosPsi1), so DD = D[ord][:, ord]; img_idx: int64 device indices of
+    the images in psi order into imgAll / ctf_half (posPath[PosPsi1]).
+    """
+    dev = D.device
+    num = int(ord_d.shape[0])
+    nS = num if nS is None else nS
+    m = num - ConOrder
+    Npix = N * N
+    Nh = N // 2 + 1
+    r = NLSAResult()
+    # a19
+    ConD = torch.empty(m * m, dtype=torch.float64, device=dev)
+    nat.nlsa_condist(D.reshape(-1), ord_d, num, ConOrder, ConD)
+    # a20
+    plan = nat.Plan([0], [m], [m], device=dev)
+    dm1 = pl.dm_embed(plan, ConD.float(), tune, num_eigs, logEps, rand=rand)[0]
+    psiC, mu = dm1.psi, dm1.mu
+    r.psiC1 = psiC.copy()
+    # a21
+    IMG1 = imgAll.index_select(0, img_idx)
+    CTF1 = ctf_half.index_select(0, img_idx).contiguous()
+    spec = torch.fft.rfft2(IMG1).contiguous()
+    ell = psiTrunc - 1
+    psiCx = np.hstack((np.ones((m, 1)), psiC[:, 0:ell]))
+    mu_psi = torch.as_tensor(mu.reshape(-1, 1) * psiCx, device=dev).contiguous()
+    sv = torch.empty((ConOrder, psiTrunc, N, Nh), dtype=torch.complex64, device=dev)
+    nat.nlsa_supervectors(spec, CTF1, num, N, ConOrder, mu_psi, snr, sv)
+    imgs = torch.fft.irfft2(sv, s=(N, N))
+    if msk2 is not None:
+        imgs = imgs * msk2
+    # column-major flatten of each image (NLSA.py:80: img.T.reshape(-1))
+    A = imgs.permute(0, 3, 2, 1).reshape(ConOrder * Npix, psiTrunc).double().contiguous()
+    del sv, imgs, spec, IMG1
+    # a22
+    G = torch.empty((psiTrunc, psiTrunc), dtype=torch.float64, device=dev)
+    nat.tall_gram(A, G)
+    Dv, Vv = np.linalg.eigh(G.cpu().numpy())
+    order = np.argsort(Dv)[::-1]
+    Dv = np.sort(Dv)[::-1]
+    Vv = Vv[:, order]
+    S = np.sqrt(Dv)
+    U = torch.empty_like(A)
+    nat.tall_apply(A, Vv @ np.diag(1.0 / S), U)
+    r.VX = Vv.T @ psiCx.T
+    r.sdiag = S
+    # a23
+    U3 = U.view(ConOrder, Npix, psiTrunc)
+    r.Topo_mean = U3.mean(dim=0).cpu().numpy()
+    n_rec = nS - 2 * ConOrder
+    n_fill = num - 2 * ConOrder
+    Ubar = U3[:, :, 0:2].permute(1, 2, 0).reshape(Npix, 2 * ConOrder).contiguous()
+    z = (S[0:2, None] * (Vv[:, 0:2].T @ psiCx.T))          # (2, m): z_r(j)
+    H = np.zeros((2 * ConOrder, n_rec))
+    jj = np.arange(ConOrder)[:, None] + np.arange(n_fill)[None, :]
+    H[:ConOrder, :n_fill] = z[0][jj]
+    H[ConOrder:, :n_fill] = z[1][jj]
+    Gq = torch.empty((2 * ConOrder, 2 * ConOrder), dtype=torch.float64, device=dev)
+    nat.center_gram(Ubar, Gq)
+    Gq_h = Gq.cpu().numpy()
+    Gram = H.T @ Gq_h @ H
+    dg = np.diag(Gram).copy()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        sd = np.sqrt(dg / Npix)
+        inv_sd = 1.0 / sd
+    if want_imgt:
+        Hd = torch.as_tensor(H, device=dev).contiguous()
+        IMGT = torch.empty((Npix, n_rec), dtype=torch.float64, device=dev)
+        nat.nlsa_imgt(Ubar, Hd, torch.as_tensor(inv_sd, device=dev).contiguous(), IMGT)
+        r.IMGT_dev = IMGT
+    # a24: L2_distance on the z-scored frames, from the factored Gram
+    with np.errstate(divide="ignore", invalid="ignore"):
+        C = Gram * np.outer(inv_sd, inv_sd)
+    aa = np.diag(C)
+    tmp = aa[:, None] + aa[None, :] - 2 * C
+    tmp[tmp < 1e-8] = 0
+    Drecon2 = np.sqrt(tmp) ** 2
+    r.Drecon2 = Drecon2
+    # second DM (NLSA.py:141-146)
+    nSrecon = min(Npix, n_rec)
+    plan2 = nat.Plan([0], [n_rec], [nSrecon], device=dev)
+    dm2 = pl.dm_embed(plan2, torch.as_tensor(Drecon2.astype(np.float32).ravel(), device=dev),
+                      tune, num_eigs, logEps, rand=rand)[0]
+    r.psirec, r.mu = dm2.psi, dm2.mu
+    # a25
+    r.a, r.b, r.tau = fit1d.fit(r.psirec)
+    if keep:
+        r.dm1, r.dm2, r.A, r.U = dm1, dm2, A, U
+    return r
