@@ -39,7 +39,20 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 path")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 3 device copies of 512 MiB (PMC byte-counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """Three 512 MiB device-to-device copies: the reference for FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    a = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}), flush=True)
 
 
 def log(*a):
@@ -104,8 +117,9 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events):
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     gram_ms = [events.elapsed_ms(p) for p in pairs]
+    stages = B.breakdown(batch, wsp, num_eigs, rand)
     return dict(t=t, t_dist=t_dist, gram_ms=float(np.mean(gram_ms)), wsp=wsp, dist=dist,
-                res=res, flags=int(dist.flag_count.item()))
+                res=res, flags=int(dist.flag_count.item()), stages=stages)
 
 
 def fp64_error(batch, wsp, dist):
@@ -200,6 +214,9 @@ def main():
               "independent, so a multi-GPU run would partition the PD list by cost with no "
               "collective (see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     import torch
     from manifoldem_python_amd import _native as nat, pipeline as pl
     dev = torch.device("cuda")
@@ -231,6 +248,7 @@ def main():
         "pds_per_s": batch.ns.size * args.steps / bf["t"],
         "distance_ms_per_step": 1e3 * bf["t_dist"] / args.steps,
         "gram_kernel_ms": bf["gram_ms"],
+        "stage_ms": bf["stages"],
         "kappa_flagged": bf["flags"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
@@ -248,6 +266,7 @@ def main():
                        "ms_per_step": 1e3 * f32["t"] / args.steps,
                        "pds_per_s": batch.ns.size * args.steps / f32["t"],
                        "gram_kernel_ms": f32["gram_ms"],
+                       "stage_ms": f32["stages"],
                        "roofline": {"bound": "mfma", "achieved": a32, "peak": PEAK_FP32_TFLOPS,
                                     "unit": "TFLOP/s", "frac": a32 / PEAK_FP32_TFLOPS}}
         if not args.no_check:

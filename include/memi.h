@@ -183,6 +183,14 @@ int memi_ferguson_sweep(const void* plan_dev, const memi_plan_info* info, const 
                         const uint32_t* mask, const double* logEps, int n_eps, double* out,
                         void* stream);
 
+/* Batched tanh fit of fergusonE.py:45-57: for each PD (block), Levenberg-Marquardt from p0
+ * (float64 [npd][4]) of f = a3 + a2 tanh(a0 x + a1) to y (float64 [npd][m]) at x [m].
+ * out float64 [npd][16] = {popt[4], J^T J at popt (upper triangle, 10), ss_res, iterations};
+ * the caller forms pcov = inv(J^T J) ss_res/(m-4) and applies the reference's restart rule.
+ * active: optional int32 [npd] (0 = skip). */
+int memi_tanh_fit(const double* x, const double* y, int m, int npd, const double* p0,
+                  const int32_t* active, int max_iter, double* out, void* stream);
+
 /* Markov normalisation, slaplacianonFly.py:42-81 with alpha = 1: W = exp(-v/sigma^2) on the
  * kernel's entries, d = column sums, W/(d_i d_j), d' = sqrt(column sums), L = |L + L^T| / 2.
  * sigma: float64 [npd]; L: float64 [info->d_elems]; deg: float64 scratch [2 * n_rows]. */

@@ -64,6 +64,7 @@ _SIGS = {
     "memi_knn_select": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_ferguson_sweep": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_markov_normalize": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "memi_tanh_fit": [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_ferguson_values": [c_vp, c_i64, c_vp, c_int, c_vp, c_vp],
     "memi_markov_coo": [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_vp],
     "memi_lanczos_step": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_int, c_int, c_vp, c_vp,
@@ -126,6 +127,20 @@ def call(name, *args):
     _check(getattr(load(), name)(*args), name)
 
 
+def _need(cond, msg):
+    """Host-side size check before any launch: a mismatch raises instead of reaching a kernel."""
+    if not cond:
+        raise RuntimeError(msg)
+
+
+def _numel(t):
+    return 0 if t is None else t.numel()
+
+
+def _bytes(t):
+    return 0 if t is None else t.numel() * t.element_size()
+
+
 # ------------------------------------------------------------------------- Stage 1 ---------
 def load_normalize(stack, src, conj, N, out):
     if src.numel() and (int(src.min()) < 0 or int(src.max()) >= stack.shape[0]):
@@ -137,14 +152,26 @@ def load_normalize(stack, src, conj, N, out):
 
 
 def spectrum_scale(spec, N, kind, Qc=1.0, order=1):
+    _need(spec.dim() == 3 and tuple(spec.shape[1:]) == (N, N // 2 + 1) and spec.is_complex()
+          and spec.dtype == torch.complex64, "spectrum_scale: spec must be complex64 (n, N, N/2+1)")
     call("memi_spectrum_scale", _p(spec), spec.shape[0], N, kind, float(Qc), int(order), _stream())
 
 
 def mask_multiply(img, N, disk=True, msk2=None):
+    _need(img.dim() == 3 and tuple(img.shape[1:]) == (N, N) and img.dtype == torch.float32,
+          "mask_multiply: img must be float32 (n, N, N)")
+    _need(msk2 is None or (msk2.numel() == N * N and msk2.dtype == torch.float32),
+          "mask_multiply: msk2 must be float32 with N*N elements")
     call("memi_mask_multiply", _p(img), img.shape[0], N, int(bool(disk)), _p(msk2), _stream())
 
 
 def spline_rotate(coef, angle_deg, N, out):
+    n = coef.shape[0]
+    _need(coef.dim() == 3 and tuple(coef.shape[1:]) == (N, N) and coef.dtype == torch.float32,
+          "spline_rotate: coef must be float32 (n, N, N)")
+    _need(out.shape == coef.shape and out.dtype == torch.float32, "spline_rotate: out shape")
+    _need(angle_deg.numel() == n and angle_deg.dtype == torch.float64,
+          "spline_rotate: angle_deg must be float64 (n,)")
     call("memi_spline_rotate", _p(coef), _p(angle_deg), coef.shape[0], N, _p(out), _stream())
 
 
@@ -153,21 +180,41 @@ def ctf_params(pix_size, cs, ekv, ampc, gauss_env):
 
 
 def ctf_pack(fy, df, N, ctf, precision, Khp, row0, op_hi, op_lo, ctf_half):
+    n = fy.shape[0]
+    Kh = N * (N // 2 + 1)
+    _need(fy.dtype == torch.complex64 and fy.numel() == n * Kh, "ctf_pack: fy (n, N, N/2+1) c64")
+    _need(df.dtype == torch.float64 and df.numel() == n, "ctf_pack: df must be float64 (n,)")
+    _need(ctf_half.dtype == torch.float32 and ctf_half.numel() >= n * Kh,
+          "ctf_pack: ctf_half needs n*Kh float32")
+    need = (int(row0) + n) * 4 * int(Khp) * (4 if precision == PREC_FP32 else 2)
+    _need(_bytes(op_hi) >= need and (precision == PREC_FP32 or _bytes(op_lo) >= need),
+          "ctf_pack: operand too small for rows row0..row0+n")
     call("memi_ctf_pack", _p(fy), _p(df), fy.shape[0], N, ctypes.byref(ctf), precision, Khp, row0,
          _p(op_hi), _p(op_lo), _p(ctf_half), _stream())
 
 
 def phase_flip(fy, ctf_half, N, out):
+    n = fy.shape[0]
+    Kh = N * (N // 2 + 1)
+    _need(fy.dtype == torch.complex64 and fy.numel() == n * Kh and out.shape == fy.shape
+          and out.dtype == fy.dtype and ctf_half.numel() >= n * Kh, "phase_flip: shapes")
     call("memi_phase_flip", _p(fy), _p(ctf_half), fy.shape[0], N, _p(out), _stream())
 
 
 def wiener_average(spec, ctf_half, N, snr, out):
+    n = spec.shape[0]
+    Kh = N * (N // 2 + 1)
+    _need(spec.dtype == torch.complex64 and spec.numel() == n * Kh and ctf_half.numel() >= n * Kh
+          and out.dtype == torch.complex64 and out.numel() == Kh, "wiener_average: shapes")
     call("memi_wiener_average", _p(spec), _p(ctf_half), spec.shape[0], N, float(snr), _p(out),
          _stream())
 
 
 def image_moments(img, s, s2):
     n = img.shape[0]
+    npix = img.numel() // max(n, 1)
+    _need(img.dtype == torch.float32 and s.numel() == npix and s2.numel() == npix
+          and s.dtype == torch.float64 and s2.dtype == torch.float64, "image_moments: shapes")
     call("memi_image_moments", _p(img), n, img.numel() // max(n, 1), _p(s), _p(s2), _stream())
 
 
@@ -213,6 +260,18 @@ class Plan:
 
 def ctf_distance(plan, op_hi, op_lo, Khp, precision, kappa, fy, ctf_half, N, D, ws, flags,
                  flag_count, events=None):
+    I = plan.info
+    Kh = N * (N // 2 + 1)
+    esz = 4 if precision == PREC_FP32 else 2
+    _need(_bytes(op_hi) >= I.n_rows * 4 * Khp * esz and
+          (precision == PREC_FP32 or _bytes(op_lo) >= I.n_rows * 4 * Khp * esz),
+          "ctf_distance: operand smaller than the plan's rows")
+    _need(fy.numel() >= I.n_rows * Kh and ctf_half.numel() >= I.n_rows * Kh,
+          "ctf_distance: fy/ctf_half smaller than the plan's rows")
+    _need(D.dtype == torch.float32 and D.numel() >= I.d_elems, "ctf_distance: D too small")
+    _need(ws.dtype == torch.float32 and ws.numel() >= I.ws_floats, "ctf_distance: ws too small")
+    _need(flags.dtype == torch.int32 and flags.numel() >= 3 * I.flag_cap and
+          flag_count.numel() >= 1, "ctf_distance: flag buffers too small")
     ev0, ev1 = (None, None) if events is None else (c_vp(events[0]), c_vp(events[1]))
     call("memi_ctf_distance", _p(plan.dev), ctypes.byref(plan.info), _p(op_hi), _p(op_lo), Khp,
          precision, float(kappa), _p(fy), _p(ctf_half), N, _p(D), _p(ws), _p(flags),
@@ -254,18 +313,40 @@ class HipEvents:
 
 # ------------------------------------------------------------------------- Stage 2 ---------
 def knn_select(plan, D, mask, idx_out=None, val_out=None):
+    I = plan.info
+    _need(D.dtype == torch.float32 and D.numel() >= I.d_elems, "knn_select: D too small")
+    _need(mask.numel() >= I.mask_words and mask.element_size() == 4, "knn_select: mask too small")
+    _need(idx_out is None or idx_out.numel() >= I.list_elems, "knn_select: idx_out too small")
+    _need(val_out is None or val_out.numel() >= I.list_elems, "knn_select: val_out too small")
     call("memi_knn_select", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask), _p(idx_out),
          _p(val_out), _stream())
 
 
 def ferguson_sweep(plan, D, mask, logEps, out):
+    I = plan.info
+    _need(D.numel() >= I.d_elems and mask.numel() >= I.mask_words, "ferguson_sweep: D/mask")
+    _need(logEps.dtype == torch.float64 and out.dtype == torch.float64 and
+          out.numel() >= I.n_pd * (3 * logEps.numel() + 2), "ferguson_sweep: out too small")
     call("memi_ferguson_sweep", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask),
          _p(logEps), logEps.shape[0], _p(out), _stream())
 
 
 def markov_normalize(plan, D, mask, sigma, L, deg):
+    I = plan.info
+    _need(D.numel() >= I.d_elems and mask.numel() >= I.mask_words, "markov_normalize: D/mask")
+    _need(sigma.dtype == torch.float64 and sigma.numel() >= I.n_pd, "markov_normalize: sigma")
+    _need(L.dtype == torch.float64 and L.numel() >= I.d_elems, "markov_normalize: L too small")
+    _need(deg.dtype == torch.float64 and deg.numel() >= 2 * I.n_rows, "markov_normalize: deg")
     call("memi_markov_normalize", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask),
          _p(sigma), _p(L), _p(deg), _stream())
+
+
+def tanh_fit(x, y, p0, out, active=None, max_iter=400):
+    npd, m = y.shape
+    if x.numel() != m or p0.shape != (npd, 4) or out.numel() < 16 * npd:
+        raise RuntimeError("tanh_fit: shape mismatch")
+    call("memi_tanh_fit", _p(x), _p(y), m, npd, _p(p0), _p(active), int(max_iter), _p(out),
+         _stream())
 
 
 def ferguson_values(vals, logEps, out):
@@ -284,6 +365,10 @@ def markov_coo(row, col, val, nS, sigma, alpha, out, deg):
 
 
 def lanczos_step(plan, L, V, m, j, active, W, ab):
+    I = plan.info
+    _need(L.dtype == torch.float64 and L.numel() >= I.d_elems, "lanczos_step: L too small")
+    _need(V.numel() >= m * I.n_rows and W.numel() >= I.n_rows and ab.numel() >= I.n_pd * m * 2
+          and active.numel() >= I.n_pd and active.dtype == torch.int32, "lanczos_step: buffers")
     call("memi_lanczos_step", _p(plan.dev), ctypes.byref(plan.info), _p(L), _p(V), int(m), int(j),
          _p(active), _p(W), _p(ab), _stream())
 
@@ -299,34 +384,62 @@ def nlsa_condist(D, ord_, n, con_order, ConD):
 
 
 def nlsa_supervectors(spec, ctf_half, n, N, con_order, mu_psi, snr, out):
+    Kh = N * (N // 2 + 1)
+    ell = mu_psi.shape[1]
+    _need(spec.dtype == torch.complex64 and spec.numel() == n * Kh, "nlsa_supervectors: spec")
+    _need(ctf_half.dtype == torch.float32 and ctf_half.numel() >= n * Kh, "nlsa_supervectors: ctf")
+    _need(mu_psi.dtype == torch.float64 and mu_psi.shape[0] == n - con_order,
+          "nlsa_supervectors: mu_psi must be float64 (n - ConOrder, ell)")
+    _need(out.dtype == torch.complex64 and out.numel() >= con_order * ell * Kh,
+          "nlsa_supervectors: out too small")
     call("memi_nlsa_supervectors", _p(spec), _p(ctf_half), n, N, con_order, _p(mu_psi),
          mu_psi.shape[1], float(snr), _p(out), _stream())
 
 
 def tall_gram(A, G):
+    _need(A.dim() == 2 and A.dtype == torch.float64 and G.numel() >= A.shape[1] ** 2 and
+          G.dtype == torch.float64, "tall_gram: A (M, c) float64, G (c, c)")
     call("memi_tall_gram", _p(A), A.shape[0], A.shape[1], _p(G), _stream())
 
 
 def tall_apply(A, B_host, Y):
     B = np.ascontiguousarray(B_host, dtype=np.float64)
+    _need(A.dim() == 2 and A.dtype == torch.float64 and B.shape[0] == A.shape[1] and
+          Y.numel() >= A.shape[0] * B.shape[1] and Y.dtype == torch.float64, "tall_apply: shapes")
     call("memi_tall_apply", _p(A), A.shape[0], A.shape[1], c_vp(B.ctypes.data), B.shape[1],
          _p(Y), _stream())
 
 
 def nlsa_imgt(Ubar, H, inv_sd, IMGT):
+    _need(Ubar.dim() == 2 and H.dim() == 2 and H.shape[0] == Ubar.shape[1] and
+          inv_sd.numel() >= H.shape[1] and IMGT.numel() >= Ubar.shape[0] * H.shape[1] and
+          all(t.dtype == torch.float64 for t in (Ubar, H, inv_sd, IMGT)), "nlsa_imgt: shapes")
     call("memi_nlsa_imgt", _p(Ubar), _p(H), _p(inv_sd), Ubar.shape[0], Ubar.shape[1], H.shape[1],
          _p(IMGT), _stream())
 
 
 def center_gram(U2, G):
+    _need(U2.dim() == 2 and U2.dtype == torch.float64 and G.numel() >= U2.shape[1] ** 2 and
+          G.dtype == torch.float64, "center_gram: shapes")
     call("memi_center_gram", _p(U2), U2.shape[0], U2.shape[1], _p(G), _stream())
 
 
 # ------------------------------------------------------------------ differentiation ---------
 def kernel_rowsums(D, eps, S, T):
+    _need(D.dim() == 2 and D.shape[0] == D.shape[1] and D.dtype == torch.float32,
+          "kernel_rowsums: D must be a float32 (n, n) matrix")
+    n = D.shape[0]
+    _need(eps.dtype == torch.float64 and S.dtype == torch.float64 and T.dtype == torch.float64
+          and S.numel() >= eps.numel() * n and T.numel() >= eps.numel() * n,
+          "kernel_rowsums: S/T must be float64 (n_eps, n)")
     call("memi_kernel_rowsums", _p(D), D.shape[0], _p(eps), eps.shape[0], _p(S), _p(T), _stream())
 
 
 def kernel_grad_weights(D, eps, S, g, Wt):
+    _need(D.dim() == 2 and D.shape[0] == D.shape[1] and D.dtype == torch.float32,
+          "kernel_grad_weights: D must be a float32 (n, n) matrix")
+    n = D.shape[0]
+    _need(S.numel() >= eps.numel() * n and g.numel() >= eps.numel() and Wt.numel() >= n * n and
+          Wt.dtype == torch.float32 and g.dtype == torch.float64, "kernel_grad_weights: shapes")
     call("memi_kernel_grad_weights", _p(D), D.shape[0], _p(eps), eps.shape[0], _p(S), _p(g),
          _p(Wt), _stream())

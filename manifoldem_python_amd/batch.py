@@ -98,6 +98,18 @@ def distances(batch, wsp, events=None):
                             events=events)
 
 
-def embed(wsp, dist, tune=3, num_eigs=15, logEps=pl.LOG_EPS, rand=None):
+def embed(wsp, dist, tune=3, num_eigs=15, logEps=pl.LOG_EPS, rand=None, timer=None):
     """a10-a16 for every PD of the batch."""
-    return pl.dm_embed(wsp.plan, dist.D, tune, num_eigs, logEps, rand=rand)
+    return pl.dm_embed(wsp.plan, dist.D, tune, num_eigs, logEps, rand=rand, timer=timer)
+
+
+def breakdown(batch, wsp, num_eigs, rand):
+    """Wall time per stage of one step (synchronising between stages), in ms."""
+    tm = pl.StageTimer()
+    tm.start()
+    prepare(batch, wsp)
+    tm.mark("prep")
+    dist = pl.ctf_distances(wsp.plan, wsp.operand, wsp.fy, wsp.ctf, batch.N, ws=wsp.ws)
+    tm.mark("gram")
+    embed(wsp, dist, num_eigs=num_eigs, rand=rand, timer=tm)
+    return {k: 1e3 * v for k, v in tm.t.items()}

@@ -356,6 +356,158 @@ __global__ __launch_bounds__(256) void k_markov_write(const int32_t* __restrict_
   }
 }
 
+// ------------------------------------------------------------------ tanh fit (a13) ----
+// Levenberg-Marquardt fit of f(x) = a3 + a2 tanh(a0 x + a1) to y (fergusonE.py:45-57), one
+// block per PD, float64.  Scaled LM in the manner of MINPACK lmdif (diagonal scaling by the
+// running max column norm of J, ftol = xtol = 1.49012e-8), analytic Jacobian.  Returns popt,
+// J^T J at popt (for pcov = inv(J^T J) ss_res / (m - 4)), ss_res and the iteration count.
+__device__ inline void tanh_terms(double x, const double a[4], double& f, double J[4]) {
+  const double u = a[0] * x + a[1];
+  const double t = tanh(u);
+  const double s2 = 1.0 - t * t;
+  f = a[3] + a[2] * t;
+  J[0] = a[2] * s2 * x;
+  J[1] = a[2] * s2;
+  J[2] = t;
+  J[3] = 1.0;
+}
+
+// block reduction of NV doubles (in place in v), 256 threads
+template <int NV>
+__device__ inline void block_reduce_vec(double* v, double (*sh)[NV]) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sh[wid][q] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = sh[0][q] + sh[1][q] + sh[2][q] + sh[3][q];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_tanh_fit(const double* __restrict__ x,
+                                                  const double* __restrict__ y, int m,
+                                                  const double* __restrict__ p0,
+                                                  const int32_t* __restrict__ active,
+                                                  int max_iter, double* __restrict__ out) {
+  __shared__ double red[4][15];
+  __shared__ double ctl[8];   // trial parameters, flags
+  const int pd = blockIdx.x;
+  if (active && !active[pd]) return;
+  const double* yp = y + (int64_t)pd * m;
+  double a[4];
+  for (int q = 0; q < 4; ++q) a[q] = p0[pd * 4 + q];
+  double diag[4] = {0, 0, 0, 0};
+  double lam = 1e-3;
+  int it = 0;
+  double cost = 0.0, JtJ[10], Jtr[4];
+  // evaluate J^T J, J^T r and cost at a
+  auto eval = [&](const double* aa, bool full, double* jtj, double* jtr, double& c) {
+    double v[15];
+    for (int q = 0; q < 15; ++q) v[q] = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      double f, J[4];
+      tanh_terms(x[i], aa, f, J);
+      const double r = yp[i] - f;
+      v[14] += r * r;
+      if (full) {
+        int k = 0;
+        for (int p = 0; p < 4; ++p)
+          for (int q = p; q < 4; ++q) v[k++] += J[p] * J[q];
+        for (int p = 0; p < 4; ++p) v[10 + p] += J[p] * r;
+      }
+    }
+    block_reduce_vec<15>(v, red);
+    c = v[14];
+    if (full) {
+      for (int k = 0; k < 10; ++k) jtj[k] = v[k];
+      for (int p = 0; p < 4; ++p) jtr[p] = v[10 + p];
+    }
+  };
+  eval(a, true, JtJ, Jtr, cost);
+  const int dg[4] = {0, 4, 7, 9};  // packed upper-triangle index of (p, p)
+  bool done = false;
+  while (!done && it < max_iter) {
+    ++it;
+    for (int p = 0; p < 4; ++p) diag[p] = fmax(diag[p], sqrt(JtJ[dg[p]]));
+    // solve (JtJ + lam diag^2) d = Jtr by Cholesky (all threads redundantly, 4x4)
+    double A[4][4];
+    {
+      int k = 0;
+      for (int p = 0; p < 4; ++p)
+        for (int q = p; q < 4; ++q) {
+          A[p][q] = JtJ[k];
+          A[q][p] = JtJ[k];
+          ++k;
+        }
+    }
+    for (int p = 0; p < 4; ++p) A[p][p] += lam * diag[p] * diag[p];
+    double Lc[4][4] = {};
+    bool ok = true;
+    for (int p = 0; p < 4 && ok; ++p) {
+      for (int q = 0; q <= p; ++q) {
+        double s = A[p][q];
+        for (int r = 0; r < q; ++r) s -= Lc[p][r] * Lc[q][r];
+        if (p == q) {
+          if (s <= 0.0) { ok = false; break; }
+          Lc[p][p] = sqrt(s);
+        } else {
+          Lc[p][q] = s / Lc[q][q];
+        }
+      }
+    }
+    if (!ok) { lam *= 10.0; if (lam > 1e32) break; continue; }
+    double z[4], d[4];
+    for (int p = 0; p < 4; ++p) {
+      double s = Jtr[p];
+      for (int r = 0; r < p; ++r) s -= Lc[p][r] * z[r];
+      z[p] = s / Lc[p][p];
+    }
+    for (int p = 3; p >= 0; --p) {
+      double s = z[p];
+      for (int r = p + 1; r < 4; ++r) s -= Lc[r][p] * d[r];
+      d[p] = s / Lc[p][p];
+    }
+    double at[4];
+    for (int p = 0; p < 4; ++p) at[p] = a[p] + d[p];
+    double cost_t;
+    double dummy[10], dummy4[4];
+    eval(at, false, dummy, dummy4, cost_t);
+    // predicted reduction of the linear model
+    double pred = 0.0;
+    for (int p = 0; p < 4; ++p) pred += d[p] * (Jtr[p] + lam * diag[p] * diag[p] * d[p]);
+    if (cost_t < cost) {
+      const double actred = (cost - cost_t) / fmax(cost, 1e-300);
+      double xn = 0.0, dn = 0.0;
+      for (int p = 0; p < 4; ++p) {
+        xn += (diag[p] * at[p]) * (diag[p] * at[p]);
+        dn += (diag[p] * d[p]) * (diag[p] * d[p]);
+      }
+      for (int p = 0; p < 4; ++p) a[p] = at[p];
+      cost = cost_t;
+      eval(a, true, JtJ, Jtr, cost);
+      lam = fmax(lam * 0.1, 1e-15);
+      const double prered = pred / fmax(cost_t + pred, 1e-300);
+      if ((actred <= 1.49012e-8 && prered <= 1.49012e-8) || sqrt(dn) <= 1.49012e-8 * sqrt(xn))
+        done = true;
+    } else {
+      lam *= 10.0;
+      if (lam > 1e32) done = true;
+    }
+  }
+  if (threadIdx.x == 0) {
+    double* o = out + (int64_t)pd * 16;
+    for (int p = 0; p < 4; ++p) o[p] = a[p];
+    for (int k = 0; k < 10; ++k) o[4 + k] = JtJ[k];
+    o[14] = cost;
+    o[15] = it;
+  }
+}
+
 // ------------------------------------------------- list forms (fergusonE / slaplacian) ----
 // fergusonE.op(D, logEps, a0) on a flat list: D2 = D*D; thr pass then the eps sweep.
 __global__ __launch_bounds__(256) void k_ferguson_values_thr(const double* __restrict__ v,
@@ -571,6 +723,17 @@ int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, i
   hipLaunchKernelGGL(k_coo_norm<2>, g, dim3(256), 0, s, row, col, val, nnz, inv_s2, alpha, deg,
                      nS, out);
   MEMI_CHECK_LAUNCH("k_coo_norm<2>");
+  return MEMI_OK;
+}
+
+int memi_tanh_fit(const double* x, const double* y, int m, int npd, const double* p0,
+                  const int32_t* active, int max_iter, double* out, void* stream) {
+  MEMI_CHECK_ARG(x && y && p0 && out && m >= 5 && npd >= 0 && max_iter > 0,
+                 "memi_tanh_fit: bad arguments");
+  if (npd == 0) return MEMI_OK;
+  hipLaunchKernelGGL(k_tanh_fit, dim3((unsigned)npd), dim3(256), 0, (hipStream_t)stream, x, y, m,
+                     p0, active, max_iter, out);
+  MEMI_CHECK_LAUNCH("k_tanh_fit");
   return MEMI_OK;
 }
 

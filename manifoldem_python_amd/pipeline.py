@@ -248,6 +248,51 @@ def ferguson_fit(logEps, logSumWij, a0, rand=None):
     return popt, resnorm, R_squared
 
 
+def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200):
+    """The fit of ``fergusonE.py:45-57`` for many PDs at once on the GPU (memi_tanh_fit):
+    fit from a0, pcov = inv(J^T J) ss_res / (m - 4) as scipy's curve_fit forms it, and refit
+    from rand(4,1) - 0.5 while sum(sqrt|diag pcov|) > 100.  Returns (popt (npd, 4), R^2)."""
+    rand = np.random.rand if rand is None else rand
+    dev = device()
+    npd, m = logS.shape
+    x = torch.as_tensor(np.ascontiguousarray(logEps, dtype=np.float64), device=dev)
+    y = torch.as_tensor(np.ascontiguousarray(logS, dtype=np.float64), device=dev)
+    p0 = np.stack([np.ravel(a) for a in a0s]).astype(np.float64)
+    popt = np.zeros((npd, 4))
+    R2 = np.zeros(npd)
+    todo = np.ones(npd, dtype=bool)
+    ss_tot = np.sum((logS - logS.mean(axis=1, keepdims=True)) ** 2, axis=1)
+    out = torch.empty(npd * 16, dtype=torch.float64, device=dev)
+    best = np.full(npd, np.inf)
+    for _ in range(max_restarts):
+        act = torch.as_tensor(todo.astype(np.int32), device=dev)
+        nat.tanh_fit(x, y, torch.as_tensor(p0, device=dev), out, act)
+        o = out.cpu().numpy().reshape(npd, 16)
+        for p in np.nonzero(todo)[0]:
+            a = o[p, 0:4]
+            J = np.zeros((4, 4))
+            J[np.triu_indices(4)] = o[p, 4:14]
+            J = J + np.triu(J, 1).T
+            ss_res = o[p, 14]
+            try:
+                pcov = np.linalg.inv(J) * (ss_res / (m - 4))
+                resnorm = np.sum(np.sqrt(np.fabs(np.diag(pcov))))
+            except np.linalg.LinAlgError:
+                resnorm = np.inf
+            if not np.isfinite(resnorm):
+                resnorm = np.inf
+            p0[p] = np.ravel(1 * (rand(4, 1) - 0.5))
+            if resnorm < best[p]:
+                best[p] = resnorm
+                popt[p] = a
+                R2[p] = 1 - ss_res / ss_tot[p]
+            if resnorm <= 100:
+                todo[p] = False
+        if not todo.any():
+            break
+    return popt, R2
+
+
 class DMResult:
     __slots__ = ("lamb", "psi", "sigma", "mu", "logEps", "logSumWij", "popt", "R_squared",
                  "slope", "V")
@@ -353,38 +398,76 @@ def ferguson_sums(plan, D, mask, logEps):
     return S, slope, fer[:, :, 2]
 
 
+class StageTimer:
+    """Opt-in wall-clock timer of pipeline stages (synchronises the device at each mark)."""
+
+    def __init__(self):
+        import time
+        self._now = time.perf_counter
+        self.t = {}
+        self._last = None
+
+    def start(self):
+        torch.cuda.synchronize()
+        self._last = self._now()
+
+    def mark(self, name):
+        torch.cuda.synchronize()
+        now = self._now()
+        self.t[name] = self.t.get(name, 0.0) + (now - self._last)
+        self._last = now
+
+
 def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, mask=None,
-             eig="lanczos"):
+             eig="lanczos", timer=None, fit="gpu"):
     """a10-a16 for every PD of the plan.  D: flat float32 device buffer.  Returns a list of
     DMResult (numpy outputs).  eig: "lanczos" (batched GPU Lanczos) or "dense" (rocSOLVER
     syevd on each PD, for checking)."""
     dev = D.device
     info = plan.info
     npd = plan.npd
+    if timer:
+        timer.start()
     if mask is None:
         mask = knn_mask(plan, D)
+    if timer:
+        timer.mark("knn")
     S, slope, _ = ferguson_sums(plan, D, mask, logEps)
+    if timer:
+        timer.mark("ferguson")
     rand = np.random.rand if rand is None else rand
     results = []
     sig = np.zeros(npd)
+    a0l = [((rand(4, 1) - 0.5) if a0s is None else a0s[p]) for p in range(npd)]
+    with np.errstate(divide="ignore"):
+        logS = np.log(S)
+    if fit == "gpu":
+        popt_b, R2_b = ferguson_fit_batch(np.asarray(logEps), logS, a0l, rand)
     for p in range(npd):
         r = DMResult()
-        a0 = (rand(4, 1) - 0.5) if a0s is None else a0s[p]
-        with np.errstate(divide="ignore"):
-            r.logSumWij = np.log(S[p])
+        r.logSumWij = logS[p]
         r.slope = slope[p]
         r.logEps = np.asarray(logEps)
-        r.popt, _, r.R_squared = ferguson_fit(r.logEps, r.logSumWij, a0, rand)
+        if fit == "gpu":
+            r.popt, r.R_squared = popt_b[p], R2_b[p]
+        else:
+            r.popt, _, r.R_squared = ferguson_fit(r.logEps, r.logSumWij, a0l[p], rand)
         r.sigma = tune * np.sqrt(2 * np.exp(-r.popt[1] / r.popt[0]))
         sig[p] = r.sigma
         results.append(r)
+    if timer:
+        timer.mark("tanh_fit_host")
     sigma = torch.as_tensor(sig, device=dev)
     L = torch.empty(max(info.d_elems, 1), dtype=torch.float64, device=dev)
     deg = torch.empty(2 * info.n_rows, dtype=torch.float64, device=dev)
     nat.markov_normalize(plan, D, mask, sigma, L, deg)
+    if timer:
+        timer.mark("normalize")
     ks = [min(num_eigs, int(n) - 3) + 1 for n in plan.ns]
     if eig == "lanczos":
         eigs = lanczos_eigs(plan, L, ks)
+        if timer:
+            timer.mark("lanczos")
     for p in range(npd):
         n = int(plan.ns[p])
         nEigs = ks[p] - 1
@@ -401,4 +484,6 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
         r.psi = psi
         r.mu = V[:, 0] ** 2
         r.V = V
+    if timer:
+        timer.mark("eig_post")
     return results

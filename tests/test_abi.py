@@ -75,3 +75,28 @@ def test_no_cpu_fallback_in_product_modules():
             if f.endswith(".py"):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in src and "from oracle" not in src, f
+
+
+def test_wrappers_check_sizes_before_launch():
+    """Shape mistakes raise on the host (regression: a flat n*n distance buffer passed where an
+    (n, n) matrix is expected must not reach the row-sum kernel)."""
+    import torch
+    from manifoldem_python_amd import _native as nat
+    D = torch.zeros(48 * 48)
+    eps = torch.ones(1, dtype=torch.float64)
+    S = torch.zeros(1, 48, dtype=torch.float64)
+    with pytest.raises(RuntimeError, match=r"\(n, n\) matrix"):
+        nat.kernel_rowsums(D, eps, S, S.clone())
+    with pytest.raises(RuntimeError, match=r"\(n, n\) matrix"):
+        nat.kernel_grad_weights(D, eps, S, eps, torch.zeros(48, 48))
+    with pytest.raises(RuntimeError, match="spline_rotate"):
+        nat.spline_rotate(torch.zeros(4, 8, 8), torch.zeros(3, dtype=torch.float64), 8,
+                          torch.zeros(4, 8, 8))
+    with pytest.raises(RuntimeError, match="ctf_pack"):
+        nat.ctf_pack(torch.zeros(2, 8, 5, dtype=torch.complex64),
+                     torch.zeros(2, dtype=torch.float64), 8, nat.ctf_params(1, 2, 300, .1, 1e9),
+                     nat.PREC_FP32, 64, 0, torch.zeros(2 * 4 * 63), None, torch.zeros(2 * 40))
+    pl = nat.Plan([0], [100], [50], device="cpu")
+    with pytest.raises(RuntimeError, match="knn_select: D too small"):
+        nat.knn_select(pl, torch.zeros(99 * 100), torch.zeros(pl.info.mask_words,
+                                                            dtype=torch.int32))

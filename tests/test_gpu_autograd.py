@@ -1,0 +1,78 @@
+"""Hand-written backward of the fused distance -> kernel -> log-sum op against a float64
+torch.autograd reference of the same formula and against central finite differences."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _ref_loss(img, C, w, eps):
+    """float64 torch reference: images -> rfft2 -> weighted half-spectrum distances -> L_e."""
+    import torch
+    X = torch.fft.rfft2(img)
+    n = X.shape[0]
+    A = (X.abs() ** 2 * w).reshape(n, -1)
+    B = (C ** 2).reshape(n, -1)
+    Y = (C * X * w.sqrt()).reshape(n, -1)
+    P = A @ B.T
+    D = P + P.T - 2 * (Y @ Y.conj().T.contiguous()).real
+    K = torch.exp(-D[None] / (2 * eps[:, None, None]))
+    return torch.log(K.sum(2)).sum(1), D
+
+
+def test_backward_matches_float64_autograd_and_fd(cuda):
+    import torch
+    from manifoldem_python_amd import _native as nat, autograd, synth
+    n, N = 48, 32
+    s = synth.make_pd(synth.seeds(7)[2], n, N)
+    rng = np.random.default_rng(0)
+    img64 = torch.as_tensor(rng.normal(size=(n, N, N)), dtype=torch.float64)
+    df = torch.as_tensor(s.df, device=cuda)
+    ctf = nat.ctf_params(synth.PIX, synth.CS, synth.EKV, synth.AMPC, np.inf)
+    # median distance sets the eps scale: 2^-20 .. 2^20 x median (41 values)
+    img = img64.to(torch.float32).to(cuda).requires_grad_(True)
+    fy = torch.fft.rfft2(img)
+    with torch.no_grad():
+        L0 = autograd.kernel_logsum(fy, df, torch.ones(1, dtype=torch.float64, device=cuda), ctf)
+    k = synth.ctf_grid_k(N, synth.PIX)[:, :N // 2 + 1]
+    C64 = torch.as_tensor(synth.ctf_values(k[None], s.df[:, None, None]))
+    w = torch.full((N // 2 + 1,), 2.0, dtype=torch.float64)
+    w[0] = 1.0
+    w[-1] = 1.0
+    _, Dref = _ref_loss(img64, C64, w, torch.ones(1, dtype=torch.float64))
+    med = float(torch.median(Dref[~torch.eye(n, dtype=torch.bool)]))
+    eps = torch.as_tensor(med * 2.0 ** np.linspace(-20, 20, 41), device=cuda).requires_grad_(True)
+    g = torch.as_tensor(rng.normal(size=41), device=cuda)
+    L = autograd.kernel_logsum(fy, df, eps, ctf)
+    (L * g).sum().backward()
+    # float64 autograd reference
+    x = img64.clone().requires_grad_(True)
+    e = eps.detach().cpu().clone().requires_grad_(True)
+    Lr, _ = _ref_loss(x, C64, w, e)
+    (Lr * g.cpu()).sum().backward()
+    np.testing.assert_allclose(L.detach().cpu().numpy(), Lr.detach().numpy(), rtol=1e-5,
+                               atol=1e-5)
+    ge = eps.grad.cpu().numpy()
+    ger = e.grad.numpy()
+    assert np.max(np.abs(ge - ger) / (np.abs(ger) + 1e-30 + 1e-6 * np.max(np.abs(ger)))) < 1e-3
+    gx = img.grad.cpu().numpy().astype(np.float64)
+    gxr = x.grad.numpy()
+    rel = np.linalg.norm(gx - gxr) / np.linalg.norm(gxr)
+    assert rel < 1e-3, rel
+    # the float64 reference itself against central differences (step 1e-3 relative)
+    for e_idx in (10, 20, 30):
+        h = 1e-3 * float(e.detach()[e_idx])
+        ep, em = e.detach().clone(), e.detach().clone()
+        ep[e_idx] += h
+        em[e_idx] -= h
+        fd = float(((_ref_loss(img64, C64, w, ep)[0] - _ref_loss(img64, C64, w, em)[0])
+                    * g.cpu()).sum()) / (2 * h)
+        assert abs(fd - ger[e_idx]) <= 1e-5 * max(1.0, abs(ger[e_idx]))
+    for (i, a, b) in ((0, 3, 5), (7, 10, 20)):
+        h = 1e-3 * float(img64.abs().mean())
+        xp, xm = img64.clone(), img64.clone()
+        xp[i, a, b] += h
+        xm[i, a, b] -= h
+        fd = float(((_ref_loss(xp, C64, w, e.detach())[0] - _ref_loss(xm, C64, w, e.detach())[0])
+                    * g.cpu()).sum()) / (2 * h)
+        assert abs(fd - gxr[i, a, b]) <= 1e-4 * max(1.0, np.max(np.abs(gxr)))

@@ -141,3 +141,28 @@ def test_batched_lanczos_matches_dense(cuda):
         Y = Y.cpu().numpy()
         for c in range(kk[p]):
             assert abs(abs(np.dot(Y[:, c], U[:, sel[c]])) - 1) < 1e-6, (p, c)
+
+
+def test_gpu_tanh_fit_matches_curve_fit(cuda):
+    """Batched LM fit of the Ferguson curve against scipy curve_fit (the reference's fitter)
+    on sweeps of several realistic kernels, from the same start points."""
+    import torch
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    from oracle import dm
+    curves, a0s = [], []
+    rng = np.random.default_rng(12)
+    for seed in range(6):
+        D = _distance_matrix(150 + 20 * seed, 40 + seed)
+        yRow, yCol, yVal = dm.assemble(D.copy(), 60)
+        curves.append(dm.ferguson_sums(np.sqrt(yVal), dm.LOG_EPS)[0])
+        a0s.append(rng.random((4, 1)) - 0.5)
+    logS = np.stack(curves)
+    r1 = np.random.default_rng(3)
+    popt, R2 = pl.ferguson_fit_batch(dm.LOG_EPS, logS, a0s, rand=lambda *sh: r1.random(sh))
+    r2 = np.random.default_rng(3)
+    for p in range(len(curves)):
+        ref, _, R2r = dm.ferguson_fit(dm.LOG_EPS, logS[p], a0s[p], r2)
+        s_ref = np.sqrt(2 * np.exp(-ref[1] / ref[0]))
+        s_gpu = np.sqrt(2 * np.exp(-popt[p][1] / popt[p][0]))
+        assert abs(s_gpu - s_ref) / s_ref < 1e-6, (p, s_gpu, s_ref)
+        assert abs(R2[p] - R2r) < 1e-9
