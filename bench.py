@@ -98,12 +98,14 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events):
     """Warm up, then time exactly `steps` full steps; returns timings and the last results."""
     import torch
     from manifoldem_python_amd import batch as B
+    from manifoldem_python_amd import autotune
     wsp = B.Workspace(batch, precision)
     rng = np.random.default_rng(0)
     rand = lambda *sh: rng.random(sh)
     for _ in range(warmup):
         dist = B.distances(batch, wsp)
         B.embed(wsp, dist, num_eigs=num_eigs, rand=rand)
+    tuned = autotune.tune_gram(wsp, batch.N)
     torch.cuda.synchronize()
     pairs = [events.new_pair() for _ in range(steps)]
     t_dist = 0.0
@@ -119,7 +121,7 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events):
     gram_ms = [events.elapsed_ms(p) for p in pairs]
     stages = B.breakdown(batch, wsp, num_eigs, rand)
     return dict(t=t, t_dist=t_dist, gram_ms=float(np.mean(gram_ms)), wsp=wsp, dist=dist,
-                res=res, flags=int(dist.flag_count.item()), stages=stages)
+                res=res, flags=int(dist.flag_count.item()), stages=stages, tuned=tuned)
 
 
 def fp64_error(batch, wsp, dist):
@@ -249,6 +251,7 @@ def main():
         "distance_ms_per_step": 1e3 * bf["t_dist"] / args.steps,
         "gram_kernel_ms": bf["gram_ms"],
         "stage_ms": bf["stages"],
+        "gram_variant_ms": bf["tuned"],
         "kappa_flagged": bf["flags"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,

@@ -155,13 +155,15 @@ int memi_plan(const memi_pd_desc* pds_in_host, int npd, int chunks, void* plan_h
  * (P + 2|R|)/D exceeds `kappa` (flags: 3*info->flag_cap int32; *flag_count: int32 on the
  * device, zeroed here, gets the number flagged).  D: float32 [info->d_elems].  fy/ctf_half:
  * the spectra and CTFs that produced the operand (rows indexed like the operand).
+ * variant: Gram kernel schedule, 0 = default, 1 = prefetch one K step (two workgroups per
+ * CU), 2 = prefetch two K steps (split-BF16: one workgroup per CU).
  * ev_start / ev_stop: optional hipEvent_t recorded on `stream` right before and after the
  * MFMA Gram kernel (NULL: not recorded), for timing that kernel alone. */
 int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const void* op_hi,
                       const void* op_lo, int64_t Khp, int precision, double kappa,
                       const void* fy, const float* ctf_half, int N, float* D, float* ws,
-                      int32_t* flags, int32_t* flag_count, void* ev_start, void* ev_stop,
-                      void* stream);
+                      int32_t* flags, int32_t* flag_count, int variant, void* ev_start,
+                      void* ev_stop, void* stream);
 
 /* ---------------- Stage 2: diffusion map (DMembeddingII / fergusonE / slaplacianonFly) ---- */
 

@@ -60,7 +60,7 @@ _SIGS = {
     "memi_image_moments": [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp],
     "memi_plan": [c_vp, c_int, c_int, c_vp, c_i64, ctypes.POINTER(PlanInfo)],
     "memi_ctf_distance": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_i64, c_int, c_dbl, c_vp,
-                          c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+                          c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp],
     "memi_knn_select": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_ferguson_sweep": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_markov_normalize": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
@@ -259,7 +259,7 @@ class Plan:
 
 
 def ctf_distance(plan, op_hi, op_lo, Khp, precision, kappa, fy, ctf_half, N, D, ws, flags,
-                 flag_count, events=None):
+                 flag_count, events=None, variant=0):
     I = plan.info
     Kh = N * (N // 2 + 1)
     esz = 4 if precision == PREC_FP32 else 2
@@ -275,7 +275,7 @@ def ctf_distance(plan, op_hi, op_lo, Khp, precision, kappa, fy, ctf_half, N, D, 
     ev0, ev1 = (None, None) if events is None else (c_vp(events[0]), c_vp(events[1]))
     call("memi_ctf_distance", _p(plan.dev), ctypes.byref(plan.info), _p(op_hi), _p(op_lo), Khp,
          precision, float(kappa), _p(fy), _p(ctf_half), N, _p(D), _p(ws), _p(flags),
-         _p(flag_count), ev0, ev1, _stream())
+         _p(flag_count), int(variant), ev0, ev1, _stream())
 
 
 class HipEvents:

@@ -48,8 +48,14 @@ struct GramCfg<MEMI_PREC_SPLIT_BF16> {
   static constexpr int NARR = 4;  // A hi, A lo, B hi, B lo
 };
 
-template <int PREC>
-__global__ __launch_bounds__(256, 2) void k_gram_tiles(
+// waves per EU the variant can sustain: depth-2 split-BF16 needs up to 512 VGPRs
+template <int PREC, int DEPTH>
+struct GramOcc {
+  static constexpr int value = (DEPTH == 2 && PREC == MEMI_PREC_SPLIT_BF16) ? 1 : 2;
+};
+
+template <int PREC, int DEPTH>
+__global__ __launch_bounds__(256, (GramOcc<PREC, DEPTH>::value)) void k_gram_tiles(
     const char* __restrict__ op_hi, const char* __restrict__ op_lo, int64_t Khp, int kc,
     const int32_t* __restrict__ items, int64_t n_items, const memi_pd_desc* __restrict__ pds,
     float* __restrict__ ws) {
@@ -90,34 +96,37 @@ __global__ __launch_bounds__(256, 2) void k_gram_tiles(
   const int lds_st = ld_row0 * LDS_ROW + ld_part * 16;
   constexpr int HALF = 64 * LDS_ROW;
 
-  uint4 sA0h, sA1h, sB0h, sB1h, sA0l, sA1l, sB0l, sB1l;
-#define MEMI_GLOAD(ks)                                                        \
+  // Two register staging sets (X, Y) give a prefetch distance of two K steps: the loads for
+  // step k+2 are issued before step k's MFMAs and land in LDS only after step k+1's.
+  uint4 X0h, X1h, X2h, X3h, X0l, X1l, X2l, X3l;
+  uint4 Y0h, Y1h, Y2h, Y3h, Y0l, Y1l, Y2l, Y3l;
+#define MEMI_GLOAD(R, ks)                                                     \
   do {                                                                        \
     const int64_t ko_ = (int64_t)(ks) * ROW_STEP;                             \
-    sA0h = *(const uint4*)(op_hi + oA0 + ko_);                                \
-    sA1h = *(const uint4*)(op_hi + oA1 + ko_);                                \
-    sB0h = *(const uint4*)(op_hi + oB0 + ko_);                                \
-    sB1h = *(const uint4*)(op_hi + oB1 + ko_);                                \
+    R##0h = *(const uint4*)(op_hi + oA0 + ko_);                               \
+    R##1h = *(const uint4*)(op_hi + oA1 + ko_);                               \
+    R##2h = *(const uint4*)(op_hi + oB0 + ko_);                               \
+    R##3h = *(const uint4*)(op_hi + oB1 + ko_);                               \
     if (NA == 4) {                                                            \
-      sA0l = *(const uint4*)(op_lo + oA0 + ko_);                              \
-      sA1l = *(const uint4*)(op_lo + oA1 + ko_);                              \
-      sB0l = *(const uint4*)(op_lo + oB0 + ko_);                              \
-      sB1l = *(const uint4*)(op_lo + oB1 + ko_);                              \
+      R##0l = *(const uint4*)(op_lo + oA0 + ko_);                             \
+      R##1l = *(const uint4*)(op_lo + oA1 + ko_);                             \
+      R##2l = *(const uint4*)(op_lo + oB0 + ko_);                             \
+      R##3l = *(const uint4*)(op_lo + oB1 + ko_);                             \
     }                                                                         \
   } while (0)
   // LDS tile order: A hi, B hi (FP32: A, B) then A lo, B lo
-#define MEMI_LSTORE(buf)                                                      \
+#define MEMI_LSTORE(R, buf)                                                   \
   do {                                                                        \
     char* b_ = lds + (buf) * NA * LDS_TILE + lds_st;                          \
-    *(uint4*)(b_ + 0 * LDS_TILE) = sA0h;                                      \
-    *(uint4*)(b_ + 0 * LDS_TILE + HALF) = sA1h;                               \
-    *(uint4*)(b_ + 1 * LDS_TILE) = sB0h;                                      \
-    *(uint4*)(b_ + 1 * LDS_TILE + HALF) = sB1h;                               \
+    *(uint4*)(b_ + 0 * LDS_TILE) = R##0h;                                     \
+    *(uint4*)(b_ + 0 * LDS_TILE + HALF) = R##1h;                              \
+    *(uint4*)(b_ + 1 * LDS_TILE) = R##2h;                                     \
+    *(uint4*)(b_ + 1 * LDS_TILE + HALF) = R##3h;                              \
     if (NA == 4) {                                                            \
-      *(uint4*)(b_ + 2 * LDS_TILE) = sA0l;                                    \
-      *(uint4*)(b_ + 2 * LDS_TILE + HALF) = sA1l;                             \
-      *(uint4*)(b_ + 3 * LDS_TILE) = sB0l;                                    \
-      *(uint4*)(b_ + 3 * LDS_TILE + HALF) = sB1l;                             \
+      *(uint4*)(b_ + 2 * LDS_TILE) = R##0l;                                   \
+      *(uint4*)(b_ + 2 * LDS_TILE + HALF) = R##1l;                            \
+      *(uint4*)(b_ + 3 * LDS_TILE) = R##2l;                                   \
+      *(uint4*)(b_ + 3 * LDS_TILE + HALF) = R##3l;                            \
     }                                                                         \
   } while (0)
 
@@ -129,59 +138,76 @@ __global__ __launch_bounds__(256, 2) void k_gram_tiles(
 
   const int frag_off = (lane & 15) * LDS_ROW + (lane >> 4) * 16;
 
-  MEMI_GLOAD(0);
-  MEMI_LSTORE(0);
-  __syncthreads();
-  for (int ks = 0; ks < nks; ++ks) {
-    const int cur = ks & 1;
-    if (ks + 1 < nks) MEMI_GLOAD(ks + 1);
-    const char* base = lds + cur * NA * LDS_TILE;
-    if constexpr (PREC == MEMI_PREC_SPLIT_BF16) {
-      bf16x8_t ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int ro = (wr * 64 + i * 16) * LDS_ROW + frag_off;
-        ah[i] = *(const bf16x8_t*)(base + 0 * LDS_TILE + ro);
-        al[i] = *(const bf16x8_t*)(base + 2 * LDS_TILE + ro);
-        const int co = (wc * 64 + i * 16) * LDS_ROW + frag_off;
-        bh[i] = *(const bf16x8_t*)(base + 1 * LDS_TILE + co);
-        bl[i] = *(const bf16x8_t*)(base + 3 * LDS_TILE + co);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-    } else {
-      floatx4_t a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        a[i] = *(const floatx4_t*)(base + 0 * LDS_TILE + (wr * 64 + i * 16) * LDS_ROW + frag_off);
-        b[i] = *(const floatx4_t*)(base + 1 * LDS_TILE + (wc * 64 + i * 16) * LDS_ROW + frag_off);
-      }
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][e], b[j][e], acc[i][j], 0, 0, 0);
-    }
-    if (ks + 1 < nks) MEMI_LSTORE(cur ^ 1);
+#define MEMI_COMPUTE(cur)                                                                   \
+  do {                                                                                      \
+    const char* base = lds + (cur) * NA * LDS_TILE;                                         \
+    if constexpr (PREC == MEMI_PREC_SPLIT_BF16) {                                           \
+      bf16x8_t fa[4], fb[4], fc[4];                                                         \
+      _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                       \
+        fa[i] = *(const bf16x8_t*)(base + 0 * LDS_TILE + (wr * 64 + i * 16) * LDS_ROW + frag_off); \
+        fb[i] = *(const bf16x8_t*)(base + 1 * LDS_TILE + (wc * 64 + i * 16) * LDS_ROW + frag_off); \
+      }                                                                                     \
+      _Pragma("unroll") for (int i = 0; i < 4; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j) \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0); \
+      _Pragma("unroll") for (int i = 0; i < 4; ++i)                                         \
+        fc[i] = *(const bf16x8_t*)(base + 3 * LDS_TILE + (wc * 64 + i * 16) * LDS_ROW + frag_off); \
+      _Pragma("unroll") for (int i = 0; i < 4; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j) \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fc[j], acc[i][j], 0, 0, 0); \
+      _Pragma("unroll") for (int i = 0; i < 4; ++i)                                         \
+        fa[i] = *(const bf16x8_t*)(base + 2 * LDS_TILE + (wr * 64 + i * 16) * LDS_ROW + frag_off); \
+      _Pragma("unroll") for (int i = 0; i < 4; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j) \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0); \
+    } else {                                                                                \
+      floatx4_t a[4], b[4];                                                                 \
+      _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                       \
+        a[i] = *(const floatx4_t*)(base + 0 * LDS_TILE + (wr * 64 + i * 16) * LDS_ROW + frag_off); \
+        b[i] = *(const floatx4_t*)(base + 1 * LDS_TILE + (wc * 64 + i * 16) * LDS_ROW + frag_off); \
+      }                                                                                     \
+      _Pragma("unroll") for (int e = 0; e < 4; ++e) _Pragma("unroll") for (int i = 0; i < 4; ++i) \
+        _Pragma("unroll") for (int j = 0; j < 4; ++j)                                       \
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][e], b[j][e], acc[i][j], 0, 0, 0); \
+    }                                                                                       \
+  } while (0)
+
+  if constexpr (DEPTH == 1) {
+    // prefetch distance one K step: loads for ks+1 are issued before step ks's MFMAs
+    MEMI_GLOAD(X, 0);
+    MEMI_LSTORE(X, 0);
     __syncthreads();
+    for (int ks = 0; ks < nks; ++ks) {
+      const int cur = ks & 1;
+      if (ks + 1 < nks) MEMI_GLOAD(X, ks + 1);
+      MEMI_COMPUTE(cur);
+      if (ks + 1 < nks) MEMI_LSTORE(X, cur ^ 1);
+      __syncthreads();
+    }
+  } else {
+    // prologue: step 0 into LDS buffer 0, step 1 in flight in Y
+    MEMI_GLOAD(X, 0);
+    if (nks > 1) MEMI_GLOAD(Y, 1);
+    MEMI_LSTORE(X, 0);
+    __syncthreads();
+    // steady state, unrolled by two so the register sets alternate by name
+    int ks = 0;
+    while (ks < nks) {
+      // even step: compute buffer 0; X is free -> fetch ks+2; Y (step ks+1) -> buffer 1
+      if (ks + 2 < nks) MEMI_GLOAD(X, ks + 2);
+      MEMI_COMPUTE(0);
+      if (ks + 1 < nks) MEMI_LSTORE(Y, 1);
+      __syncthreads();
+      ++ks;
+      if (ks >= nks) break;
+      // odd step: compute buffer 1; Y is free -> fetch ks+2; X (step ks+1) -> buffer 0
+      if (ks + 2 < nks) MEMI_GLOAD(Y, ks + 2);
+      MEMI_COMPUTE(1);
+      if (ks + 1 < nks) MEMI_LSTORE(X, 0);
+      __syncthreads();
+      ++ks;
+    }
   }
 #undef MEMI_GLOAD
 #undef MEMI_LSTORE
+#undef MEMI_COMPUTE
 
   // 16x16 output block: lane holds rows 4*(lane/16) + r, column lane % 16
   float* out = ws + (int64_t)slot * BM * BN;
@@ -396,8 +422,8 @@ int memi_plan(const memi_pd_desc* pds_in, int npd, int chunks, void* plan_host,
 int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const void* op_hi,
                       const void* op_lo, int64_t Khp, int precision, double kappa,
                       const void* fy, const float* ctf_half, int N, float* D, float* ws,
-                      int32_t* flags, int32_t* flag_count, void* ev_start, void* ev_stop,
-                      void* stream) {
+                      int32_t* flags, int32_t* flag_count, int variant, void* ev_start,
+                      void* ev_stop, void* stream) {
   MEMI_CHECK_ARG(plan_dev && info && op_hi && fy && ctf_half && D && ws && flags && flag_count,
                  "memi_ctf_distance: null pointer");
   MEMI_CHECK_ARG(precision == MEMI_PREC_FP32 || (precision == MEMI_PREC_SPLIT_BF16 && op_lo),
@@ -425,15 +451,19 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
     set_error("memi_ctf_distance: hipEventRecord failed");
     return MEMI_ELAUNCH;
   }
+  MEMI_CHECK_ARG(variant >= 0 && variant <= 2, "memi_ctf_distance: unknown variant %d", variant);
+  const int v = variant == 0 ? 1 : variant;
+  const char* lo = precision == MEMI_PREC_FP32 ? nullptr : (const char*)op_lo;
+#define MEMI_LAUNCH_GRAM(P, V)                                                                \
+  hipLaunchKernelGGL((k_gram_tiles<P, V>), dim3((unsigned)grid), dim3(256), 0, s,              \
+                     (const char*)op_hi, lo, Khp, kc, items, info->n_items, pds, ws)
   if (precision == MEMI_PREC_FP32) {
-    hipLaunchKernelGGL(k_gram_tiles<MEMI_PREC_FP32>, dim3((unsigned)grid), dim3(256), 0, s,
-                       (const char*)op_hi, (const char*)nullptr, Khp, kc, items, info->n_items,
-                       pds, ws);
+    if (v == 1) MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, 1); else MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, 2);
   } else {
-    hipLaunchKernelGGL(k_gram_tiles<MEMI_PREC_SPLIT_BF16>, dim3((unsigned)grid), dim3(256), 0,
-                       s, (const char*)op_hi, (const char*)op_lo, Khp, kc, items,
-                       info->n_items, pds, ws);
+    if (v == 1) MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, 1);
+    else MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, 2);
   }
+#undef MEMI_LAUNCH_GRAM
   MEMI_CHECK_LAUNCH("k_gram_tiles");
   if (ev_stop && hipEventRecord((hipEvent_t)ev_stop, s) != hipSuccess) {
     set_error("memi_ctf_distance: hipEventRecord failed");

@@ -208,7 +208,11 @@ class Distances:
         return self.plan.block(self.D, p)
 
 
-def ctf_distances(plan, operand, fy, ctf_half, N, kappa=None, ws=None, events=None):
+GRAM_VARIANT = {nat.PREC_FP32: 0, nat.PREC_SPLIT_BF16: 0}   # 0 = library default
+
+
+def ctf_distances(plan, operand, fy, ctf_half, N, kappa=None, ws=None, events=None,
+                  variant=None):
     """a8 for every PD of the plan: returns Distances (float32 D on the device).  `events`: an
     optional (start, stop) hipEvent_t pair recorded around the MFMA Gram kernel."""
     dev = operand.hi.device
@@ -221,7 +225,7 @@ def ctf_distances(plan, operand, fy, ctf_half, N, kappa=None, ws=None, events=No
     cnt = torch.zeros(1, dtype=torch.int32, device=dev)
     nat.ctf_distance(plan, operand.hi, operand.lo if operand.lo is not None else operand.hi,
                      operand.Khp, operand.precision, kappa, fy, ctf_half, N, D, ws, flags, cnt,
-                     events)
+                     events, GRAM_VARIANT[operand.precision] if variant is None else variant)
     return Distances(plan, D, cnt)
 
 

@@ -153,3 +153,16 @@ def test_gram_alone_vs_float64(cuda, precision):
     err = rel_offdiag(D, ref)[mask[~np.eye(n, dtype=bool)]]
     assert err.max() < 1e-5, (err.max(), np.median(err))
     assert int(dist.flag_count.item()) >= 1
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gram_schedule_variants_agree(cuda, precision):
+    """Both Gram schedules issue the same MFMAs in the same order: identical D."""
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    n, N = 333, 48
+    s = _setup(n, N, seed=3)
+    geo, out, op = _gpu_prep(s, cuda, precision)
+    plan = nat.Plan([0], [n], [n], chunks=pl.CHUNKS[precision])
+    D1 = pl.ctf_distances(plan, op, out["fy"], out["ctf"], N, variant=1).block(0).cpu().numpy()
+    D2 = pl.ctf_distances(plan, op, out["fy"], out["ctf"], N, variant=2).block(0).cpu().numpy()
+    np.testing.assert_array_equal(D1, D2)
