@@ -306,6 +306,36 @@ class DMResult:
                 self.R_squared)
 
 
+_POOL = None
+
+
+def _pool():
+    """Host thread pool for per-PD small dense work (LAPACK calls release the GIL)."""
+    global _POOL
+    if _POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        import os
+        _POOL = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+    return _POOL
+
+
+def lm_ritz(alpha, beta, k):
+    """The k largest-magnitude eigenpairs of the symmetric tridiagonal T(alpha, beta): both
+    ends of the spectrum (k each) by index range, then the k of largest |theta|.  Returns
+    (theta, S) with S the eigenvectors of T (columns)."""
+    from scipy.linalg import eigh_tridiagonal
+    J = alpha.shape[0]
+    if J <= 2 * k + 8:
+        theta, S = eigh_tridiagonal(alpha, beta)
+    else:
+        t1, S1 = eigh_tridiagonal(alpha, beta, select="i", select_range=(0, k - 1))
+        t2, S2 = eigh_tridiagonal(alpha, beta, select="i", select_range=(J - k, J - 1))
+        theta = np.concatenate([t1, t2])
+        S = np.concatenate([S1, S2], axis=1)
+    sel = np.argsort(-np.abs(theta), kind="stable")[:k]
+    return theta[sel], S[:, sel]
+
+
 def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202):
     """Largest-magnitude eigenpairs of every PD's float64 kernel block of L (sembeddingonFly.py
     semantics: the k eigenpairs of largest |lambda|, sorted by descending lambda) by batched
@@ -314,7 +344,6 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202):
     Converged when every selected Ritz pair has residual |beta_J s_J| <= tol * max|theta| or
     the Krylov space is the whole space.  Returns a list of (lam, V (n x k) device tensor,
     steps, max residual)."""
-    from scipy.linalg import eigh_tridiagonal
     dev = L.device
     npd = plan.npd
     ns = plan.ns.astype(int)
@@ -346,19 +375,27 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202):
             nat.lanczos_step(plan, L, V, m, j, active, W, ab)
             j += 1
         abh = ab.view(npd, m, 2).cpu().numpy()
-        for p in np.nonzero(active_h)[0]:
+
+        def check(p):
             J = int(min(j, steps[p]))
             alpha, beta = abh[p, :J, 0], abh[p, :J, 1]
-            theta, S = eigh_tridiagonal(alpha, beta[:J - 1])
-            k = ks[p]
-            sel = np.argsort(-np.abs(theta), kind="stable")[:k]
-            res = np.abs(beta[J - 1] * S[J - 1, sel])
-            scale = np.max(np.abs(theta[sel]))
-            done = (J >= ns[p]) or (np.all(res <= tol * scale)) or (J >= m)
+            theta, S = lm_ritz(alpha, beta[:J - 1], ks[p])
+            res = np.abs(beta[J - 1] * S[J - 1])
+            done = (J >= ns[p]) or np.all(res <= tol * np.max(np.abs(theta))) or (J >= m)
             if done:
-                order = np.argsort(-theta[sel], kind="stable")
-                sel = sel[order]
-                out[p] = (theta[sel], S[:, sel], J, float(np.max(res[order])) if J < ns[p] else 0.0)
+                order = np.argsort(-theta, kind="stable")
+                return p, (theta[order], S[:, order], J,
+                           float(np.max(res[order])) if J < ns[p] else 0.0)
+            return p, None
+
+        todo = np.nonzero(active_h)[0]
+        if len(todo) > 8:
+            results_c = list(_pool().map(check, todo))
+        else:
+            results_c = [check(p) for p in todo]
+        for p, r in results_c:
+            if r is not None:
+                out[p] = r
                 active_h[p] = 0
         active.copy_(torch.as_tensor(active_h, device=dev))
     results = []
