@@ -142,11 +142,17 @@ def _bytes(t):
 
 
 # ------------------------------------------------------------------------- Stage 1 ---------
-def load_normalize(stack, src, conj, N, out):
-    if src.numel() and (int(src.min()) < 0 or int(src.max()) >= stack.shape[0]):
-        raise RuntimeError("load_normalize: image index outside the stack")
-    if tuple(stack.shape[1:]) != (N, N) or tuple(out.shape) != (src.shape[0], N, N):
-        raise RuntimeError("load_normalize: stack/out shapes do not match N")
+def load_normalize(stack, src, conj, N, out, src_max=None):
+    """src_max: host-known largest index in src (checked against the stack); when None the
+    indices are read back from the device to check them (one synchronisation)."""
+    if src.numel():
+        hi = int(src.max()) if src_max is None else int(src_max)
+        lo = int(src.min()) if src_max is None else 0
+        _need(lo >= 0 and hi < stack.shape[0], "load_normalize: image index outside the stack")
+    _need(tuple(stack.shape[1:]) == (N, N) and tuple(out.shape) == (src.shape[0], N, N) and
+          stack.dtype == torch.float32 and src.dtype == torch.int64 and
+          (conj is None or (conj.dtype == torch.int32 and conj.numel() == src.numel())),
+          "load_normalize: stack/src/conj/out shapes or dtypes do not match")
     call("memi_load_normalize", _p(stack), stack.shape[0], _p(src), _p(conj), src.shape[0], N,
          _p(out), _stream())
 

@@ -56,6 +56,9 @@ def make_batch(stack, pds, N, ctf, filterPar, device):
         ns.append(n)
         ks.append(min(int(d.get("k", n)), n))
         geo.append(g)
+    all_src = np.concatenate(src)
+    if all_src.size and (all_src.min() < 0 or all_src.max() >= stack.shape[0]):
+        raise ValueError("make_batch: particle index outside the image stack")
     cat = lambda xs, dt: torch.as_tensor(np.concatenate(xs).astype(dt), device=device)
     return Batch(N=N, stack=stack, src=cat(src, np.int64), conj=cat(conj, np.int32),
                  rot1=cat(rot1, np.float64), rot2=cat(rot2, np.float64), df=cat(df, np.float64),
@@ -88,7 +91,7 @@ def prepare(batch, wsp):
         pl.prepare_images(batch.stack, batch.src[sl], batch.conj[sl], batch.rot1[sl],
                           batch.rot2[sl], batch.df[sl], batch.N, batch.filterPar, batch.ctf,
                           operand=wsp.operand, row0=r0, fy_out=wsp.fy[sl], ctf_out=wsp.ctf[sl],
-                          want_images=False)
+                          want_images=False, src_max=batch.stack.shape[0] - 1)
 
 
 def distances(batch, wsp, events=None):

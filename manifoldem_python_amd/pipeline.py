@@ -126,7 +126,7 @@ def filter_kind(filterPar):
 
 
 def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, operand=None,
-                   row0=0, msk2=None, fy_out=None, ctf_out=None, want_images=True):
+                   row0=0, msk2=None, fy_out=None, ctf_out=None, want_images=True, src_max=None):
     """a1-a6 for one set of images (all device tensors): load + normalise, Butterworth,
     disk mask, two cubic-spline rotations, fy = rfft2(img * msk2), CTF and operand rows.
 
@@ -135,7 +135,7 @@ def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, 
     n = int(src.shape[0])
     dev = stack.device
     y = torch.empty((n, N, N), dtype=torch.float32, device=dev)
-    nat.load_normalize(stack, src, conj, N, y)
+    nat.load_normalize(stack, src, conj, N, y, src_max=src_max)
     Y = torch.fft.rfft2(y)
     nat.spectrum_scale(Y, N, filter_kind(filterPar), filterPar["Qc"], filterPar["N"])
     y = torch.fft.irfft2(Y, s=(N, N)).contiguous()
@@ -150,10 +150,11 @@ def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, 
     if msk2 is not None:
         img = y.clone()
         nat.mask_multiply(img, N, disk=False, msk2=msk2)
-    fy = torch.fft.rfft2(img)
     if fy_out is not None:
-        fy_out.copy_(fy.reshape(fy_out.shape))
         fy = fy_out.view(n, N, N // 2 + 1)
+        torch.fft.rfft2(img, out=fy)
+    else:
+        fy = torch.fft.rfft2(img)
     ctf_half = ctf_out if ctf_out is not None else torch.empty((n, half_len(N)),
                                                               dtype=torch.float32, device=dev)
     if operand is not None:
