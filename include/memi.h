@@ -155,6 +155,9 @@ int memi_plan(const memi_pd_desc* pds_in_host, int npd, int chunks, void* plan_h
  * (P + 2|R|)/D exceeds `kappa` (flags: 3*info->flag_cap int32; *flag_count: int32 on the
  * device, zeroed here, gets the number flagged).  D: float32 [info->d_elems].  fy/ctf_half:
  * the spectra and CTFs that produced the operand (rows indexed like the operand).
+ * The K range of each segment (Khp elements, 64-byte steps) is split into info->chunks
+ * chunks of ceil(steps / chunks) steps, the last one shorter; a chunk count that leaves an
+ * empty chunk is rejected.
  * variant: Gram kernel schedule, 0 = default, 1 = prefetch one K step (two workgroups per
  * CU), 2 = prefetch two K steps (split-BF16: one workgroup per CU).
  * ev_start / ev_stop: optional hipEvent_t recorded on `stream` right before and after the

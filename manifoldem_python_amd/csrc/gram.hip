@@ -56,9 +56,9 @@ struct GramOcc {
 
 template <int PREC, int DEPTH>
 __global__ __launch_bounds__(256, (GramOcc<PREC, DEPTH>::value)) void k_gram_tiles(
-    const char* __restrict__ op_hi, const char* __restrict__ op_lo, int64_t Khp, int kc,
-    const int32_t* __restrict__ items, int64_t n_items, const memi_pd_desc* __restrict__ pds,
-    float* __restrict__ ws) {
+    const char* __restrict__ op_hi, const char* __restrict__ op_lo, int64_t Khp, int spc,
+    int steps, const int32_t* __restrict__ items, int64_t n_items,
+    const memi_pd_desc* __restrict__ pds, float* __restrict__ ws) {
   using Cfg = GramCfg<PREC>;
   constexpr int ES = Cfg::ESIZE;
   constexpr int NA = Cfg::NARR;
@@ -75,9 +75,10 @@ __global__ __launch_bounds__(256, (GramOcc<PREC, DEPTH>::value)) void k_gram_til
   const int mi = min(BM, P.n - ti * BM);
   const int mj = min(BN, P.n - tj * BN);
   const int64_t row_bytes = 4 * Khp * ES;
-  const int64_t a_k0 = ((int64_t)seg * Khp + (int64_t)chunk * kc) * ES;
-  const int64_t b_k0 = ((int64_t)seg_pair(seg) * Khp + (int64_t)chunk * kc) * ES;
-  const int nks = kc * ES / ROW_STEP;
+  // chunk c covers K steps [c*spc, min((c+1)*spc, steps)); the host guarantees none is empty
+  const int64_t a_k0 = (int64_t)seg * Khp * ES + (int64_t)chunk * spc * ROW_STEP;
+  const int64_t b_k0 = (int64_t)seg_pair(seg) * Khp * ES + (int64_t)chunk * spc * ROW_STEP;
+  const int nks = min(spc, steps - chunk * spc);
 
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
@@ -432,9 +433,12 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
                  "memi_ctf_distance: bad Khp %lld", (long long)Khp);
   const int chunks = info->chunks;
   const int kstep = ROW_STEP / (precision == MEMI_PREC_FP32 ? 4 : 2);
-  MEMI_CHECK_ARG(Khp % ((int64_t)chunks * kstep) == 0,
-                 "memi_ctf_distance: Khp=%lld not divisible into %d chunks of %d-element steps",
-                 (long long)Khp, chunks, kstep);
+  // K steps per segment, split into chunks of spc steps (the last one may be shorter)
+  const int64_t steps = Khp / kstep;
+  const int64_t spc = (steps + chunks - 1) / chunks;
+  MEMI_CHECK_ARG(steps < (1LL << 30) && (int64_t)(chunks - 1) * spc < steps,
+                 "memi_ctf_distance: %d chunks leave an empty chunk of the %lld K steps",
+                 chunks, (long long)steps);
   MEMI_CHECK_ARG(kappa > 0, "memi_ctf_distance: kappa must be positive");
   if (info->n_pd == 0) return MEMI_OK;
   hipStream_t s = (hipStream_t)stream;
@@ -442,7 +446,6 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
   const memi_pd_desc* pds = (const memi_pd_desc*)(plan + info->off_pds);
   const int32_t* items = (const int32_t*)(plan + info->off_items);
   const int32_t* ut = (const int32_t*)(plan + info->off_utiles);
-  const int kc = (int)(Khp / chunks);
   const int64_t grid = ((info->n_items + 7) / 8) * 8;
   MEMI_CHECK_ARG(grid < (int64_t)1 << 31, "memi_ctf_distance: too many work items");
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, flag_count, (int64_t)1);
@@ -456,7 +459,8 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
   const char* lo = precision == MEMI_PREC_FP32 ? nullptr : (const char*)op_lo;
 #define MEMI_LAUNCH_GRAM(P, V)                                                                \
   hipLaunchKernelGGL((k_gram_tiles<P, V>), dim3((unsigned)grid), dim3(256), 0, s,              \
-                     (const char*)op_hi, lo, Khp, kc, items, info->n_items, pds, ws)
+                     (const char*)op_hi, lo, Khp, (int)spc, (int)steps, items, info->n_items, \
+                     pds, ws)
   if (precision == MEMI_PREC_FP32) {
     if (v == 1) MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, 1); else MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, 2);
   } else {

@@ -100,3 +100,23 @@ def test_wrappers_check_sizes_before_launch():
     with pytest.raises(RuntimeError, match="knn_select: D too small"):
         nat.knn_select(pl, torch.zeros(99 * 100), torch.zeros(pl.info.mask_words,
                                                             dtype=torch.int32))
+
+
+def test_distance_rejects_a_split_with_an_empty_chunk():
+    """Uneven K splits are allowed (last chunk shorter) but one that leaves a chunk empty is
+    refused before anything touches the device: Khp = 64 split-BF16 is 2 K steps."""
+    from manifoldem_python_amd import _native as nat
+    fake = ctypes.c_void_p(256)       # never dereferenced: the argument checks fail first
+    for chunks, ok in ((2, True), (3, False)):
+        plan = nat.Plan([0], [10], [5], chunks=chunks, device="cpu")
+        if ok:
+            continue
+        with pytest.raises(RuntimeError, match="empty chunk"):
+            nat.call("memi_ctf_distance", fake, ctypes.byref(plan.info), fake, fake, 64,
+                     nat.PREC_SPLIT_BF16, 8.0, fake, fake, 8, fake, fake, fake, fake, 0, None,
+                     None, None)
+    from manifoldem_python_amd import autotune
+    assert autotune.valid_chunks(nat.PREC_SPLIT_BF16, 64, 2)
+    assert not autotune.valid_chunks(nat.PREC_SPLIT_BF16, 64, 3)
+    # C2-sized operand (N = 128): 260 split-BF16 steps, 3 chunks of 87/87/86
+    assert autotune.valid_chunks(nat.PREC_SPLIT_BF16, 8320, 3)

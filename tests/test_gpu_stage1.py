@@ -166,3 +166,31 @@ def test_gram_schedule_variants_agree(cuda, precision):
     D1 = pl.ctf_distances(plan, op, out["fy"], out["ctf"], N, variant=1).block(0).cpu().numpy()
     D2 = pl.ctf_distances(plan, op, out["fy"], out["ctf"], N, variant=2).block(0).cpu().numpy()
     np.testing.assert_array_equal(D1, D2)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("chunks", [5, 7])
+def test_gram_uneven_chunks_vs_float64(cuda, precision, chunks):
+    """K splits whose last chunk is shorter (N = 64: 66 split-BF16 / 132 FP32 steps per
+    segment) against the float64 Gram of the same spectra, and against the default split."""
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    from oracle import distance as od
+    n, N = 300, 64
+    s = _setup(n, N, seed=4)
+    geo, out, op = _gpu_prep(s, cuda, precision)
+    plan = nat.Plan([0], [n], [n], chunks=chunks)
+    D = pl.ctf_distances(plan, op, out["fy"], out["ctf"], N).block(0).cpu().numpy()
+    base = nat.Plan([0], [n], [n], chunks=pl.CHUNKS[precision])
+    D0 = pl.ctf_distances(base, op, out["fy"], out["ctf"], N).block(0).cpu().numpy()
+    Nh = N // 2 + 1
+    fy = out["fy"].cpu().numpy().astype(np.complex128)
+    full = np.empty((n, N, N), dtype=np.complex128)
+    full[:, :, :Nh] = fy
+    rows = (-np.arange(N)) % N
+    full[:, :, Nh:] = np.conj(fy[:, rows][:, :, N - np.arange(Nh, N)])
+    C = pl.full_ctf(out["ctf"], N).cpu().numpy().astype(np.float64).reshape(n, N, N)
+    ref = od.gram_distance(full, C)
+    assert rel_offdiag(D, ref).max() < 1e-5
+    assert rel_offdiag(D, D0).max() < 2e-5
+    np.testing.assert_array_equal(D, D.T)
+    assert np.all(np.diag(D) == 0)
