@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256, (GramOcc<PREC, DEPTH>::value)) void k_gram_til
 
   const int frag_off = (lane & 15) * LDS_ROW + (lane >> 4) * 16;
 
-#define MEMI_COMPUTE(cur)                                                                   \
+#define MEMI_COMPUTE(cur, G)                                                                 \
   do {                                                                                      \
     const char* base = lds + (cur) * NA * LDS_TILE;                                         \
     if constexpr (PREC == MEMI_PREC_SPLIT_BF16) {                                           \
@@ -149,15 +149,18 @@ __global__ __launch_bounds__(256, (GramOcc<PREC, DEPTH>::value)) void k_gram_til
         fb[i] = *(const bf16x8_t*)(base + 1 * LDS_TILE + (wc * 64 + i * 16) * LDS_ROW + frag_off); \
       }                                                                                     \
       _Pragma("unroll") for (int i = 0; i < 4; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j) \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0); \
+        if (!(G) || (i < nbi && j < nbj))                                                   \
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0); \
       _Pragma("unroll") for (int i = 0; i < 4; ++i)                                         \
         fc[i] = *(const bf16x8_t*)(base + 3 * LDS_TILE + (wc * 64 + i * 16) * LDS_ROW + frag_off); \
       _Pragma("unroll") for (int i = 0; i < 4; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j) \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fc[j], acc[i][j], 0, 0, 0); \
+        if (!(G) || (i < nbi && j < nbj))                                                   \
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fc[j], acc[i][j], 0, 0, 0); \
       _Pragma("unroll") for (int i = 0; i < 4; ++i)                                         \
         fa[i] = *(const bf16x8_t*)(base + 2 * LDS_TILE + (wr * 64 + i * 16) * LDS_ROW + frag_off); \
       _Pragma("unroll") for (int i = 0; i < 4; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j) \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0); \
+        if (!(G) || (i < nbi && j < nbj))                                                   \
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0); \
     } else {                                                                                \
       floatx4_t a[4], b[4];                                                                 \
       _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                       \
@@ -166,46 +169,60 @@ __global__ __launch_bounds__(256, (GramOcc<PREC, DEPTH>::value)) void k_gram_til
       }                                                                                     \
       _Pragma("unroll") for (int e = 0; e < 4; ++e) _Pragma("unroll") for (int i = 0; i < 4; ++i) \
         _Pragma("unroll") for (int j = 0; j < 4; ++j)                                       \
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][e], b[j][e], acc[i][j], 0, 0, 0); \
+          if (!(G) || (i < nbi && j < nbj))                                                 \
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][e], b[j][e], acc[i][j], 0, 0, 0); \
     }                                                                                       \
   } while (0)
 
-  if constexpr (DEPTH == 1) {
-    // prefetch distance one K step: loads for ks+1 are issued before step ks's MFMAs
-    MEMI_GLOAD(X, 0);
-    MEMI_LSTORE(X, 0);
-    __syncthreads();
-    for (int ks = 0; ks < nks; ++ks) {
-      const int cur = ks & 1;
-      if (ks + 1 < nks) MEMI_GLOAD(X, ks + 1);
-      MEMI_COMPUTE(cur);
-      if (ks + 1 < nks) MEMI_LSTORE(X, cur ^ 1);
-      __syncthreads();
-    }
+  // The K loop, G = true for edge tiles: MFMA blocks outside the tile's rows/columns are
+  // skipped (nbi x nbj of this wave's 4x4 blocks hold data), full tiles run unguarded.
+#define MEMI_KLOOP(G)\
+  do {                                                                                      \
+  if constexpr (DEPTH == 1) {                                                               \
+    /* prefetch distance one K step: loads for ks+1 are issued before step ks's MFMAs */    \
+    MEMI_GLOAD(X, 0);                                                                       \
+    MEMI_LSTORE(X, 0);                                                                      \
+    __syncthreads();                                                                        \
+    for (int ks = 0; ks < nks; ++ks) {                                                      \
+      const int cur = ks & 1;                                                               \
+      if (ks + 1 < nks) MEMI_GLOAD(X, ks + 1);                                              \
+      MEMI_COMPUTE(cur, G);                                                                 \
+      if (ks + 1 < nks) MEMI_LSTORE(X, cur ^ 1);                                            \
+      __syncthreads();                                                                      \
+    }                                                                                       \
+  } else {                                                                                  \
+    /* prologue: step 0 into LDS buffer 0, step 1 in flight in Y */                         \
+    MEMI_GLOAD(X, 0);                                                                       \
+    if (nks > 1) MEMI_GLOAD(Y, 1);                                                          \
+    MEMI_LSTORE(X, 0);                                                                      \
+    __syncthreads();                                                                        \
+    /* steady state, unrolled by two so the register sets alternate by name */              \
+    int ks = 0;                                                                             \
+    while (ks < nks) {                                                                      \
+      /* even step: compute buffer 0; X is free -> fetch ks+2; Y (step ks+1) -> buffer 1 */ \
+      if (ks + 2 < nks) MEMI_GLOAD(X, ks + 2);                                              \
+      MEMI_COMPUTE(0, G);                                                                   \
+      if (ks + 1 < nks) MEMI_LSTORE(Y, 1);                                                  \
+      __syncthreads();                                                                      \
+      ++ks;                                                                                 \
+      if (ks >= nks) break;                                                                 \
+      /* odd step: compute buffer 1; Y is free -> fetch ks+2; X (step ks+1) -> buffer 0 */  \
+      if (ks + 2 < nks) MEMI_GLOAD(Y, ks + 2);                                              \
+      MEMI_COMPUTE(1, G);                                                                   \
+      if (ks + 1 < nks) MEMI_LSTORE(X, 0);                                                  \
+      __syncthreads();                                                                      \
+      ++ks;                                                                                 \
+    }                                                                                       \
+  }                                                                                         \
+  } while (0)
+  const int nbi = max(0, min(4, (mi - wr * 64 + 15) / 16));
+  const int nbj = max(0, min(4, (mj - wc * 64 + 15) / 16));
+  if (mi == BM && mj == BN) {
+    MEMI_KLOOP(false);
   } else {
-    // prologue: step 0 into LDS buffer 0, step 1 in flight in Y
-    MEMI_GLOAD(X, 0);
-    if (nks > 1) MEMI_GLOAD(Y, 1);
-    MEMI_LSTORE(X, 0);
-    __syncthreads();
-    // steady state, unrolled by two so the register sets alternate by name
-    int ks = 0;
-    while (ks < nks) {
-      // even step: compute buffer 0; X is free -> fetch ks+2; Y (step ks+1) -> buffer 1
-      if (ks + 2 < nks) MEMI_GLOAD(X, ks + 2);
-      MEMI_COMPUTE(0);
-      if (ks + 1 < nks) MEMI_LSTORE(Y, 1);
-      __syncthreads();
-      ++ks;
-      if (ks >= nks) break;
-      // odd step: compute buffer 1; Y is free -> fetch ks+2; X (step ks+1) -> buffer 0
-      if (ks + 2 < nks) MEMI_GLOAD(Y, ks + 2);
-      MEMI_COMPUTE(1);
-      if (ks + 1 < nks) MEMI_LSTORE(X, 0);
-      __syncthreads();
-      ++ks;
-    }
+    MEMI_KLOOP(true);
   }
+#undef MEMI_KLOOP
 #undef MEMI_GLOAD
 #undef MEMI_LSTORE
 #undef MEMI_COMPUTE
