@@ -79,6 +79,7 @@ class Workspace:
         self.fy = torch.empty((batch.rows, batch.N, Nh), dtype=torch.complex64, device=dev)
         self.ctf = torch.empty((batch.rows, batch.N * Nh), dtype=torch.float32, device=dev)
         self.ws = torch.empty(max(self.plan.info.ws_floats, 1), dtype=torch.float32, device=dev)
+        self.flags = torch.empty(3 * self.plan.info.flag_cap, dtype=torch.int32, device=dev)
         self.chunk_rows = chunk_rows
 
 
@@ -94,11 +95,39 @@ def prepare(batch, wsp):
                           want_images=False, src_max=batch.stack.shape[0] - 1)
 
 
-def distances(batch, wsp, events=None):
+def distances(batch, wsp, events=None, D=None):
     """a1-a8: prepared operand then the grouped Gram; returns pl.Distances."""
     prepare(batch, wsp)
     return pl.ctf_distances(wsp.plan, wsp.operand, wsp.fy, wsp.ctf, batch.N, ws=wsp.ws,
-                            events=events)
+                            events=events, flags=wsp.flags, D=D)
+
+
+class GraphedDistances:
+    """a1-a8 of a batch captured once into a HIP graph (torch.cuda.CUDAGraph records a
+    hipGraph on ROCm) and replayed per step: one graph launch instead of the ~20 kernel and
+    FFT launches per row chunk plus the three Gram launches.
+
+    The captured work clears its own counters with kernels (no memset nodes).  Every replay
+    overwrites the same device buffers: ``run()`` returns the same Distances object each time,
+    so a caller that needs an earlier step's D must copy it before the next replay."""
+
+    def __init__(self, batch, wsp):
+        self.batch, self.wsp = batch, wsp
+        self.D = torch.empty(max(wsp.plan.info.d_elems, 1), dtype=torch.float32,
+                             device=wsp.operand.hi.device)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):           # eager run first: FFT plans and workspaces
+            distances(batch, wsp, D=self.D)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.dist = distances(batch, wsp, D=self.D)
+
+    def run(self):
+        self.graph.replay()
+        return self.dist
 
 
 def embed(wsp, dist, tune=3, num_eigs=15, logEps=pl.LOG_EPS, rand=None, timer=None):
@@ -112,7 +141,8 @@ def breakdown(batch, wsp, num_eigs, rand):
     tm.start()
     prepare(batch, wsp)
     tm.mark("prep")
-    dist = pl.ctf_distances(wsp.plan, wsp.operand, wsp.fy, wsp.ctf, batch.N, ws=wsp.ws)
+    dist = pl.ctf_distances(wsp.plan, wsp.operand, wsp.fy, wsp.ctf, batch.N, ws=wsp.ws,
+                            flags=wsp.flags)
     tm.mark("gram")
     embed(wsp, dist, num_eigs=num_eigs, rand=rand, timer=tm)
     return {k: 1e3 * v for k, v in tm.t.items()}

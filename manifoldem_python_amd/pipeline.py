@@ -2,13 +2,15 @@
 
 Stage 1 (``getDistanceCTF_local_Conj9combinedS2``): image preparation as batched rocFFT
 transforms (``torch.fft``) with the elementwise work between them in HIP kernels, then the MFMA
-distance Gram.  Stage 2 (``DMembeddingII``): kNN selection, Ferguson sweep and Markov
-normalisation in HIP kernels, the tanh fit on the host (scipy, as the reference), and a float64
-symmetric eigensolve on the GPU.
+distance Gram.  Stage 2 (``DMembeddingII``): kNN selection, Ferguson sweep, the batched
+Levenberg-Marquardt tanh fit and Markov normalisation in HIP kernels, then the largest-magnitude
+eigenpairs of every PD's float64 kernel by batched Lanczos on the GPU (a dense float64 eigh is
+kept as an option).
 
 Host-side work kept from the reference: per-image in-plane angles and the per-PD reference
-angle (``get_psi``, ``calc_avg_pd``, ``q2Spider``), the 4-parameter tanh fit of the Ferguson curve
-and the final psi/mu arithmetic on the (n x nEigs+1) eigenvectors.
+angle (``get_psi``, ``calc_avg_pd``, ``q2Spider``), the fit's restart rule (scipy's curve_fit
+remains available as ``ferguson_fit``), the small tridiagonal Ritz problems, and the final
+psi/mu arithmetic on the (n x nEigs+1) eigenvectors.
 """
 from __future__ import annotations
 
@@ -213,17 +215,20 @@ GRAM_VARIANT = {nat.PREC_FP32: 0, nat.PREC_SPLIT_BF16: 0}   # 0 = library defaul
 
 
 def ctf_distances(plan, operand, fy, ctf_half, N, kappa=None, ws=None, events=None,
-                  variant=None):
+                  variant=None, flags=None, D=None):
     """a8 for every PD of the plan: returns Distances (float32 D on the device).  `events`: an
-    optional (start, stop) hipEvent_t pair recorded around the MFMA Gram kernel."""
+    optional (start, stop) hipEvent_t pair recorded around the MFMA Gram kernel.  ws / flags /
+    D: optional preallocated buffers (partial tiles, flagged-entry queue, output)."""
     dev = operand.hi.device
     info = plan.info
     kappa = KAPPA[operand.precision] if kappa is None else kappa
-    D = torch.empty(info.d_elems, dtype=torch.float32, device=dev)
+    if D is None:
+        D = torch.empty(info.d_elems, dtype=torch.float32, device=dev)
     if ws is None or ws.numel() < info.ws_floats:
         ws = torch.empty(max(info.ws_floats, 1), dtype=torch.float32, device=dev)
-    flags = torch.empty(3 * info.flag_cap, dtype=torch.int32, device=dev)
-    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    if flags is None or flags.numel() < 3 * info.flag_cap:
+        flags = torch.empty(3 * info.flag_cap, dtype=torch.int32, device=dev)
+    cnt = torch.empty(1, dtype=torch.int32, device=dev)     # zeroed by the library
     nat.ctf_distance(plan, operand.hi, operand.lo if operand.lo is not None else operand.hi,
                      operand.Khp, operand.precision, kappa, fy, ctf_half, N, D, ws, flags, cnt,
                      events, GRAM_VARIANT[operand.precision] if variant is None else variant)
