@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--graph", type=int, default=None,
+                    help="1: replay prep + Gram as a captured HIP graph (default: on for c3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 path")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
@@ -94,34 +96,63 @@ def build_workload(name, dev):
     return batch, meta, num_eigs
 
 
-def run_precision(batch, precision, steps, warmup, num_eigs, events):
-    """Warm up, then time exactly `steps` full steps; returns timings and the last results."""
+def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False):
+    """Tune the Gram on the prepared operand, warm up, then time exactly `steps` full steps;
+    returns timings and the last results.  graph: the prep + Gram part of each step is one
+    replay of a captured HIP graph; the Gram kernel is then timed with HIP events in `steps`
+    eager launches right after the timed region (events cannot sit inside the graph)."""
     import torch
     from manifoldem_python_amd import batch as B
     from manifoldem_python_amd import autotune
+    from manifoldem_python_amd import pipeline as pl
     wsp = B.Workspace(batch, precision)
     rng = np.random.default_rng(0)
     rand = lambda *sh: rng.random(sh)
-    for _ in range(warmup):
-        dist = B.distances(batch, wsp)
-        B.embed(wsp, dist, num_eigs=num_eigs, rand=rand)
+    B.distances(batch, wsp)
     tuned = autotune.tune_gram(wsp, batch.N)
+    g = B.GraphedDistances(batch, wsp) if graph else None
+    dist_fn = (lambda ev=None: g.run()) if graph else (
+        lambda ev=None: B.distances(batch, wsp, events=ev))
+    for _ in range(warmup):
+        dist = dist_fn()
+        B.embed(wsp, dist, num_eigs=num_eigs, rand=rand)
     torch.cuda.synchronize()
     pairs = [events.new_pair() for _ in range(steps)]
     t_dist = 0.0
     t0 = time.perf_counter()
     for k in range(steps):
         a = time.perf_counter()
-        dist = B.distances(batch, wsp, events=pairs[k])
+        dist = dist_fn(pairs[k])
         torch.cuda.synchronize()
         t_dist += time.perf_counter() - a
         res = B.embed(wsp, dist, num_eigs=num_eigs, rand=rand)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
+    if graph:
+        for k in range(steps):
+            pl.ctf_distances(wsp.plan, wsp.operand, wsp.fy, wsp.ctf, batch.N, ws=wsp.ws,
+                             events=pairs[k], flags=wsp.flags)
+        torch.cuda.synchronize()
     gram_ms = [events.elapsed_ms(p) for p in pairs]
     stages = B.breakdown(batch, wsp, num_eigs, rand)
     return dict(t=t, t_dist=t_dist, gram_ms=float(np.mean(gram_ms)), wsp=wsp, dist=dist,
-                res=res, flags=int(dist.flag_count.item()), stages=stages, tuned=tuned)
+                res=res, flags=int(dist.flag_count.item()), stages=stages, tuned=tuned,
+                graph=graph)
+
+
+def issued_gram_flops(plan, Khp, products):
+    """MFMA flops the Gram kernel issues: every upper tile's 16-row/16-column blocks that hold
+    data (edge blocks beyond a PD's size are skipped), 4 segments of Khp, `products` MFMA
+    products per block (3 for split-BF16, 1 for FP32)."""
+    tot = 0
+    for n in plan.ns:
+        T = -(-int(n) // 128)
+        m = [min(128, int(n) - 128 * a) for a in range(T)]
+        b = [-(-x // 16) * 16 for x in m]
+        for a in range(T):
+            for c in range(a, T):
+                tot += b[a] * b[c]
+    return float(tot) * 2.0 * 4 * Khp * products
 
 
 def fp64_error(batch, wsp, dist):
@@ -228,13 +259,14 @@ def main():
     torch.cuda.synchronize()
     ev = nat.HipEvents()
     log("timing split-BF16")
-    bf = run_precision(batch, nat.PREC_SPLIT_BF16, args.steps, args.warmup, num_eigs, ev)
+    graph = bool(args.graph) if args.graph is not None else args.workload == "c3"
+    bf = run_precision(batch, nat.PREC_SPLIT_BF16, args.steps, args.warmup, num_eigs, ev,
+                       graph=graph)
     ns = batch.ns
     pairs = float(np.sum(ns * (ns - 1) / 2))
     Kh = pl.half_len(batch.N)
     flops = 8.0 * Kh * pairs                       # algorithmic: 8 K_h per unordered pair
-    n_items = bf["wsp"].plan.info.n_items
-    exec_flops = 3 * n_items * 2.0 * 128 * 128 * bf["wsp"].operand.Khp  # hi*hi + hi*lo + lo*hi
+    exec_flops = issued_gram_flops(bf["wsp"].plan, bf["wsp"].operand.Khp, 3)
     gram_s = bf["gram_ms"] / 1e3
     achieved = flops / gram_s / 1e12
     out = {
@@ -246,12 +278,12 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16x3 (split) Gram, fp32 prep, fp64 eigensolve",
         "data": "synthetic",
-        "config": dict(meta, precision="split-bf16"),
+        "config": dict(meta, precision="split-bf16", hip_graph=graph),
         "pds_per_s": batch.ns.size * args.steps / bf["t"],
         "distance_ms_per_step": 1e3 * bf["t_dist"] / args.steps,
         "gram_kernel_ms": bf["gram_ms"],
         "stage_ms": bf["stages"],
-        "gram_variant_ms": bf["tuned"],
+        "gram_tuned": bf["tuned"],
         "kappa_flagged": bf["flags"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
@@ -263,15 +295,20 @@ def main():
     del bf
     if not args.no_fp32:
         log("timing FP32")
-        f32 = run_precision(batch, nat.PREC_FP32, args.steps, args.warmup, num_eigs, ev)
+        f32 = run_precision(batch, nat.PREC_FP32, args.steps, args.warmup, num_eigs, ev,
+                            graph=graph)
         a32 = flops / (f32["gram_ms"] / 1e3) / 1e12
         out["fp32"] = {"value": pairs * args.steps / f32["t"],
                        "ms_per_step": 1e3 * f32["t"] / args.steps,
                        "pds_per_s": batch.ns.size * args.steps / f32["t"],
                        "gram_kernel_ms": f32["gram_ms"],
                        "stage_ms": f32["stages"],
+                       "gram_tuned": f32["tuned"],
                        "roofline": {"bound": "mfma", "achieved": a32, "peak": PEAK_FP32_TFLOPS,
-                                    "unit": "TFLOP/s", "frac": a32 / PEAK_FP32_TFLOPS}}
+                                    "unit": "TFLOP/s", "frac": a32 / PEAK_FP32_TFLOPS,
+                                    "mfma_util": issued_gram_flops(
+                                        f32["wsp"].plan, f32["wsp"].operand.Khp, 1)
+                                    / (f32["gram_ms"] / 1e3) / 1e12 / PEAK_FP32_TFLOPS}}
         if not args.no_check:
             out["fp32"]["d_rel_err_vs_fp64"] = fp64_error(batch, f32["wsp"], f32["dist"])
         out["speedup_vs_fp32"] = out["value"] / out["fp32"]["value"]
