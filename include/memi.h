@@ -221,10 +221,22 @@ int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, i
  * largest-magnitude eigenpairs that sembeddingonFly.py:27 asks ARPACK eigsh for.
  * V: float64 bases, column-major per PD at m*row0 (column 0 = unit start vector, set by the
  * caller); W: float64 [n_rows] scratch; ab: float64 [npd][m][2] gets {alpha_j, beta_j};
- * active: int32 [npd] (0 = skip).  Step j reads column j and writes column j+1. */
+ * active: int32 [npd] (0 = skip).  Step j reads column j and writes column j+1.
+ * split = 0: one workgroup per PD orthogonalises (H unused, may be NULL); split = 1: the
+ * orthogonalisation is spread over the grid (for few large PDs), H: float64 [2*npd*m]. */
 int memi_lanczos_step(const void* plan_dev, const memi_plan_info* info, const double* L,
-                      double* V, int m, int j, const int32_t* active, double* W, double* ab,
-                      void* stream);
+                      double* V, int m, int j, const int32_t* active, double* W, double* H,
+                      double* ab, int split, void* stream);
+
+/* Ritz vectors of every PD after Lanczos, the eigenvectors sembeddingonFly.py:27-35 takes
+ * from eigsh: Y_p = V_p S_p with V_p the basis of memi_lanczos_step (first J columns) and S_p
+ * the J x k eigenvectors of the tridiagonal T_p, row-major.  meta: int64 on the device,
+ * [npd][3] = {J, k, s_off} (the caller checks J <= min(n, m), k <= kmax, s_off + J*k within S);
+ * S: float64 [sum J*k]; Y: float64 [n_rows][kmax], kmax <= MEMI_RITZ_KMAX, columns from k on
+ * zero. */
+#define MEMI_RITZ_KMAX 16
+int memi_ritz_vectors(const void* plan_dev, const memi_plan_info* info, const double* V, int m,
+                      const double* S, const int64_t* meta, int kmax, double* Y, void* stream);
 
 /* ---------------- Stage 3: NLSA (NLSA.py / get_wiener.py / svdRF.py / L2_distance.py) ---- */
 

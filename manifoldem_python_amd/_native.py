@@ -68,7 +68,9 @@ _SIGS = {
     "memi_ferguson_values": [c_vp, c_i64, c_vp, c_int, c_vp, c_vp],
     "memi_markov_coo": [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_vp],
     "memi_lanczos_step": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_int, c_int, c_vp, c_vp,
-                          c_vp, c_vp],
+                          c_vp, c_vp, c_int, c_vp],
+    "memi_ritz_vectors": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_vp, c_vp, c_int, c_vp,
+                          c_vp],
     "memi_nlsa_condist": [c_vp, c_int, c_vp, c_int, c_int, c_vp, c_vp],
     "memi_nlsa_supervectors": [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_vp, c_vp],
     "memi_tall_gram": [c_vp, c_i64, c_int, c_vp, c_vp],
@@ -370,13 +372,38 @@ def markov_coo(row, col, val, nS, sigma, alpha, out, deg):
          float(alpha), _p(out), _p(deg), _stream())
 
 
-def lanczos_step(plan, L, V, m, j, active, W, ab):
+def lanczos_step(plan, L, V, m, j, active, W, ab, H=None):
+    """H: float64 [2*npd*m] scratch selects the grid-parallel orthogonalisation; None selects
+    one workgroup per PD."""
     I = plan.info
     _need(L.dtype == torch.float64 and L.numel() >= I.d_elems, "lanczos_step: L too small")
     _need(V.numel() >= m * I.n_rows and W.numel() >= I.n_rows and ab.numel() >= I.n_pd * m * 2
           and active.numel() >= I.n_pd and active.dtype == torch.int32, "lanczos_step: buffers")
+    _need(H is None or (H.dtype == torch.float64 and H.numel() >= 2 * I.n_pd * m),
+          "lanczos_step: H must be float64 with 2*npd*m elements")
     call("memi_lanczos_step", _p(plan.dev), ctypes.byref(plan.info), _p(L), _p(V), int(m), int(j),
-         _p(active), _p(W), _p(ab), _stream())
+         _p(active), _p(W), None if H is None else _p(H), _p(ab), int(H is not None), _stream())
+
+
+RITZ_KMAX = 16
+
+
+def ritz_vectors(plan, V, m, S, meta_host, meta, kmax, Y):
+    """Y[n_rows][kmax] = per-PD V_p S_p.  meta_host: int64 numpy [npd][3] {J, k, s_off} (checked
+    here against the plan and S); meta: the same on the device."""
+    I = plan.info
+    mh = np.asarray(meta_host, dtype=np.int64).reshape(-1, 3)
+    ns = np.asarray(plan.ns)
+    _need(mh.shape[0] == I.n_pd and meta.numel() == 3 * I.n_pd and meta.dtype == torch.int64,
+          "ritz_vectors: meta must be int64 [npd][3]")
+    _need(1 <= kmax <= RITZ_KMAX and np.all(mh[:, 1] >= 0) and np.all(mh[:, 1] <= kmax) and
+          np.all(mh[:, 0] >= 0) and np.all(mh[:, 0] <= np.minimum(ns, m)) and
+          np.all(mh[:, 2] >= 0) and np.all(mh[:, 2] + mh[:, 0] * mh[:, 1] <= S.numel()),
+          "ritz_vectors: J/k/s_off out of range for the plan, m, kmax or S")
+    _need(V.dtype == torch.float64 and V.numel() >= m * I.n_rows and S.dtype == torch.float64
+          and Y.dtype == torch.float64 and Y.numel() >= I.n_rows * kmax, "ritz_vectors: buffers")
+    call("memi_ritz_vectors", _p(plan.dev), ctypes.byref(plan.info), _p(V), int(m), _p(S),
+         _p(meta), int(kmax), _p(Y), _stream())
 
 
 # ------------------------------------------------------------------------- Stage 3 ---------

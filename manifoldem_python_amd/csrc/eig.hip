@@ -5,8 +5,9 @@
 // reorthogonalisation in lockstep: one step is a grouped matvec w = L v_j over all PDs
 // (k_lanczos_matvec, many workgroups per PD) and one workgroup per PD that orthogonalises w
 // against the whole basis twice (classical Gram-Schmidt, CGS2), records alpha_j, beta_j and
-// appends v_{j+1} = w / beta_j.  The host diagonalises the small tridiagonal T and forms the
-// Ritz vectors V S.
+// appends v_{j+1} = w / beta_j (or, for batches of few large PDs, the same CGS2 spread over
+// the whole grid in three kernels).  The host diagonalises the small tridiagonal T; the Ritz
+// vectors V S of every PD come from one batched kernel (k_ritz_vectors).
 //
 // Layout: L float64, PD p's n*n block at plan d_off; V float64 column-major per PD, PD p at
 // element p_vo = m * row0(p) (row0 = packed row offset), column c of length n at p_vo + c*n;
@@ -88,6 +89,116 @@ __global__ __launch_bounds__(256) void k_lanczos_orth(const memi_pd_desc* __rest
   }
 }
 
+// Grid-parallel CGS2 for large PDs (one workgroup per PD leaves most CUs idle when a batch
+// has few PDs): per pass, k_orth_dot (one wave per (PD, basis column)) then k_orth_update
+// (64 rows x 4 column groups per workgroup); k_orth_finish records alpha, beta and writes
+// v_{j+1}.  H: float64 [2][npd][m] coefficients of the two passes.
+__global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict__ pds,
+                                                  const double* __restrict__ V, int m, int j,
+                                                  const int32_t* __restrict__ active,
+                                                  const double* __restrict__ W,
+                                                  double* __restrict__ H) {
+  const int pd = blockIdx.y;
+  if (!active[pd]) return;
+  const memi_pd_desc P = pds[pd];
+  const int n = P.n;
+  if (j >= min(n, m)) return;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c > j) return;
+  const int lane = threadIdx.x & 63;
+  const double* vc = V + (int64_t)m * P.row0 + (int64_t)c * n;
+  const double* w = W + P.row0;
+  double s = 0.0;
+  for (int r = lane; r < n; r += 64) s += vc[r] * w[r];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) H[(int64_t)pd * m + c] = s;
+}
+
+__global__ __launch_bounds__(256) void k_orth_update(const memi_pd_desc* __restrict__ pds,
+                                                     const double* __restrict__ V, int m, int j,
+                                                     const int32_t* __restrict__ active,
+                                                     double* __restrict__ W,
+                                                     const double* __restrict__ H) {
+  __shared__ double part[4][64];
+  const int pd = blockIdx.y;
+  if (!active[pd]) return;
+  const memi_pd_desc P = pds[pd];
+  const int n = P.n;
+  if (j >= min(n, m) || blockIdx.x * 64 >= n) return;   // uniform over the block
+  const int r = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int cg = threadIdx.x >> 6;
+  const double* Vp = V + (int64_t)m * P.row0;
+  const double* h = H + (int64_t)pd * m;
+  double acc = 0.0;
+  if (r < n)
+    for (int c = cg; c <= j; c += 4) acc += Vp[(int64_t)c * n + r] * h[c];
+  part[cg][threadIdx.x & 63] = acc;
+  __syncthreads();
+  if (cg == 0 && r < n) {
+    const int t = threadIdx.x;
+    W[P.row0 + r] -= (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_orth_finish(const memi_pd_desc* __restrict__ pds,
+                                                     double* __restrict__ V, int m, int j,
+                                                     const int32_t* __restrict__ active,
+                                                     const double* __restrict__ W,
+                                                     const double* __restrict__ H, int64_t npd,
+                                                     double* __restrict__ ab) {
+  __shared__ double red[8];
+  const int pd = blockIdx.x;
+  if (!active[pd]) return;
+  const memi_pd_desc P = pds[pd];
+  const int n = P.n;
+  if (j >= min(n, m)) return;
+  const double* w = W + P.row0;
+  double ss = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) ss += w[r] * w[r];
+  const double beta = sqrt(block_sum(ss, red));
+  if (threadIdx.x == 0) {
+    ab[((int64_t)pd * m + j) * 2 + 0] = H[(int64_t)pd * m + j] + H[(npd + pd) * m + j];
+    ab[((int64_t)pd * m + j) * 2 + 1] = beta;
+  }
+  if (j + 1 < m) {
+    const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
+    double* vn = V + (int64_t)m * P.row0 + (int64_t)(j + 1) * n;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) vn[r] = w[r] * inv;
+  }
+}
+
+// Ritz vectors Y_p = V_p S_p of every PD in one launch, one thread per row: V_p the n x J
+// Lanczos basis (column-major at m*row0), S_p the J x k eigenvectors of T_p (row-major at
+// meta.s_off), Y [n_rows][kmax] (columns k..kmax-1 zero).  meta: int64 [npd][3] = {J, k, s_off}.
+__global__ __launch_bounds__(256) void k_ritz_vectors(const int32_t* __restrict__ rows,
+                                                      int64_t n_rows,
+                                                      const memi_pd_desc* __restrict__ pds,
+                                                      const double* __restrict__ V, int m,
+                                                      const double* __restrict__ S,
+                                                      const int64_t* __restrict__ meta, int kmax,
+                                                      double* __restrict__ Y) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_rows) return;
+  const int pd = rows[2 * g], i = rows[2 * g + 1];
+  const memi_pd_desc P = pds[pd];
+  const int n = P.n;
+  const int J = (int)meta[3 * pd], k = (int)meta[3 * pd + 1];
+  const double* Sp = S + meta[3 * pd + 2];
+  const double* v = V + (int64_t)m * P.row0 + i;
+  double acc[MEMI_RITZ_KMAX];
+#pragma unroll
+  for (int e = 0; e < MEMI_RITZ_KMAX; ++e) acc[e] = 0.0;
+  for (int j = 0; j < J; ++j) {
+    const double x = v[(int64_t)j * n];
+#pragma unroll
+    for (int e = 0; e < MEMI_RITZ_KMAX; ++e)
+      if (e < k) acc[e] += x * Sp[(int64_t)j * k + e];
+  }
+#pragma unroll
+  for (int e = 0; e < MEMI_RITZ_KMAX; ++e)
+    if (e < kmax) Y[g * kmax + e] = e < k ? acc[e] : 0.0;
+}
+
 }  // namespace memi
 
 using namespace memi;
@@ -95,11 +206,14 @@ using namespace memi;
 extern "C" {
 
 int memi_lanczos_step(const void* plan_dev, const memi_plan_info* info, const double* L,
-                      double* V, int m, int j, const int32_t* active, double* W, double* ab,
-                      void* stream) {
-  MEMI_CHECK_ARG(plan_dev && info && L && V && active && W && ab, "memi_lanczos_step: null");
+                      double* V, int m, int j, const int32_t* active, double* W, double* H,
+                      double* ab, int split, void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && L && V && active && W && ab && (!split || H),
+                 "memi_lanczos_step: null");
   MEMI_CHECK_ARG(m >= 2 && m <= 4096 && j >= 0 && j < m, "memi_lanczos_step: bad m=%d j=%d", m,
                  j);
+  MEMI_CHECK_ARG(!split || (info->n_pd <= 65535 && info->max_n <= 65535 * 64),
+                 "memi_lanczos_step: batch too large for the split orthogonalisation");
   if (info->n_pd == 0) return MEMI_OK;
   hipStream_t s = (hipStream_t)stream;
   const char* plan = (const char*)plan_dev;
@@ -109,9 +223,41 @@ int memi_lanczos_step(const void* plan_dev, const memi_plan_info* info, const do
   hipLaunchKernelGGL(k_lanczos_matvec, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, rows,
                      info->n_rows, pds, L, V, m, j, active, W);
   MEMI_CHECK_LAUNCH("k_lanczos_matvec");
-  hipLaunchKernelGGL(k_lanczos_orth, dim3((unsigned)info->n_pd), dim3(256),
-                     (size_t)m * sizeof(double), s, pds, V, m, j, active, W, ab);
-  MEMI_CHECK_LAUNCH("k_lanczos_orth");
+  if (!split) {
+    hipLaunchKernelGGL(k_lanczos_orth, dim3((unsigned)info->n_pd), dim3(256),
+                       (size_t)m * sizeof(double), s, pds, V, m, j, active, W, ab);
+    MEMI_CHECK_LAUNCH("k_lanczos_orth");
+    return MEMI_OK;
+  }
+  const unsigned npd = (unsigned)info->n_pd;
+  for (int pass = 0; pass < 2; ++pass) {
+    double* Hp = H + (int64_t)pass * info->n_pd * m;
+    hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)((j + 1 + 3) / 4), npd), dim3(256), 0, s, pds,
+                       V, m, j, active, W, Hp);
+    MEMI_CHECK_LAUNCH("k_orth_dot");
+    hipLaunchKernelGGL(k_orth_update, dim3((unsigned)((info->max_n + 63) / 64), npd), dim3(256),
+                       0, s, pds, V, m, j, active, W, Hp);
+    MEMI_CHECK_LAUNCH("k_orth_update");
+  }
+  hipLaunchKernelGGL(k_orth_finish, dim3(npd), dim3(256), 0, s, pds, V, m, j, active, W, H,
+                     info->n_pd, ab);
+  MEMI_CHECK_LAUNCH("k_orth_finish");
+  return MEMI_OK;
+}
+
+int memi_ritz_vectors(const void* plan_dev, const memi_plan_info* info, const double* V, int m,
+                      const double* S, const int64_t* meta, int kmax, double* Y, void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && V && S && meta && Y, "memi_ritz_vectors: null pointer");
+  MEMI_CHECK_ARG(m >= 1 && kmax >= 1 && kmax <= MEMI_RITZ_KMAX,
+                 "memi_ritz_vectors: bad m=%d kmax=%d (at most %d)", m, kmax, MEMI_RITZ_KMAX);
+  if (info->n_rows == 0) return MEMI_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const char* plan = (const char*)plan_dev;
+  const memi_pd_desc* pds = (const memi_pd_desc*)(plan + info->off_pds);
+  const int32_t* rows = (const int32_t*)(plan + info->off_rows);
+  hipLaunchKernelGGL(k_ritz_vectors, dim3((unsigned)((info->n_rows + 255) / 256)), dim3(256), 0,
+                     s, rows, info->n_rows, pds, V, m, S, meta, kmax, Y);
+  MEMI_CHECK_LAUNCH("k_ritz_vectors");
   return MEMI_OK;
 }
 

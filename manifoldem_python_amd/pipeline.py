@@ -342,14 +342,15 @@ def lm_ritz(alpha, beta, k):
     return theta[sel], S[:, sel]
 
 
-def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202):
+def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202, split=None):
     """Largest-magnitude eigenpairs of every PD's float64 kernel block of L (sembeddingonFly.py
     semantics: the k eigenpairs of largest |lambda|, sorted by descending lambda) by batched
     Lanczos with full reorthogonalisation on the GPU.
 
     Converged when every selected Ritz pair has residual |beta_J s_J| <= tol * max|theta| or
-    the Krylov space is the whole space.  Returns a list of (lam, V (n x k) device tensor,
-    steps, max residual)."""
+    the Krylov space is the whole space.  split: orthogonalise with the whole grid (default
+    for batches of fewer than 128 PDs or with a PD above 1024 rows) instead of one workgroup
+    per PD.  Returns a list of (lam, V (n x k) numpy, steps, max residual)."""
     dev = L.device
     npd = plan.npd
     ns = plan.ns.astype(int)
@@ -358,8 +359,11 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202):
     if m is None:
         m = int(min(ns.max(), 512))
     m = int(max(2, min(m, ns.max())))
+    if split is None:
+        split = npd < 128 or int(ns.max()) > 1024
     V = torch.zeros(m * n_rows, dtype=torch.float64, device=dev)
     W = torch.empty(n_rows, dtype=torch.float64, device=dev)
+    H = torch.empty(2 * npd * m, dtype=torch.float64, device=dev) if split else None
     ab = torch.zeros(npd * m * 2, dtype=torch.float64, device=dev)
     rng = np.random.default_rng(seed)
     v0 = np.empty(n_rows)
@@ -378,7 +382,7 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202):
     while active_h.any():
         stop = min(m, j + check_every)
         while j < stop:
-            nat.lanczos_step(plan, L, V, m, j, active, W, ab)
+            nat.lanczos_step(plan, L, V, m, j, active, W, ab, H=H)
             j += 1
         abh = ab.view(npd, m, 2).cpu().numpy()
 
@@ -404,13 +408,26 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202):
                 out[p] = r
                 active_h[p] = 0
         active.copy_(torch.as_tensor(active_h, device=dev))
+    # Ritz vectors of all PDs in one launch and one copy to the host
+    kmax = int(max(out[p][1].shape[1] for p in range(npd)))
+    if kmax > nat.RITZ_KMAX:
+        raise ValueError(f"lanczos_eigs: at most {nat.RITZ_KMAX} eigenpairs per PD")
+    meta_h = np.zeros((npd, 3), dtype=np.int64)
+    blocks, off = [], 0
+    for p in range(npd):
+        S = np.ascontiguousarray(out[p][1], dtype=np.float64)
+        meta_h[p] = (out[p][2], S.shape[1], off)
+        blocks.append(S.ravel())
+        off += S.size
+    S_all = torch.as_tensor(np.concatenate(blocks), device=dev)
+    meta = torch.as_tensor(meta_h, device=dev)
+    Y = torch.empty(n_rows * kmax, dtype=torch.float64, device=dev)
+    nat.ritz_vectors(plan, V, m, S_all, meta_h, meta, kmax, Y)
+    Yh = Y.view(n_rows, kmax).cpu().numpy()
     results = []
     for p in range(npd):
         lam, S, J, res = out[p]
-        n = ns[p]
-        Vp = V[m * rows0[p]:m * rows0[p] + J * n].view(J, n)
-        Y = (torch.as_tensor(np.ascontiguousarray(S.T), device=dev) @ Vp).T
-        results.append((lam, Y, J, res))
+        results.append((lam, Yh[rows0[p]:rows0[p] + ns[p], :S.shape[1]], J, res))
     return results
 
 
@@ -522,8 +539,7 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
             lam, V, _, _ = eigs[p]
         else:
             lam, V = top_eigs_lm(plan.block(L, p), nEigs + 1)
-            lam = lam.cpu().numpy()
-        V = V.cpu().numpy()
+            lam, V = lam.cpu().numpy(), V.cpu().numpy()
         r = results[p]
         r.lamb = lam
         psi = np.zeros((n, nEigs))

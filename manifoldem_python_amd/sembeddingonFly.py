@@ -22,4 +22,4 @@ def op(yVal, yCol, yRow, nS, options1):
     L = torch.as_tensor(l.toarray().ravel(), device=dev)
     plan = nat.Plan([0], [int(nS)], [int(nS)], device=dev)
     lam, V, _, _ = pl.lanczos_eigs(plan, L, [options.nEigs + 1])[0]
-    return lam, V.cpu().numpy()
+    return lam, V

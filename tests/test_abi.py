@@ -120,3 +120,22 @@ def test_distance_rejects_a_split_with_an_empty_chunk():
     assert not autotune.valid_chunks(nat.PREC_SPLIT_BF16, 64, 3)
     # C2-sized operand (N = 128): 260 split-BF16 steps, 3 chunks of 87/87/86
     assert autotune.valid_chunks(nat.PREC_SPLIT_BF16, 8320, 3)
+
+
+def test_ritz_vectors_checks_meta_on_the_host():
+    """A Ritz assembly whose basis length J exceeds the PD size (or k exceeds kmax) is refused
+    before launch: the kernel indexes the basis and S by these numbers."""
+    import torch
+    from manifoldem_python_amd import _native as nat
+    plan = nat.Plan([0, 10], [10, 20], [5, 5], device="cpu")
+    m, kmax = 8, 4
+    V = torch.zeros(m * 30, dtype=torch.float64)
+    S = torch.zeros(100, dtype=torch.float64)
+    Y = torch.zeros(30 * kmax, dtype=torch.float64)
+    meta = torch.zeros(6, dtype=torch.int64)
+    bad = [np.array([[11, 2, 0], [8, 2, 22]]),     # J > n of PD 0
+           np.array([[8, 5, 0], [8, 2, 40]]),      # k > kmax
+           np.array([[8, 2, 0], [8, 4, 90]])]      # s_off + J*k beyond S
+    for mh in bad:
+        with pytest.raises(RuntimeError, match="ritz_vectors"):
+            nat.ritz_vectors(plan, V, m, S, mh, meta, kmax, Y)

@@ -112,7 +112,8 @@ def test_ferguson_slope_matches_finite_differences(cuda):
     assert np.max(np.abs(np.log(S[0]) - np.array([logS(le) for le in logEps]))) < 1e-6
 
 
-def test_batched_lanczos_matches_dense(cuda):
+@pytest.mark.parametrize("split", [False, True])
+def test_batched_lanczos_matches_dense(cuda, split):
     """Three PDs of ragged sizes in one plan: GPU Lanczos eigenpairs (largest magnitude)
     against numpy eigh of each normalised kernel block."""
     import torch
@@ -129,7 +130,7 @@ def test_batched_lanczos_matches_dense(cuda):
     deg = torch.empty(2 * plan.info.n_rows, dtype=torch.float64, device=cuda)
     nat.markov_normalize(plan, flat, mask, sigma, L, deg)
     kk = [12, 16, 9]
-    res = pl.lanczos_eigs(plan, L, kk)
+    res = pl.lanczos_eigs(plan, L, kk, split=split)
     for p in range(3):
         A = plan.block(L, p).cpu().numpy()
         assert np.allclose(A, A.T)
@@ -138,7 +139,7 @@ def test_batched_lanczos_matches_dense(cuda):
         sel = sel[np.argsort(-w[sel], kind="stable")]
         lam, Y, J, r = res[p]
         np.testing.assert_allclose(lam, w[sel], atol=1e-9)
-        Y = Y.cpu().numpy()
+        assert Y.shape == (ns[p], kk[p])
         for c in range(kk[p]):
             assert abs(abs(np.dot(Y[:, c], U[:, sel[c]])) - 1) < 1e-6, (p, c)
 
