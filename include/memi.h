@@ -112,6 +112,16 @@ int memi_mask_multiply(float* img, int64_t n, int N, int disk, const float* msk2
 int memi_spline_rotate(const float* coef, const double* angle_deg, int64_t n, int N,
                        float* out, void* stream);
 
+/* Fused in-plane alignment, getDistanceCTF_local_Conj9combinedS2.py:315-330 with
+ * rotatefill.py:21-28 and annularMask.py:19-32: optionally the disk mask (disk != 0), then
+ * the periodic cubic-spline rotation by rot1_deg[i] followed by the one by rot2_deg[i], each
+ * image resident in LDS (exact recursive B-spline prefilter in place of the FFT division of
+ * memi_spectrum_scale + memi_spline_rotate).  in, out: float32 [n][N][N] (may be the same
+ * buffer); 8 <= N <= MEMI_ALIGN_MAX_N. */
+#define MEMI_ALIGN_MAX_N 200
+int memi_align_images(const float* in, const double* rot1_deg, const double* rot2_deg, int64_t n,
+                      int N, int disk, float* out, void* stream);
+
 /* CTF + packed distance operand, ctemh_cryoFrank.py:24-44 and
  * getDistanceCTF_local_Conj9combinedS2.py:336-344,392-396: from fy (rfft2 of the aligned
  * images) and per-image defocus df (A), writes ctf_half float32 [n][Kh] (C on the half

@@ -53,6 +53,7 @@ _SIGS = {
     "memi_spectrum_scale": [c_vp, c_i64, c_int, c_int, c_dbl, c_int, c_vp],
     "memi_mask_multiply": [c_vp, c_i64, c_int, c_int, c_vp, c_vp],
     "memi_spline_rotate": [c_vp, c_vp, c_i64, c_int, c_vp, c_vp],
+    "memi_align_images": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp],
     "memi_ctf_pack": [c_vp, c_vp, c_i64, c_int, ctypes.POINTER(CtfParams), c_int, c_i64, c_i64,
                       c_vp, c_vp, c_vp, c_vp],
     "memi_phase_flip": [c_vp, c_vp, c_i64, c_int, c_vp, c_vp],
@@ -181,6 +182,22 @@ def spline_rotate(coef, angle_deg, N, out):
     _need(angle_deg.numel() == n and angle_deg.dtype == torch.float64,
           "spline_rotate: angle_deg must be float64 (n,)")
     call("memi_spline_rotate", _p(coef), _p(angle_deg), coef.shape[0], N, _p(out), _stream())
+
+
+ALIGN_MIN_N, ALIGN_MAX_N = 8, 200
+
+
+def align_images(img, rot1_deg, rot2_deg, N, out, disk=True):
+    """Disk mask (optional) then the two spline rotations, one fused kernel; out may be img."""
+    n = img.shape[0]
+    _need(img.dim() == 3 and tuple(img.shape[1:]) == (N, N) and img.dtype == torch.float32,
+          "align_images: img must be float32 (n, N, N)")
+    _need(out.shape == img.shape and out.dtype == torch.float32, "align_images: out shape")
+    _need(rot1_deg.numel() == n and rot2_deg.numel() == n and rot1_deg.dtype == torch.float64
+          and rot2_deg.dtype == torch.float64, "align_images: angles must be float64 (n,)")
+    _need(ALIGN_MIN_N <= N <= ALIGN_MAX_N, f"align_images: N={N} outside [8, {ALIGN_MAX_N}]")
+    call("memi_align_images", _p(img), _p(rot1_deg), _p(rot2_deg), n, N, int(bool(disk)),
+         _p(out), _stream())
 
 
 def ctf_params(pix_size, cs, ekv, ampc, gauss_env):

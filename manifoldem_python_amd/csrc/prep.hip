@@ -159,6 +159,141 @@ __global__ __launch_bounds__(256) void k_spline_rotate(const float* __restrict__
   }
 }
 
+// ---- fused alignment: disk mask, then two cubic-spline rotations, image resident in LDS ----
+//
+// The periodic cubic B-spline coefficients c of a line s (c = s / ((4 + 2 cos w) / 6) in
+// frequency) come from the exact recursive filter with pole z = sqrt(3) - 2:
+//   c+[0] = sum_j z^j s[-j mod N] / (1 - z^N),       c+[k] = s[k] + z c+[k-1],
+//   c-[N-1] = -z / (1 - z^N) sum_j z^j c+[(N-1+j) mod N],  c-[k] = z (c-[k+1] - c+[k]),
+//   c = 6 c-,
+// the periodic sums truncated after 24 terms (z^24 < 1e-13).  Rows then columns.
+constexpr double BSPLINE_Z = -0.2679491924311227;   // sqrt(3) - 2
+constexpr int BSPLINE_NT = 24;
+
+template <bool ROWS>
+__device__ inline void bspline_lines(float* buf, int N, int S, double zN) {
+  const int es = ROWS ? 1 : S;
+  const int nt = min(N, BSPLINE_NT);
+  const double z = BSPLINE_Z;
+  for (int l = threadIdx.x; l < N; l += blockDim.x) {
+    float* x = buf + (ROWS ? l * S : l);
+    double c0 = 0.0, zj = 1.0;
+    for (int j = 0; j < nt; ++j) {
+      c0 += zj * (double)x[(j == 0 ? 0 : N - j) * es];
+      zj *= z;
+    }
+    double prev = c0 / (1.0 - zN);
+    x[0] = (float)prev;
+    for (int k = 1; k < N; ++k) {
+      prev = (double)x[k * es] + z * prev;
+      x[k * es] = (float)prev;
+    }
+    double cm = 0.0;
+    zj = 1.0;
+    for (int j = 0; j < nt; ++j) {
+      cm += zj * (double)x[(j == 0 ? N - 1 : j - 1) * es];
+      zj *= z;
+    }
+    double nxt = -z / (1.0 - zN) * cm;
+    x[(N - 1) * es] = (float)(6.0 * nxt);
+    for (int k = N - 2; k >= 0; --k) {
+      nxt = z * (nxt - (double)x[k * es]);
+      x[k * es] = (float)(6.0 * nxt);
+    }
+  }
+}
+
+__device__ inline void bspline_prefilter(float* buf, int N, int S, double zN) {
+  bspline_lines<true>(buf, N, S, zN);
+  __syncthreads();
+  bspline_lines<false>(buf, N, S, zN);
+  __syncthreads();
+}
+
+// Sample the coefficient image `src` (LDS, row stride S) rotated by angle_deg into dst (row
+// stride ds).  Source rows/columns are clamped to [-N+2, 2N-3] before the taps, which changes
+// nothing for finite angles (|src - h| <= sqrt(2) h) and keeps every tap index in one wrap of
+// [0, N) for any input.
+__device__ inline void spline_sample(const float* src, int N, int S, double angle_deg, float* dst,
+                                     int ds) {
+  double sn, cs;
+  sincospi(angle_deg / 180.0, &sn, &cs);
+  const double h = 0.5 * (N - 1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int r = wv; r < N; r += nw) {
+    const double dr = r - h;
+    for (int c = lane; c < N; c += 64) {
+      const double dc = c - h;
+      const double sr = h + cs * dr + sn * dc;
+      const double sc = h - sn * dr + cs * dc;
+      const double fr = floor(sr), fc = floor(sc);
+      float wr[4], wc[4];
+      bspline_w((float)(sr - fr), wr);
+      bspline_w((float)(sc - fc), wc);
+      const int ir = (int)fmin(fmax(fr, (double)(2 - N)), (double)(2 * N - 3)) - 1;
+      const int ic = (int)fmin(fmax(fc, (double)(2 - N)), (double)(2 * N - 3)) - 1;
+      int cols[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        int t = ic + b;
+        t += t < 0 ? N : 0;
+        t -= t >= N ? N : 0;
+        cols[b] = t;
+      }
+      float acc = 0.f;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        int t = ir + a;
+        t += t < 0 ? N : 0;
+        t -= t >= N ? N : 0;
+        const float* rowp = src + t * S;
+        float racc = 0.f;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) racc += wc[b] * rowp[cols[b]];
+        acc += wr[a] * racc;
+      }
+      dst[r * ds + c] = acc;
+    }
+  }
+}
+
+// One block per image.  NBUF = 2: mask, rotate by ang1 into the second LDS image, rotate by
+// ang2 into out.  NBUF = 1: (mask if disk,) rotate by ang1 into out; the host runs it twice.
+// in and out may be the same buffer (each block reads its whole image before writing).
+template <int NMAX, int NBUF>
+__global__ __launch_bounds__(512) void k_align_images(const float* in, const double* ang1,
+                                                      const double* ang2, int N, int disk,
+                                                      float* out) {
+  __shared__ float lds[NBUF * NMAX * (NMAX + 1)];
+  const int S = N + 1;
+  const int64_t img = blockIdx.x;
+  const float* src = in + img * N * N;
+  float* A = lds;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int r = wv; r < N; r += nw)
+    for (int c = lane; c < N; c += 64) {
+      float v = src[r * N + c];
+      if (disk) {
+        const int xd = 2 * r - N + 2, yd = 2 * c - N;     // annularMask.op(0, N/2, N, N)
+        if (xd * xd + yd * yd >= N * N) v = 0.f;
+      }
+      A[r * S + c] = v;
+    }
+  __syncthreads();
+  const double zN = pow(BSPLINE_Z, (double)N);
+  bspline_prefilter(A, N, S, zN);
+  float* o = out + img * N * N;
+  if constexpr (NBUF == 2) {
+    float* B = lds + NMAX * (NMAX + 1);
+    spline_sample(A, N, S, ang1[img], B, S);
+    __syncthreads();
+    bspline_prefilter(B, N, S, zN);
+    spline_sample(B, N, S, ang2[img], o, N);
+  } else {
+    spline_sample(A, N, S, ang1[img], o, N);
+  }
+}
+
 __device__ inline uint16_t bf16_rne(float f) {
   uint32_t u = __float_as_uint(f);
   u += 0x7FFFu + ((u >> 16) & 1u);
@@ -329,6 +464,31 @@ int memi_spline_rotate(const float* coef, const double* angle_deg, int64_t n, in
   hipLaunchKernelGGL(k_spline_rotate, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream, coef,
                      angle_deg, N, out);
   MEMI_CHECK_LAUNCH("k_spline_rotate");
+  return MEMI_OK;
+}
+
+int memi_align_images(const float* in, const double* rot1_deg, const double* rot2_deg, int64_t n,
+                      int N, int disk, float* out, void* stream) {
+  MEMI_CHECK_ARG(in && rot1_deg && rot2_deg && out && n >= 0, "memi_align_images: bad arguments");
+  MEMI_CHECK_ARG(N >= 8 && N <= MEMI_ALIGN_MAX_N,
+                 "memi_align_images: N=%d outside [8, %d]; use memi_spline_rotate", N,
+                 MEMI_ALIGN_MAX_N);
+  if (n == 0) return MEMI_OK;
+  MEMI_CHECK_ARG(n < ((int64_t)1 << 31), "memi_align_images: too many images");
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 g((unsigned)n), b(512);
+  if (N <= 64) {
+    hipLaunchKernelGGL((k_align_images<64, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
+  } else if (N <= 141) {
+    hipLaunchKernelGGL((k_align_images<141, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
+  } else {
+    hipLaunchKernelGGL((k_align_images<MEMI_ALIGN_MAX_N, 1>), g, b, 0, s, in, rot1_deg, rot2_deg,
+                       N, disk, out);
+    MEMI_CHECK_LAUNCH("k_align_images");
+    hipLaunchKernelGGL((k_align_images<MEMI_ALIGN_MAX_N, 1>), g, b, 0, s, out, rot2_deg, rot2_deg,
+                       N, 0, out);
+  }
+  MEMI_CHECK_LAUNCH("k_align_images");
   return MEMI_OK;
 }
 

@@ -127,10 +127,16 @@ def filter_kind(filterPar):
     raise ValueError("%s filter is unsupported" % t)
 
 
+ALIGN = "auto"      # "fused": memi_align_images; "fft": FFT prefilter + memi_spline_rotate
+
+
 def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, operand=None,
-                   row0=0, msk2=None, fy_out=None, ctf_out=None, want_images=True, src_max=None):
+                   row0=0, msk2=None, fy_out=None, ctf_out=None, want_images=True, src_max=None,
+                   align=None):
     """a1-a6 for one set of images (all device tensors): load + normalise, Butterworth,
     disk mask, two cubic-spline rotations, fy = rfft2(img * msk2), CTF and operand rows.
+    align: "fused" (one kernel with the images in LDS, 8 <= N <= 200), "fft" (B-spline
+    prefilter by FFT division, any N) or "auto"/None (fused when N allows).
 
     Returns a dict of device tensors: imgAll (n,N,N) f32, fy (n,N,Nh) c64, ctf (n,Kh) f32.
     """
@@ -141,13 +147,20 @@ def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, 
     Y = torch.fft.rfft2(y)
     nat.spectrum_scale(Y, N, filter_kind(filterPar), filterPar["Qc"], filterPar["N"])
     y = torch.fft.irfft2(Y, s=(N, N)).contiguous()
-    nat.mask_multiply(y, N, disk=True)
-    for ang in (rot1_deg, rot2_deg):
-        Y = torch.fft.rfft2(y)
-        nat.spectrum_scale(Y, N, nat.FILTER_BSPLINE)
-        coef = torch.fft.irfft2(Y, s=(N, N)).contiguous()
-        nat.spline_rotate(coef, ang, N, y)
-    del Y, coef
+    del Y
+    align = align or ALIGN
+    if align == "auto":
+        align = "fused" if nat.ALIGN_MIN_N <= N <= nat.ALIGN_MAX_N else "fft"
+    if align == "fused":
+        nat.align_images(y, rot1_deg, rot2_deg, N, y, disk=True)
+    else:
+        nat.mask_multiply(y, N, disk=True)
+        for ang in (rot1_deg, rot2_deg):
+            Y = torch.fft.rfft2(y)
+            nat.spectrum_scale(Y, N, nat.FILTER_BSPLINE)
+            coef = torch.fft.irfft2(Y, s=(N, N)).contiguous()
+            nat.spline_rotate(coef, ang, N, y)
+        del Y, coef
     img = y
     if msk2 is not None:
         img = y.clone()

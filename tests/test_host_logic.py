@@ -55,3 +55,31 @@ def test_tau_bins_match_oracle():
     tr, IMG1, ir = nlsa.tau_bins(tau, IMGT, 50)
     np.testing.assert_array_equal(inds, ir)
     np.testing.assert_allclose(tn, tr)
+
+
+def test_recursive_bspline_prefilter_equals_fft_division():
+    """The fused alignment kernel prefilters each image line with the exact periodic recursive
+    filter (pole z = sqrt(3) - 2, initial values from 24-term periodic sums, gain 6).  Restated
+    here in float64, it must equal the frequency-domain division by (4 + 2 cos w) / 6 that the
+    FFT path uses, including lines shorter than the 24-term sums."""
+    z = np.sqrt(3.0) - 2.0
+
+    def recursive(s, nt=24):
+        N = len(s)
+        zn = z ** N
+        cp = np.empty(N)
+        cp[0] = sum(z ** j * s[(-j) % N] for j in range(min(N, nt))) / (1 - zn)
+        for k in range(1, N):
+            cp[k] = s[k] + z * cp[k - 1]
+        cm = np.empty(N)
+        cm[N - 1] = -z / (1 - zn) * sum(z ** j * cp[(N - 1 + j) % N] for j in range(min(N, nt)))
+        for k in range(N - 2, -1, -1):
+            cm[k] = z * (cm[k + 1] - cp[k])
+        return 6 * cm
+
+    rng = np.random.default_rng(3)
+    for N in (8, 9, 23, 24, 25, 70, 128, 160, 200):
+        s = rng.normal(size=N)
+        k = np.arange(N)
+        ref = np.real(np.fft.ifft(np.fft.fft(s) / ((4 + 2 * np.cos(2 * np.pi * k / N)) / 6)))
+        assert np.max(np.abs(recursive(s) - ref)) < 1e-12 * np.max(np.abs(ref)), N
