@@ -61,14 +61,56 @@ __global__ __launch_bounds__(256) void k_load_normalize(const float* __restrict_
   }
 }
 
+// The same with the raw image staged once in LDS (row stride N + 1, so the transposed reads
+// are conflict-free): background moments in one pass (float64 sum and sum of squares; the
+// variance of the zero-filled background array is E[bg^2] - mean^2), then the transposed,
+// normalised write.  One HBM read and one write per image.
+template <int NMAX>
+__global__ __launch_bounds__(512) void k_load_normalize_lds(const float* __restrict__ stack,
+                                                            const int64_t* __restrict__ src,
+                                                            const int32_t* __restrict__ conj,
+                                                            int N, float* __restrict__ out) {
+  __shared__ float t[NMAX * (NMAX + 1)];
+  __shared__ double red[16];
+  const int64_t img = blockIdx.x;
+  const float* raw = stack + src[img] * (int64_t)N * N;
+  const bool cj = conj != nullptr && conj[img] != 0;
+  const int S = N + 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double s = 0.0, s2 = 0.0;
+  for (int a = wv; a < N; a += nw)
+    for (int b = lane; b < N; b += 64) {
+      const float v = raw[a * N + b];
+      t[a * S + b] = v;
+      // raw[a][b] lands at output (r, c) = (cj ? N-1-b : b, a)
+      const int r = cj ? N - 1 - b : b;
+      const int xd = 2 * r - N + 2, yd = 2 * a - N;      // annularMask.op(0, N/2, N, N)
+      if (xd * xd + yd * yd >= N * N) {
+        s += (double)v;
+        s2 += (double)v * v;
+      }
+    }
+  const double NN = (double)N * N;
+  const double mean = block_sum(s, red) / NN;
+  const double var = block_sum(s2, red + 8) / NN - mean * mean;
+  const double sd = sqrt(var);
+  float* o = out + img * (int64_t)N * N;
+  for (int r = wv; r < N; r += nw) {
+    const int b = cj ? N - 1 - r : r;
+    for (int c = lane; c < N; c += 64)
+      o[r * N + c] = (float)(((double)t[c * S + b] - mean) / sd);
+  }
+}
+
 // cubic B-spline transfer function (4 + 2 cos w) / 6 at FFT index i of n
 __device__ inline double bspline_hat(int i, int n) {
   return (4.0 + 2.0 * cospi(2.0 * (double)i / (double)n)) / 6.0;
 }
 
-// thread per half-spectrum bin (factor evaluated once), images looped in chunks
+// thread per half-spectrum bin and SPEC_IMG images (factor evaluated once per thread)
+constexpr int SPEC_IMG = 4;
 __global__ void k_spectrum_scale(float2* __restrict__ spec, int64_t n, int N, int kind,
-                                 double Qc, int order, int img_chunk) {
+                                 double Qc, int order) {
   const int Nh = N / 2 + 1;
   const int64_t Kh = (int64_t)N * Nh;
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -83,14 +125,15 @@ __global__ void k_spectrum_scale(float2* __restrict__ spec, int64_t n, int N, in
   }
   if (kind & MEMI_FILTER_BSPLINE) f /= bspline_hat(row, N) * bspline_hat(col, N);
   const float ff = (float)f;
-  const int64_t i0 = (int64_t)blockIdx.y * img_chunk;
-  const int64_t i1 = min(n, i0 + img_chunk);
-  for (int64_t i = i0; i < i1; ++i) {
-    float2 v = spec[i * Kh + k];
-    v.x *= ff;
-    v.y *= ff;
-    spec[i * Kh + k] = v;
-  }
+  // all loads of the chunk are issued before any store (SPEC_IMG images per thread)
+  const int64_t i0 = (int64_t)blockIdx.y * SPEC_IMG;
+  float2 v[SPEC_IMG];
+#pragma unroll
+  for (int u = 0; u < SPEC_IMG; ++u)
+    if (i0 + u < n) v[u] = spec[(i0 + u) * Kh + k];
+#pragma unroll
+  for (int u = 0; u < SPEC_IMG; ++u)
+    if (i0 + u < n) spec[(i0 + u) * Kh + k] = make_float2(v[u].x * ff, v[u].y * ff);
 }
 
 __global__ void k_mask_multiply(float* __restrict__ img, int64_t total, int N, int disk,
@@ -304,10 +347,11 @@ __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t
 // thread per half-spectrum bin k (grid.y: image chunks).  The CTF phase
 // wr = (w1/2 k^2 - w2) k^2 = a_k - c_k df is formed in float64, reduced to [-pi, pi] in float64,
 // and only then passed to the float32 sincos (|C| error ~1e-7).
+constexpr int CTF_IMG = 4;
 __global__ void k_ctf_pack(const float2* __restrict__ fy, const double* __restrict__ df,
                            int64_t n, int N, memi_ctf_params cp, int prec, int64_t Khp,
                            int64_t row0, void* __restrict__ op_hi, void* __restrict__ op_lo,
-                           float* __restrict__ ctf_half, int img_chunk) {
+                           float* __restrict__ ctf_half) {
   const int Nh = N / 2 + 1;
   const int64_t Kh = (int64_t)N * Nh;
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -328,18 +372,29 @@ __global__ void k_ctf_pack(const float2* __restrict__ fy, const double* __restri
   const double w = live ? half_weight(col, N) : 0.0;
   const double sw = sqrt(w);
   const double inv2pi = 0.15915494309189535, twopi = 6.283185307179586;
-  const int64_t i0 = (int64_t)blockIdx.y * img_chunk;
-  const int64_t i1 = min(n, i0 + img_chunk);
-  for (int64_t i = i0; i < i1; ++i) {
+  // all loads of the chunk are issued before any store (CTF_IMG images per thread)
+  const int64_t i0 = (int64_t)blockIdx.y * CTF_IMG;
+  double dfv[CTF_IMG];
+  float2 Xv[CTF_IMG];
+#pragma unroll
+  for (int u = 0; u < CTF_IMG; ++u) {
+    const bool ok = i0 + u < n;
+    dfv[u] = ok ? df[i0 + u] : 0.0;
+    Xv[u] = (ok && live) ? fy[(i0 + u) * Kh + k] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int u = 0; u < CTF_IMG; ++u) {
+    const int64_t i = i0 + u;
+    if (i >= n) break;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (live) {
-      const double wr = a_k - c_k * df[i];
+      const double wr = a_k - c_k * dfv[u];
       const double red = wr - twopi * rint(wr * inv2pi);
       float sf, cf;
       sincosf((float)red, &sf, &cf);
       const double C = ((double)sf - cp.ampc * (double)cf) * env;
       ctf_half[i * Kh + k] = (float)C;
-      const float2 X = fy[i * Kh + k];
+      const float2 X = Xv[u];
       v[0] = w * ((double)X.x * X.x + (double)X.y * X.y);
       v[1] = C * C;
       v[2] = sw * C * X.x;
@@ -421,8 +476,19 @@ int memi_load_normalize(const float* stack, int64_t n_stack, const int64_t* src,
   MEMI_CHECK_ARG(stack && src && out, "memi_load_normalize: null pointer");
   MEMI_CHECK_ARG(n >= 0 && N > 0 && n_stack >= 0, "memi_load_normalize: bad sizes");
   if (n == 0) return MEMI_OK;
-  hipLaunchKernelGGL(k_load_normalize, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream,
-                     stack, src, conj, N, out);
+  hipStream_t s = (hipStream_t)stream;
+  if (N <= 64)
+    hipLaunchKernelGGL(k_load_normalize_lds<64>, dim3((unsigned)n), dim3(512), 0, s, stack, src,
+                       conj, N, out);
+  else if (N <= 141)
+    hipLaunchKernelGGL(k_load_normalize_lds<141>, dim3((unsigned)n), dim3(512), 0, s, stack,
+                       src, conj, N, out);
+  else if (N <= 200)
+    hipLaunchKernelGGL(k_load_normalize_lds<200>, dim3((unsigned)n), dim3(512), 0, s, stack,
+                       src, conj, N, out);
+  else
+    hipLaunchKernelGGL(k_load_normalize, dim3((unsigned)n), dim3(256), 0, s, stack, src, conj, N,
+                       out);
   MEMI_CHECK_LAUNCH("k_load_normalize");
   return MEMI_OK;
 }
@@ -436,11 +502,11 @@ int memi_spectrum_scale(void* spec, int64_t n, int N, int kind, double Qc, int o
                  "memi_spectrum_scale: Qc must be positive");
   const int64_t Kh = (int64_t)N * (N / 2 + 1);
   if (n == 0) return MEMI_OK;
-  const int chunk = 16;
-  dim3 g((unsigned)((Kh + 255) / 256), (unsigned)((n + chunk - 1) / chunk));
-  MEMI_CHECK_ARG(g.y < 65536u * 16, "memi_spectrum_scale: too many images");
+  MEMI_CHECK_ARG((n + SPEC_IMG - 1) / SPEC_IMG < 65536, "memi_spectrum_scale: too many images "
+                 "(at most %d per call)", 65535 * SPEC_IMG);
+  dim3 g((unsigned)((Kh + 255) / 256), (unsigned)((n + SPEC_IMG - 1) / SPEC_IMG));
   hipLaunchKernelGGL(k_spectrum_scale, g, dim3(256), 0, (hipStream_t)stream, (float2*)spec, n, N,
-                     kind, Qc, order, chunk);
+                     kind, Qc, order);
   MEMI_CHECK_LAUNCH("k_spectrum_scale");
   return MEMI_OK;
 }
@@ -503,10 +569,11 @@ int memi_ctf_pack(const void* fy, const double* df, int64_t n, int N, const memi
                  "multiple of 64", (long long)Khp, (long long)Kh);
   MEMI_CHECK_ARG(ctf->pix_size > 0, "memi_ctf_pack: pixel size must be positive");
   if (n == 0) return MEMI_OK;
-  const int chunk = 8;
-  dim3 g((unsigned)((Khp + 255) / 256), (unsigned)((n + chunk - 1) / chunk));
+  MEMI_CHECK_ARG((n + CTF_IMG - 1) / CTF_IMG < 65536, "memi_ctf_pack: too many images (at most "
+                 "%d per call)", 65535 * CTF_IMG);
+  dim3 g((unsigned)((Khp + 255) / 256), (unsigned)((n + CTF_IMG - 1) / CTF_IMG));
   hipLaunchKernelGGL(k_ctf_pack, g, dim3(256), 0, (hipStream_t)stream, (const float2*)fy, df, n, N,
-                     *ctf, precision, Khp, row0, op_hi, op_lo, ctf_half, chunk);
+                     *ctf, precision, Khp, row0, op_hi, op_lo, ctf_half);
   MEMI_CHECK_LAUNCH("k_ctf_pack");
   return MEMI_OK;
 }
