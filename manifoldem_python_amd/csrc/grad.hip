@@ -42,7 +42,11 @@ __global__ __launch_bounds__(256) void k_kernel_rowsums(const float* __restrict_
   }
 }
 
-// Wt[i][j] = sum_e g[e] (-1/(2 eps_e)) K^e_ij (1/S[e][i] + 1/S[e][j])   (K symmetric)
+// Wt[i][j] = sum_e g[e] (-1/(2 eps_e)) K^e_ij (1/S[e][i] + 1/S[e][j])   (K symmetric), and
+// Wt[i][i] = 0: the i = j term of the gradient, w_k (X_ik C_ik^2 - C_ik C_ik X_ik) Wt_ii, is
+// identically zero, while Wt_ii itself (K_ii = 1 at every eps, so the smallest eps dominates)
+// can exceed the off-diagonal weights by 1e6 or more and would only feed rounding error of
+// that size into the two float32 products Wt B and Wt Y, which cancel it in exact arithmetic.
 __global__ void k_kernel_grad_weights(const float* __restrict__ D, int n,
                                       const double* __restrict__ eps, int n_eps,
                                       const double* __restrict__ S, const double* __restrict__ g,
@@ -58,7 +62,7 @@ __global__ void k_kernel_grad_weights(const float* __restrict__ D, int n,
       const double k = exp(-d * c);
       w += g[q] * (-c) * k * (1.0 / S[(int64_t)q * n + i] + 1.0 / S[(int64_t)q * n + j]);
     }
-    Wt[e] = (float)w;
+    Wt[e] = i == j ? 0.f : (float)w;
   }
 }
 
