@@ -135,9 +135,13 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False
         torch.cuda.synchronize()
     gram_ms = [events.elapsed_ms(p) for p in pairs]
     stages = B.breakdown(batch, wsp, num_eigs, rand)
+    J = np.array([getattr(r, "lanczos_steps", 0) for r in res])
+    lanczos = dict(mean_steps=float(J.mean()), max_steps=int(J.max()),
+                   p90_steps=float(np.percentile(J, 90)),
+                   max_residual=float(max(getattr(r, "lanczos_residual", 0.0) for r in res)))
     return dict(t=t, t_dist=t_dist, gram_ms=float(np.mean(gram_ms)), wsp=wsp, dist=dist,
                 res=res, flags=int(dist.flag_count.item()), stages=stages, tuned=tuned,
-                graph=graph)
+                graph=graph, lanczos=lanczos)
 
 
 def issued_gram_flops(plan, Khp, products):
@@ -284,6 +288,7 @@ def main():
         "gram_kernel_ms": bf["gram_ms"],
         "stage_ms": bf["stages"],
         "gram_tuned": bf["tuned"],
+        "lanczos": bf["lanczos"],
         "kappa_flagged": bf["flags"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,

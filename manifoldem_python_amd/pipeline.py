@@ -318,7 +318,7 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200):
 
 class DMResult:
     __slots__ = ("lamb", "psi", "sigma", "mu", "logEps", "logSumWij", "popt", "R_squared",
-                 "slope", "V")
+                 "slope", "V", "lanczos_steps", "lanczos_residual")
 
     def as_tuple(self):
         return (self.lamb, self.psi, self.sigma, self.mu, self.logEps, self.logSumWij, self.popt,
@@ -549,7 +549,8 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
         n = int(plan.ns[p])
         nEigs = ks[p] - 1
         if eig == "lanczos":
-            lam, V, _, _ = eigs[p]
+            lam, V, r_steps, r_res = eigs[p]
+            results[p].lanczos_steps, results[p].lanczos_residual = r_steps, r_res
         else:
             lam, V = top_eigs_lm(plan.block(L, p), nEigs + 1)
             lam, V = lam.cpu().numpy(), V.cpu().numpy()
