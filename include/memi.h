@@ -238,6 +238,20 @@ int memi_lanczos_step(const void* plan_dev, const memi_plan_info* info, const do
                       double* V, int m, int j, const int32_t* active, double* W, double* H,
                       double* ab, int split, void* stream);
 
+/* Convergence check of the batched Lanczos run, the Ritz pairs eigsh(which='LM') returns
+ * (sembeddingonFly.py:27-35), entirely on the GPU.  For every active PD with
+ * J = min(steps_done, n, m): the k = ks[pd] (<= MEMI_RITZ_KMAX) eigenvalues of largest
+ * magnitude of the tridiagonal T (ab as written by memi_lanczos_step) by Sturm bisection,
+ * sorted descending, their eigenvectors by twisted factorisation + modified Gram-Schmidt, and
+ * the residuals |beta_{J-1} s_{J-1}|.  A PD is done when every residual is <= tol * max|theta|,
+ * or J = n, or J = m: then active[pd] = 0, steps[pd] = J and res[pd] = the largest residual
+ * (0 when J = n).  theta: float64 [npd][16]; S: float64 [npd][m][16] (PD p's J x k row-major
+ * block at p*m*16, ready for memi_ritz_vectors); piv: float64 scratch [npd][16][2][m]. */
+int memi_lanczos_ritz(const void* plan_dev, const memi_plan_info* info, const double* ab, int m,
+                      int steps_done, const int32_t* ks, double tol, int32_t* active,
+                      double* theta, double* S, int32_t* steps, double* res, double* piv,
+                      void* stream);
+
 /* Ritz vectors of every PD after Lanczos, the eigenvectors sembeddingonFly.py:27-35 takes
  * from eigsh: Y_p = V_p S_p with V_p the basis of memi_lanczos_step (first J columns) and S_p
  * the J x k eigenvectors of the tridiagonal T_p, row-major.  meta: int64 on the device,

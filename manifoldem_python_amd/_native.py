@@ -72,6 +72,8 @@ _SIGS = {
                           c_vp, c_vp, c_int, c_vp],
     "memi_ritz_vectors": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_vp, c_vp, c_int, c_vp,
                           c_vp],
+    "memi_lanczos_ritz": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_int, c_vp, c_dbl, c_vp,
+                          c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_nlsa_condist": [c_vp, c_int, c_vp, c_int, c_int, c_vp, c_vp],
     "memi_nlsa_supervectors": [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_vp, c_vp],
     "memi_tall_gram": [c_vp, c_i64, c_int, c_vp, c_vp],
@@ -403,6 +405,25 @@ def lanczos_step(plan, L, V, m, j, active, W, ab, H=None):
 
 
 RITZ_KMAX = 16
+
+
+def lanczos_ritz(plan, ab, m, steps_done, ks, tol, active, theta, S, steps, res, piv):
+    """GPU convergence check of every active PD (memi_lanczos_ritz); buffers as in memi.h."""
+    I = plan.info
+    K = RITZ_KMAX
+    _need(1 <= steps_done <= m and ab.dtype == torch.float64 and ab.numel() >= I.n_pd * m * 2,
+          "lanczos_ritz: ab / steps_done")
+    _need(ks.dtype == torch.int32 and ks.numel() >= I.n_pd and active.dtype == torch.int32 and
+          active.numel() >= I.n_pd and steps.dtype == torch.int32 and steps.numel() >= I.n_pd,
+          "lanczos_ritz: ks / active / steps must be int32 [npd]")
+    _need(theta.dtype == torch.float64 and theta.numel() >= I.n_pd * K and
+          S.dtype == torch.float64 and S.numel() >= I.n_pd * m * K and
+          res.dtype == torch.float64 and res.numel() >= I.n_pd and
+          piv.dtype == torch.float64 and piv.numel() >= I.n_pd * K * 2 * m,
+          "lanczos_ritz: theta / S / res / piv sizes")
+    call("memi_lanczos_ritz", _p(plan.dev), ctypes.byref(plan.info), _p(ab), int(m),
+         int(steps_done), _p(ks), float(tol), _p(active), _p(theta), _p(S), _p(steps), _p(res),
+         _p(piv), _stream())
 
 
 def ritz_vectors(plan, V, m, S, meta_host, meta, kmax, Y):

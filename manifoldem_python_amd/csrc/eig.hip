@@ -167,6 +167,195 @@ __global__ __launch_bounds__(256) void k_orth_finish(const memi_pd_desc* __restr
   }
 }
 
+// ---- convergence check: Ritz pairs of the tridiagonal T_p on the GPU -----------------------
+// One 64-thread workgroup per active PD with J = min(steps, n, m) Lanczos steps.
+//  1. candidates: the k smallest and k largest eigenvalues of T (all J when J <= 2k), each by
+//     Sturm-count bisection on the Gershgorin interval (one thread per eigenvalue);
+//  2. the k of largest |theta| (first of equal magnitudes in candidate order), sorted by
+//     descending theta -- the pairs eigsh(which='LM') returns;
+//  3. their eigenvectors by the twisted factorisation of T - theta I (forward and backward
+//     pivots d+, d-, twist r = argmin |gamma_r|, then the two recurrences from z_r = 1),
+//     normalised, then modified Gram-Schmidt in descending-theta order;
+//  4. residuals |beta_{J-1} s_{J-1}|; the PD is done when all are <= tol * max|theta|, or
+//     J = n (the whole space), or J = m.  Done PDs get active = 0, steps = J, res = the largest
+//     residual (0 when J = n); theta [npd][16] and S (row-major J x k at pd*m*16) stay as the
+//     final Ritz pairs.
+// piv: float64 scratch [npd][16][2][m].
+__device__ inline int sturm_count(const double* ab, int J, double x, double pivmin) {
+  double q = ab[0] - x;
+  int c = q < 0.0;
+  for (int i = 1; i < J; ++i) {
+    if (fabs(q) < pivmin) q = -pivmin;
+    const double b = ab[2 * (i - 1) + 1];
+    q = ab[2 * i] - x - b * b / q;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restrict__ pds,
+                                                     const double* __restrict__ ab, int m,
+                                                     int steps_done,
+                                                     const int32_t* __restrict__ ks, double tol,
+                                                     int32_t* __restrict__ active,
+                                                     double* __restrict__ theta,
+                                                     double* __restrict__ S,
+                                                     int32_t* __restrict__ steps,
+                                                     double* __restrict__ res_out,
+                                                     double* __restrict__ piv) {
+  constexpr int KM = MEMI_RITZ_KMAX;
+  __shared__ double cand[2 * KM];
+  __shared__ double sel[KM];
+  __shared__ double resv[KM];
+  __shared__ double red[64];
+  const int pd = blockIdx.x;
+  if (!active[pd]) return;
+  const int t = threadIdx.x;
+  const int n = pds[pd].n;
+  const int J = min(steps_done, min(n, m));
+  const int k = min(ks[pd], J);
+  const double* a = ab + (int64_t)pd * m * 2;      // a[2i] = alpha_i, a[2i+1] = beta_i
+  // Gershgorin interval and pivot floor
+  double lo = INFINITY, hi = -INFINITY, bmax = 0.0;
+  for (int i = t; i < J; i += 64) {
+    const double bl = i > 0 ? fabs(a[2 * (i - 1) + 1]) : 0.0;
+    const double br = i + 1 < J ? fabs(a[2 * i + 1]) : 0.0;
+    lo = fmin(lo, a[2 * i] - bl - br);
+    hi = fmax(hi, a[2 * i] + bl + br);
+    bmax = fmax(bmax, br);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, off, 64));
+    hi = fmax(hi, __shfl_xor(hi, off, 64));
+    bmax = fmax(bmax, __shfl_xor(bmax, off, 64));
+  }
+  const double span = fmax(hi - lo, 1e-300);
+  lo -= 1e-14 * span;
+  hi += 1e-14 * span;
+  const double pivmin = fmax(1e-290, 1e-300 * bmax * bmax);
+  // 1. candidate eigenvalues
+  const int nc = J <= 2 * k ? J : 2 * k;
+  if (t < nc) {
+    const int idx = J <= 2 * k ? t : (t < k ? t : J - 2 * k + t);
+    double l = lo, h = hi;
+    for (int it = 0; it < 200 && h - l > 2.2e-16 * fmax(fabs(l), fabs(h)) + 1e-300; ++it) {
+      const double mid = 0.5 * (l + h);
+      if (mid <= l || mid >= h) break;
+      if (sturm_count(a, J, mid, pivmin) > idx) h = mid;
+      else l = mid;
+    }
+    cand[t] = 0.5 * (l + h);
+  }
+  __syncthreads();
+  // 2. selection: k largest |theta|, then descending theta
+  if (t == 0) {
+    bool used[2 * KM];
+    for (int c = 0; c < nc; ++c) used[c] = false;
+    for (int q = 0; q < k; ++q) {
+      int best = -1;
+      for (int c = 0; c < nc; ++c)
+        if (!used[c] && (best < 0 || fabs(cand[c]) > fabs(cand[best]))) best = c;
+      used[best] = true;
+      sel[q] = cand[best];
+    }
+    for (int q = 1; q < k; ++q) {            // insertion sort, descending
+      const double v = sel[q];
+      int r = q - 1;
+      while (r >= 0 && sel[r] < v) {
+        sel[r + 1] = sel[r];
+        --r;
+      }
+      sel[r + 1] = v;
+    }
+  }
+  __syncthreads();
+  // 3. eigenvectors by twisted factorisation (one thread per selected pair)
+  double* Sp = S + (int64_t)pd * m * KM;          // row-major J x k
+  if (t < k) {
+    const double th = sel[t];
+    double* dp = piv + (((int64_t)pd * KM + t) * 2) * m;
+    double* dm = dp + m;
+    double q = a[0] - th;
+    dp[0] = q;
+    for (int i = 1; i < J; ++i) {
+      if (fabs(q) < pivmin) q = -pivmin;
+      const double b = a[2 * (i - 1) + 1];
+      q = a[2 * i] - th - b * b / q;
+      dp[i] = q;
+    }
+    q = a[2 * (J - 1)] - th;
+    dm[J - 1] = q;
+    for (int i = J - 2; i >= 0; --i) {
+      if (fabs(q) < pivmin) q = -pivmin;
+      const double b = a[2 * i + 1];
+      q = a[2 * i] - th - b * b / q;
+      dm[i] = q;
+    }
+    int r = 0;
+    double gbest = INFINITY;
+    for (int i = 0; i < J; ++i) {
+      const double g = fabs(dp[i] + dm[i] - (a[2 * i] - th));
+      if (g < gbest) {
+        gbest = g;
+        r = i;
+      }
+    }
+    double zz = 1.0, nrm = 1.0;
+    Sp[(int64_t)r * k + t] = 1.0;
+    for (int i = r - 1; i >= 0; --i) {
+      const double d = fabs(dp[i]) < pivmin ? -pivmin : dp[i];
+      zz = -(a[2 * i + 1] / d) * zz;
+      Sp[(int64_t)i * k + t] = zz;
+      nrm += zz * zz;
+    }
+    zz = 1.0;
+    for (int i = r; i + 1 < J; ++i) {
+      const double d = fabs(dm[i + 1]) < pivmin ? -pivmin : dm[i + 1];
+      zz = -(a[2 * i + 1] / d) * zz;
+      Sp[(int64_t)(i + 1) * k + t] = zz;
+      nrm += zz * zz;
+    }
+    const double inv = 1.0 / sqrt(nrm);
+    for (int i = 0; i < J; ++i) Sp[(int64_t)i * k + t] *= inv;
+  }
+  __syncthreads();
+  // modified Gram-Schmidt over the k vectors (all 64 threads per dot product)
+  for (int c = 1; c < k; ++c) {
+    for (int b = 0; b < c; ++b) {
+      double d = 0.0;
+      for (int i = t; i < J; i += 64) d += Sp[(int64_t)i * k + c] * Sp[(int64_t)i * k + b];
+      for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+      for (int i = t; i < J; i += 64) Sp[(int64_t)i * k + c] -= d * Sp[(int64_t)i * k + b];
+      __syncthreads();
+    }
+    double nn = 0.0;
+    for (int i = t; i < J; i += 64) nn += Sp[(int64_t)i * k + c] * Sp[(int64_t)i * k + c];
+    for (int off = 32; off > 0; off >>= 1) nn += __shfl_xor(nn, off, 64);
+    const double inv = 1.0 / sqrt(nn);
+    for (int i = t; i < J; i += 64) Sp[(int64_t)i * k + c] *= inv;
+    __syncthreads();
+  }
+  // 4. residuals and the decision
+  const double bJ = a[2 * (J - 1) + 1];
+  if (t < k) resv[t] = fabs(bJ * Sp[(int64_t)(J - 1) * k + t]);
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0, rmax = 0.0;
+    for (int q = 0; q < k; ++q) {
+      mx = fmax(mx, fabs(sel[q]));
+      rmax = fmax(rmax, resv[q]);
+    }
+    const bool done = J >= n || J >= m || rmax <= tol * mx;
+    for (int q = 0; q < KM; ++q) theta[(int64_t)pd * KM + q] = q < k ? sel[q] : 0.0;
+    if (done) {
+      active[pd] = 0;
+      steps[pd] = J;
+      res_out[pd] = J >= n ? 0.0 : rmax;
+    }
+  }
+  (void)red;
+}
+
 // Ritz vectors Y_p = V_p S_p of every PD in one launch, one thread per row: V_p the n x J
 // Lanczos basis (column-major at m*row0), S_p the J x k eigenvectors of T_p (row-major at
 // meta.s_off), Y [n_rows][kmax] (columns k..kmax-1 zero).  meta: int64 [npd][3] = {J, k, s_off}.
@@ -258,6 +447,23 @@ int memi_ritz_vectors(const void* plan_dev, const memi_plan_info* info, const do
   hipLaunchKernelGGL(k_ritz_vectors, dim3((unsigned)((info->n_rows + 255) / 256)), dim3(256), 0,
                      s, rows, info->n_rows, pds, V, m, S, meta, kmax, Y);
   MEMI_CHECK_LAUNCH("k_ritz_vectors");
+  return MEMI_OK;
+}
+
+int memi_lanczos_ritz(const void* plan_dev, const memi_plan_info* info, const double* ab, int m,
+                      int steps_done, const int32_t* ks, double tol, int32_t* active,
+                      double* theta, double* S, int32_t* steps, double* res, double* piv,
+                      void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && ab && ks && active && theta && S && steps && res && piv,
+                 "memi_lanczos_ritz: null pointer");
+  MEMI_CHECK_ARG(m >= 2 && m <= 4096 && steps_done >= 1 && steps_done <= m && tol >= 0,
+                 "memi_lanczos_ritz: bad m=%d steps=%d", m, steps_done);
+  if (info->n_pd == 0) return MEMI_OK;
+  const char* plan = (const char*)plan_dev;
+  const memi_pd_desc* pds = (const memi_pd_desc*)(plan + info->off_pds);
+  hipLaunchKernelGGL(k_lanczos_ritz, dim3((unsigned)info->n_pd), dim3(64), 0, (hipStream_t)stream,
+                     pds, ab, m, steps_done, ks, tol, active, theta, S, steps, res, piv);
+  MEMI_CHECK_LAUNCH("k_lanczos_ritz");
   return MEMI_OK;
 }
 

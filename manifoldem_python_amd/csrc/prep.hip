@@ -480,6 +480,9 @@ int memi_load_normalize(const float* stack, int64_t n_stack, const int64_t* src,
   if (N <= 64)
     hipLaunchKernelGGL(k_load_normalize_lds<64>, dim3((unsigned)n), dim3(512), 0, s, stack, src,
                        conj, N, out);
+  else if (N <= 96)
+    hipLaunchKernelGGL(k_load_normalize_lds<96>, dim3((unsigned)n), dim3(512), 0, s, stack, src,
+                       conj, N, out);
   else if (N <= 141)
     hipLaunchKernelGGL(k_load_normalize_lds<141>, dim3((unsigned)n), dim3(512), 0, s, stack,
                        src, conj, N, out);
@@ -545,6 +548,8 @@ int memi_align_images(const float* in, const double* rot1_deg, const double* rot
   const dim3 g((unsigned)n), b(512);
   if (N <= 64) {
     hipLaunchKernelGGL((k_align_images<64, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
+  } else if (N <= 96) {          // 74.5 KB of LDS: two blocks per CU
+    hipLaunchKernelGGL((k_align_images<96, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
   } else if (N <= 141) {
     hipLaunchKernelGGL((k_align_images<141, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
   } else {

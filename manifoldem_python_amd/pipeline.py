@@ -355,7 +355,8 @@ def lm_ritz(alpha, beta, k):
     return theta[sel], S[:, sel]
 
 
-def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202, split=None):
+def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202, split=None,
+                 check="gpu"):
     """Largest-magnitude eigenpairs of every PD's float64 kernel block of L (sembeddingonFly.py
     semantics: the k eigenpairs of largest |lambda|, sorted by descending lambda) by batched
     Lanczos with full reorthogonalisation on the GPU.
@@ -363,7 +364,10 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202, 
     Converged when every selected Ritz pair has residual |beta_J s_J| <= tol * max|theta| or
     the Krylov space is the whole space.  split: orthogonalise with the whole grid (default
     for batches of fewer than 128 PDs or with a PD above 1024 rows) instead of one workgroup
-    per PD.  Returns a list of (lam, V (n x k) numpy, steps, max residual)."""
+    per PD.  check: "gpu" (memi_lanczos_ritz: Ritz pairs and the decision on the device, one
+    int per PD read back per check, every 16 steps by default) or "host" (scipy tridiagonal
+    eigensolver per PD, every 32 steps by default).  Returns a list of
+    (lam, V (n x k) numpy, steps, max residual)."""
     dev = L.device
     npd = plan.npd
     ns = plan.ns.astype(int)
@@ -374,6 +378,10 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202, 
     m = int(max(2, min(m, ns.max())))
     if split is None:
         split = npd < 128 or int(ns.max()) > 1024
+    if check_every is None:
+        check_every = 16 if check == "gpu" else 32
+    if max(ks) > nat.RITZ_KMAX:
+        raise ValueError(f"lanczos_eigs: at most {nat.RITZ_KMAX} eigenpairs per PD")
     V = torch.zeros(m * n_rows, dtype=torch.float64, device=dev)
     W = torch.empty(n_rows, dtype=torch.float64, device=dev)
     H = torch.empty(2 * npd * m, dtype=torch.float64, device=dev) if split else None
@@ -389,6 +397,14 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202, 
     V[torch.as_tensor(idx, device=dev)] = torch.as_tensor(v0, device=dev)
     active_h = np.ones(npd, dtype=np.int32)
     active = torch.as_tensor(active_h, device=dev)
+    K = nat.RITZ_KMAX
+    if check == "gpu":
+        ks_t = torch.as_tensor(np.asarray(ks, dtype=np.int32), device=dev)
+        theta = torch.zeros(npd * K, dtype=torch.float64, device=dev)
+        S_dev = torch.zeros(npd * m * K, dtype=torch.float64, device=dev)
+        steps_t = torch.zeros(npd, dtype=torch.int32, device=dev)
+        res_t = torch.zeros(npd, dtype=torch.float64, device=dev)
+        piv = torch.empty(npd * K * 2 * m, dtype=torch.float64, device=dev)
     steps = np.minimum(ns, m)
     out = [None] * npd
     j = 0
@@ -397,50 +413,64 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=32, seed=20261202, 
         while j < stop:
             nat.lanczos_step(plan, L, V, m, j, active, W, ab, H=H)
             j += 1
+        if check == "gpu":
+            nat.lanczos_ritz(plan, ab, m, j, ks_t, tol, active, theta, S_dev, steps_t, res_t,
+                             piv)
+            active_h = active.cpu().numpy()
+            continue
         abh = ab.view(npd, m, 2).cpu().numpy()
 
-        def check(p):
+        def check_host(p):
             J = int(min(j, steps[p]))
             alpha, beta = abh[p, :J, 0], abh[p, :J, 1]
-            theta, S = lm_ritz(alpha, beta[:J - 1], ks[p])
+            theta_p, S = lm_ritz(alpha, beta[:J - 1], ks[p])
             res = np.abs(beta[J - 1] * S[J - 1])
-            done = (J >= ns[p]) or np.all(res <= tol * np.max(np.abs(theta))) or (J >= m)
+            done = (J >= ns[p]) or np.all(res <= tol * np.max(np.abs(theta_p))) or (J >= m)
             if done:
-                order = np.argsort(-theta, kind="stable")
-                return p, (theta[order], S[:, order], J,
+                order = np.argsort(-theta_p, kind="stable")
+                return p, (theta_p[order], S[:, order], J,
                            float(np.max(res[order])) if J < ns[p] else 0.0)
             return p, None
 
         todo = np.nonzero(active_h)[0]
         if len(todo) > 8:
-            results_c = list(_pool().map(check, todo))
+            results_c = list(_pool().map(check_host, todo))
         else:
-            results_c = [check(p) for p in todo]
+            results_c = [check_host(p) for p in todo]
         for p, r in results_c:
             if r is not None:
                 out[p] = r
                 active_h[p] = 0
         active.copy_(torch.as_tensor(active_h, device=dev))
     # Ritz vectors of all PDs in one launch and one copy to the host
-    kmax = int(max(out[p][1].shape[1] for p in range(npd)))
-    if kmax > nat.RITZ_KMAX:
-        raise ValueError(f"lanczos_eigs: at most {nat.RITZ_KMAX} eigenpairs per PD")
-    meta_h = np.zeros((npd, 3), dtype=np.int64)
-    blocks, off = [], 0
-    for p in range(npd):
-        S = np.ascontiguousarray(out[p][1], dtype=np.float64)
-        meta_h[p] = (out[p][2], S.shape[1], off)
-        blocks.append(S.ravel())
-        off += S.size
-    S_all = torch.as_tensor(np.concatenate(blocks), device=dev)
+    if check == "gpu":
+        J_h = steps_t.cpu().numpy().astype(np.int64)
+        k_h = np.minimum(np.asarray(ks, dtype=np.int64), J_h)
+        meta_h = np.stack([J_h, k_h, np.arange(npd, dtype=np.int64) * m * K], axis=1)
+        S_all = S_dev
+        lam_h = theta.view(npd, K).cpu().numpy()
+        res_h = res_t.cpu().numpy()
+        for p in range(npd):
+            out[p] = (lam_h[p, :k_h[p]].copy(), None, int(J_h[p]), float(res_h[p]))
+    else:
+        meta_h = np.zeros((npd, 3), dtype=np.int64)
+        blocks, off = [], 0
+        for p in range(npd):
+            S = np.ascontiguousarray(out[p][1], dtype=np.float64)
+            meta_h[p] = (out[p][2], S.shape[1], off)
+            blocks.append(S.ravel())
+            off += S.size
+        S_all = torch.as_tensor(np.concatenate(blocks), device=dev)
+    kmax = int(meta_h[:, 1].max())
     meta = torch.as_tensor(meta_h, device=dev)
     Y = torch.empty(n_rows * kmax, dtype=torch.float64, device=dev)
     nat.ritz_vectors(plan, V, m, S_all, meta_h, meta, kmax, Y)
     Yh = Y.view(n_rows, kmax).cpu().numpy()
     results = []
     for p in range(npd):
-        lam, S, J, res = out[p]
-        results.append((lam, Yh[rows0[p]:rows0[p] + ns[p], :S.shape[1]], J, res))
+        lam, _, J, res = out[p]
+        k = int(meta_h[p, 1])
+        results.append((lam, Yh[rows0[p]:rows0[p] + ns[p], :k], J, res))
     return results
 
 

@@ -112,8 +112,8 @@ def test_ferguson_slope_matches_finite_differences(cuda):
     assert np.max(np.abs(np.log(S[0]) - np.array([logS(le) for le in logEps]))) < 1e-6
 
 
-@pytest.mark.parametrize("split", [False, True])
-def test_batched_lanczos_matches_dense(cuda, split):
+@pytest.mark.parametrize("split,check", [(False, "gpu"), (True, "gpu"), (False, "host")])
+def test_batched_lanczos_matches_dense(cuda, split, check):
     """Three PDs of ragged sizes in one plan: GPU Lanczos eigenpairs (largest magnitude)
     against numpy eigh of each normalised kernel block."""
     import torch
@@ -130,7 +130,7 @@ def test_batched_lanczos_matches_dense(cuda, split):
     deg = torch.empty(2 * plan.info.n_rows, dtype=torch.float64, device=cuda)
     nat.markov_normalize(plan, flat, mask, sigma, L, deg)
     kk = [12, 16, 9]
-    res = pl.lanczos_eigs(plan, L, kk, split=split)
+    res = pl.lanczos_eigs(plan, L, kk, split=split, check=check)
     for p in range(3):
         A = plan.block(L, p).cpu().numpy()
         assert np.allclose(A, A.T)
