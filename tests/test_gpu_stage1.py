@@ -90,7 +90,7 @@ def test_spline_rotate_matches_ndimage(cuda, N):
     assert err < 2e-6, err
 
 
-@pytest.mark.parametrize("N", [40, 45, 128, 160])
+@pytest.mark.parametrize("N", [40, 45, 96, 128, 160])
 def test_fused_alignment_matches_ndimage(cuda, N):
     """memi_align_images (disk mask, then two rotations, recursive B-spline prefilter in LDS)
     against the oracle's rotatefill(rotatefill(img * mask, a1), a2) -- scipy.ndimage.rotate
