@@ -213,33 +213,57 @@ __global__ __launch_bounds__(256) void k_spline_rotate(const float* __restrict__
 constexpr double BSPLINE_Z = -0.2679491924311227;   // sqrt(3) - 2
 constexpr int BSPLINE_NT = 24;
 
+// Each line is cut into nseg segments handled by one thread each (N lines x nseg <= block):
+// every segment starts from its own truncated periodic sum, so the recursions are short and
+// independent.  Phases: causal starts (read only), causal recursion, anticausal starts (read
+// the finished c+), anticausal recursion; the block synchronises between phases.
 template <bool ROWS>
 __device__ inline void bspline_lines(float* buf, int N, int S, double zN) {
   const int es = ROWS ? 1 : S;
   const int nt = min(N, BSPLINE_NT);
   const double z = BSPLINE_Z;
-  for (int l = threadIdx.x; l < N; l += blockDim.x) {
-    float* x = buf + (ROWS ? l * S : l);
-    double c0 = 0.0, zj = 1.0;
+  const int nseg = max(1, min((int)blockDim.x / N, (N + 7) / 8));
+  const int Ls = (N + nseg - 1) / nseg;
+  const int t = threadIdx.x;
+  const int l = t % N, sg = t / N;              // consecutive threads: consecutive lines
+  const int a = sg * Ls, e = min(a + Ls, N);
+  const bool act = t < N * nseg && a < N;
+  float* x = buf + (ROWS ? l * S : l);
+  double c0 = 0.0;
+  if (act) {
+    double zj = 1.0;
     for (int j = 0; j < nt; ++j) {
-      c0 += zj * (double)x[(j == 0 ? 0 : N - j) * es];
+      const int i = a - j;
+      c0 += zj * (double)x[(i < 0 ? i + N : i) * es];
       zj *= z;
     }
-    double prev = c0 / (1.0 - zN);
-    x[0] = (float)prev;
-    for (int k = 1; k < N; ++k) {
+    c0 /= 1.0 - zN;
+  }
+  __syncthreads();
+  if (act) {
+    double prev = c0;
+    x[a * es] = (float)prev;
+    for (int k = a + 1; k < e; ++k) {
       prev = (double)x[k * es] + z * prev;
       x[k * es] = (float)prev;
     }
-    double cm = 0.0;
-    zj = 1.0;
+  }
+  __syncthreads();
+  double cm = 0.0;
+  if (act) {
+    double zj = 1.0;
     for (int j = 0; j < nt; ++j) {
-      cm += zj * (double)x[(j == 0 ? N - 1 : j - 1) * es];
+      const int i = e - 1 + j;
+      cm += zj * (double)x[(i >= N ? i - N : i) * es];
       zj *= z;
     }
-    double nxt = -z / (1.0 - zN) * cm;
-    x[(N - 1) * es] = (float)(6.0 * nxt);
-    for (int k = N - 2; k >= 0; --k) {
+    cm *= -z / (1.0 - zN);
+  }
+  __syncthreads();
+  if (act) {
+    double nxt = cm;
+    x[(e - 1) * es] = (float)(6.0 * nxt);
+    for (int k = e - 2; k >= a; --k) {
       nxt = z * (nxt - (double)x[k * es]);
       x[k * es] = (float)(6.0 * nxt);
     }

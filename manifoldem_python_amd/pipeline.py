@@ -287,30 +287,38 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200):
     ss_tot = np.sum((logS - logS.mean(axis=1, keepdims=True)) ** 2, axis=1)
     out = torch.empty(npd * 16, dtype=torch.float64, device=dev)
     best = np.full(npd, np.inf)
+    iu = np.triu_indices(4)
     for _ in range(max_restarts):
         act = torch.as_tensor(todo.astype(np.int32), device=dev)
         nat.tanh_fit(x, y, torch.as_tensor(p0, device=dev), out, act)
         o = out.cpu().numpy().reshape(npd, 16)
-        for p in np.nonzero(todo)[0]:
-            a = o[p, 0:4]
-            J = np.zeros((4, 4))
-            J[np.triu_indices(4)] = o[p, 4:14]
-            J = J + np.triu(J, 1).T
-            ss_res = o[p, 14]
+        idx = np.nonzero(todo)[0]
+        oo = o[idx]
+        J = np.zeros((len(idx), 4, 4))
+        J[:, iu[0], iu[1]] = oo[:, 4:14]
+        J = J + np.triu(J, 1).transpose(0, 2, 1)
+        ss_res = oo[:, 14]
+        with np.errstate(all="ignore"):
             try:
-                pcov = np.linalg.inv(J) * (ss_res / (m - 4))
-                resnorm = np.sum(np.sqrt(np.fabs(np.diag(pcov))))
-            except np.linalg.LinAlgError:
-                resnorm = np.inf
-            if not np.isfinite(resnorm):
-                resnorm = np.inf
+                Jinv = np.linalg.inv(J)
+            except np.linalg.LinAlgError:        # a singular J^T J somewhere: one at a time
+                Jinv = np.full_like(J, np.nan)
+                for q in range(len(idx)):
+                    try:
+                        Jinv[q] = np.linalg.inv(J[q])
+                    except np.linalg.LinAlgError:
+                        pass
+            diag = np.diagonal(Jinv, axis1=1, axis2=2) * (ss_res / (m - 4))[:, None]
+            resnorm = np.sum(np.sqrt(np.fabs(diag)), axis=1)
+        resnorm[~np.isfinite(resnorm)] = np.inf
+        for p in idx:                                # fergusonE.py:57: a new start after each fit
             p0[p] = np.ravel(1 * (rand(4, 1) - 0.5))
-            if resnorm < best[p]:
-                best[p] = resnorm
-                popt[p] = a
-                R2[p] = 1 - ss_res / ss_tot[p]
-            if resnorm <= 100:
-                todo[p] = False
+        better = resnorm < best[idx]
+        bi = idx[better]
+        best[bi] = resnorm[better]
+        popt[bi] = oo[better, 0:4]
+        R2[bi] = 1 - ss_res[better] / ss_tot[bi]
+        todo[idx[resnorm <= 100]] = False
         if not todo.any():
             break
     return popt, R2
