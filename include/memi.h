@@ -227,16 +227,19 @@ int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, i
                     int64_t nS, double sigma, double alpha, double* out, double* deg,
                     void* stream);
 
-/* One Lanczos step (full reorthogonalisation, CGS2) on every active PD's L, for the
+/* nsteps Lanczos steps (full reorthogonalisation, CGS2) on every active PD's L, for the
  * largest-magnitude eigenpairs that sembeddingonFly.py:27 asks ARPACK eigsh for.
  * V: float64 bases, column-major per PD at m*row0 (column 0 = unit start vector, set by the
  * caller); W: float64 [n_rows] scratch; ab: float64 [npd][m][2] gets {alpha_j, beta_j};
- * active: int32 [npd] (0 = skip).  Step j reads column j and writes column j+1.
+ * active: int32 [npd] (0 = skip).  The step index j lives in device memory (*j_dev, int32,
+ * zero before the first step): step j reads column j, writes column j+1 and advances *j_dev,
+ * and steps with j >= min(n, m) do nothing -- so the launches of a block of steps can be
+ * captured once in a HIP graph and replayed.
  * split = 0: one workgroup per PD orthogonalises (H unused, may be NULL); split = 1: the
  * orthogonalisation is spread over the grid (for few large PDs), H: float64 [2*npd*m]. */
-int memi_lanczos_step(const void* plan_dev, const memi_plan_info* info, const double* L,
-                      double* V, int m, int j, const int32_t* active, double* W, double* H,
-                      double* ab, int split, void* stream);
+int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const double* L,
+                       double* V, int m, int32_t* j_dev, int nsteps, const int32_t* active,
+                       double* W, double* H, double* ab, int split, void* stream);
 
 /* Convergence check of the batched Lanczos run, the Ritz pairs eigsh(which='LM') returns
  * (sembeddingonFly.py:27-35), entirely on the GPU.  For every active PD with

@@ -68,8 +68,8 @@ _SIGS = {
     "memi_tanh_fit": [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_ferguson_values": [c_vp, c_i64, c_vp, c_int, c_vp, c_vp],
     "memi_markov_coo": [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_vp],
-    "memi_lanczos_step": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_int, c_int, c_vp, c_vp,
-                          c_vp, c_vp, c_int, c_vp],
+    "memi_lanczos_steps": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_int, c_vp, c_int, c_vp,
+                           c_vp, c_vp, c_vp, c_int, c_vp],
     "memi_ritz_vectors": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_vp, c_vp, c_int, c_vp,
                           c_vp],
     "memi_lanczos_ritz": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_int, c_vp, c_dbl, c_vp,
@@ -391,17 +391,20 @@ def markov_coo(row, col, val, nS, sigma, alpha, out, deg):
          float(alpha), _p(out), _p(deg), _stream())
 
 
-def lanczos_step(plan, L, V, m, j, active, W, ab, H=None):
-    """H: float64 [2*npd*m] scratch selects the grid-parallel orthogonalisation; None selects
-    one workgroup per PD."""
+def lanczos_steps(plan, L, V, m, j_dev, nsteps, active, W, ab, H=None):
+    """nsteps Lanczos steps from the step index held in j_dev (int32 [1] on the device).  H:
+    float64 [2*npd*m] scratch selects the grid-parallel orthogonalisation; None selects one
+    workgroup per PD."""
     I = plan.info
-    _need(L.dtype == torch.float64 and L.numel() >= I.d_elems, "lanczos_step: L too small")
+    _need(L.dtype == torch.float64 and L.numel() >= I.d_elems, "lanczos_steps: L too small")
     _need(V.numel() >= m * I.n_rows and W.numel() >= I.n_rows and ab.numel() >= I.n_pd * m * 2
-          and active.numel() >= I.n_pd and active.dtype == torch.int32, "lanczos_step: buffers")
+          and active.numel() >= I.n_pd and active.dtype == torch.int32, "lanczos_steps: buffers")
+    _need(j_dev.dtype == torch.int32 and j_dev.numel() >= 1, "lanczos_steps: j_dev int32 [1]")
     _need(H is None or (H.dtype == torch.float64 and H.numel() >= 2 * I.n_pd * m),
-          "lanczos_step: H must be float64 with 2*npd*m elements")
-    call("memi_lanczos_step", _p(plan.dev), ctypes.byref(plan.info), _p(L), _p(V), int(m), int(j),
-         _p(active), _p(W), None if H is None else _p(H), _p(ab), int(H is not None), _stream())
+          "lanczos_steps: H must be float64 with 2*npd*m elements")
+    call("memi_lanczos_steps", _p(plan.dev), ctypes.byref(plan.info), _p(L), _p(V), int(m),
+         _p(j_dev), int(nsteps), _p(active), _p(W), None if H is None else _p(H), _p(ab),
+         int(H is not None), _stream())
 
 
 RITZ_KMAX = 16

@@ -21,8 +21,9 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restric
                                                         const memi_pd_desc* __restrict__ pds,
                                                         const double* __restrict__ L,
                                                         const double* __restrict__ V, int m,
-                                                        int j, const int32_t* __restrict__ active,
+                                                        const int32_t* __restrict__ jp, const int32_t* __restrict__ active,
                                                         double* __restrict__ W) {
+  const int j = *jp;
   const int lane = threadIdx.x & 63;
   const int64_t gr = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // one wave per row
   if (gr >= n_rows) return;
@@ -41,10 +42,11 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restric
 
 // one block per PD: CGS2 against columns 0..j, then normalise into column j+1
 __global__ __launch_bounds__(256) void k_lanczos_orth(const memi_pd_desc* __restrict__ pds,
-                                                      double* __restrict__ V, int m, int j,
+                                                      double* __restrict__ V, int m, const int32_t* __restrict__ jp,
                                                       const int32_t* __restrict__ active,
                                                       double* __restrict__ W,
                                                       double* __restrict__ ab) {
+  const int j = *jp;
   extern __shared__ double h[];  // m coefficients
   __shared__ double red[8];
   const int pd = blockIdx.x;
@@ -89,15 +91,22 @@ __global__ __launch_bounds__(256) void k_lanczos_orth(const memi_pd_desc* __rest
   }
 }
 
+// the step counter lives in device memory so that a block of steps can be captured once in a
+// HIP graph and replayed; this advances it after each step (one thread, one vector store)
+__global__ void k_lanczos_advance(int32_t* __restrict__ jp) {
+  if (threadIdx.x == 0) jp[0] = jp[0] + 1;
+}
+
 // Grid-parallel CGS2 for large PDs (one workgroup per PD leaves most CUs idle when a batch
 // has few PDs): per pass, k_orth_dot (one wave per (PD, basis column)) then k_orth_update
 // (64 rows x 4 column groups per workgroup); k_orth_finish records alpha, beta and writes
 // v_{j+1}.  H: float64 [2][npd][m] coefficients of the two passes.
 __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict__ pds,
-                                                  const double* __restrict__ V, int m, int j,
+                                                  const double* __restrict__ V, int m, const int32_t* __restrict__ jp,
                                                   const int32_t* __restrict__ active,
                                                   const double* __restrict__ W,
                                                   double* __restrict__ H) {
+  const int j = *jp;
   const int pd = blockIdx.y;
   if (!active[pd]) return;
   const memi_pd_desc P = pds[pd];
@@ -115,10 +124,11 @@ __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict
 }
 
 __global__ __launch_bounds__(256) void k_orth_update(const memi_pd_desc* __restrict__ pds,
-                                                     const double* __restrict__ V, int m, int j,
+                                                     const double* __restrict__ V, int m, const int32_t* __restrict__ jp,
                                                      const int32_t* __restrict__ active,
                                                      double* __restrict__ W,
                                                      const double* __restrict__ H) {
+  const int j = *jp;
   __shared__ double part[4][64];
   const int pd = blockIdx.y;
   if (!active[pd]) return;
@@ -141,11 +151,12 @@ __global__ __launch_bounds__(256) void k_orth_update(const memi_pd_desc* __restr
 }
 
 __global__ __launch_bounds__(256) void k_orth_finish(const memi_pd_desc* __restrict__ pds,
-                                                     double* __restrict__ V, int m, int j,
+                                                     double* __restrict__ V, int m, const int32_t* __restrict__ jp,
                                                      const int32_t* __restrict__ active,
                                                      const double* __restrict__ W,
                                                      const double* __restrict__ H, int64_t npd,
                                                      double* __restrict__ ab) {
+  const int j = *jp;
   __shared__ double red[8];
   const int pd = blockIdx.x;
   if (!active[pd]) return;
@@ -394,43 +405,48 @@ using namespace memi;
 
 extern "C" {
 
-int memi_lanczos_step(const void* plan_dev, const memi_plan_info* info, const double* L,
-                      double* V, int m, int j, const int32_t* active, double* W, double* H,
-                      double* ab, int split, void* stream) {
-  MEMI_CHECK_ARG(plan_dev && info && L && V && active && W && ab && (!split || H),
-                 "memi_lanczos_step: null");
-  MEMI_CHECK_ARG(m >= 2 && m <= 4096 && j >= 0 && j < m, "memi_lanczos_step: bad m=%d j=%d", m,
-                 j);
+int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const double* L,
+                       double* V, int m, int32_t* j_dev, int nsteps, const int32_t* active,
+                       double* W, double* H, double* ab, int split, void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && L && V && j_dev && active && W && ab && (!split || H),
+                 "memi_lanczos_steps: null pointer");
+  MEMI_CHECK_ARG(m >= 2 && m <= 4096 && nsteps >= 0, "memi_lanczos_steps: bad m=%d nsteps=%d",
+                 m, nsteps);
   MEMI_CHECK_ARG(!split || (info->n_pd <= 65535 && info->max_n <= 65535 * 64),
-                 "memi_lanczos_step: batch too large for the split orthogonalisation");
-  if (info->n_pd == 0) return MEMI_OK;
+                 "memi_lanczos_steps: batch too large for the split orthogonalisation");
+  if (info->n_pd == 0 || nsteps == 0) return MEMI_OK;
   hipStream_t s = (hipStream_t)stream;
   const char* plan = (const char*)plan_dev;
   const memi_pd_desc* pds = (const memi_pd_desc*)(plan + info->off_pds);
   const int32_t* rows = (const int32_t*)(plan + info->off_rows);
-  const int64_t waves = info->n_rows;
-  hipLaunchKernelGGL(k_lanczos_matvec, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, rows,
-                     info->n_rows, pds, L, V, m, j, active, W);
-  MEMI_CHECK_LAUNCH("k_lanczos_matvec");
-  if (!split) {
-    hipLaunchKernelGGL(k_lanczos_orth, dim3((unsigned)info->n_pd), dim3(256),
-                       (size_t)m * sizeof(double), s, pds, V, m, j, active, W, ab);
-    MEMI_CHECK_LAUNCH("k_lanczos_orth");
-    return MEMI_OK;
-  }
   const unsigned npd = (unsigned)info->n_pd;
-  for (int pass = 0; pass < 2; ++pass) {
-    double* Hp = H + (int64_t)pass * info->n_pd * m;
-    hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)((j + 1 + 3) / 4), npd), dim3(256), 0, s, pds,
-                       V, m, j, active, W, Hp);
-    MEMI_CHECK_LAUNCH("k_orth_dot");
-    hipLaunchKernelGGL(k_orth_update, dim3((unsigned)((info->max_n + 63) / 64), npd), dim3(256),
-                       0, s, pds, V, m, j, active, W, Hp);
-    MEMI_CHECK_LAUNCH("k_orth_update");
+  const int32_t* jp = j_dev;
+  for (int st = 0; st < nsteps; ++st) {
+    hipLaunchKernelGGL(k_lanczos_matvec, dim3((unsigned)((info->n_rows + 3) / 4)), dim3(256), 0,
+                       s, rows, info->n_rows, pds, L, V, m, jp, active, W);
+    MEMI_CHECK_LAUNCH("k_lanczos_matvec");
+    if (!split) {
+      hipLaunchKernelGGL(k_lanczos_orth, dim3(npd), dim3(256), (size_t)m * sizeof(double), s,
+                         pds, V, m, jp, active, W, ab);
+      MEMI_CHECK_LAUNCH("k_lanczos_orth");
+    } else {
+      // column blocks sized for the largest possible j (< m); blocks beyond j exit at once
+      for (int pass = 0; pass < 2; ++pass) {
+        double* Hp = H + (int64_t)pass * info->n_pd * m;
+        hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)((m + 3) / 4), npd), dim3(256), 0, s, pds,
+                           V, m, jp, active, W, Hp);
+        MEMI_CHECK_LAUNCH("k_orth_dot");
+        hipLaunchKernelGGL(k_orth_update, dim3((unsigned)((info->max_n + 63) / 64), npd),
+                           dim3(256), 0, s, pds, V, m, jp, active, W, Hp);
+        MEMI_CHECK_LAUNCH("k_orth_update");
+      }
+      hipLaunchKernelGGL(k_orth_finish, dim3(npd), dim3(256), 0, s, pds, V, m, jp, active, W, H,
+                         info->n_pd, ab);
+      MEMI_CHECK_LAUNCH("k_orth_finish");
+    }
+    hipLaunchKernelGGL(k_lanczos_advance, dim3(1), dim3(64), 0, s, j_dev);
+    MEMI_CHECK_LAUNCH("k_lanczos_advance");
   }
-  hipLaunchKernelGGL(k_orth_finish, dim3(npd), dim3(256), 0, s, pds, V, m, j, active, W, H,
-                     info->n_pd, ab);
-  MEMI_CHECK_LAUNCH("k_orth_finish");
   return MEMI_OK;
 }
 

@@ -364,7 +364,7 @@ def lm_ritz(alpha, beta, k):
 
 
 def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202, split=None,
-                 check="gpu"):
+                 check="gpu", graph=True):
     """Largest-magnitude eigenpairs of every PD's float64 kernel block of L (sembeddingonFly.py
     semantics: the k eigenpairs of largest |lambda|, sorted by descending lambda) by batched
     Lanczos with full reorthogonalisation on the GPU.
@@ -374,8 +374,9 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     for batches of fewer than 128 PDs or with a PD above 1024 rows) instead of one workgroup
     per PD.  check: "gpu" (memi_lanczos_ritz: Ritz pairs and the decision on the device, one
     int per PD read back per check, every 16 steps by default) or "host" (scipy tridiagonal
-    eigensolver per PD, every 32 steps by default).  Returns a list of
-    (lam, V (n x k) numpy, steps, max residual)."""
+    eigensolver per PD, every 32 steps by default).  graph: the launches of one block of
+    check_every steps are captured once in a HIP graph and replayed (the step index lives on
+    the device).  Returns a list of (lam, V (n x k) numpy, steps, max residual)."""
     dev = L.device
     npd = plan.npd
     ns = plan.ns.astype(int)
@@ -415,12 +416,19 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
         piv = torch.empty(npd * K * 2 * m, dtype=torch.float64, device=dev)
     steps = np.minimum(ns, m)
     out = [None] * npd
+    j_dev = torch.zeros(1, dtype=torch.int32, device=dev)
+    g = None
+    if graph:
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            nat.lanczos_steps(plan, L, V, m, j_dev, check_every, active, W, ab, H=H)
     j = 0
     while active_h.any():
-        stop = min(m, j + check_every)
-        while j < stop:
-            nat.lanczos_step(plan, L, V, m, j, active, W, ab, H=H)
-            j += 1
+        if g is not None:
+            g.replay()
+        else:
+            nat.lanczos_steps(plan, L, V, m, j_dev, check_every, active, W, ab, H=H)
+        j = min(m, j + check_every)
         if check == "gpu":
             nat.lanczos_ritz(plan, ab, m, j, ks_t, tol, active, theta, S_dev, steps_t, res_t,
                              piv)
