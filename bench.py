@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--lanczos-graph", type=int, default=1,
+                    help="replay blocks of Lanczos steps from a captured HIP graph (default 1)")
+    ap.add_argument("--lanczos-split", type=int, default=None,
+                    help="1/0: force the grid-parallel / per-PD orthogonalisation")
     ap.add_argument("--graph", type=int, default=None,
                     help="1: replay prep + Gram as a captured HIP graph (default: on for c3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -258,6 +262,8 @@ def main():
     from manifoldem_python_amd import _native as nat, pipeline as pl
     dev = torch.device("cuda")
     nat.load()
+    pl.LANCZOS.update(graph=bool(args.lanczos_graph),
+                      split=None if args.lanczos_split is None else bool(args.lanczos_split))
     log("building workload", args.workload)
     batch, meta, num_eigs = build_workload(args.workload, dev)
     torch.cuda.synchronize()
@@ -282,7 +288,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16x3 (split) Gram, fp32 prep, fp64 eigensolve",
         "data": "synthetic",
-        "config": dict(meta, precision="split-bf16", hip_graph=graph),
+        "config": dict(meta, precision="split-bf16", hip_graph=graph,
+                       lanczos=dict(pl.LANCZOS)),
         "pds_per_s": batch.ns.size * args.steps / bf["t"],
         "distance_ms_per_step": 1e3 * bf["t_dist"] / args.steps,
         "gram_kernel_ms": bf["gram_ms"],

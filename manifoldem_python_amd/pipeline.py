@@ -363,6 +363,9 @@ def lm_ritz(alpha, beta, k):
     return theta[sel], S[:, sel]
 
 
+LANCZOS = dict(graph=True, split=None, check_every=None)   # dm_embed's Lanczos settings
+
+
 def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202, split=None,
                  check="gpu", graph=True):
     """Largest-magnitude eigenpairs of every PD's float64 kernel block of L (sembeddingonFly.py
@@ -391,10 +394,37 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
         check_every = 16 if check == "gpu" else 32
     if max(ks) > nat.RITZ_KMAX:
         raise ValueError(f"lanczos_eigs: at most {nat.RITZ_KMAX} eigenpairs per PD")
-    V = torch.zeros(m * n_rows, dtype=torch.float64, device=dev)
-    W = torch.empty(n_rows, dtype=torch.float64, device=dev)
-    H = torch.empty(2 * npd * m, dtype=torch.float64, device=dev) if split else None
-    ab = torch.zeros(npd * m * 2, dtype=torch.float64, device=dev)
+    K = nat.RITZ_KMAX
+    # buffers (and the captured block of steps) persist on the plan between calls with the
+    # same L buffer and settings, so repeated embeddings only replay the graph
+    key = (L.data_ptr(), L.numel(), m, bool(split), check_every, bool(graph), check)
+    st = getattr(plan, "_lanczos", None)
+    if st is None or st["key"] != key:
+        st = dict(key=key)
+        st["V"] = torch.zeros(m * n_rows, dtype=torch.float64, device=dev)
+        st["W"] = torch.empty(n_rows, dtype=torch.float64, device=dev)
+        st["H"] = torch.empty(2 * npd * m, dtype=torch.float64, device=dev) if split else None
+        st["ab"] = torch.zeros(npd * m * 2, dtype=torch.float64, device=dev)
+        st["active"] = torch.zeros(npd, dtype=torch.int32, device=dev)
+        st["j"] = torch.zeros(1, dtype=torch.int32, device=dev)
+        if check == "gpu":
+            st["theta"] = torch.zeros(npd * K, dtype=torch.float64, device=dev)
+            st["S"] = torch.zeros(npd * m * K, dtype=torch.float64, device=dev)
+            st["steps"] = torch.zeros(npd, dtype=torch.int32, device=dev)
+            st["res"] = torch.zeros(npd, dtype=torch.float64, device=dev)
+            st["piv"] = torch.empty(npd * K * 2 * m, dtype=torch.float64, device=dev)
+        st["graph"] = None
+        if graph:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                nat.lanczos_steps(plan, L, st["V"], m, st["j"], check_every, st["active"],
+                                  st["W"], st["ab"], H=st["H"])
+            st["graph"] = g
+        plan._lanczos = st
+    V, W, H, ab, active, j_dev, g = (st["V"], st["W"], st["H"], st["ab"], st["active"], st["j"],
+                                     st["graph"])
+    if check == "gpu":
+        theta, S_dev, steps_t, res_t, piv = st["theta"], st["S"], st["steps"], st["res"], st["piv"]
     rng = np.random.default_rng(seed)
     v0 = np.empty(n_rows)
     idx = np.empty(n_rows, dtype=np.int64)
@@ -403,25 +433,15 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
         v = 1.0 + 0.1 * rng.standard_normal(n)
         v0[rows0[p]:rows0[p] + n] = v / np.linalg.norm(v)
         idx[rows0[p]:rows0[p] + n] = m * rows0[p] + np.arange(n)
+    V.zero_()
+    ab.zero_()
+    j_dev.zero_()
     V[torch.as_tensor(idx, device=dev)] = torch.as_tensor(v0, device=dev)
     active_h = np.ones(npd, dtype=np.int32)
-    active = torch.as_tensor(active_h, device=dev)
-    K = nat.RITZ_KMAX
-    if check == "gpu":
-        ks_t = torch.as_tensor(np.asarray(ks, dtype=np.int32), device=dev)
-        theta = torch.zeros(npd * K, dtype=torch.float64, device=dev)
-        S_dev = torch.zeros(npd * m * K, dtype=torch.float64, device=dev)
-        steps_t = torch.zeros(npd, dtype=torch.int32, device=dev)
-        res_t = torch.zeros(npd, dtype=torch.float64, device=dev)
-        piv = torch.empty(npd * K * 2 * m, dtype=torch.float64, device=dev)
+    active.copy_(torch.as_tensor(active_h, device=dev))
+    ks_t = torch.as_tensor(np.asarray(ks, dtype=np.int32), device=dev)
     steps = np.minimum(ns, m)
     out = [None] * npd
-    j_dev = torch.zeros(1, dtype=torch.int32, device=dev)
-    g = None
-    if graph:
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            nat.lanczos_steps(plan, L, V, m, j_dev, check_every, active, W, ab, H=H)
     j = 0
     while active_h.any():
         if g is not None:
@@ -581,14 +601,18 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
     if timer:
         timer.mark("tanh_fit_host")
     sigma = torch.as_tensor(sig, device=dev)
-    L = torch.empty(max(info.d_elems, 1), dtype=torch.float64, device=dev)
+    # the kernel matrix buffer is kept on the plan so that the Lanczos graph stays valid
+    L = getattr(plan, "_L", None)
+    if L is None or L.numel() < max(info.d_elems, 1) or L.device != dev:
+        L = torch.empty(max(info.d_elems, 1), dtype=torch.float64, device=dev)
+        plan._L = L
     deg = torch.empty(2 * info.n_rows, dtype=torch.float64, device=dev)
     nat.markov_normalize(plan, D, mask, sigma, L, deg)
     if timer:
         timer.mark("normalize")
     ks = [min(num_eigs, int(n) - 3) + 1 for n in plan.ns]
     if eig == "lanczos":
-        eigs = lanczos_eigs(plan, L, ks)
+        eigs = lanczos_eigs(plan, L, ks, **LANCZOS)
         if timer:
             timer.mark("lanczos")
     for p in range(npd):
