@@ -376,7 +376,8 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     the Krylov space is the whole space.  split: orthogonalise with the whole grid (default
     for batches of fewer than 128 PDs or with a PD above 1024 rows) instead of one workgroup
     per PD.  check: "gpu" (memi_lanczos_ritz: Ritz pairs and the decision on the device, one
-    int per PD read back per check, every 16 steps by default) or "host" (scipy tridiagonal
+    int per PD read back per check, every 8 steps from the first step count at which the
+    selected pairs can have converged, by default) or "host" (scipy tridiagonal
     eigensolver per PD, every 32 steps by default).  graph: the launches of one block of
     check_every steps are captured once in a HIP graph and replayed (the step index lives on
     the device).  Returns a list of (lam, V (n x k) numpy, steps, max residual)."""
@@ -391,7 +392,9 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     if split is None:
         split = npd < 128 or int(ns.max()) > 1024
     if check_every is None:
-        check_every = 16 if check == "gpu" else 32
+        check_every = 8 if check == "gpu" else 32
+    # no Ritz pair can have converged before the Krylov space holds both ends of the spectrum
+    first_check = min(m, max(check_every, 2 * max(ks) + 8)) if check == "gpu" else check_every
     if max(ks) > nat.RITZ_KMAX:
         raise ValueError(f"lanczos_eigs: at most {nat.RITZ_KMAX} eigenpairs per PD")
     K = nat.RITZ_KMAX
@@ -444,11 +447,13 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     out = [None] * npd
     j = 0
     while active_h.any():
-        if g is not None:
-            g.replay()
-        else:
-            nat.lanczos_steps(plan, L, V, m, j_dev, check_every, active, W, ab, H=H)
-        j = min(m, j + check_every)
+        target = first_check if j == 0 else min(m, j + check_every)
+        while j < target:
+            if g is not None:
+                g.replay()
+            else:
+                nat.lanczos_steps(plan, L, V, m, j_dev, check_every, active, W, ab, H=H)
+            j = min(m, j + check_every)
         if check == "gpu":
             nat.lanczos_ritz(plan, ab, m, j, ks_t, tol, active, theta, S_dev, steps_t, res_t,
                              piv)
