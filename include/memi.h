@@ -231,10 +231,11 @@ int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, i
  * largest-magnitude eigenpairs that sembeddingonFly.py:27 asks ARPACK eigsh for.
  * V: float64 bases, column-major per PD at m*row0 (column 0 = unit start vector, set by the
  * caller); W: float64 [n_rows] scratch; ab: float64 [npd][m][2] gets {alpha_j, beta_j};
- * active: int32 [npd] (0 = skip).  The step index j lives in device memory (*j_dev, int32,
- * zero before the first step): step j reads column j, writes column j+1 and advances *j_dev,
- * and steps with j >= min(n, m) do nothing -- so the launches of a block of steps can be
- * captured once in a HIP graph and replayed.
+ * active: int32 [npd] (0 = skip).  The step index j lives in device memory: j_dev is int32
+ * [2] = {j, arrival counter}, both zero before the first step.  Step j reads column j, writes
+ * column j+1, and its last kernel advances j_dev[0] (the last of its workgroups to finish);
+ * steps with j >= min(n, m) do nothing -- so the launches of a block of steps can be captured
+ * once in a HIP graph and replayed.
  * split = 0: one workgroup per PD orthogonalises (H unused, may be NULL); split = 1: the
  * orthogonalisation is spread over the grid (for few large PDs), H: float64 [2*npd*m]. */
 int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const double* L,

@@ -16,6 +16,20 @@
 
 namespace memi {
 
+// Every block of the last kernel of a step arrives here; the last to arrive advances the step
+// index jp[0] and re-arms the arrival counter jp[1] (all blocks read jp[0] before arriving).
+__device__ inline void step_arrive(int32_t* jp, int j) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const int prev = atomicAdd(&jp[1], 1);
+    if (prev == (int)(gridDim.x * gridDim.y) - 1) {
+      jp[1] = 0;
+      jp[0] = j + 1;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restrict__ rows,
                                                         int64_t n_rows,
                                                         const memi_pd_desc* __restrict__ pds,
@@ -34,21 +48,24 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restric
   if (j >= min(n, m)) return;
   const double* Lr = L + P.d_off + (int64_t)i * n;
   const double* v = V + (int64_t)m * P.row0 + (int64_t)j * n;
-  double s = 0.0;
-  for (int c = lane; c < n; c += 64) s += Lr[c] * v[c];
+  double s = 0.0, s1 = 0.0;
+  int c = lane;
+  for (; c + 64 < n; c += 128) {
+    s += Lr[c] * v[c];
+    s1 += Lr[c + 64] * v[c + 64];
+  }
+  if (c < n) s += Lr[c] * v[c];
+  s += s1;
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) W[P.row0 + i] = s;
 }
 
 // one block per PD: CGS2 against columns 0..j, then normalise into column j+1
-__global__ __launch_bounds__(256) void k_lanczos_orth(const memi_pd_desc* __restrict__ pds,
-                                                      double* __restrict__ V, int m, const int32_t* __restrict__ jp,
-                                                      const int32_t* __restrict__ active,
-                                                      double* __restrict__ W,
-                                                      double* __restrict__ ab) {
-  const int j = *jp;
-  extern __shared__ double h[];  // m coefficients
-  __shared__ double red[8];
+__device__ inline void lanczos_orth_body(const memi_pd_desc* __restrict__ pds,
+                                         double* __restrict__ V, int m, int j,
+                                         const int32_t* __restrict__ active,
+                                         double* __restrict__ W, double* __restrict__ ab,
+                                         double* h, double* red) {
   const int pd = blockIdx.x;
   if (!active[pd]) return;
   const memi_pd_desc P = pds[pd];
@@ -91,36 +108,45 @@ __global__ __launch_bounds__(256) void k_lanczos_orth(const memi_pd_desc* __rest
   }
 }
 
-// the step counter lives in device memory so that a block of steps can be captured once in a
-// HIP graph and replayed; this advances it after each step (one thread, one vector store)
-__global__ void k_lanczos_advance(int32_t* __restrict__ jp) {
-  if (threadIdx.x == 0) jp[0] = jp[0] + 1;
+__global__ __launch_bounds__(256) void k_lanczos_orth(const memi_pd_desc* __restrict__ pds,
+                                                      double* __restrict__ V, int m,
+                                                      int32_t* __restrict__ jp,
+                                                      const int32_t* __restrict__ active,
+                                                      double* __restrict__ W,
+                                                      double* __restrict__ ab) {
+  extern __shared__ double h[];  // m coefficients
+  __shared__ double red[8];
+  const int j = jp[0];
+  lanczos_orth_body(pds, V, m, j, active, W, ab, h, red);
+  step_arrive(jp, j);
 }
+
 
 // Grid-parallel CGS2 for large PDs (one workgroup per PD leaves most CUs idle when a batch
 // has few PDs): per pass, k_orth_dot (one wave per (PD, basis column)) then k_orth_update
 // (64 rows x 4 column groups per workgroup); k_orth_finish records alpha, beta and writes
 // v_{j+1}.  H: float64 [2][npd][m] coefficients of the two passes.
 __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict__ pds,
-                                                  const double* __restrict__ V, int m, const int32_t* __restrict__ jp,
+                                                  const double* __restrict__ V, int m,
+                                                  const int32_t* __restrict__ jp,
                                                   const int32_t* __restrict__ active,
                                                   const double* __restrict__ W,
                                                   double* __restrict__ H) {
+  __shared__ double red[8];
   const int j = *jp;
   const int pd = blockIdx.y;
   if (!active[pd]) return;
   const memi_pd_desc P = pds[pd];
   const int n = P.n;
   if (j >= min(n, m)) return;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int c = blockIdx.x;                       // one block per basis column
   if (c > j) return;
-  const int lane = threadIdx.x & 63;
   const double* vc = V + (int64_t)m * P.row0 + (int64_t)c * n;
   const double* w = W + P.row0;
   double s = 0.0;
-  for (int r = lane; r < n; r += 64) s += vc[r] * w[r];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) H[(int64_t)pd * m + c] = s;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) s += vc[r] * w[r];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) H[(int64_t)pd * m + c] = s;
 }
 
 __global__ __launch_bounds__(256) void k_orth_update(const memi_pd_desc* __restrict__ pds,
@@ -150,14 +176,12 @@ __global__ __launch_bounds__(256) void k_orth_update(const memi_pd_desc* __restr
   }
 }
 
-__global__ __launch_bounds__(256) void k_orth_finish(const memi_pd_desc* __restrict__ pds,
-                                                     double* __restrict__ V, int m, const int32_t* __restrict__ jp,
-                                                     const int32_t* __restrict__ active,
-                                                     const double* __restrict__ W,
-                                                     const double* __restrict__ H, int64_t npd,
-                                                     double* __restrict__ ab) {
-  const int j = *jp;
-  __shared__ double red[8];
+__device__ inline void orth_finish_body(const memi_pd_desc* __restrict__ pds,
+                                        double* __restrict__ V, int m, int j,
+                                        const int32_t* __restrict__ active,
+                                        const double* __restrict__ W,
+                                        const double* __restrict__ H, int64_t npd,
+                                        double* __restrict__ ab, double* red) {
   const int pd = blockIdx.x;
   if (!active[pd]) return;
   const memi_pd_desc P = pds[pd];
@@ -178,6 +202,19 @@ __global__ __launch_bounds__(256) void k_orth_finish(const memi_pd_desc* __restr
   }
 }
 
+__global__ __launch_bounds__(256) void k_orth_finish(const memi_pd_desc* __restrict__ pds,
+                                                     double* __restrict__ V, int m,
+                                                     int32_t* __restrict__ jp,
+                                                     const int32_t* __restrict__ active,
+                                                     const double* __restrict__ W,
+                                                     const double* __restrict__ H, int64_t npd,
+                                                     double* __restrict__ ab) {
+  __shared__ double red[8];
+  const int j = jp[0];
+  orth_finish_body(pds, V, m, j, active, W, H, npd, ab, red);
+  step_arrive(jp, j);
+}
+
 // ---- convergence check: Ritz pairs of the tridiagonal T_p on the GPU -----------------------
 // One 64-thread workgroup per active PD with J = min(steps, n, m) Lanczos steps.
 //  1. candidates: the k smallest and k largest eigenvalues of T (all J when J <= 2k), each by
@@ -192,17 +229,20 @@ __global__ __launch_bounds__(256) void k_orth_finish(const memi_pd_desc* __restr
 //     residual (0 when J = n); theta [npd][16] and S (row-major J x k at pd*m*16) stay as the
 //     final Ritz pairs.
 // piv: float64 scratch [npd][16][2][m].
-__device__ inline int sturm_count(const double* ab, int J, double x, double pivmin) {
-  double q = ab[0] - x;
+// Sturm count (eigenvalues below x) from LDS copies of alpha and beta^2
+__device__ inline int sturm_count(const double* al, const double* b2, int J, double x,
+                                  double pivmin) {
+  double q = al[0] - x;
   int c = q < 0.0;
   for (int i = 1; i < J; ++i) {
     if (fabs(q) < pivmin) q = -pivmin;
-    const double b = ab[2 * (i - 1) + 1];
-    q = ab[2 * i] - x - b * b / q;
+    q = al[i] - x - b2[i - 1] / q;
     c += q < 0.0;
   }
   return c;
 }
+
+constexpr int RITZ_LDS_M = 4096;   // the largest m memi_lanczos_ritz accepts
 
 __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restrict__ pds,
                                                      const double* __restrict__ ab, int m,
@@ -215,6 +255,7 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
                                                      double* __restrict__ res_out,
                                                      double* __restrict__ piv) {
   constexpr int KM = MEMI_RITZ_KMAX;
+  __shared__ double al[RITZ_LDS_M], b2[RITZ_LDS_M];
   __shared__ double cand[2 * KM];
   __shared__ double sel[KM];
   __shared__ double resv[KM];
@@ -226,6 +267,11 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
   const int J = min(steps_done, min(n, m));
   const int k = min(ks[pd], J);
   const double* a = ab + (int64_t)pd * m * 2;      // a[2i] = alpha_i, a[2i+1] = beta_i
+  for (int i = t; i < J; i += 64) {
+    al[i] = a[2 * i];
+    b2[i] = a[2 * i + 1] * a[2 * i + 1];
+  }
+  __syncthreads();
   // Gershgorin interval and pivot floor
   double lo = INFINITY, hi = -INFINITY, bmax = 0.0;
   for (int i = t; i < J; i += 64) {
@@ -252,7 +298,7 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
     for (int it = 0; it < 200 && h - l > 2.2e-16 * fmax(fabs(l), fabs(h)) + 1e-300; ++it) {
       const double mid = 0.5 * (l + h);
       if (mid <= l || mid >= h) break;
-      if (sturm_count(a, J, mid, pivmin) > idx) h = mid;
+      if (sturm_count(al, b2, J, mid, pivmin) > idx) h = mid;
       else l = mid;
     }
     cand[t] = 0.5 * (l + h);
@@ -286,26 +332,24 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
     const double th = sel[t];
     double* dp = piv + (((int64_t)pd * KM + t) * 2) * m;
     double* dm = dp + m;
-    double q = a[0] - th;
+    double q = al[0] - th;
     dp[0] = q;
     for (int i = 1; i < J; ++i) {
       if (fabs(q) < pivmin) q = -pivmin;
-      const double b = a[2 * (i - 1) + 1];
-      q = a[2 * i] - th - b * b / q;
+      q = al[i] - th - b2[i - 1] / q;
       dp[i] = q;
     }
-    q = a[2 * (J - 1)] - th;
+    q = al[J - 1] - th;
     dm[J - 1] = q;
     for (int i = J - 2; i >= 0; --i) {
       if (fabs(q) < pivmin) q = -pivmin;
-      const double b = a[2 * i + 1];
-      q = a[2 * i] - th - b * b / q;
+      q = al[i] - th - b2[i] / q;
       dm[i] = q;
     }
     int r = 0;
     double gbest = INFINITY;
     for (int i = 0; i < J; ++i) {
-      const double g = fabs(dp[i] + dm[i] - (a[2 * i] - th));
+      const double g = fabs(dp[i] + dm[i] - (al[i] - th));
       if (g < gbest) {
         gbest = g;
         r = i;
@@ -427,25 +471,23 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const d
     MEMI_CHECK_LAUNCH("k_lanczos_matvec");
     if (!split) {
       hipLaunchKernelGGL(k_lanczos_orth, dim3(npd), dim3(256), (size_t)m * sizeof(double), s,
-                         pds, V, m, jp, active, W, ab);
+                         pds, V, m, j_dev, active, W, ab);
       MEMI_CHECK_LAUNCH("k_lanczos_orth");
     } else {
       // column blocks sized for the largest possible j (< m); blocks beyond j exit at once
       for (int pass = 0; pass < 2; ++pass) {
         double* Hp = H + (int64_t)pass * info->n_pd * m;
-        hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)((m + 3) / 4), npd), dim3(256), 0, s, pds,
-                           V, m, jp, active, W, Hp);
+        hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)m, npd), dim3(256), 0, s, pds, V, m, jp,
+                           active, W, Hp);
         MEMI_CHECK_LAUNCH("k_orth_dot");
         hipLaunchKernelGGL(k_orth_update, dim3((unsigned)((info->max_n + 63) / 64), npd),
                            dim3(256), 0, s, pds, V, m, jp, active, W, Hp);
         MEMI_CHECK_LAUNCH("k_orth_update");
       }
-      hipLaunchKernelGGL(k_orth_finish, dim3(npd), dim3(256), 0, s, pds, V, m, jp, active, W, H,
-                         info->n_pd, ab);
+      hipLaunchKernelGGL(k_orth_finish, dim3(npd), dim3(256), 0, s, pds, V, m, j_dev, active, W,
+                         H, info->n_pd, ab);
       MEMI_CHECK_LAUNCH("k_orth_finish");
     }
-    hipLaunchKernelGGL(k_lanczos_advance, dim3(1), dim3(64), 0, s, j_dev);
-    MEMI_CHECK_LAUNCH("k_lanczos_advance");
   }
   return MEMI_OK;
 }

@@ -241,6 +241,8 @@ __global__ __launch_bounds__(256, (GramOcc<PREC, DEPTH>::value)) void k_gram_til
       }
 }
 
+constexpr int FIN_SPLIT = 8;   // workgroups per upper tile in the finalize
+
 // Reduce the segment partials of one upper tile in float64, write D and its mirror,
 // and queue entries with a large cancellation ratio for direct recomputation.
 __global__ __launch_bounds__(256) void k_gram_finalize(const int32_t* __restrict__ utiles,
@@ -256,7 +258,9 @@ __global__ __launch_bounds__(256) void k_gram_finalize(const int32_t* __restrict
   const memi_pd_desc P = pds[pd];
   const int n = P.n;
   const int64_t tile = (int64_t)BM * BN;
-  for (int e = threadIdx.x; e < BM * BN; e += blockDim.x) {
+  // blockIdx.y: which FIN_SPLIT-th of the tile's entries this block reduces
+  const int e0 = blockIdx.y * (BM * BN / FIN_SPLIT), e1 = e0 + BM * BN / FIN_SPLIT;
+  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const int r = e / BN, c = e % BN;
     const int i = ti * BM + r, j = tj * BN + c;
     if (i >= n || j >= n) continue;
@@ -490,8 +494,8 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
     set_error("memi_ctf_distance: hipEventRecord failed");
     return MEMI_ELAUNCH;
   }
-  hipLaunchKernelGGL(k_gram_finalize, dim3((unsigned)info->n_utiles), dim3(256), 0, s, ut, pds,
-                     ws, chunks, kappa, D, flags, info->flag_cap, flag_count);
+  hipLaunchKernelGGL(k_gram_finalize, dim3((unsigned)info->n_utiles, FIN_SPLIT), dim3(256), 0, s,
+                     ut, pds, ws, chunks, kappa, D, flags, info->flag_cap, flag_count);
   MEMI_CHECK_LAUNCH("k_gram_finalize");
   hipLaunchKernelGGL(k_gram_recompute, dim3(1024), dim3(256), 0, s, flags, flag_count,
                      info->flag_cap, pds, (const float2*)fy, ctf_half, N, D);

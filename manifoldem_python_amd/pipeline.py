@@ -409,7 +409,7 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
         st["H"] = torch.empty(2 * npd * m, dtype=torch.float64, device=dev) if split else None
         st["ab"] = torch.zeros(npd * m * 2, dtype=torch.float64, device=dev)
         st["active"] = torch.zeros(npd, dtype=torch.int32, device=dev)
-        st["j"] = torch.zeros(1, dtype=torch.int32, device=dev)
+        st["j"] = torch.zeros(2, dtype=torch.int32, device=dev)
         if check == "gpu":
             st["theta"] = torch.zeros(npd * K, dtype=torch.float64, device=dev)
             st["S"] = torch.zeros(npd * m * K, dtype=torch.float64, device=dev)
