@@ -277,6 +277,16 @@ def main():
     Kh = pl.half_len(batch.N)
     flops = 8.0 * Kh * pairs                       # algorithmic: 8 K_h per unordered pair
     exec_flops = issued_gram_flops(bf["wsp"].plan, bf["wsp"].operand.Khp, 3)
+    traffic, traffic_note = None, None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "k_gram_tiles_c2_traffic.json")
+    if args.workload == "c2" and os.path.exists(tfile):
+        with open(tfile) as f:
+            tr = json.load(f)
+        traffic = tr["bytes_per_launch"]
+        traffic_note = (f"rocprofv3 FETCH_SIZE x {tr['fetch_scale']:.3f} + WRITE_SIZE x "
+                        f"{tr['write_scale']:.3f} (copy-calibrated) per launch of variant "
+                        f"{tr['variant']} with {tr['chunks']} K chunks; {tr['source']}")
     gram_s = bf["gram_ms"] / 1e3
     achieved = flops / gram_s / 1e12
     out = {
@@ -298,7 +308,8 @@ def main():
         "lanczos": bf["lanczos"],
         "kappa_flagged": bf["flags"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
+                     "traffic_note": traffic_note,
                      "kernel": "k_gram_tiles<split-bf16>",
                      "mfma_util": exec_flops / gram_s / 1e12 / PEAK_BF16_TFLOPS},
     }
