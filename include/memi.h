@@ -198,6 +198,17 @@ int memi_ferguson_sweep(const void* plan_dev, const memi_plan_info* info, const 
                         const uint32_t* mask, const double* logEps, int n_eps, double* out,
                         void* stream);
 
+/* The same sums for a log-uniform eps grid (logEps[e] = le0 + e*dle, dle > 0) by bucket
+ * moments: positive entry values go into log-uniform buckets whose edges contain every
+ * eps's threshold cut, each bucket keeps the power sums of its values' offsets from its
+ * centre, and each eps sums a 17-term series per bucket below its cut (~1e-14 relative to
+ * the direct sums; cost O(n_eps * buckets) per PD instead of O(n_eps * entries)).
+ * out as memi_ferguson_sweep; ws: float64 scratch of npd * (6 + 8 + 544 * 18) doubles. */
+int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, const float* D,
+                          const uint32_t* mask, const double* logEps, int n_eps, double le0,
+                          double dle, double* out, double* ws, int64_t ws_doubles,
+                          void* stream);
+
 /* Batched tanh fit of fergusonE.py:45-57: for each PD (block), Levenberg-Marquardt from p0
  * (float64 [npd][4]) of f = a3 + a2 tanh(a0 x + a1) to y (float64 [npd][m]) at x [m].
  * out float64 [npd][16] = {popt[4], J^T J at popt (upper triangle, 10), ss_res, iterations};

@@ -64,6 +64,8 @@ _SIGS = {
                           c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp],
     "memi_knn_select": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_ferguson_sweep": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
+    "memi_ferguson_moments": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_int, c_dbl,
+                              c_dbl, c_vp, c_vp, c_i64, c_vp],
     "memi_markov_normalize": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_tanh_fit": [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_ferguson_values": [c_vp, c_i64, c_vp, c_int, c_vp, c_vp],
@@ -356,6 +358,22 @@ def ferguson_sweep(plan, D, mask, logEps, out):
           out.numel() >= I.n_pd * (3 * logEps.numel() + 2), "ferguson_sweep: out too small")
     call("memi_ferguson_sweep", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask),
          _p(logEps), logEps.shape[0], _p(out), _stream())
+
+
+FM_WS_PER_PD = 6 + 8 + 544 * 18
+
+
+def ferguson_moments(plan, D, mask, logEps, le0, dle, out, ws):
+    """memi_ferguson_moments: the sweep for a log-uniform grid logEps[e] = le0 + e*dle."""
+    I = plan.info
+    _need(D.numel() >= I.d_elems and mask.numel() >= I.mask_words, "ferguson_moments: D/mask")
+    _need(logEps.dtype == torch.float64 and out.dtype == torch.float64 and
+          out.numel() >= I.n_pd * (3 * logEps.numel() + 2), "ferguson_moments: out too small")
+    _need(ws.dtype == torch.float64 and ws.numel() >= I.n_pd * FM_WS_PER_PD,
+          "ferguson_moments: ws too small")
+    call("memi_ferguson_moments", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask),
+         _p(logEps), logEps.shape[0], float(le0), float(dle), _p(out), _p(ws), ws.numel(),
+         _stream())
 
 
 def markov_normalize(plan, D, mask, sigma, L, deg):

@@ -288,6 +288,236 @@ __global__ __launch_bounds__(256) void k_ferguson_sweep(const int32_t* __restric
   }
 }
 
+// ---------------------------------------------------- Ferguson sweep by bucket moments ----
+// For a log-uniform eps grid (logEps[e] = le0 + e*dle) the threshold cuts of all eps,
+// v < 2 thr eps_e, are log-uniform too.  Positive entry values go into log-uniform buckets
+// whose edges E_k = exp(base + k w), base = log(2 thr) + le0, w = dle / s, contain every cut
+// (the cut of eps_e is E_{e s}), so bucket k is wholly inside (k < e s) or outside the sum of
+// eps_e.  Per bucket, with centre c_k and half-width h_k, the power sums M_m = sum u^m of
+// u = (v - c_k) / h_k (m < FM_NM) give, for x = v / (2 eps) and t = h_k / (2 eps),
+//   sum exp(-x)   = exp(-c_k / (2 eps)) sum_m (-t)^m M_m / m!,
+//   sum x exp(-x) = exp(-c_k / (2 eps)) sum_m (-t)^m (c_k/(2 eps) M_m + t M_{m+1}) / m!,
+// truncated at m = 16: t <= thr * (e^{w/2} - 1) <= 0.8 leaves < 1e-14 relative.  s is the
+// largest of 1..8 that keeps the PD's value range within FM_BMAX buckets.  Zero entries
+// (v = 0, x = 0) count 1 each.  Results equal the direct sweep to ~1e-14 relative.
+constexpr int FM_BMAX = 544;  // buckets per PD: ln(3.4e38 / 1e-6) / 0.2 + 2 < 544
+constexpr int FM_NM = 18;     // power sums M_0 .. M_17
+constexpr int FM_GP = 8;      // grid parameters per PD
+
+__device__ inline unsigned long long dbits(double v) { return (unsigned long long)__double_as_longlong(v); }
+
+__global__ void k_fer_init(int64_t npd, double* __restrict__ st) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npd;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    st[6 * p + 0] = 0.0;                                   // entries
+    st[6 * p + 1] = 0.0;                                   // sum exp(-v / (2 max eps))
+    st[6 * p + 2] = 0.0;                                   // zero entries
+    st[6 * p + 3] = __longlong_as_double(0x7FF0000000000000LL);   // min positive v (+inf)
+    st[6 * p + 4] = 0.0;                                   // max v
+    st[6 * p + 5] = 0.0;
+  }
+}
+
+// per PD: #entries, the threshold sum, #zero entries, min and max positive value
+__global__ __launch_bounds__(256) void k_fer_stats(const int32_t* __restrict__ ftiles,
+                                                   const memi_pd_desc* __restrict__ pds,
+                                                   const float* __restrict__ D,
+                                                   const uint32_t* __restrict__ mask,
+                                                   const double* __restrict__ logEps, int n_eps,
+                                                   double* __restrict__ st) {
+  __shared__ double red[8];
+  __shared__ double mx_sh[4];
+  __shared__ unsigned long long mm_sh[8];
+  double mx = -INFINITY;
+  for (int e = threadIdx.x; e < n_eps; e += blockDim.x) mx = fmax(mx, logEps[e]);
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) mx_sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  mx = fmax(fmax(mx_sh[0], mx_sh[1]), fmax(mx_sh[2], mx_sh[3]));
+  const double cmin = 1.0 / (2.0 * exp(mx));
+  const int32_t* f = ftiles + 4 * (int64_t)blockIdx.x;
+  const memi_pd_desc P = pds[f[0]];
+  const int W = (P.n + 31) / 32;
+  const bool full = P.k >= P.n;
+  double sum = 0.0, cnt = 0.0, zer = 0.0;
+  unsigned long long vmin = dbits(INFINITY), vmax = 0ull;
+  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
+    const int i = f[1] * TB + e / TB, j = f[2] * TB + e % TB;
+    if (i >= P.n || j >= P.n) continue;
+    const double v = entry_value(D, mask, P, W, full, i, j);
+    if (v < 0) continue;
+    sum += exp(-v * cmin);
+    cnt += 1.0;
+    if (v == 0.0) {
+      zer += 1.0;
+    } else {
+      vmin = min(vmin, dbits(v));
+      vmax = max(vmax, dbits(v));
+    }
+  }
+  sum = block_sum(sum, red);
+  cnt = block_sum(cnt, red);
+  zer = block_sum(zer, red);
+  for (int off = 32; off > 0; off >>= 1) {
+    vmin = min(vmin, (unsigned long long)__shfl_down((long long)vmin, off, 64));
+    vmax = max(vmax, (unsigned long long)__shfl_down((long long)vmax, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    mm_sh[threadIdx.x >> 6] = vmin;
+    mm_sh[4 + (threadIdx.x >> 6)] = vmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      vmin = min(vmin, mm_sh[w]);
+      vmax = max(vmax, mm_sh[4 + w]);
+    }
+    double* sp = st + 6 * (int64_t)f[0];
+    atomicAdd(sp + 0, cnt);
+    atomicAdd(sp + 1, sum);
+    atomicAdd(sp + 2, zer);
+    atomicMin((unsigned long long*)(sp + 3), vmin);
+    atomicMax((unsigned long long*)(sp + 4), vmax);
+  }
+}
+
+__device__ inline int fm_bucket(double v, double base, double w) {
+  int k = (int)floor((log(v) - base) / w);
+  if (v < exp(base + k * w)) --k;
+  else if (v >= exp(base + (k + 1) * w)) ++k;
+  return k;
+}
+
+// per PD (one thread each): thr and the bucket grid {thr, w, base, b0, B, s, zeros, -}
+__global__ void k_fer_grid(int64_t npd, const double* __restrict__ st, double le0, double dle,
+                           double* __restrict__ gp) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npd;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const double* sp = st + 6 * p;
+    double* g = gp + FM_GP * p;
+    // fergusonE.py:25-31
+    const double thr = fmax(-log(0.01 * sp[1] / sp[0]), 10.0);
+    const double vmin = sp[3], vmax = sp[4];
+    int B = 0, s = 1, b0 = 0;
+    double w = dle, base = log(2.0 * thr) + le0;
+    if (vmax > 0.0 && vmin <= vmax) {
+      const double range = fmax(log(vmax) - log(vmin), 1e-300);
+      s = (int)fmin(8.0, fmax(1.0, floor((FM_BMAX - 32) * dle / range)));
+      w = dle / s;
+      b0 = fm_bucket(vmin, base, w);
+      B = fm_bucket(vmax, base, w) - b0 + 1;
+      B = min(max(B, 0), FM_BMAX);      // bounds the tables for any input (NaN, inf)
+    }
+    g[0] = thr;
+    g[1] = w;
+    g[2] = base;
+    g[3] = b0;
+    g[4] = B;
+    g[5] = s;
+    g[6] = sp[2];
+    g[7] = 0.0;
+  }
+}
+
+// power sums of every positive entry into its bucket (LDS accumulation, then one atomic
+// per nonzero sum); M: float64 [npd][FM_BMAX][FM_NM], zeroed by the caller
+__global__ __launch_bounds__(256) void k_fer_moments(const int32_t* __restrict__ ftiles,
+                                                     const memi_pd_desc* __restrict__ pds,
+                                                     const float* __restrict__ D,
+                                                     const uint32_t* __restrict__ mask,
+                                                     const double* __restrict__ gp,
+                                                     double* __restrict__ M) {
+  __shared__ double ms[FM_BMAX * FM_NM];
+  const int32_t* f = ftiles + 4 * (int64_t)blockIdx.x;
+  const int pd = f[0];
+  const memi_pd_desc P = pds[pd];
+  const double* g = gp + FM_GP * (int64_t)pd;
+  const double w = g[1], base = g[2];
+  const int b0 = (int)g[3], B = (int)g[4];
+  if (B <= 0) return;                                 // uniform over the block
+  for (int q = threadIdx.x; q < B * FM_NM; q += blockDim.x) ms[q] = 0.0;
+  __syncthreads();
+  const int W = (P.n + 31) / 32;
+  const bool full = P.k >= P.n;
+  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
+    const int i = f[1] * TB + e / TB, j = f[2] * TB + e % TB;
+    if (i >= P.n || j >= P.n) continue;
+    const double v = entry_value(D, mask, P, W, full, i, j);
+    if (v <= 0.0) continue;
+    const int k = min(max(fm_bucket(v, base, w) - b0, 0), B - 1);
+    const double lo = exp(base + (b0 + k) * w), hi = exp(base + (b0 + k + 1) * w);
+    const double u = (v - 0.5 * (lo + hi)) / (0.5 * (hi - lo));
+    double pw = 1.0;
+    for (int m = 0; m < FM_NM; ++m) {
+      atomicAdd(&ms[k * FM_NM + m], pw);
+      pw *= u;
+    }
+  }
+  __syncthreads();
+  double* Mp = M + (int64_t)pd * FM_BMAX * FM_NM;
+  for (int q = threadIdx.x; q < B * FM_NM; q += blockDim.x)
+    if (ms[q] != 0.0) atomicAdd(&Mp[q], ms[q]);
+}
+
+__global__ void k_fer_thr_copy(int64_t npd, const double* __restrict__ st,
+                               double* __restrict__ dst) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npd;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    dst[2 * p] = st[6 * p];
+    dst[2 * p + 1] = st[6 * p + 1];
+  }
+}
+
+// one thread per eps: the series over the buckets below the eps's cut
+__global__ __launch_bounds__(256) void k_fer_sweep_moments(const double* __restrict__ logEps,
+                                                           int n_eps,
+                                                           const double* __restrict__ gp,
+                                                           const double* __restrict__ M,
+                                                           double* __restrict__ out) {
+  __shared__ double cen[FM_BMAX], hw[FM_BMAX];
+  const int pd = blockIdx.y;
+  const double* g = gp + FM_GP * (int64_t)pd;
+  const double w = g[1], base = g[2];
+  const int b0 = (int)g[3], B = (int)g[4], s = (int)g[5];
+  const double zeros = g[6];
+  for (int k = threadIdx.x; k < B; k += blockDim.x) {
+    const double lo = exp(base + (b0 + k) * w), hi = exp(base + (b0 + k + 1) * w);
+    cen[k] = 0.5 * (lo + hi);
+    hw[k] = 0.5 * (hi - lo);
+  }
+  __syncthreads();
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_eps) return;
+  constexpr double invf[FM_NM - 1] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
+                                      1.0 / 5040, 1.0 / 40320, 1.0 / 362880, 1.0 / 3628800,
+                                      1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0,
+                                      1.0 / 87178291200.0, 1.0 / 1307674368000.0,
+                                      1.0 / 20922789888000.0};
+  const double c = 1.0 / (2.0 * exp(logEps[e]));
+  const int jmax = min(B, max(0, e * s - b0));
+  const double* Mp = M + (int64_t)pd * FM_BMAX * FM_NM;
+  double S = zeros, SX = 0.0, CN = zeros;
+  for (int k = 0; k < jmax; ++k) {
+    const double xb = cen[k] * c;
+    if (xb > 800.0) break;                            // later buckets underflow too
+    const double t = -hw[k] * c;
+    const double* m = Mp + (int64_t)k * FM_NM;
+    double s0 = m[FM_NM - 2] * invf[FM_NM - 2], s1 = m[FM_NM - 1] * invf[FM_NM - 2];
+    for (int q = FM_NM - 3; q >= 0; --q) {
+      s0 = s0 * t + m[q] * invf[q];
+      s1 = s1 * t + m[q + 1] * invf[q];
+    }
+    const double ex = exp(-xb);
+    S += ex * s0;
+    SX += ex * (xb * s0 - t * s1);
+    CN += m[0];
+  }
+  double* o = out + ((int64_t)pd * n_eps + e) * 3;
+  o[0] = S;
+  o[1] = SX;
+  o[2] = CN;
+}
+
 // ------------------------------------------------------------------ normalisation ------
 __device__ inline double kernel_weight(double v, double inv_s2) {
   return v < 0 ? 0.0 : (v == 0.0 ? 1.0 : exp(-v * inv_s2));
@@ -657,6 +887,48 @@ int memi_ferguson_sweep(const void* plan_dev, const memi_plan_info* info, const 
   hipLaunchKernelGGL(k_ferguson_sweep, grid, dim3(256), 0, s, v.ftiles, v.pds, D, mask, logEps,
                      n_eps, thr_acc, out);
   MEMI_CHECK_LAUNCH("k_ferguson_sweep");
+  return MEMI_OK;
+}
+
+int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, const float* D,
+                          const uint32_t* mask, const double* logEps, int n_eps, double le0,
+                          double dle, double* out, double* ws, int64_t ws_doubles,
+                          void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && D && mask && logEps && out && ws && n_eps > 0,
+                 "memi_ferguson_moments: bad arguments");
+  MEMI_CHECK_ARG(dle > 0.0 && std::isfinite(le0), "memi_ferguson_moments: the eps grid must be "
+                 "log-uniform with a positive step (dle = %g)", dle);
+  const int64_t npd = info->n_pd;
+  const int64_t need = npd * (6 + FM_GP + (int64_t)FM_BMAX * FM_NM);
+  MEMI_CHECK_ARG(ws_doubles >= need, "memi_ferguson_moments: workspace %lld < %lld doubles",
+                 (long long)ws_doubles, (long long)need);
+  MEMI_CHECK_ARG(npd < 65536 && n_eps <= (1 << 24), "memi_ferguson_moments: too many PDs/eps");
+  if (npd == 0) return MEMI_OK;
+  hipStream_t s = (hipStream_t)stream;
+  PlanView v = view(plan_dev, info);
+  double* st = ws;
+  double* gp = st + 6 * npd;
+  double* M = gp + FM_GP * npd;
+  const int64_t nm = npd * (int64_t)FM_BMAX * FM_NM;
+  hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(nm)), dim3(256), 0, s, M, nm);
+  MEMI_CHECK_LAUNCH("k_zero_f64");
+  hipLaunchKernelGGL(k_fer_init, dim3(grid_for(npd)), dim3(256), 0, s, npd, st);
+  MEMI_CHECK_LAUNCH("k_fer_init");
+  hipLaunchKernelGGL(k_fer_stats, dim3((unsigned)info->n_ftiles), dim3(256), 0, s, v.ftiles,
+                     v.pds, D, mask, logEps, n_eps, st);
+  MEMI_CHECK_LAUNCH("k_fer_stats");
+  hipLaunchKernelGGL(k_fer_grid, dim3(grid_for(npd)), dim3(256), 0, s, npd, st, le0, dle, gp);
+  MEMI_CHECK_LAUNCH("k_fer_grid");
+  hipLaunchKernelGGL(k_fer_moments, dim3((unsigned)info->n_ftiles), dim3(256), 0, s, v.ftiles,
+                     v.pds, D, mask, gp, M);
+  MEMI_CHECK_LAUNCH("k_fer_moments");
+  hipLaunchKernelGGL(k_fer_sweep_moments, dim3((unsigned)((n_eps + 255) / 256), (unsigned)npd),
+                     dim3(256), 0, s, logEps, n_eps, gp, M, out);
+  MEMI_CHECK_LAUNCH("k_fer_sweep_moments");
+  // the threshold sums after the results, as memi_ferguson_sweep leaves them
+  hipLaunchKernelGGL(k_fer_thr_copy, dim3(grid_for(npd)), dim3(256), 0, s, npd, st,
+                     out + npd * (int64_t)n_eps * 3);
+  MEMI_CHECK_LAUNCH("k_fer_thr_copy");
   return MEMI_OK;
 }
 

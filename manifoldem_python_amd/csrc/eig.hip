@@ -229,15 +229,27 @@ __global__ __launch_bounds__(256) void k_orth_finish(const memi_pd_desc* __restr
 //     residual (0 when J = n); theta [npd][16] and S (row-major J x k at pd*m*16) stay as the
 //     final Ritz pairs.
 // piv: float64 scratch [npd][16][2][m].
-// Sturm count (eigenvalues below x) from LDS copies of alpha and beta^2
-__device__ inline int sturm_count(const double* al, const double* b2, int J, double x,
-                                  double pivmin) {
-  double q = al[0] - x;
-  int c = q < 0.0;
+// Sturm count (eigenvalues below x) from LDS copies of alpha and beta^2, division free: the
+// sign changes of p_0 = 1, p_i = (alpha_i - x) p_{i-1} - beta_{i-1}^2 p_{i-2} (a zero p_i
+// counts as a change, as a -pivmin pivot does in the ratio form), rescaled by exact powers of
+// two when |p| leaves [2^-600, 2^600].
+__device__ inline int sturm_count(const double* al, const double* b2, int J, double x) {
+  double pm = 1.0, p = al[0] - x;
+  if (p == 0.0) p = -1e-300;
+  int c = p < 0.0;
   for (int i = 1; i < J; ++i) {
-    if (fabs(q) < pivmin) q = -pivmin;
-    q = al[i] - x - b2[i - 1] / q;
-    c += q < 0.0;
+    double pn = (al[i] - x) * p - b2[i - 1] * pm;
+    if (pn == 0.0) pn = p < 0.0 ? 1e-300 : -1e-300;
+    c += (pn < 0.0) != (p < 0.0);
+    pm = p;
+    p = pn;
+    const double a = fabs(p);
+    if (a > 0x1p600 || a < 0x1p-600) {
+      int e;
+      frexp(a, &e);
+      pm = ldexp(pm, -e);
+      p = ldexp(p, -e);
+    }
   }
   return c;
 }
@@ -298,7 +310,7 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
     for (int it = 0; it < 200 && h - l > 2.2e-16 * fmax(fabs(l), fabs(h)) + 1e-300; ++it) {
       const double mid = 0.5 * (l + h);
       if (mid <= l || mid >= h) break;
-      if (sturm_count(al, b2, J, mid, pivmin) > idx) h = mid;
+      if (sturm_count(al, b2, J, mid) > idx) h = mid;
       else l = mid;
     }
     cand[t] = 0.5 * (l + h);

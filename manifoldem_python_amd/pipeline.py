@@ -532,13 +532,40 @@ def knn_mask(plan, D):
     return mask
 
 
-def ferguson_sums(plan, D, mask, logEps):
-    """a12 sums: (npd, n_eps) arrays of sum W, d log(sum W)/d log eps and #terms."""
+FERGUSON = "auto"   # "moments" (log-uniform eps grids), "direct", or "auto"
+
+
+def uniform_grid(logEps):
+    """(le0, dle) when logEps[e] = le0 + e*dle with dle > 0 (as np.arange builds it), else
+    None."""
+    le = np.asarray(logEps, dtype=np.float64)
+    if le.size < 2:
+        return None
+    dle = (le[-1] - le[0]) / (le.size - 1)
+    if not dle > 0:
+        return None
+    if np.max(np.abs(le - (le[0] + np.arange(le.size) * dle))) > 1e-9 * max(1.0, abs(dle)):
+        return None
+    return float(le[0]), float(dle)
+
+
+def ferguson_sums(plan, D, mask, logEps, method=None):
+    """a12 sums: (npd, n_eps) arrays of sum W, d log(sum W)/d log eps and #terms.  method:
+    "moments" (bucket moments, needs a log-uniform grid), "direct" (every entry at every eps)
+    or "auto" (moments when the grid allows)."""
     npd = plan.npd
     le = torch.as_tensor(np.asarray(logEps, dtype=np.float64), device=D.device)
     n_eps = le.shape[0]
     fer = torch.empty(npd * n_eps * 3 + 2 * npd, dtype=torch.float64, device=D.device)
-    nat.ferguson_sweep(plan, D, mask, le, fer)
+    method = method or FERGUSON
+    grid = uniform_grid(logEps) if method in ("auto", "moments") else None
+    if method == "moments" and grid is None:
+        raise ValueError("ferguson_sums: the moments method needs a log-uniform eps grid")
+    if grid is not None:
+        ws = torch.empty(npd * nat.FM_WS_PER_PD, dtype=torch.float64, device=D.device)
+        nat.ferguson_moments(plan, D, mask, le, grid[0], grid[1], fer, ws)
+    else:
+        nat.ferguson_sweep(plan, D, mask, le, fer)
     fer = fer.cpu().numpy()[:npd * n_eps * 3].reshape(npd, n_eps, 3)
     S = fer[:, :, 0]
     with np.errstate(divide="ignore", invalid="ignore"):

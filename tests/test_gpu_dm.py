@@ -167,3 +167,28 @@ def test_gpu_tanh_fit_matches_curve_fit(cuda):
         s_gpu = np.sqrt(2 * np.exp(-popt[p][1] / popt[p][0]))
         assert abs(s_gpu - s_ref) / s_ref < 1e-6, (p, s_gpu, s_ref)
         assert abs(R2[p] - R2r) < 1e-9
+
+
+def test_ferguson_moments_match_direct_sweep(cuda):
+    """Bucket-moment sums (memi_ferguson_moments) against the direct sweep over every entry at
+    every eps, for a batch of PDs with kNN and full kernels, duplicate points (zero entries)
+    and value spreads from one to eight decades, on the default 1501-point grid and a coarse
+    one."""
+    import torch
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    ns, ks = [150, 400, 257, 90], [150, 60, 257, 30]
+    D = [_distance_matrix(n, 40 + i) for i, n in enumerate(ns)]
+    D[1][5, 6] = D[1][6, 5] = 0.0                     # a duplicate pair: zero entries
+    D[2] = D[2] ** 4 / 1e12                           # eight decades of values
+    rows0 = np.concatenate(([0], np.cumsum(ns)[:-1]))
+    plan = nat.Plan(rows0, ns, ks, chunks=1)
+    flat = torch.as_tensor(np.concatenate([d.astype(np.float32).ravel() for d in D]),
+                           device=cuda)
+    mask = pl.knn_mask(plan, flat)
+    for le in (pl.LOG_EPS, np.arange(-40, 40.5, 0.5)):
+        S1, sl1, c1 = pl.ferguson_sums(plan, flat, mask, le, method="moments")
+        S0, sl0, c0 = pl.ferguson_sums(plan, flat, mask, le, method="direct")
+        np.testing.assert_array_equal(c1, c0)
+        ok = S0 > 0
+        assert np.max(np.abs(np.log(S1[ok]) - np.log(S0[ok]))) < 1e-9
+        assert np.max(np.abs(sl1[ok] - sl0[ok])) < 1e-9

@@ -83,3 +83,13 @@ def test_recursive_bspline_prefilter_equals_fft_division():
         k = np.arange(N)
         ref = np.real(np.fft.ifft(np.fft.fft(s) / ((4 + 2 * np.cos(2 * np.pi * k / N)) / 6)))
         assert np.max(np.abs(recursive(s) - ref)) < 1e-12 * np.max(np.abs(ref)), N
+
+
+def test_uniform_grid_detection():
+    """The moment-based Ferguson sweep is used only for log-uniform eps grids."""
+    from manifoldem_python_amd import pipeline as pl
+    g = pl.uniform_grid(pl.LOG_EPS)
+    assert g is not None and g[0] == -150.0 and abs(g[1] - 0.2) < 1e-12
+    assert pl.uniform_grid(np.array([0.0, 1.0, 3.0])) is None
+    assert pl.uniform_grid(np.array([1.0])) is None
+    assert pl.uniform_grid(np.array([2.0, 1.0, 0.0])) is None
