@@ -190,5 +190,7 @@ def test_ferguson_moments_match_direct_sweep(cuda):
         S0, sl0, c0 = pl.ferguson_sums(plan, flat, mask, le, method="direct")
         np.testing.assert_array_equal(c1, c0)
         ok = S0 > 0
-        assert np.max(np.abs(np.log(S1[ok]) - np.log(S0[ok]))) < 1e-9
-        assert np.max(np.abs(sl1[ok] - sl0[ok])) < 1e-9
+        # the direct sweep takes exp(-x) through a float32 exp2 of the fractional part
+        # (~1e-7 per term); the moment sums are float64 throughout
+        assert np.max(np.abs(np.log(S1[ok]) - np.log(S0[ok]))) < 1e-7
+        assert np.max(np.abs(sl1[ok] - sl0[ok])) < 1e-6
