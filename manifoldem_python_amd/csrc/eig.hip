@@ -254,8 +254,10 @@ __device__ inline int sturm_count(const double* al, const double* b2, int J, dou
   return c;
 }
 
-constexpr int RITZ_LDS_M = 4096;   // the largest m memi_lanczos_ritz accepts
-
+// MAXM: the largest m of the instantiation.  MAXM <= 512 keeps alpha, beta^2 and the twisted
+// factorisation's pivots in LDS (pivot (i, side, pair) at (2 i + side) * 16 + pair, so the
+// pairs' threads hit consecutive banks); larger MAXM keeps the pivots in `piv` (global).
+template <int MAXM>
 __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restrict__ pds,
                                                      const double* __restrict__ ab, int m,
                                                      int steps_done,
@@ -267,11 +269,13 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
                                                      double* __restrict__ res_out,
                                                      double* __restrict__ piv) {
   constexpr int KM = MEMI_RITZ_KMAX;
-  __shared__ double al[RITZ_LDS_M], b2[RITZ_LDS_M];
+  constexpr bool PLDS = MAXM <= 512;
+  __shared__ double al[MAXM], b2[MAXM];
+  __shared__ double pv_lds[PLDS ? 2 * MAXM * KM : 1];
   __shared__ double cand[2 * KM];
   __shared__ double sel[KM];
   __shared__ double resv[KM];
-  __shared__ double red[64];
+  __shared__ int done_sh;
   const int pd = blockIdx.x;
   if (!active[pd]) return;
   const int t = threadIdx.x;
@@ -289,8 +293,8 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
   for (int i = t; i < J; i += 64) {
     const double bl = i > 0 ? fabs(a[2 * (i - 1) + 1]) : 0.0;
     const double br = i + 1 < J ? fabs(a[2 * i + 1]) : 0.0;
-    lo = fmin(lo, a[2 * i] - bl - br);
-    hi = fmax(hi, a[2 * i] + bl + br);
+    lo = fmin(lo, al[i] - bl - br);
+    hi = fmax(hi, al[i] + bl + br);
     bmax = fmax(bmax, br);
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -302,12 +306,13 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
   lo -= 1e-14 * span;
   hi += 1e-14 * span;
   const double pivmin = fmax(1e-290, 1e-300 * bmax * bmax);
+  const double atol = 1e-15 * span;    // eigenvalues to 1e-15 of the spectrum's width
   // 1. candidate eigenvalues
   const int nc = J <= 2 * k ? J : 2 * k;
   if (t < nc) {
     const int idx = J <= 2 * k ? t : (t < k ? t : J - 2 * k + t);
     double l = lo, h = hi;
-    for (int it = 0; it < 200 && h - l > 2.2e-16 * fmax(fabs(l), fabs(h)) + 1e-300; ++it) {
+    for (int it = 0; it < 200 && h - l > fmax(2.2e-16 * fmax(fabs(l), fabs(h)), atol); ++it) {
       const double mid = 0.5 * (l + h);
       if (mid <= l || mid >= h) break;
       if (sturm_count(al, b2, J, mid) > idx) h = mid;
@@ -318,13 +323,12 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
   __syncthreads();
   // 2. selection: k largest |theta|, then descending theta
   if (t == 0) {
-    bool used[2 * KM];
-    for (int c = 0; c < nc; ++c) used[c] = false;
+    uint32_t used = 0u;                       // bit c: candidate c taken (nc <= 32)
     for (int q = 0; q < k; ++q) {
       int best = -1;
       for (int c = 0; c < nc; ++c)
-        if (!used[c] && (best < 0 || fabs(cand[c]) > fabs(cand[best]))) best = c;
-      used[best] = true;
+        if (!((used >> c) & 1u) && (best < 0 || fabs(cand[c]) > fabs(cand[best]))) best = c;
+      used |= 1u << best;
       sel[q] = cand[best];
     }
     for (int q = 1; q < k; ++q) {            // insertion sort, descending
@@ -338,73 +342,55 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
     }
   }
   __syncthreads();
-  // 3. eigenvectors by twisted factorisation (one thread per selected pair)
-  double* Sp = S + (int64_t)pd * m * KM;          // row-major J x k
+  // 3. twisted factorisation per selected pair: pivots, twist, then the vector's norm and last
+  //    component (the vector itself is written only once the PD is done)
+  double* pvb = PLDS ? pv_lds : piv + (int64_t)pd * 2 * m * KM;
+  auto DP = [&](int i) -> double& { return pvb[(2 * i) * KM + t]; };
+  auto DM = [&](int i) -> double& { return pvb[(2 * i + 1) * KM + t]; };
+  int r = 0;
+  double inv = 0.0, th = 0.0;
   if (t < k) {
-    const double th = sel[t];
-    double* dp = piv + (((int64_t)pd * KM + t) * 2) * m;
-    double* dm = dp + m;
+    th = sel[t];
     double q = al[0] - th;
-    dp[0] = q;
+    DP(0) = q;
     for (int i = 1; i < J; ++i) {
       if (fabs(q) < pivmin) q = -pivmin;
       q = al[i] - th - b2[i - 1] / q;
-      dp[i] = q;
+      DP(i) = q;
     }
     q = al[J - 1] - th;
-    dm[J - 1] = q;
+    DM(J - 1) = q;
     for (int i = J - 2; i >= 0; --i) {
       if (fabs(q) < pivmin) q = -pivmin;
       q = al[i] - th - b2[i] / q;
-      dm[i] = q;
+      DM(i) = q;
     }
-    int r = 0;
     double gbest = INFINITY;
     for (int i = 0; i < J; ++i) {
-      const double g = fabs(dp[i] + dm[i] - (al[i] - th));
+      const double g = fabs(DP(i) + DM(i) - (al[i] - th));
       if (g < gbest) {
         gbest = g;
         r = i;
       }
     }
-    double zz = 1.0, nrm = 1.0;
-    Sp[(int64_t)r * k + t] = 1.0;
+    // norm and last component from z_r = 1
+    double zz = 1.0, nrm = 1.0, zlast = r == J - 1 ? 1.0 : 0.0;
     for (int i = r - 1; i >= 0; --i) {
-      const double d = fabs(dp[i]) < pivmin ? -pivmin : dp[i];
+      const double d = fabs(DP(i)) < pivmin ? -pivmin : DP(i);
       zz = -(a[2 * i + 1] / d) * zz;
-      Sp[(int64_t)i * k + t] = zz;
       nrm += zz * zz;
     }
     zz = 1.0;
     for (int i = r; i + 1 < J; ++i) {
-      const double d = fabs(dm[i + 1]) < pivmin ? -pivmin : dm[i + 1];
+      const double d = fabs(DM(i + 1)) < pivmin ? -pivmin : DM(i + 1);
       zz = -(a[2 * i + 1] / d) * zz;
-      Sp[(int64_t)(i + 1) * k + t] = zz;
       nrm += zz * zz;
+      if (i + 1 == J - 1) zlast = zz;
     }
-    const double inv = 1.0 / sqrt(nrm);
-    for (int i = 0; i < J; ++i) Sp[(int64_t)i * k + t] *= inv;
+    inv = 1.0 / sqrt(nrm);
+    // 4. residual |beta_{J-1} s_{J-1}|
+    resv[t] = fabs(a[2 * (J - 1) + 1] * zlast * inv);
   }
-  __syncthreads();
-  // modified Gram-Schmidt over the k vectors (all 64 threads per dot product)
-  for (int c = 1; c < k; ++c) {
-    for (int b = 0; b < c; ++b) {
-      double d = 0.0;
-      for (int i = t; i < J; i += 64) d += Sp[(int64_t)i * k + c] * Sp[(int64_t)i * k + b];
-      for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
-      for (int i = t; i < J; i += 64) Sp[(int64_t)i * k + c] -= d * Sp[(int64_t)i * k + b];
-      __syncthreads();
-    }
-    double nn = 0.0;
-    for (int i = t; i < J; i += 64) nn += Sp[(int64_t)i * k + c] * Sp[(int64_t)i * k + c];
-    for (int off = 32; off > 0; off >>= 1) nn += __shfl_xor(nn, off, 64);
-    const double inv = 1.0 / sqrt(nn);
-    for (int i = t; i < J; i += 64) Sp[(int64_t)i * k + c] *= inv;
-    __syncthreads();
-  }
-  // 4. residuals and the decision
-  const double bJ = a[2 * (J - 1) + 1];
-  if (t < k) resv[t] = fabs(bJ * Sp[(int64_t)(J - 1) * k + t]);
   __syncthreads();
   if (t == 0) {
     double mx = 0.0, rmax = 0.0;
@@ -414,13 +400,53 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
     }
     const bool done = J >= n || J >= m || rmax <= tol * mx;
     for (int q = 0; q < KM; ++q) theta[(int64_t)pd * KM + q] = q < k ? sel[q] : 0.0;
+    done_sh = done;
     if (done) {
       active[pd] = 0;
       steps[pd] = J;
       res_out[pd] = J >= n ? 0.0 : rmax;
     }
   }
-  (void)red;
+  __syncthreads();
+  if (!done_sh) return;                               // uniform over the block
+  // 5. the converged vectors: the recurrences again, normalised, into S (row-major J x k)
+  double* Sp = S + (int64_t)pd * m * KM;
+  if (t < k) {
+    double zz = 1.0;
+    Sp[(int64_t)r * k + t] = inv;
+    for (int i = r - 1; i >= 0; --i) {
+      const double d = fabs(DP(i)) < pivmin ? -pivmin : DP(i);
+      zz = -(a[2 * i + 1] / d) * zz;
+      Sp[(int64_t)i * k + t] = zz * inv;
+    }
+    zz = 1.0;
+    for (int i = r; i + 1 < J; ++i) {
+      const double d = fabs(DM(i + 1)) < pivmin ? -pivmin : DM(i + 1);
+      zz = -(a[2 * i + 1] / d) * zz;
+      Sp[(int64_t)(i + 1) * k + t] = zz * inv;
+    }
+  }
+  __syncthreads();
+  // modified Gram-Schmidt, descending theta, only within clusters (relative gap < 1e-6):
+  // vectors of well-separated pairs are orthogonal to ~1e-16 / gap already
+  double mx = 0.0;
+  for (int q = 0; q < k; ++q) mx = fmax(mx, fabs(sel[q]));
+  for (int c = 1; c < k; ++c) {
+    for (int b = 0; b < c; ++b) {
+      if (fabs(sel[c] - sel[b]) > 1e-6 * mx) continue;   // uniform over the block
+      double d = 0.0;
+      for (int i = t; i < J; i += 64) d += Sp[(int64_t)i * k + c] * Sp[(int64_t)i * k + b];
+      for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+      for (int i = t; i < J; i += 64) Sp[(int64_t)i * k + c] -= d * Sp[(int64_t)i * k + b];
+      __syncthreads();
+      double nn = 0.0;
+      for (int i = t; i < J; i += 64) nn += Sp[(int64_t)i * k + c] * Sp[(int64_t)i * k + c];
+      for (int off = 32; off > 0; off >>= 1) nn += __shfl_xor(nn, off, 64);
+      const double iv = 1.0 / sqrt(nn);
+      for (int i = t; i < J; i += 64) Sp[(int64_t)i * k + c] *= iv;
+      __syncthreads();
+    }
+  }
 }
 
 // Ritz vectors Y_p = V_p S_p of every PD in one launch, one thread per row: V_p the n x J
@@ -531,8 +557,14 @@ int memi_lanczos_ritz(const void* plan_dev, const memi_plan_info* info, const do
   if (info->n_pd == 0) return MEMI_OK;
   const char* plan = (const char*)plan_dev;
   const memi_pd_desc* pds = (const memi_pd_desc*)(plan + info->off_pds);
-  hipLaunchKernelGGL(k_lanczos_ritz, dim3((unsigned)info->n_pd), dim3(64), 0, (hipStream_t)stream,
-                     pds, ab, m, steps_done, ks, tol, active, theta, S, steps, res, piv);
+  if (m <= 512)
+    hipLaunchKernelGGL(k_lanczos_ritz<512>, dim3((unsigned)info->n_pd), dim3(64), 0,
+                       (hipStream_t)stream, pds, ab, m, steps_done, ks, tol, active, theta, S,
+                       steps, res, piv);
+  else
+    hipLaunchKernelGGL(k_lanczos_ritz<4096>, dim3((unsigned)info->n_pd), dim3(64), 0,
+                       (hipStream_t)stream, pds, ab, m, steps_done, ks, tol, active, theta, S,
+                       steps, res, piv);
   MEMI_CHECK_LAUNCH("k_lanczos_ritz");
   return MEMI_OK;
 }
