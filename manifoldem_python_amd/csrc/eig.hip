@@ -149,30 +149,69 @@ __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict
   if (threadIdx.x == 0) H[(int64_t)pd * m + c] = s;
 }
 
-__global__ __launch_bounds__(256) void k_orth_update(const memi_pd_desc* __restrict__ pds,
-                                                     const double* __restrict__ V, int m, const int32_t* __restrict__ jp,
-                                                     const int32_t* __restrict__ active,
-                                                     double* __restrict__ W,
-                                                     const double* __restrict__ H) {
+// Many-PD form of the dot pass: a fixed grid whose waves walk the npd x (j + 1) live
+// (PD, column) pairs, one wave per pair (grids sized for the largest j leave most blocks
+// empty when there are hundreds of PDs).
+__global__ __launch_bounds__(256) void k_orth_dot_pairs(const memi_pd_desc* __restrict__ pds,
+                                                        int64_t npd,
+                                                        const double* __restrict__ V, int m,
+                                                        const int32_t* __restrict__ jp,
+                                                        const int32_t* __restrict__ active,
+                                                        const double* __restrict__ W,
+                                                        double* __restrict__ H) {
   const int j = *jp;
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t total = npd * (int64_t)(j + 1);
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < total;
+       q += nwaves) {
+    const int pd = (int)(q / (j + 1)), c = (int)(q % (j + 1));
+    if (!active[pd]) continue;                    // uniform over the wave
+    const memi_pd_desc P = pds[pd];
+    const int n = P.n;
+    if (j >= min(n, m)) continue;
+    const double* vc = V + (int64_t)m * P.row0 + (int64_t)c * n;
+    const double* w = W + P.row0;
+    double s = 0.0;
+    for (int r = lane; r < n; r += 64) s += vc[r] * w[r];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) H[(int64_t)pd * m + c] = s;
+  }
+}
+
+// Row-list form of the update pass: 64 consecutive rows of the plan per block (rows of two
+// PDs may share a block), 4 column groups each, w_r -= sum_c V[c][r] h[c].
+__global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restrict__ rows,
+                                                          int64_t n_rows,
+                                                          const memi_pd_desc* __restrict__ pds,
+                                                          const double* __restrict__ V, int m,
+                                                          const int32_t* __restrict__ jp,
+                                                          const int32_t* __restrict__ active,
+                                                          double* __restrict__ W,
+                                                          const double* __restrict__ H) {
   __shared__ double part[4][64];
-  const int pd = blockIdx.y;
-  if (!active[pd]) return;
-  const memi_pd_desc P = pds[pd];
-  const int n = P.n;
-  if (j >= min(n, m) || blockIdx.x * 64 >= n) return;   // uniform over the block
-  const int r = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = *jp;
+  const int64_t g = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int cg = threadIdx.x >> 6;
-  const double* Vp = V + (int64_t)m * P.row0;
-  const double* h = H + (int64_t)pd * m;
   double acc = 0.0;
-  if (r < n)
-    for (int c = cg; c <= j; c += 4) acc += Vp[(int64_t)c * n + r] * h[c];
+  bool live = false;
+  int64_t wrow = 0;
+  if (g < n_rows) {
+    const int pd = rows[2 * g], i = rows[2 * g + 1];
+    const memi_pd_desc P = pds[pd];
+    if (active[pd] && j < min(P.n, m)) {
+      live = true;
+      wrow = P.row0 + i;
+      const double* Vp = V + (int64_t)m * P.row0 + i;
+      const double* h = H + (int64_t)pd * m;
+      for (int c = cg; c <= j; c += 4) acc += Vp[(int64_t)c * P.n] * h[c];
+    }
+  }
   part[cg][threadIdx.x & 63] = acc;
   __syncthreads();
-  if (cg == 0 && r < n) {
+  if (cg == 0 && live) {
     const int t = threadIdx.x;
-    W[P.row0 + r] -= (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+    W[wrow] -= (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
   }
 }
 
@@ -515,12 +554,17 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const d
       // column blocks sized for the largest possible j (< m); blocks beyond j exit at once
       for (int pass = 0; pass < 2; ++pass) {
         double* Hp = H + (int64_t)pass * info->n_pd * m;
-        hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)m, npd), dim3(256), 0, s, pds, V, m, jp,
-                           active, W, Hp);
+        if (npd < 64) {       // few PDs: a 256-thread block per (column, PD)
+          hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)m, npd), dim3(256), 0, s, pds, V, m,
+                             jp, active, W, Hp);
+        } else {              // many PDs: waves walk the live (PD, column) pairs
+          hipLaunchKernelGGL(k_orth_dot_pairs, dim3(1024), dim3(256), 0, s, pds, info->n_pd, V,
+                             m, jp, active, W, Hp);
+        }
         MEMI_CHECK_LAUNCH("k_orth_dot");
-        hipLaunchKernelGGL(k_orth_update, dim3((unsigned)((info->max_n + 63) / 64), npd),
-                           dim3(256), 0, s, pds, V, m, jp, active, W, Hp);
-        MEMI_CHECK_LAUNCH("k_orth_update");
+        hipLaunchKernelGGL(k_orth_update_rows, dim3((unsigned)((info->n_rows + 63) / 64)),
+                           dim3(256), 0, s, rows, info->n_rows, pds, V, m, jp, active, W, Hp);
+        MEMI_CHECK_LAUNCH("k_orth_update_rows");
       }
       hipLaunchKernelGGL(k_orth_finish, dim3(npd), dim3(256), 0, s, pds, V, m, j_dev, active, W,
                          H, info->n_pd, ab);

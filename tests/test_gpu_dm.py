@@ -194,3 +194,35 @@ def test_ferguson_moments_match_direct_sweep(cuda):
         # (~1e-7 per term); the moment sums are float64 throughout
         assert np.max(np.abs(np.log(S1[ok]) - np.log(S0[ok]))) < 1e-7
         assert np.max(np.abs(sl1[ok] - sl0[ok])) < 1e-6
+
+
+@pytest.mark.parametrize("split", [True, False])
+def test_batched_lanczos_many_pds(cuda, split):
+    """80 PDs of 40-160 points: the many-PD orthogonalisation (waves walking the live
+    (PD, column) pairs, row-list updates) against dense eigh per PD."""
+    import torch
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    rng = np.random.default_rng(9)
+    ns = rng.integers(40, 160, size=80)
+    ks = np.minimum(ns, 30)
+    D = [_distance_matrix(int(n), 100 + i) for i, n in enumerate(ns)]
+    rows0 = np.concatenate(([0], np.cumsum(ns)[:-1]))
+    plan = nat.Plan(rows0, ns, ks, chunks=1)
+    flat = torch.as_tensor(np.concatenate([d.astype(np.float32).ravel() for d in D]),
+                           device=cuda)
+    mask = pl.knn_mask(plan, flat)
+    sigma = torch.as_tensor([np.sqrt(np.median(d)) for d in D], device=cuda)
+    L = torch.empty(plan.info.d_elems, dtype=torch.float64, device=cuda)
+    deg = torch.empty(2 * plan.info.n_rows, dtype=torch.float64, device=cuda)
+    nat.markov_normalize(plan, flat, mask, sigma, L, deg)
+    kk = [9] * len(ns)
+    res = pl.lanczos_eigs(plan, L, kk, split=split)
+    for p in range(0, len(ns), 7):
+        A = plan.block(L, p).cpu().numpy()
+        w, U = np.linalg.eigh(A)
+        sel = np.argsort(-np.abs(w), kind="stable")[:9]
+        sel = sel[np.argsort(-w[sel], kind="stable")]
+        lam, Y, J, r = res[p]
+        np.testing.assert_allclose(lam, w[sel], atol=1e-9)
+        for c in range(9):
+            assert abs(abs(np.dot(Y[:, c], U[:, sel[c]])) - 1) < 1e-6, (p, c)
