@@ -123,9 +123,10 @@ __global__ __launch_bounds__(256) void k_lanczos_orth(const memi_pd_desc* __rest
 
 
 // Grid-parallel CGS2 for large PDs (one workgroup per PD leaves most CUs idle when a batch
-// has few PDs): per pass, k_orth_dot (one wave per (PD, basis column)) then k_orth_update
-// (64 rows x 4 column groups per workgroup); k_orth_finish records alpha, beta and writes
-// v_{j+1}.  H: float64 [2][npd][m] coefficients of the two passes.
+// has few PDs): per pass, a dot kernel (k_orth_dot: one workgroup per (column, PD), or
+// k_orth_dot_pairs for many PDs) then k_orth_update_rows (64 rows x 4 column groups per
+// workgroup); k_orth_finish records alpha, beta and writes v_{j+1}.  H: float64
+// [2][npd][m] coefficients of the two passes.
 __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict__ pds,
                                                   const double* __restrict__ V, int m,
                                                   const int32_t* __restrict__ jp,
