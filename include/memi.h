@@ -118,7 +118,7 @@ int memi_spline_rotate(const float* coef, const double* angle_deg, int64_t n, in
  * image resident in LDS (exact recursive B-spline prefilter in place of the FFT division of
  * memi_spectrum_scale + memi_spline_rotate).  in, out: float32 [n][N][N] (may be the same
  * buffer); 8 <= N <= MEMI_ALIGN_MAX_N. */
-#define MEMI_ALIGN_MAX_N 200
+#define MEMI_ALIGN_MAX_N 196
 int memi_align_images(const float* in, const double* rot1_deg, const double* rot2_deg, int64_t n,
                       int N, int disk, float* out, void* stream);
 

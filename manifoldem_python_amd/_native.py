@@ -188,7 +188,7 @@ def spline_rotate(coef, angle_deg, N, out):
     call("memi_spline_rotate", _p(coef), _p(angle_deg), coef.shape[0], N, _p(out), _stream())
 
 
-ALIGN_MIN_N, ALIGN_MAX_N = 8, 200
+ALIGN_MIN_N, ALIGN_MAX_N = 8, 196
 
 
 def align_images(img, rot1_deg, rot2_deg, N, out, disk=True):
@@ -199,7 +199,8 @@ def align_images(img, rot1_deg, rot2_deg, N, out, disk=True):
     _need(out.shape == img.shape and out.dtype == torch.float32, "align_images: out shape")
     _need(rot1_deg.numel() == n and rot2_deg.numel() == n and rot1_deg.dtype == torch.float64
           and rot2_deg.dtype == torch.float64, "align_images: angles must be float64 (n,)")
-    _need(ALIGN_MIN_N <= N <= ALIGN_MAX_N, f"align_images: N={N} outside [8, {ALIGN_MAX_N}]")
+    _need(ALIGN_MIN_N <= N <= ALIGN_MAX_N,
+          f"align_images: N={N} outside [{ALIGN_MIN_N}, {ALIGN_MAX_N}]")
     call("memi_align_images", _p(img), _p(rot1_deg), _p(rot2_deg), n, N, int(bool(disk)),
          _p(out), _stream())
 

@@ -216,7 +216,8 @@ constexpr int BSPLINE_NT = 24;
 // Each line is cut into nseg segments handled by one thread each (N lines x nseg <= block):
 // every segment starts from its own truncated periodic sum, so the recursions are short and
 // independent.  Phases: causal starts (read only), causal recursion, anticausal starts (read
-// the finished c+), anticausal recursion; the block synchronises between phases.
+// the finished c+), anticausal recursion; the block synchronises between phases.  The start
+// sums run in two pieces (before and after the line's wrap) so no term needs a modulo.
 template <bool ROWS>
 __device__ inline void bspline_lines(float* buf, int N, int S, double zN) {
   const int es = ROWS ? 1 : S;
@@ -231,10 +232,15 @@ __device__ inline void bspline_lines(float* buf, int N, int S, double zN) {
   float* x = buf + (ROWS ? l * S : l);
   double c0 = 0.0;
   if (act) {
+    // sum_{j < nt} z^j x[a - j mod N]: j <= a directly, then from the line's end
     double zj = 1.0;
-    for (int j = 0; j < nt; ++j) {
-      const int i = a - j;
-      c0 += zj * (double)x[(i < 0 ? i + N : i) * es];
+    const int j1 = min(a + 1, nt);
+    for (int j = 0; j < j1; ++j) {
+      c0 += zj * (double)x[(a - j) * es];
+      zj *= z;
+    }
+    for (int j = j1; j < nt; ++j) {
+      c0 += zj * (double)x[(N + a - j) * es];
       zj *= z;
     }
     c0 /= 1.0 - zN;
@@ -251,10 +257,15 @@ __device__ inline void bspline_lines(float* buf, int N, int S, double zN) {
   __syncthreads();
   double cm = 0.0;
   if (act) {
+    // sum_{j < nt} z^j c+[e - 1 + j mod N]: up to the line's end, then from its start
     double zj = 1.0;
-    for (int j = 0; j < nt; ++j) {
-      const int i = e - 1 + j;
-      cm += zj * (double)x[(i >= N ? i - N : i) * es];
+    const int j1 = min(N - e + 1, nt);
+    for (int j = 0; j < j1; ++j) {
+      cm += zj * (double)x[(e - 1 + j) * es];
+      zj *= z;
+    }
+    for (int j = j1; j < nt; ++j) {
+      cm += zj * (double)x[(e - 1 + j - N) * es];
       zj *= z;
     }
     cm *= -z / (1.0 - zN);
@@ -270,54 +281,63 @@ __device__ inline void bspline_lines(float* buf, int N, int S, double zN) {
   }
 }
 
+// The image is stored with row stride S = N + 3 and three extra rows and columns that repeat
+// rows / columns 0..2 (the periodic continuation), so the 4x4 spline taps never wrap.
+__device__ inline void refresh_pad(float* buf, int N, int S) {
+  for (int q = threadIdx.x; q < N * 3; q += blockDim.x) {
+    const int r = q / 3, c = q % 3;
+    buf[r * S + N + c] = buf[r * S + c];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 3 * S; q += blockDim.x) {
+    const int r = q / S, c = q % S;
+    buf[(N + r) * S + c] = buf[r * S + c];
+  }
+  __syncthreads();
+}
+
 __device__ inline void bspline_prefilter(float* buf, int N, int S, double zN) {
   bspline_lines<true>(buf, N, S, zN);
   __syncthreads();
   bspline_lines<false>(buf, N, S, zN);
   __syncthreads();
+  refresh_pad(buf, N, S);
 }
 
-// Sample the coefficient image `src` (LDS, row stride S) rotated by angle_deg into dst (row
-// stride ds).  Source rows/columns are clamped to [-N+2, 2N-3] before the taps, which changes
-// nothing for finite angles (|src - h| <= sqrt(2) h) and keeps every tap index in one wrap of
-// [0, N) for any input.
+// Sample the coefficient image `src` (LDS, row stride S, padded as above) rotated by
+// angle_deg into dst (row stride ds).  Source rows/columns are clamped to [-N+2, 2N-3] before
+// the taps, which changes nothing for finite angles (|src - h| <= sqrt(2) h) and, with one
+// wrap of the first tap into [0, N), keeps all four taps inside the padded image.
 __device__ inline void spline_sample(const float* src, int N, int S, double angle_deg, float* dst,
                                      int ds) {
   double sn, cs;
   sincospi(angle_deg / 180.0, &sn, &cs);
   const double h = 0.5 * (N - 1);
+  const double lo = 2 - N, hi = 2 * N - 3;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int r = wv; r < N; r += nw) {
     const double dr = r - h;
+    const double br = h + cs * dr, bc = h - sn * dr;
     for (int c = lane; c < N; c += 64) {
       const double dc = c - h;
-      const double sr = h + cs * dr + sn * dc;
-      const double sc = h - sn * dr + cs * dc;
+      const double sr = br + sn * dc;
+      const double sc = bc + cs * dc;
       const double fr = floor(sr), fc = floor(sc);
       float wr[4], wc[4];
       bspline_w((float)(sr - fr), wr);
       bspline_w((float)(sc - fc), wc);
-      const int ir = (int)fmin(fmax(fr, (double)(2 - N)), (double)(2 * N - 3)) - 1;
-      const int ic = (int)fmin(fmax(fc, (double)(2 - N)), (double)(2 * N - 3)) - 1;
-      int cols[4];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        int t = ic + b;
-        t += t < 0 ? N : 0;
-        t -= t >= N ? N : 0;
-        cols[b] = t;
-      }
+      int ir = (int)fmin(fmax(fr, lo), hi) - 1;
+      int ic = (int)fmin(fmax(fc, lo), hi) - 1;
+      ir += ir < 0 ? N : 0;
+      ir -= ir >= N ? N : 0;
+      ic += ic < 0 ? N : 0;
+      ic -= ic >= N ? N : 0;
+      const float* p = src + ir * S + ic;
       float acc = 0.f;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        int t = ir + a;
-        t += t < 0 ? N : 0;
-        t -= t >= N ? N : 0;
-        const float* rowp = src + t * S;
-        float racc = 0.f;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) racc += wc[b] * rowp[cols[b]];
-        acc += wr[a] * racc;
+      for (int q = 0; q < 4; ++q) {
+        const float* rp = p + q * S;
+        acc += wr[q] * (wc[0] * rp[0] + wc[1] * rp[1] + wc[2] * rp[2] + wc[3] * rp[3]);
       }
       dst[r * ds + c] = acc;
     }
@@ -331,8 +351,8 @@ template <int NMAX, int NBUF>
 __global__ __launch_bounds__(512) void k_align_images(const float* in, const double* ang1,
                                                       const double* ang2, int N, int disk,
                                                       float* out) {
-  __shared__ float lds[NBUF * NMAX * (NMAX + 1)];
-  const int S = N + 1;
+  __shared__ float lds[NBUF * (NMAX + 3) * (NMAX + 3)];
+  const int S = N + 3;
   const int64_t img = blockIdx.x;
   const float* src = in + img * N * N;
   float* A = lds;
@@ -351,7 +371,7 @@ __global__ __launch_bounds__(512) void k_align_images(const float* in, const dou
   bspline_prefilter(A, N, S, zN);
   float* o = out + img * N * N;
   if constexpr (NBUF == 2) {
-    float* B = lds + NMAX * (NMAX + 1);
+    float* B = lds + (NMAX + 3) * (NMAX + 3);
     spline_sample(A, N, S, ang1[img], B, S);
     __syncthreads();
     bspline_prefilter(B, N, S, zN);
@@ -572,10 +592,10 @@ int memi_align_images(const float* in, const double* rot1_deg, const double* rot
   const dim3 g((unsigned)n), b(512);
   if (N <= 64) {
     hipLaunchKernelGGL((k_align_images<64, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
-  } else if (N <= 96) {          // 74.5 KB of LDS: two blocks per CU
+  } else if (N <= 96) {          // 78 KB of LDS: two blocks per CU
     hipLaunchKernelGGL((k_align_images<96, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
-  } else if (N <= 141) {
-    hipLaunchKernelGGL((k_align_images<141, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
+  } else if (N <= 140) {
+    hipLaunchKernelGGL((k_align_images<140, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
   } else {
     hipLaunchKernelGGL((k_align_images<MEMI_ALIGN_MAX_N, 1>), g, b, 0, s, in, rot1_deg, rot2_deg,
                        N, disk, out);
