@@ -112,6 +112,23 @@ int memi_mask_multiply(float* img, int64_t n, int N, int disk, const float* msk2
 int memi_spline_rotate(const float* coef, const double* angle_deg, int64_t n, int N,
                        float* out, void* stream);
 
+/* Image preparation in one kernel, getDistanceCTF_local_Conj9combinedS2.py:244-349: for each
+ * of the n images (stack index src[i], conjugate flag conj[i] or conj = NULL), load +
+ * normalise (a1), Butterworth / Gaussian filter (a2, filter_kind as memi_spectrum_scale, no
+ * B-spline bit), disk mask and the rotations by rot1_deg[i] then rot2_deg[i] (a4), the
+ * half spectrum fy = rfft2 (complex64 [n][N][N/2+1], a6), the CTF (float32 [n][N(N/2+1)],
+ * a5) and the packed operand rows row0 + i (as memi_ctf_pack).  img_out: float32 [n][N][N]
+ * aligned images, or NULL.  The image stays in LDS from load to operand (mixed-radix FFTs in
+ * LDS); 8 <= N <= MEMI_PREP_MAX_N with prime factors 2, 3, 5, 7.  ws: float64 scratch of
+ * 6 N(N/2+1) doubles (per-bin tables). */
+#define MEMI_PREP_MAX_N 128
+int memi_prep_images(const float* stack, int64_t n_stack, const int64_t* src,
+                     const int32_t* conj, const double* rot1_deg, const double* rot2_deg,
+                     const double* df, int64_t n, int N, int filter_kind, double Qc,
+                     int order, const memi_ctf_params* ctf, int precision, int64_t Khp,
+                     int64_t row0, void* op_hi, void* op_lo, void* fy, float* ctf_half,
+                     float* img_out, double* ws, int64_t ws_doubles, void* stream);
+
 /* Fused in-plane alignment, getDistanceCTF_local_Conj9combinedS2.py:315-330 with
  * rotatefill.py:21-28 and annularMask.py:19-32: optionally the disk mask (disk != 0), then
  * the periodic cubic-spline rotation by rot1_deg[i] followed by the one by rot2_deg[i], each

@@ -54,6 +54,9 @@ _SIGS = {
     "memi_mask_multiply": [c_vp, c_i64, c_int, c_int, c_vp, c_vp],
     "memi_spline_rotate": [c_vp, c_vp, c_i64, c_int, c_vp, c_vp],
     "memi_align_images": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp],
+    "memi_prep_images": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dbl,
+                         c_int, ctypes.POINTER(CtfParams), c_int, c_i64, c_i64, c_vp, c_vp, c_vp,
+                         c_vp, c_vp, c_vp, c_i64, c_vp],
     "memi_ctf_pack": [c_vp, c_vp, c_i64, c_int, ctypes.POINTER(CtfParams), c_int, c_i64, c_i64,
                       c_vp, c_vp, c_vp, c_vp],
     "memi_phase_flip": [c_vp, c_vp, c_i64, c_int, c_vp, c_vp],
@@ -203,6 +206,53 @@ def align_images(img, rot1_deg, rot2_deg, N, out, disk=True):
           f"align_images: N={N} outside [{ALIGN_MIN_N}, {ALIGN_MAX_N}]")
     call("memi_align_images", _p(img), _p(rot1_deg), _p(rot2_deg), n, N, int(bool(disk)),
          _p(out), _stream())
+
+
+PREP_MAX_N = 128
+
+
+def prep_supported(N):
+    """True when memi_prep_images handles side N (8..128, prime factors 2, 3, 5, 7)."""
+    if not 8 <= N <= PREP_MAX_N:
+        return False
+    n = N
+    for p in (2, 3, 5, 7):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+def prep_images(stack, src, conj, rot1, rot2, df, N, filter_kind, Qc, order, ctf, precision,
+                Khp, row0, op_hi, op_lo, fy, ctf_half, ws, img_out=None, src_max=None):
+    """memi_prep_images: a1-a6 for len(src) images in one kernel (see include/memi.h)."""
+    n = int(src.shape[0])
+    Kh = N * (N // 2 + 1)
+    _need(prep_supported(N), f"prep_images: N={N} not supported by the fused kernel")
+    _need(stack.dtype == torch.float32 and tuple(stack.shape[1:]) == (N, N),
+          "prep_images: stack must be float32 (n_stack, N, N)")
+    _need(src.dtype == torch.int64 and (conj is None or (conj.dtype == torch.int32 and
+                                                        conj.numel() == n)),
+          "prep_images: src int64 / conj int32")
+    for t, name in ((rot1, "rot1"), (rot2, "rot2"), (df, "df")):
+        _need(t.dtype == torch.float64 and t.numel() == n, f"prep_images: {name} float64 (n,)")
+    if n:
+        hi = int(src.max()) if src_max is None else int(src_max)
+        lo = int(src.min()) if src_max is None else 0
+        _need(lo >= 0 and hi < stack.shape[0], "prep_images: image index outside the stack")
+    esz = 4 if precision == PREC_FP32 else 2
+    _need(_bytes(op_hi) >= (row0 + n) * 4 * Khp * esz and
+          (precision == PREC_FP32 or _bytes(op_lo) >= (row0 + n) * 4 * Khp * esz),
+          "prep_images: operand smaller than row0 + n rows")
+    _need(fy.dtype == torch.complex64 and fy.numel() >= n * Kh and ctf_half.numel() >= n * Kh,
+          "prep_images: fy / ctf sizes")
+    _need(ws.dtype == torch.float64 and ws.numel() >= 6 * Kh, "prep_images: ws too small")
+    _need(img_out is None or (img_out.dtype == torch.float32 and img_out.numel() >= n * N * N),
+          "prep_images: img_out size")
+    call("memi_prep_images", _p(stack), stack.shape[0], _p(src), None if conj is None else _p(conj),
+         _p(rot1), _p(rot2), _p(df), n, N, int(filter_kind), float(Qc), int(order),
+         ctypes.byref(ctf), int(precision), int(Khp), int(row0), _p(op_hi),
+         _p(op_lo) if op_lo is not None else None, _p(fy), _p(ctf_half),
+         None if img_out is None else _p(img_out), _p(ws), ws.numel(), _stream())
 
 
 def ctf_params(pix_size, cs, ekv, ampc, gauss_env):

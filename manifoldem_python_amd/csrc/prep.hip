@@ -2,6 +2,7 @@
 // Reference: getDistanceCTF_local_Conj9combinedS2.py:244-367 and its leaves
 // annularMask.py:19-32, rotatefill.py:21-28, ctemh_cryoFrank.py:24-44.
 #include "common.h"
+#include "fft_lds.h"
 
 namespace memi {
 
@@ -509,6 +510,154 @@ __global__ void k_image_moments(const float* __restrict__ img, int64_t n, int64_
   }
 }
 
+// ---- fused image preparation (a1-a6 in one kernel) ------------------------------------
+// Per half-spectrum bin k (row-major [ky][kx]): {Butterworth factor / N^2, a_k, c_k,
+// envelope, w_k, sqrt(w_k)} with the CTF phase wr = a_k - c_k df (ctemh_cryoFrank.py:24-44).
+constexpr int PT = 6;
+
+__global__ void k_prep_tables(int N, int kind, double Qc, int order, memi_ctf_params cp,
+                              double* __restrict__ tab) {
+  const int Nh = N / 2 + 1;
+  const int64_t Kh = (int64_t)N * Nh;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < Kh;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int row = (int)(k / Nh), col = (int)(k % Nh);
+    double f = 1.0;
+    if (kind & (MEMI_FILTER_BUTTER | MEMI_FILTER_GAUSS)) {
+      const double q = radial_q(row, col, N) / Qc;
+      if (kind & MEMI_FILTER_BUTTER) f *= sqrt(1.0 / (1.0 + pow(q, 2.0 * order)));
+      if (kind & MEMI_FILTER_GAUSS) f *= exp(-(log(2.0) / 2.0) * q * q);
+    }
+    const double kk = radial_q(row, col, N) / (2.0 * cp.pix_size);
+    const double cs = cp.cs_mm * 1.0e7;
+    const double wav = 12.3986 / sqrt((2.0 * 511.0 + cp.ekv) * cp.ekv);
+    const double k2 = kk * kk;
+    double env = 1.0;
+    if (isfinite(cp.gauss_env)) {
+      const double sigm = cp.gauss_env / sqrt(2.0 * log(2.0));
+      env = exp(-k2 / (2.0 * sigm * sigm));
+    }
+    const double w = half_weight(col, N);
+    double* t = tab + k * PT;
+    t[0] = f / ((double)N * N);
+    t[1] = 0.5 * (M_PI * cs * wav * wav * wav) * k2 * k2;
+    t[2] = M_PI * wav * k2;
+    t[3] = env;
+    t[4] = w;
+    t[5] = sqrt(w);
+  }
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(512) void k_prep_images(
+    const float* __restrict__ stack, const int64_t* __restrict__ src,
+    const int32_t* __restrict__ conj, const double* __restrict__ rot1,
+    const double* __restrict__ rot2, const double* __restrict__ df, int N,
+    const double* __restrict__ tab, double ampc, int prec, int64_t Khp, int64_t row0,
+    void* __restrict__ op_hi, void* __restrict__ op_lo, float2* __restrict__ fy,
+    float* __restrict__ ctf_half, float* __restrict__ img_out) {
+  constexpr int BUF = (NMAX + 3) * (NMAX + 3);
+  __shared__ float lds[2 * BUF];
+  __shared__ cpx tw[NMAX];
+  __shared__ double red[16];
+  float* A = lds;
+  float* B = lds + BUF;
+  const int S = N + 3;
+  const int Nh = N / 2 + 1;
+  const int64_t Kh = (int64_t)N * Nh;
+  const int64_t img = blockIdx.x;
+  int rad[8];
+  const int nrad = radix_plan(N, rad);
+  fft_twiddles(tw, N);
+  // a1: load (conjugates flipped, SPIDER transpose) and background statistics
+  const float* raw = stack + src[img] * (int64_t)N * N;
+  const bool cj = conj != nullptr && conj[img] != 0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double s1 = 0.0, s2 = 0.0;
+  for (int a = wv; a < N; a += nw)
+    for (int b = lane; b < N; b += 64) {
+      const float v = raw[a * N + b];
+      const int r = cj ? N - 1 - b : b;           // raw[a][b] lands at (r, a)
+      A[r * S + a] = v;
+      const int xd = 2 * r - N + 2, yd = 2 * a - N;
+      if (xd * xd + yd * yd >= N * N) {
+        s1 += (double)v;
+        s2 += (double)v * v;
+      }
+    }
+  const double NN = (double)N * N;
+  const double mean = block_sum(s1, red) / NN;
+  const double sd = sqrt(block_sum(s2, red + 8) / NN - mean * mean);
+  for (int r = wv; r < N; r += nw)
+    for (int c = lane; c < N; c += 64) A[r * S + c] = (float)(((double)A[r * S + c] - mean) / sd);
+  __syncthreads();
+  // a2: Butterworth in frequency (the table folds in the 1/N^2 of the inverse)
+  cpx* H = rfft2_lds(A, B, N, S, tw, rad, nrad);
+  for (int64_t q = threadIdx.x; q < Kh; q += blockDim.x) {
+    const float f = (float)tab[q * PT];
+    H[q] = make_float2(H[q].x * f, H[q].y * f);
+  }
+  __syncthreads();
+  irfft2_lds(H, H == (cpx*)A ? (cpx*)B : (cpx*)A, A, N, S, tw, rad, nrad);
+  // a4: disk mask, then the two spline rotations (A -> B -> A)
+  for (int r = wv; r < N; r += nw)
+    for (int c = lane; c < N; c += 64) {
+      const int xd = 2 * r - N + 2, yd = 2 * c - N;
+      if (xd * xd + yd * yd >= N * N) A[r * S + c] = 0.f;
+    }
+  __syncthreads();
+  const double zN = pow(BSPLINE_Z, (double)N);
+  bspline_prefilter(A, N, S, zN);
+  spline_sample(A, N, S, rot1[img], B, S);
+  __syncthreads();
+  bspline_prefilter(B, N, S, zN);
+  spline_sample(B, N, S, rot2[img], A, S);
+  __syncthreads();
+  if (img_out) {
+    float* o = img_out + img * (int64_t)N * N;
+    for (int r = wv; r < N; r += nw)
+      for (int c = lane; c < N; c += 64) o[r * N + c] = A[r * S + c];
+  }
+  // a5/a6: spectrum of the aligned image, CTF, operand rows (as k_ctf_pack)
+  cpx* F = rfft2_lds(A, B, N, S, tw, rad, nrad);
+  const double dfi = df[img];
+  const double inv2pi = 0.15915494309189535, twopi = 6.283185307179586;
+  const int64_t base = (row0 + img) * 4 * Khp;
+  for (int64_t q = threadIdx.x; q < Khp; q += blockDim.x) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (q < Kh) {
+      const double* t = tab + q * PT;
+      const double wr = t[1] - t[2] * dfi;
+      const double rd = wr - twopi * rint(wr * inv2pi);
+      float sf, cf;
+      sincosf((float)rd, &sf, &cf);
+      const double C = ((double)sf - ampc * (double)cf) * t[3];
+      const cpx X = F[q];
+      ctf_half[img * Kh + q] = (float)C;
+      fy[img * Kh + q] = X;
+      v[0] = t[4] * ((double)X.x * X.x + (double)X.y * X.y);
+      v[1] = C * C;
+      v[2] = t[5] * C * X.x;
+      v[3] = t[5] * C * X.y;
+    }
+    if (prec == MEMI_PREC_FP32) {
+      float* o = (float*)op_hi;
+#pragma unroll
+      for (int sg = 0; sg < 4; ++sg) o[base + sg * Khp + q] = (float)v[sg];
+    } else {
+      uint16_t* oh = (uint16_t*)op_hi;
+      uint16_t* ol = (uint16_t*)op_lo;
+#pragma unroll
+      for (int sg = 0; sg < 4; ++sg) {
+        const float f = (float)v[sg];
+        const uint16_t h = bf16_rne(f);
+        oh[base + sg * Khp + q] = h;
+        ol[base + sg * Khp + q] = bf16_rne((float)((double)f - (double)bf16_to_f(h)));
+      }
+    }
+  }
+}
+
 }  // namespace memi
 
 using namespace memi;
@@ -604,6 +753,47 @@ int memi_align_images(const float* in, const double* rot1_deg, const double* rot
                        N, 0, out);
   }
   MEMI_CHECK_LAUNCH("k_align_images");
+  return MEMI_OK;
+}
+
+int memi_prep_images(const float* stack, int64_t n_stack, const int64_t* src,
+                     const int32_t* conj, const double* rot1_deg, const double* rot2_deg,
+                     const double* df, int64_t n, int N, int filter_kind, double Qc,
+                     int order, const memi_ctf_params* ctf, int precision, int64_t Khp,
+                     int64_t row0, void* op_hi, void* op_lo, void* fy, float* ctf_half,
+                     float* img_out, double* ws, int64_t ws_doubles, void* stream) {
+  MEMI_CHECK_ARG(stack && src && rot1_deg && rot2_deg && df && ctf && op_hi && fy && ctf_half &&
+                 ws && n >= 0 && n_stack >= 0, "memi_prep_images: null pointer or bad size");
+  MEMI_CHECK_ARG(precision == MEMI_PREC_FP32 || (precision == MEMI_PREC_SPLIT_BF16 && op_lo),
+                 "memi_prep_images: precision %d needs its operand buffers", precision);
+  int rad[8];
+  MEMI_CHECK_ARG(N >= 8 && N <= MEMI_PREP_MAX_N && radix_plan(N, rad) > 0,
+                 "memi_prep_images: N=%d unsupported (8 <= N <= %d, factors 2, 3, 5, 7)", N,
+                 MEMI_PREP_MAX_N);
+  const int64_t Kh = (int64_t)N * (N / 2 + 1);
+  MEMI_CHECK_ARG(Khp >= Kh && Khp % 64 == 0, "memi_prep_images: bad Khp %lld", (long long)Khp);
+  MEMI_CHECK_ARG(ws_doubles >= Kh * PT, "memi_prep_images: workspace %lld < %lld doubles",
+                 (long long)ws_doubles, (long long)(Kh * PT));
+  MEMI_CHECK_ARG((filter_kind & ~(MEMI_FILTER_BUTTER | MEMI_FILTER_GAUSS)) == 0 &&
+                     (!filter_kind || Qc > 0),
+                 "memi_prep_images: bad filter kind %d / Qc", filter_kind);
+  MEMI_CHECK_ARG(ctf->pix_size > 0, "memi_prep_images: pixel size must be positive");
+  if (n == 0) return MEMI_OK;
+  MEMI_CHECK_ARG(n < ((int64_t)1 << 31), "memi_prep_images: too many images");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_prep_tables, dim3(grid_for(Kh)), dim3(256), 0, s, N, filter_kind, Qc,
+                     order, *ctf, ws);
+  MEMI_CHECK_LAUNCH("k_prep_tables");
+  const dim3 g((unsigned)n), b(512);
+#define MEMI_PREP(NM)                                                                          \
+  hipLaunchKernelGGL(k_prep_images<NM>, g, b, 0, s, stack, src, conj, rot1_deg, rot2_deg, df, N, \
+                     ws, ctf->ampc, precision, Khp, row0, op_hi, op_lo, (float2*)fy, ctf_half,  \
+                     img_out)
+  if (N <= 64) MEMI_PREP(64);
+  else if (N <= 96) MEMI_PREP(96);
+  else MEMI_PREP(128);
+#undef MEMI_PREP
+  MEMI_CHECK_LAUNCH("k_prep_images");
   return MEMI_OK;
 }
 

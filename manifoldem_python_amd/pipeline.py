@@ -128,20 +128,43 @@ def filter_kind(filterPar):
 
 
 ALIGN = "auto"      # "fused": memi_align_images; "fft": FFT prefilter + memi_spline_rotate
+PREP = "auto"       # "fused": memi_prep_images (one kernel); "staged": the kernel sequence
 
 
 def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, operand=None,
                    row0=0, msk2=None, fy_out=None, ctf_out=None, want_images=True, src_max=None,
-                   align=None):
+                   align=None, prep=None):
     """a1-a6 for one set of images (all device tensors): load + normalise, Butterworth,
     disk mask, two cubic-spline rotations, fy = rfft2(img * msk2), CTF and operand rows.
-    align: "fused" (one kernel with the images in LDS, 8 <= N <= 200), "fft" (B-spline
-    prefilter by FFT division, any N) or "auto"/None (fused when N allows).
+    prep: "fused" (memi_prep_images: the whole chain in one kernel, images in LDS), "staged"
+    (rocFFT transforms with the elementwise kernels between them) or "auto"/None (fused when N
+    and the inputs allow).  align (staged only): "fused" (memi_align_images, 8 <= N <= 196),
+    "fft" (B-spline prefilter by FFT division, any N) or "auto"/None.
 
     Returns a dict of device tensors: imgAll (n,N,N) f32, fy (n,N,Nh) c64, ctf (n,Kh) f32.
     """
     n = int(src.shape[0])
     dev = stack.device
+    prep = prep or PREP
+    if prep == "auto":
+        prep = "fused" if (nat.prep_supported(N) and msk2 is None and
+                           filterPar.get("type", "Butter") in ("Butter", "Gauss")) else "staged"
+    if prep == "fused":
+        if msk2 is not None:
+            raise ValueError("prepare_images: the fused kernel has no mask-volume input")
+        op = operand if operand is not None else Operand(n, N, nat.PREC_FP32, dev)
+        r0 = row0 if operand is not None else 0
+        fy = (fy_out.view(n, N, N // 2 + 1) if fy_out is not None else
+              torch.empty((n, N, N // 2 + 1), dtype=torch.complex64, device=dev))
+        ctf_half = ctf_out if ctf_out is not None else torch.empty((n, half_len(N)),
+                                                                  dtype=torch.float32, device=dev)
+        img = torch.empty((n, N, N), dtype=torch.float32, device=dev) if want_images else None
+        ws = torch.empty(6 * half_len(N), dtype=torch.float64, device=dev)
+        nat.prep_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filter_kind(filterPar),
+                        filterPar["Qc"], filterPar["N"], ctf, op.precision, op.Khp, r0, op.hi,
+                        op.lo if op.lo is not None else op.hi, fy, ctf_half, ws, img_out=img,
+                        src_max=src_max)
+        return dict(imgAll=img, fy=fy, ctf=ctf_half)
     y = torch.empty((n, N, N), dtype=torch.float32, device=dev)
     nat.load_normalize(stack, src, conj, N, y, src_max=src_max)
     Y = torch.fft.rfft2(y)

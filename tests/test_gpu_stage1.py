@@ -23,7 +23,7 @@ def _oracle(s):
     return od.op([s.ind, s.q, s.df, None, 0], FILTER, s.raw, s.nStot, dict(avgOnly=False), p)
 
 
-def _gpu_prep(s, cuda, precision):
+def _gpu_prep(s, cuda, precision, prep=None):
     import torch
     from manifoldem_python_amd import _native as nat, pipeline as pl
     N = s.N
@@ -40,7 +40,8 @@ def _gpu_prep(s, cuda, precision):
     ctf = nat.ctf_params(PARAMS["pix_size"], PARAMS["Cs"], PARAMS["EkV"], PARAMS["AmpContrast"],
                          PARAMS["gaussEnv"])
     op = pl.Operand(n, N, precision, cuda)
-    out = pl.prepare_images(stack, t_src, t_conj, rot1, rot2, df, N, FILTER, ctf, operand=op)
+    out = pl.prepare_images(stack, t_src, t_conj, rot1, rot2, df, N, FILTER, ctf, operand=op,
+                            prep=prep)
     return geo, out, op
 
 
@@ -234,3 +235,35 @@ def test_gram_uneven_chunks_vs_float64(cuda, precision, chunks):
     assert rel_offdiag(D, D0).max() < 2e-5
     np.testing.assert_array_equal(D, D.T)
     assert np.all(np.diag(D) == 0)
+
+
+def _operand_values(op):
+    import torch
+    if op.lo is None:
+        return op.hi.cpu().numpy().astype(np.float64)
+    f = lambda t: (t.cpu().numpy().astype(np.int32) & 0xFFFF).astype(np.uint32) << 16
+    return (f(op.hi).view(np.float32).astype(np.float64) +
+            f(op.lo).view(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("N", [40, 45, 70, 96, 128])
+def test_fused_prep_matches_staged(cuda, precision, N):
+    """memi_prep_images (load, filter, rotations, FFTs and CTF packing in one kernel with LDS
+    FFTs) against the staged path (rocFFT + separate kernels) on the same inputs: aligned
+    images, half spectra, CTF and operand rows."""
+    n = 37
+    s = _setup(n, N, seed=6, conj_frac=0.3)
+    _, a, opa = _gpu_prep(s, cuda, precision, prep="fused")
+    _, b, opb = _gpu_prep(s, cuda, precision, prep="staged")
+    ia, ib = a["imgAll"].cpu().numpy(), b["imgAll"].cpu().numpy()
+    assert np.max(np.abs(ia - ib)) < 2e-6 * np.max(np.abs(ib))
+    fa, fb = a["fy"].cpu().numpy(), b["fy"].cpu().numpy()
+    assert np.max(np.abs(fa - fb)) < 2e-6 * np.max(np.abs(fb))
+    np.testing.assert_allclose(a["ctf"].cpu().numpy(), b["ctf"].cpu().numpy(), atol=1e-6)
+    va, vb = _operand_values(opa), _operand_values(opb)
+    Kp = opa.Khp
+    for sg in range(4):       # per segment, relative to the segment's scale
+        xa = va.reshape(n, 4, Kp)[:, sg]
+        xb = vb.reshape(n, 4, Kp)[:, sg]
+        assert np.max(np.abs(xa - xb)) < 1e-5 * np.max(np.abs(xb)), sg
