@@ -111,6 +111,14 @@ struct Dft<5> : DftOdd<5> {};
 template <>
 struct Dft<7> : DftOdd<7> {};
 
+// q / d for 0 <= q < 2^22, d >= 1, from a float reciprocal and one correction each way
+__device__ inline int fdivi(int q, int d, float inv_d) {
+  int t = (int)((float)q * inv_d);
+  t += (t + 1) * d <= q;
+  t -= t * d > q;
+  return t;
+}
+
 // One Stockham pass of radix R over L lines of length N (line l at base + l*ls, element i at
 // + i*es) from src to dst; Ns = product of the radices of the earlier passes.
 // y[(j/Ns) Ns R + j%Ns + r Ns] = DFT_R( x[j + q N/R] w_N^{q (j%Ns) N/(Ns R)} )_r
@@ -120,11 +128,13 @@ __device__ inline void stockham_pass(const cpx* src, cpx* dst, int N, int L, int
   const int m = N / R;
   const int total = L * m;
   const int tstep = N / (Ns * R);
+  const float inv_m = 1.0f / m, inv_ns = 1.0f / Ns;
   for (int q = threadIdx.x; q < total; q += blockDim.x) {
-    const int line = q / m, j = q - line * m;
+    const int line = fdivi(q, m, inv_m), j = q - line * m;
     const cpx* s = src + line * ls;
     cpx* d = dst + line * ls;
-    const int k = j % Ns;
+    const int jg = fdivi(j, Ns, inv_ns);
+    const int k = j - jg * Ns;
     cpx v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = s[(j + r * m) * es];
@@ -135,7 +145,7 @@ __device__ inline void stockham_pass(const cpx* src, cpx* dst, int N, int L, int
       v[r] = cmul(v[r], w);
     }
     Dft<R>::run(v, inv);
-    const int idx = (j / Ns) * Ns * R + k;
+    const int idx = jg * Ns * R + k;
 #pragma unroll
     for (int r = 0; r < R; ++r) d[(idx + r * Ns) * es] = v[r];
   }
@@ -182,8 +192,9 @@ __device__ inline cpx* rfft2_lds(float* re, float* other, int N, int S, const cp
                                  const int* rad, int nrad) {
   const int Nh = N / 2 + 1, P = (N + 1) / 2;       // row pairs
   cpx* X = (cpx*)other;
+  const float inv_n = 1.0f / N, inv_nh = 1.0f / Nh;
   for (int q = threadIdx.x; q < P * N; q += blockDim.x) {
-    const int l = q / N, c = q - l * N;
+    const int l = fdivi(q, N, inv_n), c = q - l * N;
     const float a = re[(2 * l) * S + c];
     const float b = 2 * l + 1 < N ? re[(2 * l + 1) * S + c] : 0.f;
     X[l * N + c] = make_float2(a, b);
@@ -193,7 +204,7 @@ __device__ inline cpx* rfft2_lds(float* re, float* other, int N, int S, const cp
   cpx* H = Z == X ? (cpx*)re : X;
   // unpack: X_a[k] = (Z[k] + conj Z[-k]) / 2, X_b[k] = (Z[k] - conj Z[-k]) / (2i)
   for (int q = threadIdx.x; q < P * Nh; q += blockDim.x) {
-    const int l = q / Nh, k = q - l * Nh;
+    const int l = fdivi(q, Nh, inv_nh), k = q - l * Nh;
     const cpx z = Z[l * N + k], zc = Z[l * N + (k == 0 ? 0 : N - k)];
     H[(2 * l) * Nh + k] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
     if (2 * l + 1 < N)
@@ -208,11 +219,12 @@ __device__ inline cpx* rfft2_lds(float* re, float* other, int N, int S, const cp
 __device__ inline void irfft2_lds(cpx* H, cpx* spare, float* dst_re, int N, int S,
                                   const cpx* tw, const int* rad, int nrad) {
   const int Nh = N / 2 + 1, P = (N + 1) / 2;
+  const float inv_n = 1.0f / N;
   cpx* G = fft_lines(H, spare, N, Nh, 1, Nh, tw, rad, nrad, true);
   cpx* Zb = G == H ? spare : H;
   // row pairs: Z[k] = G_a[k] + i G_b[k] with G_r[k] = conj G_r[N-k] for k > N/2
   for (int q = threadIdx.x; q < P * N; q += blockDim.x) {
-    const int l = q / N, k = q - l * N;
+    const int l = fdivi(q, N, inv_n), k = q - l * N;
     const bool lowk = k < Nh;
     const int kk = lowk ? k : N - k;
     cpx ga = G[(2 * l) * Nh + kk];
@@ -228,7 +240,7 @@ __device__ inline void irfft2_lds(cpx* H, cpx* spare, float* dst_re, int N, int 
   // Z is in Zb or G; write the real rows into dst_re, which must not be Z's buffer
   float* tmp = (float*)(Z == Zb ? G : Zb);
   for (int q = threadIdx.x; q < P * N; q += blockDim.x) {
-    const int l = q / N, c = q - l * N;
+    const int l = fdivi(q, N, inv_n), c = q - l * N;
     const cpx z = Z[l * N + c];
     tmp[(2 * l) * S + c] = z.x;
     if (2 * l + 1 < N) tmp[(2 * l + 1) * S + c] = z.y;

@@ -574,17 +574,30 @@ __global__ __launch_bounds__(512) void k_prep_images(
   const bool cj = conj != nullptr && conj[img] != 0;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double s1 = 0.0, s2 = 0.0;
-  for (int a = wv; a < N; a += nw)
-    for (int b = lane; b < N; b += 64) {
-      const float v = raw[a * N + b];
+  // every load of this thread in flight at once, then the LDS stores and the statistics
+  constexpr int PER = (NMAX * NMAX + 511) / 512;
+  const int NNi = N * N;
+  float vv[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int p = threadIdx.x + u * 512;
+    vv[u] = p < NNi ? raw[p] : 0.f;
+  }
+  const float inv_n = 1.0f / N;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int p = threadIdx.x + u * 512;
+    if (p < NNi) {
+      const int a = fdivi(p, N, inv_n), b = p - a * N;
       const int r = cj ? N - 1 - b : b;           // raw[a][b] lands at (r, a)
-      A[r * S + a] = v;
+      A[r * S + a] = vv[u];
       const int xd = 2 * r - N + 2, yd = 2 * a - N;
       if (xd * xd + yd * yd >= N * N) {
-        s1 += (double)v;
-        s2 += (double)v * v;
+        s1 += (double)vv[u];
+        s2 += (double)vv[u] * vv[u];
       }
     }
+  }
   const double NN = (double)N * N;
   const double mean = block_sum(s1, red) / NN;
   const double sd = sqrt(block_sum(s2, red + 8) / NN - mean * mean);

@@ -129,6 +129,7 @@ def filter_kind(filterPar):
 
 ALIGN = "auto"      # "fused": memi_align_images; "fft": FFT prefilter + memi_spline_rotate
 PREP = "auto"       # "fused": memi_prep_images (one kernel); "staged": the kernel sequence
+PREP_FUSED_MAX_N = 96   # "auto" uses the fused kernel up to this image side
 
 
 def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, operand=None,
@@ -147,7 +148,10 @@ def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, 
     dev = stack.device
     prep = prep or PREP
     if prep == "auto":
-        prep = "fused" if (nat.prep_supported(N) and msk2 is None and
+        # measured on MI355X: the one-kernel path wins while two images fit in a CU's LDS
+        # (N <= 96: 89.6 vs 97.5 ms for 237k 96x96 images); at N = 128 one image per CU
+        # leaves its LDS FFTs latency-bound and rocFFT + the staged kernels are faster
+        prep = "fused" if (nat.prep_supported(N) and N <= PREP_FUSED_MAX_N and msk2 is None and
                            filterPar.get("type", "Butter") in ("Butter", "Gauss")) else "staged"
     if prep == "fused":
         if msk2 is not None:
