@@ -298,56 +298,72 @@ def ferguson_fit(logEps, logSumWij, a0, rand=None):
     return popt, resnorm, R_squared
 
 
-def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200):
+def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_per_round=8):
     """The fit of ``fergusonE.py:45-57`` for many PDs at once on the GPU (memi_tanh_fit):
     fit from a0, pcov = inv(J^T J) ss_res / (m - 4) as scipy's curve_fit forms it, and refit
-    from rand(4,1) - 0.5 while sum(sqrt|diag pcov|) > 100.  Returns (popt (npd, 4), R^2)."""
+    from rand(4,1) - 0.5 while sum(sqrt|diag pcov|) > 100.  Returns (popt (npd, 4), R^2).
+
+    Each round fits up to `starts_per_round` starts of every unfinished PD in parallel: its
+    pending start followed by the next draws in the order the reference's loop makes them (one
+    after every fit), and keeps the first acceptable one in that order -- the same result as
+    the sequential restarts, in fewer round trips."""
     rand = np.random.rand if rand is None else rand
     dev = device()
     npd, m = logS.shape
     x = torch.as_tensor(np.ascontiguousarray(logEps, dtype=np.float64), device=dev)
-    y = torch.as_tensor(np.ascontiguousarray(logS, dtype=np.float64), device=dev)
-    p0 = np.stack([np.ravel(a) for a in a0s]).astype(np.float64)
+    y_all = torch.as_tensor(np.ascontiguousarray(logS, dtype=np.float64), device=dev)
+    pending = np.stack([np.ravel(a) for a in a0s]).astype(np.float64)
     popt = np.zeros((npd, 4))
     R2 = np.zeros(npd)
     todo = np.ones(npd, dtype=bool)
     ss_tot = np.sum((logS - logS.mean(axis=1, keepdims=True)) ** 2, axis=1)
-    out = torch.empty(npd * 16, dtype=torch.float64, device=dev)
     best = np.full(npd, np.inf)
     iu = np.triu_indices(4)
-    for _ in range(max_restarts):
-        act = torch.as_tensor(todo.astype(np.int32), device=dev)
-        nat.tanh_fit(x, y, torch.as_tensor(p0, device=dev), out, act)
-        o = out.cpu().numpy().reshape(npd, 16)
+    fits = 0
+    while todo.any() and fits < max_restarts:
         idx = np.nonzero(todo)[0]
-        oo = o[idx]
-        J = np.zeros((len(idx), 4, 4))
-        J[:, iu[0], iu[1]] = oo[:, 4:14]
-        J = J + np.triu(J, 1).transpose(0, 2, 1)
-        ss_res = oo[:, 14]
+        R = int(max(1, min(starts_per_round, max_restarts - fits)))
+        starts = np.empty((len(idx), R, 4))
+        for q, p in enumerate(idx):
+            starts[q, 0] = pending[p]
+            for r in range(1, R):
+                starts[q, r] = np.ravel(1 * (rand(4, 1) - 0.5))
+        B = len(idx) * R
+        yy = y_all[torch.as_tensor(np.repeat(idx, R), device=dev)].contiguous()
+        out = torch.empty(B * 16, dtype=torch.float64, device=dev)
+        act = torch.ones(B, dtype=torch.int32, device=dev)
+        nat.tanh_fit(x, yy, torch.as_tensor(starts.reshape(B, 4), device=dev), out, act)
+        o = out.cpu().numpy().reshape(len(idx), R, 16)
+        J = np.zeros((len(idx), R, 4, 4))
+        J[:, :, iu[0], iu[1]] = o[:, :, 4:14]
+        J = J + np.triu(J, 1).transpose(0, 1, 3, 2)
+        ss_res = o[:, :, 14]
         with np.errstate(all="ignore"):
+            Jf = J.reshape(-1, 4, 4)
             try:
-                Jinv = np.linalg.inv(J)
+                Jinv = np.linalg.inv(Jf)
             except np.linalg.LinAlgError:        # a singular J^T J somewhere: one at a time
-                Jinv = np.full_like(J, np.nan)
-                for q in range(len(idx)):
+                Jinv = np.full_like(Jf, np.nan)
+                for q in range(len(Jf)):
                     try:
-                        Jinv[q] = np.linalg.inv(J[q])
+                        Jinv[q] = np.linalg.inv(Jf[q])
                     except np.linalg.LinAlgError:
                         pass
-            diag = np.diagonal(Jinv, axis1=1, axis2=2) * (ss_res / (m - 4))[:, None]
-            resnorm = np.sum(np.sqrt(np.fabs(diag)), axis=1)
+            diag = np.diagonal(Jinv, axis1=1, axis2=2).reshape(len(idx), R, 4)
+            resnorm = np.sum(np.sqrt(np.fabs(diag * (ss_res / (m - 4))[:, :, None])), axis=2)
         resnorm[~np.isfinite(resnorm)] = np.inf
-        for p in idx:                                # fergusonE.py:57: a new start after each fit
-            p0[p] = np.ravel(1 * (rand(4, 1) - 0.5))
-        better = resnorm < best[idx]
-        bi = idx[better]
-        best[bi] = resnorm[better]
-        popt[bi] = oo[better, 0:4]
-        R2[bi] = 1 - ss_res[better] / ss_tot[bi]
-        todo[idx[resnorm <= 100]] = False
-        if not todo.any():
-            break
+        for q, p in enumerate(idx):
+            for r in range(R):                   # in the order the sequential loop would try
+                if resnorm[q, r] < best[p]:
+                    best[p] = resnorm[q, r]
+                    popt[p] = o[q, r, 0:4]
+                    R2[p] = 1 - ss_res[q, r] / ss_tot[p]
+                if resnorm[q, r] <= 100:
+                    todo[p] = False
+                    break
+            if todo[p]:
+                pending[p] = np.ravel(1 * (rand(4, 1) - 0.5))   # the draw after the last fit
+        fits += R
     return popt, R2
 
 
