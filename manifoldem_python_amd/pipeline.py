@@ -303,10 +303,11 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
     fit from a0, pcov = inv(J^T J) ss_res / (m - 4) as scipy's curve_fit forms it, and refit
     from rand(4,1) - 0.5 while sum(sqrt|diag pcov|) > 100.  Returns (popt (npd, 4), R^2).
 
-    Each round fits up to `starts_per_round` starts of every unfinished PD in parallel: its
-    pending start followed by the next draws in the order the reference's loop makes them (one
-    after every fit), and keeps the first acceptable one in that order -- the same result as
-    the sequential restarts, in fewer round trips."""
+    The first round fits every PD's own start; each later round fits up to `starts_per_round`
+    starts of every unfinished PD in parallel: its pending start followed by the next draws in
+    the order the reference's loop makes them (one after every fit), keeping the first
+    acceptable one in that order -- the same result as sequential restarts, in fewer round
+    trips."""
     rand = np.random.rand if rand is None else rand
     dev = device()
     npd, m = logS.shape
@@ -322,7 +323,8 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
     fits = 0
     while todo.any() and fits < max_restarts:
         idx = np.nonzero(todo)[0]
-        R = int(max(1, min(starts_per_round, max_restarts - fits)))
+        # most PDs accept their first start: one fit each first, then several per round
+        R = 1 if fits == 0 else int(max(1, min(starts_per_round, max_restarts - fits)))
         starts = np.empty((len(idx), R, 4))
         for q, p in enumerate(idx):
             starts[q, 0] = pending[p]
