@@ -326,10 +326,10 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
         # most PDs accept their first start: one fit each first, then several per round
         R = 1 if fits == 0 else int(max(1, min(starts_per_round, max_restarts - fits)))
         starts = np.empty((len(idx), R, 4))
-        for q, p in enumerate(idx):
-            starts[q, 0] = pending[p]
-            for r in range(1, R):
-                starts[q, r] = np.ravel(1 * (rand(4, 1) - 0.5))
+        starts[:, 0] = pending[idx]
+        if R > 1:       # PD after PD, in the order per-PD rand(4, 1) calls would consume them
+            starts[:, 1:] = np.reshape(rand(len(idx) * (R - 1) * 4, 1),
+                                       (len(idx), R - 1, 4)) - 0.5
         B = len(idx) * R
         yy = y_all[torch.as_tensor(np.repeat(idx, R), device=dev)].contiguous()
         out = torch.empty(B * 16, dtype=torch.float64, device=dev)
@@ -354,17 +354,23 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
             diag = np.diagonal(Jinv, axis1=1, axis2=2).reshape(len(idx), R, 4)
             resnorm = np.sum(np.sqrt(np.fabs(diag * (ss_res / (m - 4))[:, :, None])), axis=2)
         resnorm[~np.isfinite(resnorm)] = np.inf
-        for q, p in enumerate(idx):
-            for r in range(R):                   # in the order the sequential loop would try
-                if resnorm[q, r] < best[p]:
-                    best[p] = resnorm[q, r]
-                    popt[p] = o[q, r, 0:4]
-                    R2[p] = 1 - ss_res[q, r] / ss_tot[p]
-                if resnorm[q, r] <= 100:
-                    todo[p] = False
-                    break
-            if todo[p]:
-                pending[p] = np.ravel(1 * (rand(4, 1) - 0.5))   # the draw after the last fit
+        # per PD, in the order the sequential loop tries them: the first acceptable start;
+        # if none, the best of this round competes with the best so far
+        ok = resnorm <= 100
+        hit = ok.any(axis=1)
+        first = np.argmax(ok, axis=1)
+        pick = np.where(hit, first, np.argmin(resnorm, axis=1))
+        rq = np.arange(len(idx))
+        rn = resnorm[rq, pick]
+        upd = hit | (rn < best[idx])
+        pu = idx[upd]
+        best[pu] = rn[upd]
+        popt[pu] = o[rq[upd], pick[upd], 0:4]
+        R2[pu] = 1 - ss_res[rq[upd], pick[upd]] / ss_tot[pu]
+        todo[idx[hit]] = False
+        rest = idx[~hit]
+        if len(rest):   # the draw after each unfinished PD's last fit
+            pending[rest] = np.reshape(rand(len(rest) * 4, 1), (len(rest), 4)) - 0.5
         fits += R
     return popt, R2
 
