@@ -186,7 +186,7 @@ int memi_plan(const memi_pd_desc* pds_in_host, int npd, int chunks, void* plan_h
  * chunks of ceil(steps / chunks) steps, the last one shorter; a chunk count that leaves an
  * empty chunk is rejected.
  * variant: Gram kernel schedule, 0 = default, 1 = prefetch one K step (two workgroups per
- * CU), 2 = prefetch two K steps (split-BF16: one workgroup per CU).
+ * CU), 2 = prefetch two K steps (two workgroups per CU as well).
  * ev_start / ev_stop: optional hipEvent_t recorded on `stream` right before and after the
  * MFMA Gram kernel (NULL: not recorded), for timing that kernel alone. */
 int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const void* op_hi,
