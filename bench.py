@@ -12,6 +12,15 @@ U(0.5, 3.5) um, kNN k=200, 10 eigenvectors.  --workload c3: 500 PDs, lognormal s
 
 value = image-pair distances per second over the whole step (split-BF16 Gram);
 fp32 = the same with the plain FP32 MFMA Gram.
+
+Secondary workloads (their own JSON line, same contract):
+  --workload c4: the differentiable distance -> Gaussian kernel -> log-sum op at n=1000,
+    128x128, 41 eps values 2^-20..2^20 x median(D); a step is forward + hand-written backward
+    (dL/d eps and dL/dX).
+  --workload c5: the full per-PD path with NLSA: PDs of n=1500 160x160 particles (shift
+    8 px), stage 1 + diffusion map (8 psi), then NLSA (ConOrder 50, 8 topos) along each of the
+    8 psi.  --c5-pds PDs are timed end to end; the 200-PD figure is extrapolated from them
+    and labelled so.
 """
 from __future__ import annotations
 
@@ -35,7 +44,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--c5-pds", type=int, default=2,
+                    help="PDs of the NLSA workload timed end to end (c5 only)")
     ap.add_argument("--lanczos-graph", type=int, default=1,
                     help="replay blocks of Lanczos steps from a captured HIP graph (default 1)")
     ap.add_argument("--lanczos-split", type=int, default=None,
@@ -248,6 +259,199 @@ def cpu_baseline(meta, batch):
                 sample=sample, seconds_per_step=t_total)
 
 
+def _ctf():
+    from manifoldem_python_amd import synth, _native as nat
+    return nat.ctf_params(synth.PIX, synth.CS, synth.EKV, synth.AMPC, np.inf)
+
+
+def run_c4(args, dev):
+    """Forward + backward of the fused distance -> Gaussian kernel -> log-sum op."""
+    import torch
+    from manifoldem_python_amd import synth, autograd as ag, pipeline as pl, _native as nat
+    n, N = 1000, 128
+    s = synth.make_pd(synth.seeds(5)[3], n, N, device=dev, as_torch=True)
+    ctf = _ctf()
+    g = pl.pd_geometry(s.q)
+    out = pl.prepare_images(s.raw, torch.arange(n, device=dev),
+                            torch.zeros(n, dtype=torch.int32, device=dev),
+                            torch.as_tensor(-(180 / math.pi) * g["Psi"], device=dev),
+                            torch.full((n,), -g["psi_p"], dtype=torch.float64, device=dev),
+                            torch.as_tensor(s.df, device=dev), N, FILTER, ctf, want_images=False)
+    fy0 = out["fy"].contiguous()
+    df = torch.as_tensor(s.df, device=dev)
+    P = n * (n - 1) / 2
+    Kh = pl.half_len(N)
+    res = {}
+    for name, prec in (("fp32", nat.PREC_FP32), ("split-bf16", nat.PREC_SPLIT_BF16)):
+        fy = fy0.clone().requires_grad_(True)
+        plan = nat.Plan([0], [n], [n], chunks=pl.CHUNKS[prec], device=dev)
+        op = pl.Operand(n, N, prec, dev)
+        ch = torch.empty((n, Kh), dtype=torch.float32, device=dev)
+        nat.ctf_pack(fy0, df, N, ctf, prec, op.Khp, 0, op.hi, op.lo if op.lo is not None else op.hi,
+                     ch)
+        Dm = pl.ctf_distances(plan, op, fy0, ch, N).block(0)
+        med = float(Dm[~torch.eye(n, dtype=torch.bool, device=dev)].median())
+        eps = (2.0 ** torch.linspace(-20, 20, 41, dtype=torch.float64, device=dev)) * med
+        eps = eps.requires_grad_(True)
+
+        def step():
+            fy.grad = None
+            eps.grad = None
+            L = ag.kernel_logsum(fy, df, eps, ctf, prec)
+            L.sum().backward()
+            return L
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            with torch.no_grad():
+                ag.kernel_logsum(fy.detach(), df, eps.detach(), ctf, prec)
+        torch.cuda.synchronize()
+        t_fwd = (time.perf_counter() - t0) / args.steps
+        res[name] = dict(ms_per_step=1e3 * t, forward_ms=1e3 * t_fwd, value=P / t,
+                         grad_finite=bool(torch.isfinite(fy.grad).all() and
+                                          torch.isfinite(eps.grad).all()))
+    out = {
+        "metric": "image-pair distances/s through forward + backward of the kernel log-sum op",
+        "value": res["fp32"]["value"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 Gram and backward GEMMs, fp64 row sums", "data": "synthetic",
+        "config": dict(workload="c4_autograd", particles=n, image=N, n_eps=41,
+                       eps="2^-20..2^20 x median(D)", precision="fp32"),
+        "gram_flop_per_step": 8.0 * Kh * P,
+        "backward_gemm_flop_per_step": 3 * 2.0 * n * n * N * (N // 2 + 1),
+        "split_bf16": res["split-bf16"], "fp32": res["fp32"],
+    }
+    return out
+
+
+def run_c5(args, dev):
+    """PDs of n=1500 160x160 particles: stage 1 + diffusion map, then NLSA along 8 psi."""
+    import torch
+    from manifoldem_python_amd import synth, batch as B, pipeline as pl, spectral
+    from manifoldem_python_amd import _native as nat, autotune
+    n, N, con, n_psi = 1500, 160, 50, 8
+    npd = max(1, args.c5_pds)
+    ss = np.random.SeedSequence(synth.SEED + 5).spawn(npd)
+    stacks, pds = [], []
+    for p in range(npd):
+        rng = np.random.default_rng(ss[p])
+        s = synth.make_pd(ss[p], n, N, shift_px=8.0, axis=synth.random_axis(rng), device=dev,
+                          as_torch=True)
+        stacks.append(s.raw)
+        pds.append(dict(src=np.arange(n) + p * n, conj=np.zeros(n, np.int32), q=s.q, df=s.df,
+                        k=n))
+    batch = B.make_batch(torch.cat(stacks), pds, N, _ctf(), FILTER, dev)
+    del stacks
+    wsp = B.Workspace(batch, nat.PREC_SPLIT_BF16)
+    imgs = torch.empty((batch.rows, N, N), dtype=torch.float32, device=dev)
+    rng = np.random.default_rng(0)
+    rand = lambda *sh: rng.random(sh)
+    ev = nat.HipEvents()
+    rows0 = batch.rows0
+
+    def stage1(events=None):
+        for p in range(npd):
+            sl = slice(int(rows0[p]), int(rows0[p]) + n)
+            o = pl.prepare_images(batch.stack, batch.src[sl], batch.conj[sl], batch.rot1[sl],
+                                  batch.rot2[sl], batch.df[sl], N, FILTER, batch.ctf,
+                                  operand=wsp.operand, row0=int(rows0[p]), fy_out=wsp.fy[sl],
+                                  ctf_out=wsp.ctf[sl], want_images=True,
+                                  src_max=batch.stack.shape[0] - 1)
+            imgs[sl].copy_(o["imgAll"])
+        dist = pl.ctf_distances(wsp.plan, wsp.operand, wsp.fy, wsp.ctf, N, ws=wsp.ws,
+                                events=events, flags=wsp.flags)
+        return dist, B.embed(wsp, dist, num_eigs=n_psi, rand=rand)
+
+    def nlsa_pd(p, dist, res, psis):
+        sl = slice(int(rows0[p]), int(rows0[p]) + n)
+        D = dist.block(p)
+        out = []
+        for k in psis:
+            od = torch.as_tensor(np.argsort(res[p].psi[:, k]), device=dev)
+            out.append(spectral.nlsa(D, od, od, imgs[sl], wsp.ctf[sl], N, con, 8, nS=n, tune=3,
+                                     num_eigs=15, rand=rand))
+        return out
+
+    dist, res = stage1()
+    autotune.tune_gram(wsp, N)
+    log("c5 warmup")
+    for _ in range(max(1, args.warmup)):
+        dist, res = stage1()
+        nlsa_pd(0, dist, res, [0])
+    torch.cuda.synchronize()
+    steps = max(1, args.steps)
+    pair = [ev.new_pair() for _ in range(steps)]
+    t1 = 0.0
+    t_psi = []
+    t_start = time.perf_counter()
+    for k in range(steps):
+        a = time.perf_counter()
+        dist, res = stage1(pair[k])
+        torch.cuda.synchronize()
+        t1 += time.perf_counter() - a
+        log(f"c5 step {k}: stage 1 + DM {1e3 * (time.perf_counter() - a):.1f} ms")
+        for p in range(npd):
+            for j in range(n_psi):
+                a = time.perf_counter()
+                nlsa_pd(p, dist, res, [j])
+                torch.cuda.synchronize()
+                t_psi.append(time.perf_counter() - a)
+    t_all = time.perf_counter() - t_start
+    gram_ms = float(np.mean([ev.elapsed_ms(q) for q in pair]))
+    ev.close()
+    # every NLSA call of the last step again, split by stage (device synchronised between
+    # stages, so these sum to a little more than the timed calls)
+    per_call = []
+    for p in range(npd):
+        sl = slice(int(rows0[p]), int(rows0[p]) + n)
+        for j in range(n_psi):
+            tm = pl.StageTimer()
+            od = torch.as_tensor(np.argsort(res[p].psi[:, j]), device=dev)
+            spectral.nlsa(dist.block(p), od, od, imgs[sl], wsp.ctf[sl], N, con, 8, nS=n,
+                          tune=3, num_eigs=15, rand=rand, timer=tm)
+            per_call.append(tm.t)
+    stage_names = list(per_call[0])
+    t_pd = t_all / steps / npd
+    P = n * (n - 1) / 2
+    Kh = pl.half_len(N)
+    F = 8.0 * Kh * P * npd
+    ach = F / (gram_ms / 1e3) / 1e12
+    out = {
+        "metric": "PDs/s through the full per-PD path with NLSA (8 psi per PD)",
+        "value": 1.0 / t_pd, "unit": "PDs/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t_all / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3 (split) Gram, fp32 prep/NLSA transforms, fp64 NLSA sums/SVD/eigensolve",
+        "data": "synthetic",
+        "config": dict(workload="c5_nlsa", n_pd_timed=npd, particles=n, image=N,
+                       con_order=con, n_psi=n_psi, topos=8, knn="n (full kernel)"),
+        "stage1_dm_ms_per_step": 1e3 * t1 / steps,
+        "nlsa_ms_per_psi": dict(mean=1e3 * float(np.mean(t_psi)),
+                                min=1e3 * float(np.min(t_psi)), max=1e3 * float(np.max(t_psi))),
+        "ms_per_pd": 1e3 * t_pd,
+        "extrapolated_200_pd_s": 200 * t_pd,
+        "extrapolation_note": "200-PD time = 200 x measured per-PD time of the timed PDs "
+                              "(stage 1 batched over them, NLSA run per psi)",
+        "nlsa_stage_ms": {k: dict(mean=1e3 * float(np.mean([c[k] for c in per_call])),
+                                  max=1e3 * float(np.max([c[k] for c in per_call])))
+                          for k in stage_names},
+        "gram_kernel_ms": gram_ms,
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_BF16_TFLOPS,
+                     "unit": "TFLOP/s", "frac": ach / PEAK_BF16_TFLOPS, "traffic": None,
+                     "kernel": "k_gram_tiles<split-bf16>"},
+    }
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -264,6 +468,10 @@ def main():
     nat.load()
     pl.LANCZOS.update(graph=bool(args.lanczos_graph),
                       split=None if args.lanczos_split is None else bool(args.lanczos_split))
+    if args.workload in ("c4", "c5"):
+        log("timing", args.workload)
+        print(json.dumps((run_c4 if args.workload == "c4" else run_c5)(args, dev)), flush=True)
+        return
     log("building workload", args.workload)
     batch, meta, num_eigs = build_workload(args.workload, dev)
     torch.cuda.synchronize()

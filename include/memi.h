@@ -323,9 +323,31 @@ int memi_tall_apply(const double* A, int64_t M, int c, const double* B_host, int
 int memi_nlsa_imgt(const double* Ubar, const double* H, const double* inv_sd, int64_t npix,
                    int q_dim, int ncols, double* IMGT, void* stream);
 
+/* Distances between the z-scored reconstructed frames, NLSA.py:112-146 with
+ * L2_distance.py:8-41, from the factored Gram: T = Gq H, Gram = H^T T (Gq float64
+ * [q_dim][q_dim] = Ubar^T Ubar, H float64 [q_dim][ncols]), inv_sd[j] = 1/sqrt(Gram_jj / npix),
+ * C_ij = Gram_ij inv_sd_i inv_sd_j, d_ij = C_ii + C_jj - 2 C_ij with d < 1e-8 set to 0,
+ * Drecon2[i][j] = (float)(sqrt(d_ij)^2) (float32 [ncols][ncols]).
+ * ws: float64 scratch of (q_dim + 1) * ncols.  Replaces the host products and L2_distance. */
+int memi_nlsa_recon_dist(const double* Gq, const double* H, int q_dim, int ncols, int64_t npix,
+                         double* inv_sd, float* Drecon2, double* ws, void* stream);
+
 /* Column-centre U2 (float64 [npix][q_dim]) in place and return its Gram Ubar^T Ubar
  * (float64 [q_dim][q_dim]); NLSA.py:112-138 + L2_distance.py:26-41 in factored form. */
 int memi_center_gram(double* U2, int64_t npix, int q_dim, double* G, void* stream);
+
+/* One tau update of the 1-D open-manifold fit, fit_1D_open_manifold_3D.py:64-160 with
+ * solve_d_R_d_tau_p_3D.py:35-60 and R_p.py:41-49 (replaces the per-point np.roots loop):
+ * for every point x_i (float64 [n][3]) and the curve ab_host = {a1, a2, a3, b1, b2, b3}
+ * (host), tau[i] = the candidate of least R_p among arccos(real roots in [-1, 1] of the
+ * quintic)/pi, 0 and 1.  flag[i] (int32) = 1 where the reference's LAPACK root classification
+ * or arg min is within rounding of a tie (near-double root, root at +-1, equal R, zero leading
+ * or trailing coefficient): the caller re-solves those points with np.roots.
+ * sums (float64 [13], device): sum cos^2(j pi tau) [3], sum cos(j pi tau) [3],
+ * sum x_j cos(j pi tau) [3], sum x_j [3], number flagged [1].
+ * ws: float64 scratch of 13 * ceil(n / 256). */
+int memi_fit1d_tau(const double* x, int n, const double* ab_host, double* tau, int32_t* flag,
+                   double* sums, double* ws, void* stream);
 
 /* ---------------- Differentiation of the fused distance -> kernel op --------------------- */
 

@@ -85,6 +85,8 @@ _SIGS = {
     "memi_tall_apply": [c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp],
     "memi_nlsa_imgt": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp],
     "memi_center_gram": [c_vp, c_i64, c_int, c_vp, c_vp],
+    "memi_nlsa_recon_dist": [c_vp, c_vp, c_int, c_int, c_i64, c_vp, c_vp, c_vp, c_vp],
+    "memi_fit1d_tau": [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_kernel_rowsums": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp],
     "memi_kernel_grad_weights": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
 }
@@ -552,6 +554,40 @@ def tall_apply(A, B_host, Y):
           Y.numel() >= A.shape[0] * B.shape[1] and Y.dtype == torch.float64, "tall_apply: shapes")
     call("memi_tall_apply", _p(A), A.shape[0], A.shape[1], c_vp(B.ctypes.data), B.shape[1],
          _p(Y), _stream())
+
+
+def nlsa_recon_dist(Gq, H, npix, inv_sd, Drecon2, ws):
+    q, ncols = H.shape
+    _need(Gq.dtype == torch.float64 and Gq.numel() == q * q and H.dtype == torch.float64 and
+          H.is_contiguous() and Gq.is_contiguous(), "nlsa_recon_dist: Gq (q, q), H (q, ncols)")
+    _need(inv_sd.dtype == torch.float64 and inv_sd.numel() >= ncols and
+          Drecon2.dtype == torch.float32 and Drecon2.numel() >= ncols * ncols and
+          ws.dtype == torch.float64 and ws.numel() >= (q + 1) * ncols,
+          "nlsa_recon_dist: inv_sd / Drecon2 / ws sizes")
+    call("memi_nlsa_recon_dist", _p(Gq), _p(H), q, ncols, int(npix), _p(inv_sd), _p(Drecon2),
+         _p(ws), _stream())
+
+
+FIT1D_BLOCK = 256
+
+
+def fit1d_ws_len(n):
+    return 13 * (-(-int(n) // FIT1D_BLOCK))
+
+
+def fit1d_tau(x, a, b, tau, flag, sums, ws):
+    """One tau update of the 1-D fit for points x (n, 3) float64 on the device at curve (a, b)."""
+    n = x.shape[0]
+    _need(x.dtype == torch.float64 and x.is_contiguous() and x.dim() == 2 and x.shape[1] == 3,
+          "fit1d_tau: x must be contiguous float64 (n, 3)")
+    _need(tau.dtype == torch.float64 and tau.numel() >= n and flag.dtype == torch.int32 and
+          flag.numel() >= n and sums.dtype == torch.float64 and sums.numel() >= 13 and
+          ws.dtype == torch.float64 and ws.numel() >= fit1d_ws_len(n),
+          "fit1d_tau: tau / flag / sums / ws sizes")
+    ab = np.ascontiguousarray(np.concatenate([np.ravel(a), np.ravel(b)]), dtype=np.float64)
+    _need(ab.size == 6, "fit1d_tau: a and b must have 3 entries each")
+    call("memi_fit1d_tau", _p(x), n, c_vp(ab.ctypes.data), _p(tau), _p(flag), _p(sums), _p(ws),
+         _stream())
 
 
 def nlsa_imgt(Ubar, H, inv_sd, IMGT):

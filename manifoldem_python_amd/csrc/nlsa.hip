@@ -261,6 +261,276 @@ __global__ __launch_bounds__(256) void k_imgt(const double* __restrict__ U,
   }
 }
 
+// ---------------------------------------------------------- reconstruction distances ----
+// T[a][j] = sum_b Gq[a][b] H[b][j]   (Gq [q][q], H [q][ncols])
+__global__ void k_gq_h(const double* __restrict__ Gq, const double* __restrict__ H, int q,
+                       int ncols, double* __restrict__ T) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (j >= ncols) return;
+  double s = 0.0;
+  for (int b = 0; b < q; ++b) s = fma(Gq[a * q + b], H[(int64_t)b * ncols + j], s);
+  T[(int64_t)a * ncols + j] = s;
+}
+
+// Gram_jj = sum_a H[a][j] T[a][j] in increasing a with fma, the order k_recon_dist's tiles use,
+// so its diagonal entries equal these bit for bit; inv_sd_j = 1 / sqrt(Gram_jj / npix).
+__global__ void k_recon_diag(const double* __restrict__ H, const double* __restrict__ T, int q,
+                             int ncols, double npix, double* __restrict__ dg,
+                             double* __restrict__ inv_sd) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  double s = 0.0;
+  for (int a = 0; a < q; ++a) s = fma(H[(int64_t)a * ncols + j], T[(int64_t)a * ncols + j], s);
+  dg[j] = s;
+  inv_sd[j] = 1.0 / sqrt(s / npix);
+}
+
+// 64x64 tiles of Gram = H^T T with the L2_distance epilogue of NLSA.py:141-146:
+// C_ij = Gram_ij (inv_i inv_j), d = (C_ii + C_jj) - 2 C_ij, d < 1e-8 -> 0, out = sqrt(d)^2.
+__global__ __launch_bounds__(256) void k_recon_dist(const double* __restrict__ H,
+                                                    const double* __restrict__ T,
+                                                    const double* __restrict__ dg,
+                                                    const double* __restrict__ inv_sd, int q,
+                                                    int ncols, float* __restrict__ out) {
+  __shared__ double hs[16][65], ts[16][65];
+  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  double acc[4][4] = {};
+  for (int k0 = 0; k0 < q; k0 += 16) {
+    for (int e = threadIdx.x; e < 16 * 64; e += blockDim.x) {
+      const int kk = e / 64, x = e % 64;
+      const bool kin = k0 + kk < q;
+      hs[kk][x] = (kin && i0 + x < ncols) ? H[(int64_t)(k0 + kk) * ncols + i0 + x] : 0.0;
+      ts[kk][x] = (kin && j0 + x < ncols) ? T[(int64_t)(k0 + kk) * ncols + j0 + x] : 0.0;
+    }
+    __syncthreads();
+    const int kend = min(16, q - k0);
+    for (int kk = 0; kk < kend; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        a[x] = hs[kk][tr * 4 + x];
+        b[x] = ts[kk][tc * 4 + x];
+      }
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    const int i = i0 + tr * 4 + x;
+    if (i >= ncols) continue;
+    const double ii = inv_sd[i];
+    const double aai = dg[i] * (ii * ii);
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int j = j0 + tc * 4 + y;
+      if (j >= ncols) continue;
+      const double ij = inv_sd[j];
+      const double c = acc[x][y] * (ii * ij);
+      double d = (aai + dg[j] * (ij * ij)) - 2.0 * c;
+      if (d < 1e-8) d = 0.0;
+      const double r = sqrt(d);
+      out[(int64_t)i * ncols + j] = (float)(r * r);
+    }
+  }
+}
+
+// ------------------------------------------------------------- 1-D open-manifold fit ----
+// tau update of fit_1D_open_manifold_3D.py:64-160: for a point x and curve (a, b),
+// R(tau) = sum_j (x_j - b_j - a_j cos(j pi tau))^2 (R_p.py:41-49) is stationary in (0, 1)
+// where beta = cos(pi tau) is a root of the quintic of solve_d_R_d_tau_p_3D.py:35-60; tau is
+// the candidate arccos(beta)/pi (real beta in [-1, 1]) or 0 or 1 of least R, first in order on
+// ties.  The reference takes the roots from np.roots (a LAPACK eigensolve of the companion
+// matrix) and keeps those with zero imaginary part.  Here the real roots come from monotone
+// segments: the roots of p^(k) bracket those of p^(k-1), so isolating from the linear fourth
+// derivative up gives every real root of p in [-1.01, 1.01] by safeguarded Newton steps.
+// Where LAPACK's real/complex decision or the arg min is within rounding of a tie (a stationary
+// value of p within 1e-10 of its scale: a near-double root; p(+-1) that small: a root at the
+// edge; two candidates with R equal to 1e-12 of its scale; a zero leading or trailing
+// coefficient, which np.roots treats specially) the point is flagged for the host to re-solve.
+constexpr int FIT_BLOCK = 256;
+constexpr double FIT_LO = -1.01, FIT_HI = 1.01;
+constexpr double FIT_EPS = 2.220446049250313e-16;   // double precision unit roundoff x 2
+
+struct FitCurve {
+  double a[3], b[3];
+};
+
+__device__ inline double peval(const double* e, int d, double x) {
+  double r = e[d];
+  for (int k = d - 1; k >= 0; --k) r = fma(r, x, e[k]);
+  return r;
+}
+
+// root of e (degree d) in the bracket [u, v] where e(u), e(v) have opposite signs and e is
+// monotone; de is its derivative (degree d - 1)
+__device__ double monotone_root(const double* e, int d, const double* de, double u, double v,
+                                double pu) {
+  double x = 0.5 * (u + v);
+  for (int it = 0; it < 200; ++it) {
+    const double px = peval(e, d, x);
+    if (px == 0.0) return x;
+    if ((px < 0.0) == (pu < 0.0)) {
+      u = x;
+      pu = px;
+    } else {
+      v = x;
+    }
+    if (v - u <= 4.0 * FIT_EPS * fmax(fabs(u), fabs(v)) || v - u <= 1e-300) break;
+    const double dx = peval(de, d - 1, x);
+    double xn = dx != 0.0 ? x - px / dx : u;
+    if (!(xn > u && xn < v)) xn = 0.5 * (u + v);
+    if (xn == x) break;
+    x = xn;
+  }
+  return x;
+}
+
+// real roots of e (degree d) in [FIT_LO, FIT_HI], ascending, given the ascending roots cr[0..nc)
+// of its derivative strictly inside the interval
+__device__ int segment_roots(const double* e, int d, const double* de, const double* cr, int nc,
+                             double* out) {
+  int n = 0;
+  double u = FIT_LO, pu = peval(e, d, u);
+  if (pu == 0.0) out[n++] = u;
+  for (int s = 0; s <= nc; ++s) {
+    const double v = s < nc ? cr[s] : FIT_HI;
+    const double pv = peval(e, d, v);
+    if (pv == 0.0) {
+      if (n == 0 || out[n - 1] != v) out[n++] = v;
+    } else if (pu != 0.0 && ((pu < 0.0) != (pv < 0.0))) {
+      out[n++] = monotone_root(e, d, de, u, v, pu);
+    }
+    u = v;
+    pu = pv;
+  }
+  return n;
+}
+
+__device__ inline int interior(const double* r, int n, double* out) {
+  int m = 0;
+  for (int i = 0; i < n; ++i)
+    if (r[i] > FIT_LO && r[i] < FIT_HI) out[m++] = r[i];
+  return m;
+}
+
+__device__ inline double fit_residual(const double x[3], const FitCurve& c, double tau) {
+  double r = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double e = x[j] - c.b[j] - c.a[j] * cos(tau * (double)(j + 1) * M_PI);
+    r += e * e;
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(FIT_BLOCK) void k_fit1d_tau(const double* __restrict__ X, int n,
+                                                         FitCurve c, double* __restrict__ tau,
+                                                         int32_t* __restrict__ flag,
+                                                         double* __restrict__ part) {
+  __shared__ double red[13][FIT_BLOCK];
+  const int i = blockIdx.x * FIT_BLOCK + threadIdx.x;
+  double acc[13];
+#pragma unroll
+  for (int k = 0; k < 13; ++k) acc[k] = 0.0;
+  if (i < n) {
+    const double x[3] = {X[i * 3], X[i * 3 + 1], X[i * 3 + 2]};
+    // np.roots order c0..c5 (solve_d_R_d_tau_p_3D.py:35-60), stored ascending: e[k] = c_{5-k}
+    double e[6];
+    e[5] = 48.0 * c.a[2] * c.a[2];
+    e[4] = 0.0;
+    e[3] = 8.0 * c.a[1] * c.a[1] - 48.0 * c.a[2] * c.a[2];
+    e[2] = -12.0 * c.a[2] * (x[2] - c.b[2]);
+    e[1] = c.a[0] * c.a[0] - 4.0 * c.a[1] * c.a[1] + 9.0 * c.a[2] * c.a[2] -
+           4.0 * c.a[1] * (x[1] - c.b[1]);
+    e[0] = -c.a[0] * (x[0] - c.b[0]) + 3.0 * c.a[2] * (x[2] - c.b[2]);
+    int amb = (e[5] == 0.0 || e[0] == 0.0) ? 1 : 0;
+    double t_best = 0.0;
+    if (!amb) {
+      // derivatives, each ascending
+      double d1[5], d2[4], d3[3], d4[2];
+      for (int k = 0; k < 5; ++k) d1[k] = (k + 1) * e[k + 1];
+      for (int k = 0; k < 4; ++k) d2[k] = (k + 1) * d1[k + 1];
+      for (int k = 0; k < 3; ++k) d3[k] = (k + 1) * d2[k + 1];
+      for (int k = 0; k < 2; ++k) d4[k] = (k + 1) * d3[k + 1];
+      double r4[1], r3[3], r2[4], r1[5], r0[6], tmp[6];
+      int n4 = 0;
+      {
+        const double z = -d4[0] / d4[1];
+        if (z > FIT_LO && z < FIT_HI) r4[n4++] = z;
+      }
+      int n3 = interior(tmp, segment_roots(d3, 2, d4, r4, n4, tmp), r3);
+      int n2 = interior(tmp, segment_roots(d2, 3, d3, r3, n3, tmp), r2);
+      int n1 = interior(tmp, segment_roots(d1, 4, d2, r2, n2, tmp), r1);
+      const int n0 = segment_roots(e, 5, d1, r1, n1, r0);
+      const double scale = fabs(e[0]) + fabs(e[1]) + fabs(e[2]) + fabs(e[3]) + fabs(e[5]);
+      const double tol = 1e-10 * scale;
+      for (int k = 0; k < n1; ++k)
+        if (fabs(peval(e, 5, r1[k])) <= tol) amb = 1;
+      if (fabs(peval(e, 5, 1.0)) <= tol || fabs(peval(e, 5, -1.0)) <= tol) amb = 1;
+      if (scale > 1e8 * fabs(e[5])) amb = 1;
+      // candidates: real roots in [-1, 1], then 0, then 1
+      double cand[8], rc[8];
+      int nc = 0;
+      for (int k = 0; k < n0; ++k)
+        if (fabs(r0[k]) <= 1.0) cand[nc++] = acos(r0[k]) / M_PI;
+      cand[nc++] = 0.0;
+      cand[nc++] = 1.0;
+      int best = 0;
+      for (int k = 0; k < nc; ++k) {
+        rc[k] = fit_residual(x, c, cand[k]);
+        if (rc[k] < rc[best]) best = k;
+      }
+      double rs = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double m = fabs(x[j] - c.b[j]) + fabs(c.a[j]);
+        rs += m * m;
+      }
+      for (int k = 0; k < nc; ++k)
+        if (k != best && fabs(cand[k] - cand[best]) > 1e-12 && rc[k] - rc[best] <= 1e-12 * rs)
+          amb = 1;
+      t_best = cand[best];
+    }
+    tau[i] = t_best;
+    flag[i] = amb;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      // tau . (pi j), the operand order of fit_1D_open_manifold_3D.py:85
+      const double cj = cos(t_best * (M_PI * (double)(j + 1)));
+      acc[j] = cj * cj;
+      acc[3 + j] = cj;
+      acc[6 + j] = x[j] * cj;
+      acc[9 + j] = x[j];
+    }
+    acc[12] = amb;
+  }
+#pragma unroll
+  for (int k = 0; k < 13; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = FIT_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 13; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 13) part[blockIdx.x * 13 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// sums[k] = sum over blocks of part[b][k], in block order
+__global__ void k_fit1d_sums(const double* __restrict__ part, int nb, double* __restrict__ sums) {
+  const int k = threadIdx.x;
+  if (k >= 13) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[b * 13 + k];
+  sums[k] = s;
+}
+
 __global__ void k_zero_d(double* p, int64_t n) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x)
@@ -362,6 +632,45 @@ int memi_nlsa_imgt(const double* Ubar, const double* H, const double* inv_sd, in
   hipLaunchKernelGGL(k_imgt, g, dim3(256), 0, (hipStream_t)stream, Ubar, H, inv_sd, npix, q_dim,
                      ncols, IMGT);
   MEMI_CHECK_LAUNCH("k_imgt");
+  return MEMI_OK;
+}
+
+int memi_nlsa_recon_dist(const double* Gq, const double* H, int q_dim, int ncols, int64_t npix,
+                         double* inv_sd, float* Drecon2, double* ws, void* stream) {
+  MEMI_CHECK_ARG(Gq && H && inv_sd && Drecon2 && ws && q_dim > 0 && q_dim <= 4096 &&
+                     ncols > 0 && npix > 0,
+                 "memi_nlsa_recon_dist: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  double* T = ws;
+  double* dg = ws + (int64_t)q_dim * ncols;
+  hipLaunchKernelGGL(k_gq_h, dim3((ncols + 255) / 256, q_dim), dim3(256), 0, s, Gq, H, q_dim,
+                     ncols, T);
+  MEMI_CHECK_LAUNCH("k_gq_h");
+  hipLaunchKernelGGL(k_recon_diag, dim3((ncols + 255) / 256), dim3(256), 0, s, H, T, q_dim, ncols,
+                     (double)npix, dg, inv_sd);
+  MEMI_CHECK_LAUNCH("k_recon_diag");
+  const unsigned nt = (unsigned)((ncols + 63) / 64);
+  hipLaunchKernelGGL(k_recon_dist, dim3(nt, nt), dim3(256), 0, s, H, T, dg, inv_sd, q_dim, ncols,
+                     Drecon2);
+  MEMI_CHECK_LAUNCH("k_recon_dist");
+  return MEMI_OK;
+}
+
+int memi_fit1d_tau(const double* x, int n, const double* ab_host, double* tau, int32_t* flag,
+                   double* sums, double* ws, void* stream) {
+  MEMI_CHECK_ARG(x && ab_host && tau && flag && sums && ws && n > 0,
+                 "memi_fit1d_tau: bad arguments");
+  FitCurve c;
+  for (int j = 0; j < 3; ++j) {
+    c.a[j] = ab_host[j];
+    c.b[j] = ab_host[3 + j];
+  }
+  const int nb = (n + FIT_BLOCK - 1) / FIT_BLOCK;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fit1d_tau, dim3(nb), dim3(FIT_BLOCK), 0, s, x, n, c, tau, flag, ws);
+  MEMI_CHECK_LAUNCH("k_fit1d_tau");
+  hipLaunchKernelGGL(k_fit1d_sums, dim3(1), dim3(64), 0, s, ws, nb, sums);
+  MEMI_CHECK_LAUNCH("k_fit1d_sums");
   return MEMI_OK;
 }
 

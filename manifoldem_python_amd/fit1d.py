@@ -1,15 +1,21 @@
-"""1-D open-manifold fit x_j = a_j cos(j pi tau) + b_j on the host, vectorised over points.
+"""1-D open-manifold fit x_j = a_j cos(j pi tau) + b_j (a25).
 
 Reference: ``fit_1D_open_manifold_3D.py:64-160``, ``get_fit_1D_open_manifold_3D_param.py:12-95``,
 ``solve_d_R_d_tau_p_3D.py:35-60``, ``R_p.py:41-49``.  The reference solves one quintic per point
-with ``np.roots`` inside a Python loop; here all points' companion matrices go through one
-batched ``np.linalg.eigvals`` call (the same LAPACK routine ``np.roots`` uses, so the real/complex
-classification of each root is the same), with ``np.roots`` kept for the degenerate
-polynomials it treats specially (zero leading or trailing coefficients).
+with ``np.roots`` inside a Python loop, every iteration.  Here each tau update of all points and
+the sums the next (a, b) needs run in one HIP launch (memi_fit1d_tau: real roots by monotone
+segments, arg min of R_p), and only the 2x2 least-squares solves and the convergence test stay
+on the host.  np.roots keeps its real roots where they are exactly real in LAPACK's output; the
+kernel flags the rare points where that decision (or the arg min) is within rounding of a tie,
+and ``solve_tau`` re-solves exactly those with np.roots' own LAPACK routine, so the
+classification is the reference's.
 """
 from __future__ import annotations
 
 import numpy as np
+import torch
+
+from . import _native as nat
 
 EPS = 1e-4
 
@@ -81,26 +87,52 @@ def initial_guess(psi):
     return np.array([a1, a2, a3]), np.array([b1, b2, b3])
 
 
-def fit(psi, max_iter=100, delta_a_max=1, delta_b_max=1, delta_tau_max=0.01):
+def _lstsq_curve(s, nS):
+    """(a, b) from the tau sums, ``fit_1D_open_manifold_3D.py:85-100``."""
+    A11, A12, b1, b2 = s[0:3], s[3:6], s[6:9], s[9:12]
+    coeff = np.zeros((2, 3))
+    for q in range(3):
+        coeff[:, q] = np.linalg.lstsq(np.array([[A11[q], A12[q]], [A12[q], nS]]),
+                                      np.array([b1[q], b2[q]]), rcond=None)[0]
+    return coeff[0], coeff[1]
+
+
+def fit(psi, max_iter=100, delta_a_max=1, delta_b_max=1, delta_tau_max=0.01, device=None):
     """(a, b, tau (nS, 1)).  Like the reference, the tau-change test is identically zero
     (``fit_1D_open_manifold_3D.py:218`` binds tau_old to the array it then updates)."""
-    x = np.asarray(psi[:, 0:3], dtype=np.float64)
+    x = np.ascontiguousarray(np.asarray(psi)[:, 0:3], dtype=np.float64)
     a, b = initial_guess(psi)
     nS = x.shape[0]
-    tau = solve_tau(x, a, b).reshape(nS, 1)
+    dev = device if device is not None else torch.device("cuda")
+    xd = torch.as_tensor(x, device=dev)
+    tau_d = torch.empty(nS, dtype=torch.float64, device=dev)
+    flag_d = torch.empty(nS, dtype=torch.int32, device=dev)
+    sums_d = torch.empty(13, dtype=torch.float64, device=dev)
+    ws = torch.empty(nat.fit1d_ws_len(nS), dtype=torch.float64, device=dev)
+    jpi = np.pi * np.array([[1, 2, 3]])
+
+    def update(a, b):
+        """tau for (a, b) and its sums; tau on the host only when points were re-solved."""
+        nat.fit1d_tau(xd, a, b, tau_d, flag_d, sums_d, ws)
+        s = sums_d.cpu().numpy()
+        if s[12] == 0:
+            return s, None
+        tau = tau_d.cpu().numpy()
+        fl = flag_d.cpu().numpy().astype(bool)
+        tau[fl] = solve_tau(x[fl], a, b)
+        cj = np.cos(np.dot(tau[:, None], jpi))
+        s = np.concatenate([np.sum(cj ** 2, axis=0), np.sum(cj, axis=0),
+                            np.sum(x * cj, axis=0), np.sum(x, axis=0), [fl.sum()]])
+        return s, tau
+
+    s, tau_h = update(a, b)
     for _ in range(max_iter):
         a_old, b_old = a, b
-        cj = np.cos(tau * np.pi * np.array([[1, 2, 3]]))
-        A11, A12 = np.sum(cj ** 2, axis=0), np.sum(cj, axis=0)
-        b1, b2 = np.sum(x * cj, axis=0), np.sum(x, axis=0)
-        coeff = np.zeros((2, 3))
-        for q in range(3):
-            coeff[:, q] = np.linalg.lstsq(np.array([[A11[q], A12[q]], [A12[q], nS]]),
-                                          np.array([b1[q], b2[q]]), rcond=None)[0]
-        a, b = coeff[0], coeff[1]
-        tau[:, 0] = solve_tau(x, a, b)
+        a, b = _lstsq_curve(s, nS)
+        s, tau_h = update(a, b)
         da = np.max(np.fabs(a - a_old) / (np.fabs(a) + EPS)) * 100
         db = np.max(np.fabs(b - b_old) / (np.fabs(b) + EPS)) * 100
         if da < delta_a_max and db < delta_b_max and 0.0 < delta_tau_max:
             break
-    return a, b, tau
+    tau = tau_h if tau_h is not None else tau_d.cpu().numpy()
+    return a, b, tau.reshape(nS, 1)

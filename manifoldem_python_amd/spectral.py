@@ -11,9 +11,11 @@ Stages and where they run:
   a23 rank-2 reconstruction IMGT + z-score    factored form: IMGT = Ubar H diag(1/sd), Ubar the
                                               pixel-centred 2*ConOrder columns of U, H the Hankel
                                               matrix of the rank-2 chronos (HIP GEMM)
-  a24 L2 distances of IMGT frames             from the factored Gram H^T (Ubar^T Ubar) H
+  a24 L2 distances of IMGT frames             from the factored Gram H^T (Ubar^T Ubar) H with the
+                                              L2_distance epilogue fused (memi_nlsa_recon_dist)
   a20 second diffusion map                    Stage-2 GPU path
-  a25 1-D open-manifold fit                   host, vectorised (fit1d)
+  a25 1-D open-manifold fit                   HIP tau update + sums per iteration
+                                              (memi_fit1d_tau), 2x2 solves on host
 """
 from __future__ import annotations
 
@@ -36,13 +38,17 @@ class NLSAResult:
 
 def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tune=3,
          num_eigs=15, msk2=None, logEps=pl.LOG_EPS, rand=None, snr=5.0, want_imgt=True,
-         keep=False):
+         keep=False, timer=None):
     """One psi of psiAnalysisParS2 / NLSA.op.
 
     D: (n, n) float32/float64 device distances of the PD (posPath order); ord_d: int64 device
     sort order along psi (PosPsi1), so DD = D[ord][:, ord]; img_idx: int64 device indices of
-    the images in psi order into imgAll / ctf_half (posPath[PosPsi1]).
+    the images in psi order into imgAll / ctf_half (posPath[PosPsi1]).  timer: an optional
+    pipeline.StageTimer that gets one mark per stage (it synchronises the device at each).
     """
+    mark = timer.mark if timer is not None else (lambda name: None)
+    if timer is not None:
+        timer.start()
     dev = D.device
     num = int(ord_d.shape[0])
     nS = num if nS is None else nS
@@ -53,11 +59,13 @@ def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tu
     # a19
     ConD = torch.empty(m * m, dtype=torch.float64, device=dev)
     nat.nlsa_condist(D.reshape(-1), ord_d, num, ConOrder, ConD)
+    mark("condist")
     # a20
     plan = nat.Plan([0], [m], [m], device=dev)
     dm1 = pl.dm_embed(plan, ConD.float(), tune, num_eigs, logEps, rand=rand)[0]
     psiC, mu = dm1.psi, dm1.mu
     r.psiC1 = psiC.copy()
+    mark("dm1")
     # a21
     IMG1 = imgAll.index_select(0, img_idx)
     CTF1 = ctf_half.index_select(0, img_idx).contiguous()
@@ -73,6 +81,7 @@ def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tu
     # column-major flatten of each image (NLSA.py:80: img.T.reshape(-1))
     A = imgs.permute(0, 3, 2, 1).reshape(ConOrder * Npix, psiTrunc).double().contiguous()
     del sv, imgs, spec, IMG1
+    mark("supervectors")
     # a22
     G = torch.empty((psiTrunc, psiTrunc), dtype=torch.float64, device=dev)
     nat.tall_gram(A, G)
@@ -85,6 +94,7 @@ def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tu
     nat.tall_apply(A, Vv @ np.diag(1.0 / S), U)
     r.VX = Vv.T @ psiCx.T
     r.sdiag = S
+    mark("svd")
     # a23
     U3 = U.view(ConOrder, Npix, psiTrunc)
     r.Topo_mean = U3.mean(dim=0).cpu().numpy()
@@ -98,33 +108,28 @@ def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tu
     H[ConOrder:, :n_fill] = z[1][jj]
     Gq = torch.empty((2 * ConOrder, 2 * ConOrder), dtype=torch.float64, device=dev)
     nat.center_gram(Ubar, Gq)
-    Gq_h = Gq.cpu().numpy()
-    Gram = H.T @ Gq_h @ H
-    dg = np.diag(Gram).copy()
-    with np.errstate(divide="ignore", invalid="ignore"):
-        sd = np.sqrt(dg / Npix)
-        inv_sd = 1.0 / sd
+    Hd = torch.as_tensor(H, device=dev).contiguous()
+    inv_sd = torch.empty(n_rec, dtype=torch.float64, device=dev)
+    # a24: L2_distance on the z-scored frames, from the factored Gram H^T Gq H
+    Drecon2 = torch.empty((n_rec, n_rec), dtype=torch.float32, device=dev)
+    ws = torch.empty((2 * ConOrder + 1) * n_rec, dtype=torch.float64, device=dev)
+    nat.nlsa_recon_dist(Gq, Hd, Npix, inv_sd, Drecon2, ws)
+    del ws
     if want_imgt:
-        Hd = torch.as_tensor(H, device=dev).contiguous()
         IMGT = torch.empty((Npix, n_rec), dtype=torch.float64, device=dev)
-        nat.nlsa_imgt(Ubar, Hd, torch.as_tensor(inv_sd, device=dev).contiguous(), IMGT)
+        nat.nlsa_imgt(Ubar, Hd, inv_sd, IMGT)
         r.IMGT_dev = IMGT
-    # a24: L2_distance on the z-scored frames, from the factored Gram
-    with np.errstate(divide="ignore", invalid="ignore"):
-        C = Gram * np.outer(inv_sd, inv_sd)
-    aa = np.diag(C)
-    tmp = aa[:, None] + aa[None, :] - 2 * C
-    tmp[tmp < 1e-8] = 0
-    Drecon2 = np.sqrt(tmp) ** 2
     r.Drecon2 = Drecon2
+    mark("reconstruct")
     # second DM (NLSA.py:141-146)
     nSrecon = min(Npix, n_rec)
     plan2 = nat.Plan([0], [n_rec], [nSrecon], device=dev)
-    dm2 = pl.dm_embed(plan2, torch.as_tensor(Drecon2.astype(np.float32).ravel(), device=dev),
-                      tune, num_eigs, logEps, rand=rand)[0]
+    dm2 = pl.dm_embed(plan2, Drecon2.reshape(-1), tune, num_eigs, logEps, rand=rand)[0]
     r.psirec, r.mu = dm2.psi, dm2.mu
+    mark("dm2")
     # a25
-    r.a, r.b, r.tau = fit1d.fit(r.psirec)
+    r.a, r.b, r.tau = fit1d.fit(r.psirec, device=dev)
+    mark("fit1d")
     if keep:
         r.dm1, r.dm2, r.A, r.U = dm1, dm2, A, U
     return r
