@@ -416,8 +416,10 @@ def run_c5(args, dev):
         for j in range(n_psi):
             tm = pl.StageTimer()
             od = torch.as_tensor(np.argsort(res[p].psi[:, j]), device=dev)
-            spectral.nlsa(dist.block(p), od, od, imgs[sl], wsp.ctf[sl], N, con, 8, nS=n,
-                          tune=3, num_eigs=15, rand=rand, timer=tm)
+            r = spectral.nlsa(dist.block(p), od, od, imgs[sl], wsp.ctf[sl], N, con, 8, nS=n,
+                              tune=3, num_eigs=15, rand=rand, timer=tm, keep=True)
+            tm.t["dm1.lanczos_steps"] = 1e-3 * r.dm1.lanczos_steps
+            tm.t["dm2.lanczos_steps"] = 1e-3 * r.dm2.lanczos_steps
             per_call.append(tm.t)
     stage_names = list(per_call[0])
     t_pd = t_all / steps / npd

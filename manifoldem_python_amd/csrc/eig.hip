@@ -256,9 +256,9 @@ __global__ __launch_bounds__(256) void k_orth_finish(const memi_pd_desc* __restr
 }
 
 // ---- convergence check: Ritz pairs of the tridiagonal T_p on the GPU -----------------------
-// One 64-thread workgroup per active PD with J = min(steps, n, m) Lanczos steps.
+// One workgroup per active PD with J = min(steps, n, m) Lanczos steps.
 //  1. candidates: the k smallest and k largest eigenvalues of T (all J when J <= 2k), each by
-//     Sturm-count bisection on the Gershgorin interval (one thread per eigenvalue);
+//     Sturm-count multisection on the Gershgorin interval;
 //  2. the k of largest |theta| (first of equal magnitudes in candidate order), sorted by
 //     descending theta -- the pairs eigsh(which='LM') returns;
 //  3. their eigenvectors by the twisted factorisation of T - theta I (forward and backward
@@ -297,17 +297,23 @@ __device__ inline int sturm_count(const double* al, const double* b2, int J, dou
 // MAXM: the largest m of the instantiation.  MAXM <= 512 keeps alpha, beta^2 and the twisted
 // factorisation's pivots in LDS (pivot (i, side, pair) at (2 i + side) * 16 + pair, so the
 // pairs' threads hit consecutive banks); larger MAXM keeps the pivots in `piv` (global).
+// RITZ_NT threads: all of them multisect the candidate eigenvalues (a group of G lanes of one
+// wave per eigenvalue evaluates G Sturm counts per round, so each round narrows the bracket
+// G + 1 fold instead of 2); the selection, vectors and Gram-Schmidt use the first wave.
+constexpr int RITZ_NT = 1024;
+
 template <int MAXM>
-__global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restrict__ pds,
-                                                     const double* __restrict__ ab, int m,
-                                                     int steps_done,
-                                                     const int32_t* __restrict__ ks, double tol,
-                                                     int32_t* __restrict__ active,
-                                                     double* __restrict__ theta,
-                                                     double* __restrict__ S,
-                                                     int32_t* __restrict__ steps,
-                                                     double* __restrict__ res_out,
-                                                     double* __restrict__ piv) {
+__global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __restrict__ pds,
+                                                          const double* __restrict__ ab, int m,
+                                                          int steps_done,
+                                                          const int32_t* __restrict__ ks,
+                                                          double tol,
+                                                          int32_t* __restrict__ active,
+                                                          double* __restrict__ theta,
+                                                          double* __restrict__ S,
+                                                          int32_t* __restrict__ steps,
+                                                          double* __restrict__ res_out,
+                                                          double* __restrict__ piv) {
   constexpr int KM = MEMI_RITZ_KMAX;
   constexpr bool PLDS = MAXM <= 512;
   __shared__ double al[MAXM], b2[MAXM];
@@ -315,50 +321,73 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
   __shared__ double cand[2 * KM];
   __shared__ double sel[KM];
   __shared__ double resv[KM];
+  __shared__ double gersh[3];
   __shared__ int done_sh;
   const int pd = blockIdx.x;
-  if (!active[pd]) return;
+  if (!active[pd]) return;                            // uniform over the block
   const int t = threadIdx.x;
   const int n = pds[pd].n;
   const int J = min(steps_done, min(n, m));
   const int k = min(ks[pd], J);
   const double* a = ab + (int64_t)pd * m * 2;      // a[2i] = alpha_i, a[2i+1] = beta_i
-  for (int i = t; i < J; i += 64) {
+  for (int i = t; i < J; i += RITZ_NT) {
     al[i] = a[2 * i];
     b2[i] = a[2 * i + 1] * a[2 * i + 1];
   }
+  // Gershgorin interval and pivot floor (first wave)
+  if (t < 64) {
+    double lo = INFINITY, hi = -INFINITY, bmax = 0.0;
+    for (int i = t; i < J; i += 64) {
+      const double bl = i > 0 ? fabs(a[2 * (i - 1) + 1]) : 0.0;
+      const double br = i + 1 < J ? fabs(a[2 * i + 1]) : 0.0;
+      lo = fmin(lo, a[2 * i] - bl - br);
+      hi = fmax(hi, a[2 * i] + bl + br);
+      bmax = fmax(bmax, br);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = fmin(lo, __shfl_xor(lo, off, 64));
+      hi = fmax(hi, __shfl_xor(hi, off, 64));
+      bmax = fmax(bmax, __shfl_xor(bmax, off, 64));
+    }
+    if (t == 0) {
+      gersh[0] = lo;
+      gersh[1] = hi;
+      gersh[2] = bmax;
+    }
+  }
   __syncthreads();
-  // Gershgorin interval and pivot floor
-  double lo = INFINITY, hi = -INFINITY, bmax = 0.0;
-  for (int i = t; i < J; i += 64) {
-    const double bl = i > 0 ? fabs(a[2 * (i - 1) + 1]) : 0.0;
-    const double br = i + 1 < J ? fabs(a[2 * i + 1]) : 0.0;
-    lo = fmin(lo, al[i] - bl - br);
-    hi = fmax(hi, al[i] + bl + br);
-    bmax = fmax(bmax, br);
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    lo = fmin(lo, __shfl_xor(lo, off, 64));
-    hi = fmax(hi, __shfl_xor(hi, off, 64));
-    bmax = fmax(bmax, __shfl_xor(bmax, off, 64));
-  }
+  double lo = gersh[0], hi = gersh[1];
+  const double bmax = gersh[2];
   const double span = fmax(hi - lo, 1e-300);
   lo -= 1e-14 * span;
   hi += 1e-14 * span;
   const double pivmin = fmax(1e-290, 1e-300 * bmax * bmax);
   const double atol = 1e-15 * span;    // eigenvalues to 1e-15 of the spectrum's width
-  // 1. candidate eigenvalues
+  // 1. candidate eigenvalues by multisection: G lanes (a power of two, within one wave) per
+  //    candidate; lane g counts the eigenvalues below X(g + 1), X(j) = l + (h - l) j / (G + 1),
+  //    and the bracket becomes [X(f), X(f + 1)] around the first point with more than idx below
   const int nc = J <= 2 * k ? J : 2 * k;
-  if (t < nc) {
-    const int idx = J <= 2 * k ? t : (t < k ? t : J - 2 * k + t);
+  int G = 64;
+  while (G > 1 && G * nc > RITZ_NT) G >>= 1;
+  const int ci = t / G, g = t % G;
+  if (ci < nc) {                                      // whole groups, so ballots stay in-group
+    const int idx = J <= 2 * k ? ci : (ci < k ? ci : J - 2 * k + ci);
+    const int base = (t & 63) & ~(G - 1);
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
     double l = lo, h = hi;
-    for (int it = 0; it < 200 && h - l > fmax(2.2e-16 * fmax(fabs(l), fabs(h)), atol); ++it) {
-      const double mid = 0.5 * (l + h);
-      if (mid <= l || mid >= h) break;
-      if (sturm_count(al, b2, J, mid) > idx) h = mid;
-      else l = mid;
+    for (int it = 0; it < 64 && h - l > fmax(2.2e-16 * fmax(fabs(l), fabs(h)), atol); ++it) {
+      const double w = h - l;
+      const double x = l + (w * (double)(g + 1)) / (double)(G + 1);
+      const bool above = sturm_count(al, b2, J, x) > idx;
+      const uint64_t bits = (__ballot(above) >> base) & gmask;
+      const int f = bits ? __ffsll((unsigned long long)bits) - 1 : G;
+      const double nl = f == 0 ? l : l + (w * (double)f) / (double)(G + 1);
+      const double nh = f == G ? h : l + (w * (double)(f + 1)) / (double)(G + 1);
+      if (nl <= l && nh >= h) break;                  // no progress: the bracket is at ulps
+      l = nl;
+      h = nh;
     }
-    cand[t] = 0.5 * (l + h);
+    if (g == 0) cand[ci] = 0.5 * (l + h);
   }
   __syncthreads();
   // 2. selection: k largest |theta|, then descending theta
@@ -474,16 +503,20 @@ __global__ __launch_bounds__(64) void k_lanczos_ritz(const memi_pd_desc* __restr
   for (int c = 1; c < k; ++c) {
     for (int b = 0; b < c; ++b) {
       if (fabs(sel[c] - sel[b]) > 1e-6 * mx) continue;   // uniform over the block
-      double d = 0.0;
-      for (int i = t; i < J; i += 64) d += Sp[(int64_t)i * k + c] * Sp[(int64_t)i * k + b];
-      for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
-      for (int i = t; i < J; i += 64) Sp[(int64_t)i * k + c] -= d * Sp[(int64_t)i * k + b];
+      if (t < 64) {                                     // first wave only
+        double d = 0.0;
+        for (int i = t; i < J; i += 64) d += Sp[(int64_t)i * k + c] * Sp[(int64_t)i * k + b];
+        for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+        for (int i = t; i < J; i += 64) Sp[(int64_t)i * k + c] -= d * Sp[(int64_t)i * k + b];
+      }
       __syncthreads();
-      double nn = 0.0;
-      for (int i = t; i < J; i += 64) nn += Sp[(int64_t)i * k + c] * Sp[(int64_t)i * k + c];
-      for (int off = 32; off > 0; off >>= 1) nn += __shfl_xor(nn, off, 64);
-      const double iv = 1.0 / sqrt(nn);
-      for (int i = t; i < J; i += 64) Sp[(int64_t)i * k + c] *= iv;
+      if (t < 64) {
+        double nn = 0.0;
+        for (int i = t; i < J; i += 64) nn += Sp[(int64_t)i * k + c] * Sp[(int64_t)i * k + c];
+        for (int off = 32; off > 0; off >>= 1) nn += __shfl_xor(nn, off, 64);
+        const double iv = 1.0 / sqrt(nn);
+        for (int i = t; i < J; i += 64) Sp[(int64_t)i * k + c] *= iv;
+      }
       __syncthreads();
     }
   }
@@ -603,11 +636,11 @@ int memi_lanczos_ritz(const void* plan_dev, const memi_plan_info* info, const do
   const char* plan = (const char*)plan_dev;
   const memi_pd_desc* pds = (const memi_pd_desc*)(plan + info->off_pds);
   if (m <= 512)
-    hipLaunchKernelGGL(k_lanczos_ritz<512>, dim3((unsigned)info->n_pd), dim3(64), 0,
+    hipLaunchKernelGGL(k_lanczos_ritz<512>, dim3((unsigned)info->n_pd), dim3(RITZ_NT), 0,
                        (hipStream_t)stream, pds, ab, m, steps_done, ks, tol, active, theta, S,
                        steps, res, piv);
   else
-    hipLaunchKernelGGL(k_lanczos_ritz<4096>, dim3((unsigned)info->n_pd), dim3(64), 0,
+    hipLaunchKernelGGL(k_lanczos_ritz<4096>, dim3((unsigned)info->n_pd), dim3(RITZ_NT), 0,
                        (hipStream_t)stream, pds, ab, m, steps_done, ks, tol, active, theta, S,
                        steps, res, piv);
   MEMI_CHECK_LAUNCH("k_lanczos_ritz");

@@ -111,8 +111,9 @@ __global__ __launch_bounds__(256) void k_supervectors(const float2* __restrict__
   for (int t = 0; t < SG; ++t) {
     const int ii = g0 + t;
     if (ii >= C) break;
-    for (int c = 0; c < ell; ++c)
-      out[((int64_t)ii * ell + c) * Kh + k] = make_float2(acc[t][c][0], acc[t][c][1]);
+#pragma unroll
+    for (int c = 0; c < ELLMAX; ++c)       // constant indices keep acc in registers
+      if (c < ell) out[((int64_t)ii * ell + c) * Kh + k] = make_float2(acc[t][c][0], acc[t][c][1]);
   }
 }
 
@@ -383,6 +384,15 @@ __device__ double monotone_root(const double* e, int d, const double* de, double
     }
     if (v - u <= 4.0 * FIT_EPS * fmax(fabs(u), fabs(v)) || v - u <= 1e-300) break;
     const double dx = peval(de, d - 1, x);
+    if (dx != 0.0) {
+      const double step = px / dx;
+      // a Newton step of a few ulps: x is the root to rounding (further steps only bounce
+      // between the bracket's ends while bisection halves it to the last bit)
+      if (fabs(step) <= 16.0 * FIT_EPS * fabs(x)) {
+        const double xn = x - step;
+        return (xn > u && xn < v) ? xn : x;
+      }
+    }
     double xn = dx != 0.0 ? x - px / dx : u;
     if (!(xn > u && xn < v)) xn = 0.5 * (u + v);
     if (xn == x) break;

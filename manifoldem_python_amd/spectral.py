@@ -27,6 +27,20 @@ from . import fit1d
 from . import pipeline as pl
 
 
+_plans = {}
+
+
+def _plan(n, k, dev):
+    """One-PD plan of size n with k neighbours, reused by every call of that size: the plan
+    keeps the kernel-matrix buffer and the captured Lanczos graph, so the eight psi of a PD
+    (and PDs of equal size) replay one graph instead of capturing their own."""
+    key = (int(n), int(k), str(dev))
+    p = _plans.get(key)
+    if p is None:
+        p = _plans[key] = nat.Plan([0], [n], [k], device=dev)
+    return p
+
+
 class NLSAResult:
     __slots__ = ("IMGT", "Topo_mean", "psirec", "psiC1", "sdiag", "VX", "mu", "tau", "a", "b",
                  "IMGT_dev", "Drecon2", "dm1", "dm2", "A", "U")
@@ -61,11 +75,14 @@ def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tu
     nat.nlsa_condist(D.reshape(-1), ord_d, num, ConOrder, ConD)
     mark("condist")
     # a20
-    plan = nat.Plan([0], [m], [m], device=dev)
-    dm1 = pl.dm_embed(plan, ConD.float(), tune, num_eigs, logEps, rand=rand)[0]
+    plan = _plan(m, m, dev)
+    sub = pl.StageTimer() if timer is not None else None
+    dm1 = pl.dm_embed(plan, ConD.float(), tune, num_eigs, logEps, rand=rand, timer=sub)[0]
     psiC, mu = dm1.psi, dm1.mu
     r.psiC1 = psiC.copy()
     mark("dm1")
+    if timer is not None:
+        timer.t.update({"dm1." + k: v for k, v in sub.t.items()})
     # a21
     IMG1 = imgAll.index_select(0, img_idx)
     CTF1 = ctf_half.index_select(0, img_idx).contiguous()
@@ -123,10 +140,13 @@ def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tu
     mark("reconstruct")
     # second DM (NLSA.py:141-146)
     nSrecon = min(Npix, n_rec)
-    plan2 = nat.Plan([0], [n_rec], [nSrecon], device=dev)
-    dm2 = pl.dm_embed(plan2, Drecon2.reshape(-1), tune, num_eigs, logEps, rand=rand)[0]
+    plan2 = _plan(n_rec, nSrecon, dev)
+    sub = pl.StageTimer() if timer is not None else None
+    dm2 = pl.dm_embed(plan2, Drecon2.reshape(-1), tune, num_eigs, logEps, rand=rand, timer=sub)[0]
     r.psirec, r.mu = dm2.psi, dm2.mu
     mark("dm2")
+    if timer is not None:
+        timer.t.update({"dm2." + k: v for k, v in sub.t.items()})
     # a25
     r.a, r.b, r.tau = fit1d.fit(r.psirec, device=dev)
     mark("fit1d")

@@ -112,13 +112,20 @@ def test_ferguson_slope_matches_finite_differences(cuda):
     assert np.max(np.abs(np.log(S[0]) - np.array([logS(le) for le in logEps]))) < 1e-6
 
 
-@pytest.mark.parametrize("split,check", [(False, "gpu"), (True, "gpu"), (False, "host")])
-def test_batched_lanczos_matches_dense(cuda, split, check):
+@pytest.mark.parametrize("split,check,ns,kk", [
+    (False, "gpu", [150, 400, 257], [12, 16, 9]),
+    (True, "gpu", [150, 400, 257], [12, 16, 9]),
+    (False, "host", [150, 400, 257], [12, 16, 9]),
+    # two PDs exhaust their n < m steps before the first check
+    (True, "gpu", [20, 333, 16], [8, 14, 6]),
+    (False, "gpu", [20, 333, 16], [8, 14, 6]),
+])
+def test_batched_lanczos_matches_dense(cuda, split, check, ns, kk):
     """Three PDs of ragged sizes in one plan: GPU Lanczos eigenpairs (largest magnitude)
     against numpy eigh of each normalised kernel block."""
     import torch
     from manifoldem_python_amd import _native as nat, pipeline as pl
-    ns, ks = [150, 400, 257], [150, 120, 257]
+    ks = [n if n < 300 else 120 for n in ns]
     D = [_distance_matrix(n, 20 + i) for i, n in enumerate(ns)]
     rows0 = np.concatenate(([0], np.cumsum(ns)[:-1]))
     plan = nat.Plan(rows0, ns, ks, chunks=1)
@@ -129,7 +136,6 @@ def test_batched_lanczos_matches_dense(cuda, split, check):
     L = torch.empty(plan.info.d_elems, dtype=torch.float64, device=cuda)
     deg = torch.empty(2 * plan.info.n_rows, dtype=torch.float64, device=cuda)
     nat.markov_normalize(plan, flat, mask, sigma, L, deg)
-    kk = [12, 16, 9]
     res = pl.lanczos_eigs(plan, L, kk, split=split, check=check)
     for p in range(3):
         A = plan.block(L, p).cpu().numpy()
