@@ -265,7 +265,7 @@ int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, i
  * steps with j >= min(n, m) do nothing -- so the launches of a block of steps can be captured
  * once in a HIP graph and replayed.
  * split = 0: one workgroup per PD orthogonalises (H unused, may be NULL); split = 1: the
- * orthogonalisation is spread over the grid (for few large PDs), H: float64 [2*npd*m]. */
+ * orthogonalisation is spread over the grid (for few large PDs), H: float64 [2*npd*(m+1)]. */
 int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const double* L,
                        double* V, int m, int32_t* j_dev, int nsteps, const int32_t* active,
                        double* W, double* H, double* ab, int split, void* stream);

@@ -704,9 +704,10 @@ __global__ __launch_bounds__(256) void k_tanh_fit(const double* __restrict__ x,
     }
     double at[4];
     for (int p = 0; p < 4; ++p) at[p] = a[p] + d[p];
-    double cost_t;
-    double dummy[10], dummy4[4];
-    eval(at, false, dummy, dummy4, cost_t);
+    // the trial point's J^T J and J^T r come with its cost: an accepted step then needs no
+    // second pass (they are the values a re-evaluation at the new point would give)
+    double cost_t, JtJ_t[10], Jtr_t[4];
+    eval(at, true, JtJ_t, Jtr_t, cost_t);
     // predicted reduction of the linear model
     double pred = 0.0;
     for (int p = 0; p < 4; ++p) pred += d[p] * (Jtr[p] + lam * diag[p] * diag[p] * d[p]);
@@ -719,7 +720,8 @@ __global__ __launch_bounds__(256) void k_tanh_fit(const double* __restrict__ x,
       }
       for (int p = 0; p < 4; ++p) a[p] = at[p];
       cost = cost_t;
-      eval(a, true, JtJ, Jtr, cost);
+      for (int k = 0; k < 10; ++k) JtJ[k] = JtJ_t[k];
+      for (int p = 0; p < 4; ++p) Jtr[p] = Jtr_t[p];
       lam = fmax(lam * 0.1, 1e-15);
       const double prered = pred / fmax(cost_t + pred, 1e-300);
       if ((actred <= 1.49012e-8 && prered <= 1.49012e-8) || sqrt(dn) <= 1.49012e-8 * sqrt(xn))

@@ -125,14 +125,19 @@ __global__ __launch_bounds__(256) void k_lanczos_orth(const memi_pd_desc* __rest
 // Grid-parallel CGS2 for large PDs (one workgroup per PD leaves most CUs idle when a batch
 // has few PDs): per pass, a dot kernel (k_orth_dot: one workgroup per (column, PD), or
 // k_orth_dot_pairs for many PDs) then k_orth_update_rows (64 rows x 4 column groups per
-// workgroup); k_orth_finish records alpha, beta and writes v_{j+1}.  H: float64
-// [2][npd][m] coefficients of the two passes.
+// workgroup).  The second dot pass also forms w.w, so the second update can finish the step
+// itself: with V orthonormal, ||w - V h2||^2 = w.w - ||h2||^2, beta is known before w - V h2 is,
+// and each row of v_{j+1} = (w - V h2) / beta is written where it is computed (no separate
+// normalisation launch).  H: float64 [2][npd][m + 1]: the coefficients of the two passes, slot
+// m of the second pass holds w.w.
+constexpr int H_STRIDE_EXTRA = 1;
+
 __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict__ pds,
                                                   const double* __restrict__ V, int m,
                                                   const int32_t* __restrict__ jp,
                                                   const int32_t* __restrict__ active,
                                                   const double* __restrict__ W,
-                                                  double* __restrict__ H) {
+                                                  double* __restrict__ H, int with_norm) {
   __shared__ double red[8];
   const int j = *jp;
   const int pd = blockIdx.y;
@@ -140,71 +145,81 @@ __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict
   const memi_pd_desc P = pds[pd];
   const int n = P.n;
   if (j >= min(n, m)) return;
-  const int c = blockIdx.x;                       // one block per basis column
-  if (c > j) return;
-  const double* vc = V + (int64_t)m * P.row0 + (int64_t)c * n;
+  const int c = blockIdx.x;                       // one block per basis column (m: w itself)
+  const bool norm = with_norm && c == m;
+  if (c > j && !norm) return;
   const double* w = W + P.row0;
+  const double* vc = norm ? w : V + (int64_t)m * P.row0 + (int64_t)c * n;
   double s = 0.0;
   for (int r = threadIdx.x; r < n; r += blockDim.x) s += vc[r] * w[r];
   s = block_sum(s, red);
-  if (threadIdx.x == 0) H[(int64_t)pd * m + c] = s;
+  if (threadIdx.x == 0) H[(int64_t)pd * (m + H_STRIDE_EXTRA) + c] = s;
 }
 
 // Many-PD form of the dot pass: a fixed grid whose waves walk the npd x (j + 1) live
-// (PD, column) pairs, one wave per pair (grids sized for the largest j leave most blocks
-// empty when there are hundreds of PDs).
+// (PD, column) pairs (j + 2 with the w.w entry), one wave per pair (grids sized for the
+// largest j leave most blocks empty when there are hundreds of PDs).
 __global__ __launch_bounds__(256) void k_orth_dot_pairs(const memi_pd_desc* __restrict__ pds,
                                                         int64_t npd,
                                                         const double* __restrict__ V, int m,
                                                         const int32_t* __restrict__ jp,
                                                         const int32_t* __restrict__ active,
                                                         const double* __restrict__ W,
-                                                        double* __restrict__ H) {
+                                                        double* __restrict__ H, int with_norm) {
   const int j = *jp;
   const int lane = threadIdx.x & 63;
+  const int per = j + 1 + (with_norm ? 1 : 0);
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  const int64_t total = npd * (int64_t)(j + 1);
+  const int64_t total = npd * (int64_t)per;
   for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < total;
        q += nwaves) {
-    const int pd = (int)(q / (j + 1)), c = (int)(q % (j + 1));
+    const int pd = (int)(q / per), c = (int)(q % per);
     if (!active[pd]) continue;                    // uniform over the wave
     const memi_pd_desc P = pds[pd];
     const int n = P.n;
     if (j >= min(n, m)) continue;
-    const double* vc = V + (int64_t)m * P.row0 + (int64_t)c * n;
+    const bool norm = c == j + 1;
     const double* w = W + P.row0;
+    const double* vc = norm ? w : V + (int64_t)m * P.row0 + (int64_t)c * n;
     double s = 0.0;
     for (int r = lane; r < n; r += 64) s += vc[r] * w[r];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) H[(int64_t)pd * m + c] = s;
+    if (lane == 0) H[(int64_t)pd * (m + H_STRIDE_EXTRA) + (norm ? m : c)] = s;
   }
 }
 
 // Row-list form of the update pass: 64 consecutive rows of the plan per block (rows of two
-// PDs may share a block), 4 column groups each, w_r -= sum_c V[c][r] h[c].
+// PDs may share a block), 4 column groups each, w_r -= sum_c V[c][r] h[c].  finish (second
+// pass): instead of storing w_r, write v_{j+1}[r] = w_r / beta with beta^2 = w.w - ||h2||^2,
+// the PD's first row records alpha_j = h1[j] + h2[j] and beta_j, and the grid advances j.
 __global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restrict__ rows,
                                                           int64_t n_rows,
                                                           const memi_pd_desc* __restrict__ pds,
-                                                          const double* __restrict__ V, int m,
-                                                          const int32_t* __restrict__ jp,
+                                                          double* __restrict__ V, int m,
+                                                          int32_t* __restrict__ jp,
                                                           const int32_t* __restrict__ active,
                                                           double* __restrict__ W,
-                                                          const double* __restrict__ H) {
+                                                          const double* __restrict__ H,
+                                                          int64_t npd, int finish,
+                                                          double* __restrict__ ab) {
   __shared__ double part[4][64];
-  const int j = *jp;
+  const int j = jp[0];
   const int64_t g = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int cg = threadIdx.x >> 6;
+  const int64_t hs = m + H_STRIDE_EXTRA;
+  const double* Hp = H + (finish ? npd * hs : 0);
   double acc = 0.0;
   bool live = false;
-  int64_t wrow = 0;
+  int pd = 0, i = 0;
+  memi_pd_desc P{};
   if (g < n_rows) {
-    const int pd = rows[2 * g], i = rows[2 * g + 1];
-    const memi_pd_desc P = pds[pd];
+    pd = rows[2 * g];
+    i = rows[2 * g + 1];
+    P = pds[pd];
     if (active[pd] && j < min(P.n, m)) {
       live = true;
-      wrow = P.row0 + i;
       const double* Vp = V + (int64_t)m * P.row0 + i;
-      const double* h = H + (int64_t)pd * m;
+      const double* h = Hp + (int64_t)pd * hs;
       for (int c = cg; c <= j; c += 4) acc += Vp[(int64_t)c * P.n] * h[c];
     }
   }
@@ -212,47 +227,25 @@ __global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restr
   __syncthreads();
   if (cg == 0 && live) {
     const int t = threadIdx.x;
-    W[wrow] -= (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+    const double w = W[P.row0 + i] - ((part[0][t] + part[1][t]) + (part[2][t] + part[3][t]));
+    if (!finish) {
+      W[P.row0 + i] = w;
+    } else {
+      const double* h2 = Hp + (int64_t)pd * hs;
+      double q = 0.0;
+      for (int c = 0; c <= j; ++c) q += h2[c] * h2[c];
+      const double beta = sqrt(fmax(h2[m] - q, 0.0));
+      if (j + 1 < m) {
+        const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
+        V[(int64_t)m * P.row0 + (int64_t)(j + 1) * P.n + i] = w * inv;
+      }
+      if (i == 0) {
+        ab[((int64_t)pd * m + j) * 2 + 0] = H[(int64_t)pd * hs + j] + h2[j];
+        ab[((int64_t)pd * m + j) * 2 + 1] = beta;
+      }
+    }
   }
-}
-
-__device__ inline void orth_finish_body(const memi_pd_desc* __restrict__ pds,
-                                        double* __restrict__ V, int m, int j,
-                                        const int32_t* __restrict__ active,
-                                        const double* __restrict__ W,
-                                        const double* __restrict__ H, int64_t npd,
-                                        double* __restrict__ ab, double* red) {
-  const int pd = blockIdx.x;
-  if (!active[pd]) return;
-  const memi_pd_desc P = pds[pd];
-  const int n = P.n;
-  if (j >= min(n, m)) return;
-  const double* w = W + P.row0;
-  double ss = 0.0;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) ss += w[r] * w[r];
-  const double beta = sqrt(block_sum(ss, red));
-  if (threadIdx.x == 0) {
-    ab[((int64_t)pd * m + j) * 2 + 0] = H[(int64_t)pd * m + j] + H[(npd + pd) * m + j];
-    ab[((int64_t)pd * m + j) * 2 + 1] = beta;
-  }
-  if (j + 1 < m) {
-    const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
-    double* vn = V + (int64_t)m * P.row0 + (int64_t)(j + 1) * n;
-    for (int r = threadIdx.x; r < n; r += blockDim.x) vn[r] = w[r] * inv;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_orth_finish(const memi_pd_desc* __restrict__ pds,
-                                                     double* __restrict__ V, int m,
-                                                     int32_t* __restrict__ jp,
-                                                     const int32_t* __restrict__ active,
-                                                     const double* __restrict__ W,
-                                                     const double* __restrict__ H, int64_t npd,
-                                                     double* __restrict__ ab) {
-  __shared__ double red[8];
-  const int j = jp[0];
-  orth_finish_body(pds, V, m, j, active, W, H, npd, ab, red);
-  step_arrive(jp, j);
+  if (finish) step_arrive(jp, j);
 }
 
 // ---- convergence check: Ritz pairs of the tridiagonal T_p on the GPU -----------------------
@@ -587,22 +580,20 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const d
     } else {
       // column blocks sized for the largest possible j (< m); blocks beyond j exit at once
       for (int pass = 0; pass < 2; ++pass) {
-        double* Hp = H + (int64_t)pass * info->n_pd * m;
-        if (npd < 64) {       // few PDs: a 256-thread block per (column, PD)
-          hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)m, npd), dim3(256), 0, s, pds, V, m,
-                             jp, active, W, Hp);
+        const int nrm = pass == 1;
+        if (npd < 64) {       // few PDs: a 256-thread block per (column, PD), + w.w in pass 2
+          hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)(m + nrm), npd), dim3(256), 0, s, pds,
+                             V, m, jp, active, W, H + (int64_t)pass * info->n_pd * (m + 1), nrm);
         } else {              // many PDs: waves walk the live (PD, column) pairs
           hipLaunchKernelGGL(k_orth_dot_pairs, dim3(1024), dim3(256), 0, s, pds, info->n_pd, V,
-                             m, jp, active, W, Hp);
+                             m, jp, active, W, H + (int64_t)pass * info->n_pd * (m + 1), nrm);
         }
         MEMI_CHECK_LAUNCH("k_orth_dot");
         hipLaunchKernelGGL(k_orth_update_rows, dim3((unsigned)((info->n_rows + 63) / 64)),
-                           dim3(256), 0, s, rows, info->n_rows, pds, V, m, jp, active, W, Hp);
+                           dim3(256), 0, s, rows, info->n_rows, pds, V, m, j_dev, active, W, H,
+                           info->n_pd, nrm, ab);
         MEMI_CHECK_LAUNCH("k_orth_update_rows");
       }
-      hipLaunchKernelGGL(k_orth_finish, dim3(npd), dim3(256), 0, s, pds, V, m, j_dev, active, W,
-                         H, info->n_pd, ab);
-      MEMI_CHECK_LAUNCH("k_orth_finish");
     }
   }
   return MEMI_OK;

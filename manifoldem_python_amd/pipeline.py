@@ -457,7 +457,7 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
         st = dict(key=key)
         st["V"] = torch.zeros(m * n_rows, dtype=torch.float64, device=dev)
         st["W"] = torch.empty(n_rows, dtype=torch.float64, device=dev)
-        st["H"] = torch.empty(2 * npd * m, dtype=torch.float64, device=dev) if split else None
+        st["H"] = torch.empty(2 * npd * (m + 1), dtype=torch.float64, device=dev) if split else None
         st["ab"] = torch.zeros(npd * m * 2, dtype=torch.float64, device=dev)
         st["active"] = torch.zeros(npd, dtype=torch.int32, device=dev)
         st["j"] = torch.zeros(2, dtype=torch.int32, device=dev)
