@@ -493,10 +493,12 @@ def main():
     if args.workload == "c2" and os.path.exists(tfile):
         with open(tfile) as f:
             tr = json.load(f)
-        traffic = tr["bytes_per_launch"]
-        traffic_note = (f"rocprofv3 FETCH_SIZE x {tr['fetch_scale']:.3f} + WRITE_SIZE x "
-                        f"{tr['write_scale']:.3f} (copy-calibrated) per launch of variant "
-                        f"{tr['variant']} with {tr['chunks']} K chunks; {tr['source']}")
+        cfg = tr["configs"].get(f"v{bf['tuned']['best']}c{bf['tuned']['chunks']}")
+        if cfg is not None:        # the counters of the (variant, chunks) pair this run tuned to
+            traffic = cfg["bytes_per_launch"]
+            traffic_note = (f"rocprofv3 FETCH_SIZE x {tr['fetch_scale']:.3f} + WRITE_SIZE x "
+                            f"{tr['write_scale']:.3f} (copy-calibrated) per launch of variant "
+                            f"{cfg['variant']} with {cfg['chunks']} K chunks; {tr['source']}")
     gram_s = bf["gram_ms"] / 1e3
     achieved = flops / gram_s / 1e12
     out = {
