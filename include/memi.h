@@ -273,11 +273,12 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const d
 /* Convergence check of the batched Lanczos run, the Ritz pairs eigsh(which='LM') returns
  * (sembeddingonFly.py:27-35), entirely on the GPU.  For every active PD with
  * J = min(steps_done, n, m): the k = ks[pd] (<= MEMI_RITZ_KMAX) eigenvalues of largest
- * magnitude of the tridiagonal T (ab as written by memi_lanczos_step) by Sturm bisection,
- * sorted descending, their eigenvectors by twisted factorisation + modified Gram-Schmidt, and
- * the residuals |beta_{J-1} s_{J-1}|.  A PD is done when every residual is <= tol * max|theta|,
- * or J = n, or J = m: then active[pd] = 0, steps[pd] = J and res[pd] = the largest residual
- * (0 when J = n).  theta: float64 [npd][16]; S: float64 [npd][m][16] (PD p's J x k row-major
+ * magnitude of the tridiagonal T (ab as written by memi_lanczos_steps) by Sturm-count
+ * multisection, sorted descending, their eigenvectors by twisted factorisation + modified
+ * Gram-Schmidt, and the residuals |beta_{J-1} s_{J-1}|; res[pd] = the largest residual (0 when
+ * J = n) at every check.  A PD is done when every residual is <= tol * max|theta|, or J = n,
+ * or J = m: then active[pd] = 0 and steps[pd] = J.  theta: float64 [npd][16] (every check);
+ * S: float64 [npd][m][16] (PD p's J x k row-major
  * block at p*m*16, ready for memi_ritz_vectors); piv: float64 scratch [npd][16][2][m]. */
 int memi_lanczos_ritz(const void* plan_dev, const memi_plan_info* info, const double* ab, int m,
                       int steps_done, const int32_t* ks, double tol, int32_t* active,

@@ -258,32 +258,48 @@ __global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restr
 //     pivots d+, d-, twist r = argmin |gamma_r|, then the two recurrences from z_r = 1),
 //     normalised, then modified Gram-Schmidt in descending-theta order;
 //  4. residuals |beta_{J-1} s_{J-1}|; the PD is done when all are <= tol * max|theta|, or
-//     J = n (the whole space), or J = m.  Done PDs get active = 0, steps = J, res = the largest
-//     residual (0 when J = n); theta [npd][16] and S (row-major J x k at pd*m*16) stay as the
-//     final Ritz pairs.
+//     J = n (the whole space), or J = m.  res = the largest residual (0 when J = n), written at
+//     every check; done PDs get active = 0, steps = J; theta [npd][16] and S (row-major J x k at
+//     pd*m*16) stay as the final Ritz pairs.
 // piv: float64 scratch [npd][16][2][m].
-// Sturm count (eigenvalues below x) from LDS copies of alpha and beta^2, division free: the
-// sign changes of p_0 = 1, p_i = (alpha_i - x) p_{i-1} - beta_{i-1}^2 p_{i-2} (a zero p_i
-// counts as a change, as a -pivmin pivot does in the ratio form), rescaled by exact powers of
-// two when |p| leaves [2^-600, 2^600].
-__device__ inline int sturm_count(const double* al, const double* b2, int J, double x) {
-  double pm = 1.0, p = al[0] - x;
+// Sturm count (eigenvalues below x) from the LDS copy q2[i] = {alpha_i, beta_{i-1}^2},
+// division free: the sign changes of p_0 = 1, p_i = (alpha_i - x) p_{i-1} - beta_{i-1}^2 p_{i-2}
+// (a zero p_i counts as a change, as a -pivmin pivot does in the ratio form), rescaled by exact
+// powers of two when |p| leaves [2^-600, 2^600].  Four steps per load batch and range check: the
+// loads go out together instead of one LDS round trip per step, and four steps move |p| by far
+// less than the 2^400 of headroom on either side.
+__device__ inline void sturm_step(double2 q, double x, double& p, double& pm, int& c) {
+  double pn = (q.x - x) * p - q.y * pm;
+  if (pn == 0.0) pn = p < 0.0 ? 1e-300 : -1e-300;
+  c += (pn < 0.0) != (p < 0.0);
+  pm = p;
+  p = pn;
+}
+
+__device__ inline void sturm_rescale(double& p, double& pm) {
+  const double a = fabs(p);
+  if (a > 0x1p600 || a < 0x1p-600) {
+    int e;
+    frexp(a, &e);
+    pm = ldexp(pm, -e);
+    p = ldexp(p, -e);
+  }
+}
+
+__device__ inline int sturm_count(const double2* q2, int J, double x) {
+  double pm = 1.0, p = q2[0].x - x;
   if (p == 0.0) p = -1e-300;
   int c = p < 0.0;
-  for (int i = 1; i < J; ++i) {
-    double pn = (al[i] - x) * p - b2[i - 1] * pm;
-    if (pn == 0.0) pn = p < 0.0 ? 1e-300 : -1e-300;
-    c += (pn < 0.0) != (p < 0.0);
-    pm = p;
-    p = pn;
-    const double a = fabs(p);
-    if (a > 0x1p600 || a < 0x1p-600) {
-      int e;
-      frexp(a, &e);
-      pm = ldexp(pm, -e);
-      p = ldexp(p, -e);
-    }
+  int i = 1;
+  for (; i + 3 < J; i += 4) {
+    const double2 u0 = q2[i], u1 = q2[i + 1], u2 = q2[i + 2], u3 = q2[i + 3];
+    sturm_step(u0, x, p, pm, c);
+    sturm_step(u1, x, p, pm, c);
+    sturm_step(u2, x, p, pm, c);
+    sturm_step(u3, x, p, pm, c);
+    sturm_rescale(p, pm);
   }
+  for (; i < J; ++i) sturm_step(q2[i], x, p, pm, c);
   return c;
 }
 
@@ -309,7 +325,8 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
                                                           double* __restrict__ piv) {
   constexpr int KM = MEMI_RITZ_KMAX;
   constexpr bool PLDS = MAXM <= 512;
-  __shared__ double al[MAXM], b2[MAXM];
+  __shared__ double2 q2[MAXM];                       // {alpha_i, beta_{i-1}^2}
+  __shared__ double bt[MAXM];                         // beta_i
   __shared__ double pv_lds[PLDS ? 2 * MAXM * KM : 1];
   __shared__ double cand[2 * KM];
   __shared__ double sel[KM];
@@ -324,8 +341,9 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
   const int k = min(ks[pd], J);
   const double* a = ab + (int64_t)pd * m * 2;      // a[2i] = alpha_i, a[2i+1] = beta_i
   for (int i = t; i < J; i += RITZ_NT) {
-    al[i] = a[2 * i];
-    b2[i] = a[2 * i + 1] * a[2 * i + 1];
+    const double bp = i > 0 ? a[2 * (i - 1) + 1] : 0.0;
+    q2[i] = make_double2(a[2 * i], bp * bp);
+    bt[i] = a[2 * i + 1];
   }
   // Gershgorin interval and pivot floor (first wave)
   if (t < 64) {
@@ -371,7 +389,7 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
     for (int it = 0; it < 64 && h - l > fmax(2.2e-16 * fmax(fabs(l), fabs(h)), atol); ++it) {
       const double w = h - l;
       const double x = l + (w * (double)(g + 1)) / (double)(G + 1);
-      const bool above = sturm_count(al, b2, J, x) > idx;
+      const bool above = sturm_count(q2, J, x) > idx;
       const uint64_t bits = (__ballot(above) >> base) & gmask;
       const int f = bits ? __ffsll((unsigned long long)bits) - 1 : G;
       const double nl = f == 0 ? l : l + (w * (double)f) / (double)(G + 1);
@@ -404,54 +422,75 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
     }
   }
   __syncthreads();
-  // 3. twisted factorisation per selected pair: pivots, twist, then the vector's norm and last
-  //    component (the vector itself is written only once the PD is done)
+  // 3. twisted factorisation per selected pair, two adjacent threads of the first wave per
+  //    pair (the forward pivots d+ on one, the backward d- on the other; then each takes half
+  //    of the twist search and one side of the vector's recurrences): pivots, twist, the
+  //    vector's norm and last component (the vector itself is written only once the PD is done)
   double* pvb = PLDS ? pv_lds : piv + (int64_t)pd * 2 * m * KM;
-  auto DP = [&](int i) -> double& { return pvb[(2 * i) * KM + t]; };
-  auto DM = [&](int i) -> double& { return pvb[(2 * i + 1) * KM + t]; };
+  const int e = t >> 1, side = t & 1;
+  const bool tw = t < 2 * k;
+  auto DP = [&](int i) -> double& { return pvb[(2 * i) * KM + e]; };
+  auto DM = [&](int i) -> double& { return pvb[(2 * i + 1) * KM + e]; };
   int r = 0;
   double inv = 0.0, th = 0.0;
-  if (t < k) {
-    th = sel[t];
-    double q = al[0] - th;
-    DP(0) = q;
-    for (int i = 1; i < J; ++i) {
-      if (fabs(q) < pivmin) q = -pivmin;
-      q = al[i] - th - b2[i - 1] / q;
-      DP(i) = q;
-    }
-    q = al[J - 1] - th;
-    DM(J - 1) = q;
-    for (int i = J - 2; i >= 0; --i) {
-      if (fabs(q) < pivmin) q = -pivmin;
-      q = al[i] - th - b2[i] / q;
-      DM(i) = q;
-    }
-    double gbest = INFINITY;
-    for (int i = 0; i < J; ++i) {
-      const double g = fabs(DP(i) + DM(i) - (al[i] - th));
-      if (g < gbest) {
-        gbest = g;
-        r = i;
+  if (tw) {
+    th = sel[e];
+    if (side == 0) {
+      double q = q2[0].x - th;
+      DP(0) = q;
+      for (int i = 1; i < J; ++i) {
+        if (fabs(q) < pivmin) q = -pivmin;
+        q = q2[i].x - th - q2[i].y / q;
+        DP(i) = q;
+      }
+    } else {
+      double q = q2[J - 1].x - th;
+      DM(J - 1) = q;
+      for (int i = J - 2; i >= 0; --i) {
+        if (fabs(q) < pivmin) q = -pivmin;
+        q = q2[i].x - th - q2[i + 1].y / q;
+        DM(i) = q;
       }
     }
-    // norm and last component from z_r = 1
-    double zz = 1.0, nrm = 1.0, zlast = r == J - 1 ? 1.0 : 0.0;
-    for (int i = r - 1; i >= 0; --i) {
-      const double d = fabs(DP(i)) < pivmin ? -pivmin : DP(i);
-      zz = -(a[2 * i + 1] / d) * zz;
-      nrm += zz * zz;
+  }
+  __syncthreads();
+  if (tw) {
+    // twist r = argmin |d+_i + d-_i - (alpha_i - theta)|, first index on ties: the lower half
+    // on side 0, the upper on side 1
+    const int hmid = (J + 1) / 2;
+    double gb = INFINITY;
+    int rb = side ? hmid : 0;
+    for (int i = side ? hmid : 0; i < (side ? J : hmid); ++i) {
+      const double g = fabs(DP(i) + DM(i) - (q2[i].x - th));
+      if (g < gb) {
+        gb = g;
+        rb = i;
+      }
     }
-    zz = 1.0;
-    for (int i = r; i + 1 < J; ++i) {
-      const double d = fabs(DM(i + 1)) < pivmin ? -pivmin : DM(i + 1);
-      zz = -(a[2 * i + 1] / d) * zz;
-      nrm += zz * zz;
-      if (i + 1 == J - 1) zlast = zz;
+    const double go = __shfl_xor(gb, 1, 64);
+    const int ro = __shfl_xor(rb, 1, 64);
+    r = side == 0 ? (go < gb ? ro : rb) : (gb < go ? rb : ro);
+    // norm and last component from z_r = 1: side 0 the rows below r, side 1 the rows after
+    double zz = 1.0, part = 0.0, zl = r == J - 1 ? 1.0 : 0.0;
+    if (side == 0) {
+      for (int i = r - 1; i >= 0; --i) {
+        const double d = fabs(DP(i)) < pivmin ? -pivmin : DP(i);
+        zz = -(bt[i] / d) * zz;
+        part += zz * zz;
+      }
+    } else {
+      for (int i = r; i + 1 < J; ++i) {
+        const double d = fabs(DM(i + 1)) < pivmin ? -pivmin : DM(i + 1);
+        zz = -(bt[i] / d) * zz;
+        part += zz * zz;
+        if (i + 1 == J - 1) zl = zz;
+      }
     }
-    inv = 1.0 / sqrt(nrm);
+    const double po = __shfl_xor(part, 1, 64);
+    const double zo = __shfl_xor(zl, 1, 64);
+    inv = 1.0 / sqrt(1.0 + (side == 0 ? part + po : po + part));
     // 4. residual |beta_{J-1} s_{J-1}|
-    resv[t] = fabs(a[2 * (J - 1) + 1] * zlast * inv);
+    if (side == 0) resv[e] = fabs(bt[J - 1] * zo * inv);
   }
   __syncthreads();
   if (t == 0) {
@@ -463,29 +502,31 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
     const bool done = J >= n || J >= m || rmax <= tol * mx;
     for (int q = 0; q < KM; ++q) theta[(int64_t)pd * KM + q] = q < k ? sel[q] : 0.0;
     done_sh = done;
+    res_out[pd] = J >= n ? 0.0 : rmax;               // every check: the caller paces the next
     if (done) {
       active[pd] = 0;
       steps[pd] = J;
-      res_out[pd] = J >= n ? 0.0 : rmax;
     }
   }
   __syncthreads();
   if (!done_sh) return;                               // uniform over the block
   // 5. the converged vectors: the recurrences again, normalised, into S (row-major J x k)
   double* Sp = S + (int64_t)pd * m * KM;
-  if (t < k) {
+  if (tw) {
     double zz = 1.0;
-    Sp[(int64_t)r * k + t] = inv;
-    for (int i = r - 1; i >= 0; --i) {
-      const double d = fabs(DP(i)) < pivmin ? -pivmin : DP(i);
-      zz = -(a[2 * i + 1] / d) * zz;
-      Sp[(int64_t)i * k + t] = zz * inv;
-    }
-    zz = 1.0;
-    for (int i = r; i + 1 < J; ++i) {
-      const double d = fabs(DM(i + 1)) < pivmin ? -pivmin : DM(i + 1);
-      zz = -(a[2 * i + 1] / d) * zz;
-      Sp[(int64_t)(i + 1) * k + t] = zz * inv;
+    if (side == 0) {
+      Sp[(int64_t)r * k + e] = inv;
+      for (int i = r - 1; i >= 0; --i) {
+        const double d = fabs(DP(i)) < pivmin ? -pivmin : DP(i);
+        zz = -(bt[i] / d) * zz;
+        Sp[(int64_t)i * k + e] = zz * inv;
+      }
+    } else {
+      for (int i = r; i + 1 < J; ++i) {
+        const double d = fabs(DM(i + 1)) < pivmin ? -pivmin : DM(i + 1);
+        zz = -(bt[i] / d) * zz;
+        Sp[(int64_t)(i + 1) * k + e] = zz * inv;
+      }
     }
   }
   __syncthreads();

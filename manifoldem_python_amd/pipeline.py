@@ -417,6 +417,26 @@ def lm_ritz(alpha, beta, k):
 LANCZOS = dict(graph=True, split=None, check_every=None)   # dm_embed's Lanczos settings
 
 
+def _next_check_gap(j, res, thmax, active, tol, step, hist, slack=0.6, cap=4):
+    """Steps to the next convergence check.  Each check reports every live PD's largest
+    residual r against its target tol * max|theta|; with two checks, log(r / target) is
+    extrapolated linearly in the step count (Lanczos residuals usually fall faster than that,
+    hence `slack`), and the next check goes to the earliest predicted convergence over the live
+    PDs, in whole blocks of `step`, between 1 and `cap` blocks away."""
+    live = active.astype(bool)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lr = np.log(np.maximum(res, 1e-300) / (tol * np.maximum(thmax, 1e-300)))
+    gap = step
+    if hist is not None and live.any():
+        j0, lr0 = hist
+        rate = (lr - lr0) / max(j - j0, 1)                # per step; negative while converging
+        ok = live & np.isfinite(rate) & (rate < 0) & (lr > 0)
+        if ok.any() and ok.sum() == live.sum():
+            need = np.min(lr[ok] / -rate[ok]) * slack
+            gap = int(min(cap, max(1, need // step))) * step
+    return gap, (j, lr)
+
+
 def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202, split=None,
                  check="gpu", graph=True):
     """Largest-magnitude eigenpairs of every PD's float64 kernel block of L (sembeddingonFly.py
@@ -427,8 +447,10 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     the Krylov space is the whole space.  split: orthogonalise with the whole grid (default
     for batches of fewer than 128 PDs or with a PD above 1024 rows) instead of one workgroup
     per PD.  check: "gpu" (memi_lanczos_ritz: Ritz pairs and the decision on the device, one
-    int per PD read back per check, every 8 steps from the first step count at which the
-    selected pairs can have converged, by default) or "host" (scipy tridiagonal
+    int per PD read back per check; the first check at the first step count at which the
+    selected pairs can have converged, later ones every 8 steps or, once two checks show how
+    fast every live PD's residual falls, at the extrapolated convergence step, up to 32 steps
+    on) or "host" (scipy tridiagonal
     eigensolver per PD, every 32 steps by default).  graph: the launches of one block of
     check_every steps are captured once in a HIP graph and replayed (the step index lives on
     the device).  Returns a list of (lam, V (n x k) numpy, steps, max residual)."""
@@ -497,8 +519,10 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     steps = np.minimum(ns, m)
     out = [None] * npd
     j = 0
+    gap = check_every
+    hist = None                      # (j, log of residual / target) at the previous check
     while active_h.any():
-        target = first_check if j == 0 else min(m, j + check_every)
+        target = first_check if j == 0 else min(m, j + gap)
         while j < target:
             if g is not None:
                 g.replay()
@@ -508,7 +532,11 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
         if check == "gpu":
             nat.lanczos_ritz(plan, ab, m, j, ks_t, tol, active, theta, S_dev, steps_t, res_t,
                              piv)
-            active_h = active.cpu().numpy()
+            # one copy: active flags, largest residuals, max |theta| per PD
+            h = torch.stack([active.double(), res_t,
+                             theta.view(npd, K).abs().amax(dim=1)]).cpu().numpy()
+            active_h = h[0].astype(np.int32)
+            gap, hist = _next_check_gap(j, h[1], h[2], active_h, tol, check_every, hist)
             continue
         abh = ab.view(npd, m, 2).cpu().numpy()
 
