@@ -345,14 +345,71 @@ __device__ inline void spline_sample(const float* src, int N, int S, double angl
   }
 }
 
-// One block per image.  NBUF = 2: mask, rotate by ang1 into the second LDS image, rotate by
-// ang2 into out.  NBUF = 1: (mask if disk,) rotate by ang1 into out; the host runs it twice.
-// in and out may be the same buffer (each block reads its whole image before writing).
+// spline_sample into the same LDS image: every thread keeps its outputs in registers (a
+// 512-thread block gives each thread at most ceil(NMAX/8) rows x ceil(NMAX/64) columns), the
+// block synchronises, then writes them back -- one image of LDS per rotation instead of two.
+template <int NMAX>
+__device__ inline void spline_sample_inplace(float* buf, int N, int S, double angle_deg) {
+  constexpr int NW = 8, RI = (NMAX + NW - 1) / NW, CI = (NMAX + 63) / 64;
+  double sn, cs;
+  sincospi(angle_deg / 180.0, &sn, &cs);
+  const double h = 0.5 * (N - 1);
+  const double lo = 2 - N, hi = 2 * N - 3;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float v[RI][CI];
+#pragma unroll
+  for (int i = 0; i < RI; ++i) {
+    const int r = wv + i * NW;
+    const double dr = r - h;
+    const double br = h + cs * dr, bc = h - sn * dr;
+#pragma unroll
+    for (int q = 0; q < CI; ++q) {
+      const int c = lane + q * 64;
+      float acc = 0.f;
+      if (r < N && c < N) {
+        const double dc = c - h;
+        const double sr = br + sn * dc;
+        const double sc = bc + cs * dc;
+        const double fr = floor(sr), fc = floor(sc);
+        float wr[4], wc[4];
+        bspline_w((float)(sr - fr), wr);
+        bspline_w((float)(sc - fc), wc);
+        int ir = (int)fmin(fmax(fr, lo), hi) - 1;
+        int ic = (int)fmin(fmax(fc, lo), hi) - 1;
+        ir += ir < 0 ? N : 0;
+        ir -= ir >= N ? N : 0;
+        ic += ic < 0 ? N : 0;
+        ic -= ic >= N ? N : 0;
+        const float* p = buf + ir * S + ic;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float* rp = p + u * S;
+          acc += wr[u] * (wc[0] * rp[0] + wc[1] * rp[1] + wc[2] * rp[2] + wc[3] * rp[3]);
+        }
+      }
+      v[i][q] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < RI; ++i)
+#pragma unroll
+    for (int q = 0; q < CI; ++q) {
+      const int r = wv + i * NW, c = lane + q * 64;
+      if (r < N && c < N) buf[r * S + c] = v[i][q];
+    }
+  __syncthreads();
+}
+
+// One block of 512 threads per image, the image (row stride N + 3, padded) in LDS.
+// NBUF = 2: mask, rotate by ang1 in place, rotate by ang2 into out.  NBUF = 1: (mask if disk,)
+// rotate by ang1 into out; the host runs it twice.  in and out may be the same buffer (each
+// block reads its whole image before writing).
 template <int NMAX, int NBUF>
 __global__ __launch_bounds__(512) void k_align_images(const float* in, const double* ang1,
                                                       const double* ang2, int N, int disk,
                                                       float* out) {
-  __shared__ float lds[NBUF * (NMAX + 3) * (NMAX + 3)];
+  __shared__ float lds[(NMAX + 3) * (NMAX + 3)];
   const int S = N + 3;
   const int64_t img = blockIdx.x;
   const float* src = in + img * N * N;
@@ -372,11 +429,9 @@ __global__ __launch_bounds__(512) void k_align_images(const float* in, const dou
   bspline_prefilter(A, N, S, zN);
   float* o = out + img * N * N;
   if constexpr (NBUF == 2) {
-    float* B = lds + (NMAX + 3) * (NMAX + 3);
-    spline_sample(A, N, S, ang1[img], B, S);
-    __syncthreads();
-    bspline_prefilter(B, N, S, zN);
-    spline_sample(B, N, S, ang2[img], o, N);
+    spline_sample_inplace<NMAX>(A, N, S, ang1[img]);
+    bspline_prefilter(A, N, S, zN);
+    spline_sample(A, N, S, ang2[img], o, N);
   } else {
     spline_sample(A, N, S, ang1[img], o, N);
   }
@@ -754,7 +809,7 @@ int memi_align_images(const float* in, const double* rot1_deg, const double* rot
   const dim3 g((unsigned)n), b(512);
   if (N <= 64) {
     hipLaunchKernelGGL((k_align_images<64, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
-  } else if (N <= 96) {          // 78 KB of LDS: two blocks per CU
+  } else if (N <= 96) {          // 39 KB of LDS: four blocks per CU
     hipLaunchKernelGGL((k_align_images<96, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
   } else if (N <= 140) {
     hipLaunchKernelGGL((k_align_images<140, 2>), g, b, 0, s, in, rot1_deg, rot2_deg, N, disk, out);
