@@ -602,8 +602,11 @@ __device__ inline void tanh_terms(double x, const double a[4], double& f, double
   J[3] = 1.0;
 }
 
-// block reduction of NV doubles (in place in v), 256 threads
-template <int NV>
+// block reduction of NV doubles (in place in v) over NW waves: each wave sums by shuffles,
+// thread q < NV adds the NW wave sums of value q in wave order (row NW of sh), and every thread
+// reads the NV totals back.  (Summing all NV x NW partials in every thread had the compiler
+// issue every LDS read at once, holding NV x NW doubles in registers.)
+template <int NV, int NW>
 __device__ inline void block_reduce_vec(double* v, double (*sh)[NV]) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -614,129 +617,197 @@ __device__ inline void block_reduce_vec(double* v, double (*sh)[NV]) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) sh[wid][q] = v[q];
   __syncthreads();
+  if (threadIdx.x < NV) {
+    double t = 0.0;
 #pragma unroll
-  for (int q = 0; q < NV; ++q) v[q] = sh[0][q] + sh[1][q] + sh[2][q] + sh[3][q];
+    for (int w = 0; w < NW; ++w) t += sh[w][threadIdx.x];
+    sh[NW][threadIdx.x] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = sh[NW][q];
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_tanh_fit(const double* __restrict__ x,
-                                                  const double* __restrict__ y, int m,
-                                                  const double* __restrict__ p0,
-                                                  const int32_t* __restrict__ active,
-                                                  int max_iter, double* __restrict__ out) {
-  __shared__ double red[4][15];
-  __shared__ double ctl[8];   // trial parameters, flags
-  const int pd = blockIdx.x;
-  if (active && !active[pd]) return;
-  const double* yp = y + (int64_t)pd * m;
-  double a[4];
-  for (int q = 0; q < 4; ++q) a[q] = p0[pd * 4 + q];
-  double diag[4] = {0, 0, 0, 0};
-  double lam = 1e-3;
-  int it = 0;
-  double cost = 0.0, JtJ[10], Jtr[4];
-  // evaluate J^T J, J^T r and cost at a
-  auto eval = [&](const double* aa, bool full, double* jtj, double* jtr, double& c) {
-    double v[15];
-    for (int q = 0; q < 15; ++q) v[q] = 0.0;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      double f, J[4];
-      tanh_terms(x[i], aa, f, J);
-      const double r = yp[i] - f;
-      v[14] += r * r;
-      if (full) {
-        int k = 0;
-        for (int p = 0; p < 4; ++p)
-          for (int q = p; q < 4; ++q) v[k++] += J[p] * J[q];
-        for (int p = 0; p < 4; ++p) v[10 + p] += J[p] * r;
-      }
-    }
-    block_reduce_vec<15>(v, red);
-    c = v[14];
-    if (full) {
-      for (int k = 0; k < 10; ++k) jtj[k] = v[k];
-      for (int p = 0; p < 4; ++p) jtr[p] = v[10 + p];
-    }
-  };
-  eval(a, true, JtJ, Jtr, cost);
-  const int dg[4] = {0, 4, 7, 9};  // packed upper-triangle index of (p, p)
-  bool done = false;
-  while (!done && it < max_iter) {
-    ++it;
-    for (int p = 0; p < 4; ++p) diag[p] = fmax(diag[p], sqrt(JtJ[dg[p]]));
-    // solve (JtJ + lam diag^2) d = Jtr by Cholesky (all threads redundantly, 4x4)
+// 1024 threads per PD evaluate the curve (~1.5 points each); the LM bookkeeping (scaling, the
+// damped 4x4 Cholesky solve, accept / reject) runs on thread 0 with the state in LDS, so the
+// evaluation's registers and the solve's are never live together.
+constexpr int TANH_NT = 1024;
+
+struct LmState {
+  double a[4], at[4], diag[4], d[4];
+  double jtj[10], jtr[4];          // at a
+  double cost, lam;
+  int it, done;
+};
+
+// cost, J^T J and J^T r at the point pt (LDS), summed over the curve into v; every thread gets
+// them (a plain inlined function: a by-reference lambda here kept its captures in scratch)
+__device__ __forceinline__ void tanh_eval(const double* __restrict__ x,
+                                          const double* __restrict__ yp, int m,
+                                          const double* pt, double* v,
+                                          double (*red)[15]) {
+  double aa[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) aa[q] = pt[q];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) v[q] = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    double f, J[4];
+    tanh_terms(x[i], aa, f, J);
+    const double r = yp[i] - f;
+    v[14] += r * r;
+    int k = 0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = p; q < 4; ++q) v[k++] += J[p] * J[q];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) v[10 + p] += J[p] * r;
+  }
+  block_reduce_vec<15, TANH_NT / 64>(v, red);
+}
+
+// thread 0: scaling update and the damped step from the current point into st.at (more
+// damping while the system is not positive definite); st.done when the fit must stop
+__device__ inline void lm_propose(LmState& st, int max_iter) {
+  const int dg[4] = {0, 4, 7, 9};  // packed (p, p)
+  for (;;) {
+    ++st.it;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) st.diag[p] = fmax(st.diag[p], sqrt(st.jtj[dg[p]]));
     double A[4][4];
     {
       int k = 0;
+#pragma unroll
       for (int p = 0; p < 4; ++p)
+#pragma unroll
         for (int q = p; q < 4; ++q) {
-          A[p][q] = JtJ[k];
-          A[q][p] = JtJ[k];
+          A[p][q] = st.jtj[k];
+          A[q][p] = st.jtj[k];
           ++k;
         }
     }
-    for (int p = 0; p < 4; ++p) A[p][p] += lam * diag[p] * diag[p];
-    double Lc[4][4] = {};
+#pragma unroll
+    for (int p = 0; p < 4; ++p) A[p][p] += st.lam * st.diag[p] * st.diag[p];
+    double Lc[4][4];
     bool ok = true;
-    for (int p = 0; p < 4 && ok; ++p) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+#pragma unroll
       for (int q = 0; q <= p; ++q) {
-        double s = A[p][q];
-        for (int r = 0; r < q; ++r) s -= Lc[p][r] * Lc[q][r];
+        double sm = A[p][q];
+#pragma unroll
+        for (int r = 0; r < q; ++r) sm -= Lc[p][r] * Lc[q][r];
         if (p == q) {
-          if (s <= 0.0) { ok = false; break; }
-          Lc[p][p] = sqrt(s);
+          ok = ok && sm > 0.0;
+          Lc[p][p] = sqrt(fmax(sm, 1e-300));
         } else {
-          Lc[p][q] = s / Lc[q][q];
+          Lc[p][q] = sm / Lc[q][q];
         }
       }
     }
-    if (!ok) { lam *= 10.0; if (lam > 1e32) break; continue; }
-    double z[4], d[4];
+    if (!ok) {                     // not positive definite: more damping, same point
+      st.lam *= 10.0;
+      if (st.lam > 1e32 || st.it >= max_iter) {
+        st.done = 1;
+        return;
+      }
+      continue;
+    }
+    double z[4];
+#pragma unroll
     for (int p = 0; p < 4; ++p) {
-      double s = Jtr[p];
-      for (int r = 0; r < p; ++r) s -= Lc[p][r] * z[r];
-      z[p] = s / Lc[p][p];
+      double sm = st.jtr[p];
+#pragma unroll
+      for (int r = 0; r < p; ++r) sm -= Lc[p][r] * z[r];
+      z[p] = sm / Lc[p][p];
     }
+#pragma unroll
     for (int p = 3; p >= 0; --p) {
-      double s = z[p];
-      for (int r = p + 1; r < 4; ++r) s -= Lc[r][p] * d[r];
-      d[p] = s / Lc[p][p];
+      double sm = z[p];
+#pragma unroll
+      for (int r = p + 1; r < 4; ++r) sm -= Lc[r][p] * st.d[r];
+      st.d[p] = sm / Lc[p][p];
     }
-    double at[4];
-    for (int p = 0; p < 4; ++p) at[p] = a[p] + d[p];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) st.at[p] = st.a[p] + st.d[p];
+    return;
+  }
+}
+
+__global__ __launch_bounds__(TANH_NT) void k_tanh_fit(const double* __restrict__ x,
+                                                      const double* __restrict__ y, int m,
+                                                      const double* __restrict__ p0,
+                                                      const int32_t* __restrict__ active,
+                                                      int max_iter, double* __restrict__ out) {
+  __shared__ double red[TANH_NT / 64 + 1][15];
+  __shared__ LmState st;
+  const int pd = blockIdx.x;
+  if (active && !active[pd]) return;
+  const double* yp = y + (int64_t)pd * m;
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; ++q) {
+      st.a[q] = p0[pd * 4 + q];
+      st.diag[q] = 0.0;
+    }
+    st.lam = 1e-3;
+    st.it = 0;
+    st.done = 0;
+  }
+  __syncthreads();
+  {
+    double v[15];
+    tanh_eval(x, yp, m, st.a, v, red);
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < 10; ++k) st.jtj[k] = v[k];
+      for (int p = 0; p < 4; ++p) st.jtr[p] = v[10 + p];
+      st.cost = v[14];
+      if (max_iter <= 0) st.done = 1;
+      else lm_propose(st, max_iter);
+    }
+  }
+  __syncthreads();
+  while (!st.done) {                                  // uniform: read after a barrier
     // the trial point's J^T J and J^T r come with its cost: an accepted step then needs no
     // second pass (they are the values a re-evaluation at the new point would give)
-    double cost_t, JtJ_t[10], Jtr_t[4];
-    eval(at, true, JtJ_t, Jtr_t, cost_t);
-    // predicted reduction of the linear model
-    double pred = 0.0;
-    for (int p = 0; p < 4; ++p) pred += d[p] * (Jtr[p] + lam * diag[p] * diag[p] * d[p]);
-    if (cost_t < cost) {
-      const double actred = (cost - cost_t) / fmax(cost, 1e-300);
-      double xn = 0.0, dn = 0.0;
-      for (int p = 0; p < 4; ++p) {
-        xn += (diag[p] * at[p]) * (diag[p] * at[p]);
-        dn += (diag[p] * d[p]) * (diag[p] * d[p]);
+    double v[15];
+    tanh_eval(x, yp, m, st.at, v, red);
+    if (threadIdx.x == 0) {
+      const double cost_t = v[14];
+      double pred = 0.0;
+      for (int p = 0; p < 4; ++p)
+        pred += st.d[p] * (st.jtr[p] + st.lam * st.diag[p] * st.diag[p] * st.d[p]);
+      if (cost_t < st.cost) {
+        const double actred = (st.cost - cost_t) / fmax(st.cost, 1e-300);
+        double xn = 0.0, dn = 0.0;
+        for (int p = 0; p < 4; ++p) {
+          xn += (st.diag[p] * st.at[p]) * (st.diag[p] * st.at[p]);
+          dn += (st.diag[p] * st.d[p]) * (st.diag[p] * st.d[p]);
+        }
+        for (int p = 0; p < 4; ++p) st.a[p] = st.at[p];
+        st.cost = cost_t;
+        for (int k = 0; k < 10; ++k) st.jtj[k] = v[k];
+        for (int p = 0; p < 4; ++p) st.jtr[p] = v[10 + p];
+        st.lam = fmax(st.lam * 0.1, 1e-15);
+        const double prered = pred / fmax(cost_t + pred, 1e-300);
+        if ((actred <= 1.49012e-8 && prered <= 1.49012e-8) || sqrt(dn) <= 1.49012e-8 * sqrt(xn))
+          st.done = 1;
+      } else {
+        st.lam *= 10.0;
+        if (st.lam > 1e32) st.done = 1;
       }
-      for (int p = 0; p < 4; ++p) a[p] = at[p];
-      cost = cost_t;
-      for (int k = 0; k < 10; ++k) JtJ[k] = JtJ_t[k];
-      for (int p = 0; p < 4; ++p) Jtr[p] = Jtr_t[p];
-      lam = fmax(lam * 0.1, 1e-15);
-      const double prered = pred / fmax(cost_t + pred, 1e-300);
-      if ((actred <= 1.49012e-8 && prered <= 1.49012e-8) || sqrt(dn) <= 1.49012e-8 * sqrt(xn))
-        done = true;
-    } else {
-      lam *= 10.0;
-      if (lam > 1e32) done = true;
+      if (!st.done && st.it >= max_iter) st.done = 1;
+      if (!st.done) lm_propose(st, max_iter);
     }
+    __syncthreads();
   }
   if (threadIdx.x == 0) {
     double* o = out + (int64_t)pd * 16;
-    for (int p = 0; p < 4; ++p) o[p] = a[p];
-    for (int k = 0; k < 10; ++k) o[4 + k] = JtJ[k];
-    o[14] = cost;
-    o[15] = it;
+    for (int p = 0; p < 4; ++p) o[p] = st.a[p];
+    for (int k = 0; k < 10; ++k) o[4 + k] = st.jtj[k];
+    o[14] = st.cost;
+    o[15] = st.it;
   }
 }
 
@@ -1005,7 +1076,7 @@ int memi_tanh_fit(const double* x, const double* y, int m, int npd, const double
   MEMI_CHECK_ARG(x && y && p0 && out && m >= 5 && npd >= 0 && max_iter > 0,
                  "memi_tanh_fit: bad arguments");
   if (npd == 0) return MEMI_OK;
-  hipLaunchKernelGGL(k_tanh_fit, dim3((unsigned)npd), dim3(256), 0, (hipStream_t)stream, x, y, m,
+  hipLaunchKernelGGL(k_tanh_fit, dim3((unsigned)npd), dim3(TANH_NT), 0, (hipStream_t)stream, x, y, m,
                      p0, active, max_iter, out);
   MEMI_CHECK_LAUNCH("k_tanh_fit");
   return MEMI_OK;
