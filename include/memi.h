@@ -236,9 +236,10 @@ int memi_tanh_fit(const double* x, const double* y, int m, int npd, const double
 
 /* Markov normalisation, slaplacianonFly.py:42-81 with alpha = 1: W = exp(-v/sigma^2) on the
  * kernel's entries, d = column sums, W/(d_i d_j), d' = sqrt(column sums), L = |L + L^T| / 2.
- * sigma: float64 [npd]; L: float64 [info->d_elems]; deg: float64 scratch [2 * n_rows]. */
+ * Sums in float64; L is stored as float32 (its entries come from float32 distances).
+ * sigma: float64 [npd]; L: float32 [info->d_elems]; deg: float64 scratch [2 * n_rows]. */
 int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, const float* D,
-                          const uint32_t* mask, const double* sigma, double* L, double* deg,
+                          const uint32_t* mask, const double* sigma, float* L, double* deg,
                           void* stream);
 
 /* fergusonE.py:25-43 on a flat value list (the reference's D argument; squared here):
@@ -257,7 +258,8 @@ int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, i
 
 /* nsteps Lanczos steps (full reorthogonalisation, CGS2) on every active PD's L, for the
  * largest-magnitude eigenpairs that sembeddingonFly.py:27 asks ARPACK eigsh for.
- * V: float64 bases, column-major per PD at m*row0 (column 0 = unit start vector, set by the
+ * L: float32 (l_is_f64 = 0, as memi_markov_normalize writes it) or float64 (l_is_f64 = 1, a
+ * caller's own float64 kernel); products and sums in float64.  V: float64 bases, column-major per PD at m*row0 (column 0 = unit start vector, set by the
  * caller); W: float64 [n_rows] scratch; ab: float64 [npd][m][2] gets {alpha_j, beta_j};
  * active: int32 [npd] (0 = skip).  The step index j lives in device memory: j_dev is int32
  * [2] = {j, arrival counter}, both zero before the first step.  Step j reads column j, writes
@@ -266,9 +268,10 @@ int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, i
  * once in a HIP graph and replayed.
  * split = 0: one workgroup per PD orthogonalises (H unused, may be NULL); split = 1: the
  * orthogonalisation is spread over the grid (for few large PDs), H: float64 [2*npd*(m+1)]. */
-int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const double* L,
-                       double* V, int m, int32_t* j_dev, int nsteps, const int32_t* active,
-                       double* W, double* H, double* ab, int split, void* stream);
+int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const void* L,
+                       int l_is_f64, double* V, int m, int32_t* j_dev, int nsteps,
+                       const int32_t* active, double* W, double* H, double* ab, int split,
+                       void* stream);
 
 /* Convergence check of the batched Lanczos run, the Ritz pairs eigsh(which='LM') returns
  * (sembeddingonFly.py:27-35), entirely on the GPU.  For every active PD with

@@ -73,8 +73,8 @@ _SIGS = {
     "memi_tanh_fit": [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_ferguson_values": [c_vp, c_i64, c_vp, c_int, c_vp, c_vp],
     "memi_markov_coo": [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_vp],
-    "memi_lanczos_steps": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_int, c_vp, c_int, c_vp,
-                           c_vp, c_vp, c_vp, c_int, c_vp],
+    "memi_lanczos_steps": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_vp, c_int, c_vp, c_int,
+                           c_vp, c_vp, c_vp, c_vp, c_int, c_vp],
     "memi_ritz_vectors": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_vp, c_vp, c_int, c_vp,
                           c_vp],
     "memi_lanczos_ritz": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_int, c_vp, c_dbl, c_vp,
@@ -433,7 +433,8 @@ def markov_normalize(plan, D, mask, sigma, L, deg):
     I = plan.info
     _need(D.numel() >= I.d_elems and mask.numel() >= I.mask_words, "markov_normalize: D/mask")
     _need(sigma.dtype == torch.float64 and sigma.numel() >= I.n_pd, "markov_normalize: sigma")
-    _need(L.dtype == torch.float64 and L.numel() >= I.d_elems, "markov_normalize: L too small")
+    _need(L.dtype == torch.float32 and L.numel() >= I.d_elems,
+          "markov_normalize: L must be float32 with d_elems elements")
     _need(deg.dtype == torch.float64 and deg.numel() >= 2 * I.n_rows, "markov_normalize: deg")
     call("memi_markov_normalize", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask),
          _p(sigma), _p(L), _p(deg), _stream())
@@ -468,13 +469,15 @@ def lanczos_steps(plan, L, V, m, j_dev, nsteps, active, W, ab, H=None):
     float64 [2*npd*(m+1)] scratch selects the grid-parallel orthogonalisation; None selects one
     workgroup per PD."""
     I = plan.info
-    _need(L.dtype == torch.float64 and L.numel() >= I.d_elems, "lanczos_steps: L too small")
+    _need(L.dtype in (torch.float32, torch.float64) and L.numel() >= I.d_elems,
+          "lanczos_steps: L must be float32 or float64 with d_elems elements")
     _need(V.numel() >= m * I.n_rows and W.numel() >= I.n_rows and ab.numel() >= I.n_pd * m * 2
           and active.numel() >= I.n_pd and active.dtype == torch.int32, "lanczos_steps: buffers")
     _need(j_dev.dtype == torch.int32 and j_dev.numel() >= 2, "lanczos_steps: j_dev int32 [2]")
     _need(H is None or (H.dtype == torch.float64 and H.numel() >= 2 * I.n_pd * (m + 1)),
           "lanczos_steps: H must be float64 with 2*npd*(m+1) elements")
-    call("memi_lanczos_steps", _p(plan.dev), ctypes.byref(plan.info), _p(L), _p(V), int(m),
+    call("memi_lanczos_steps", _p(plan.dev), ctypes.byref(plan.info), _p(L),
+         int(L.dtype == torch.float64), _p(V), int(m),
          _p(j_dev), int(nsteps), _p(active), _p(W), None if H is None else _p(H), _p(ab),
          int(H is not None), _stream())
 

@@ -557,7 +557,7 @@ __global__ __launch_bounds__(256) void k_markov_write(const int32_t* __restrict_
                                                       const uint32_t* __restrict__ mask,
                                                       const double* __restrict__ sigma,
                                                       const double* __restrict__ deg,
-                                                      int64_t n_rows, double* __restrict__ L) {
+                                                      int64_t n_rows, float* __restrict__ L) {
   const int32_t* f = ftiles + 4 * (int64_t)blockIdx.x;
   const int pd = f[0];
   const memi_pd_desc P = pds[pd];
@@ -582,7 +582,9 @@ __global__ __launch_bounds__(256) void k_markov_write(const int32_t* __restrict_
     // slaplacianonFly.py:63-78: W/(d_i d_j), then /(d'_i d'_j), then |l + l^T| / 2
     const double lij = wij / (di * dj) / (dpi * dpj);
     const double lji = wji / (dj * di) / (dpj * dpi);
-    L[P.d_off + (int64_t)i * P.n + j] = fabs(lij + lji) / 2.0;
+    // stored in float32: the entries come from float32 distances, so a float64 copy would carry
+    // no more of the data and cost the Lanczos matvecs twice the bytes
+    L[P.d_off + (int64_t)i * P.n + j] = (float)(fabs(lij + lji) / 2.0);
   }
 }
 
@@ -1006,7 +1008,7 @@ int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, cons
 }
 
 int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, const float* D,
-                          const uint32_t* mask, const double* sigma, double* L, double* deg,
+                          const uint32_t* mask, const double* sigma, float* L, double* deg,
                           void* stream) {
   MEMI_CHECK_ARG(plan_dev && info && D && mask && sigma && L && deg,
                  "memi_markov_normalize: null pointer");

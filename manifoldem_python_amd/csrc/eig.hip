@@ -9,7 +9,7 @@
 // the whole grid in three kernels).  The host diagonalises the small tridiagonal T; the Ritz
 // vectors V S of every PD come from one batched kernel (k_ritz_vectors).
 //
-// Layout: L float64, PD p's n*n block at plan d_off; V float64 column-major per PD, PD p at
+// Layout: L float32 or float64, PD p's n*n block at plan d_off; V float64 column-major per PD, PD p at
 // element p_vo = m * row0(p) (row0 = packed row offset), column c of length n at p_vo + c*n;
 // W float64 [n_rows]; ab float64 [npd][m][2] = {alpha_j, beta_j}; active int32 [npd].
 #include "common.h"
@@ -30,10 +30,11 @@ __device__ inline void step_arrive(int32_t* jp, int j) {
   }
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restrict__ rows,
                                                         int64_t n_rows,
                                                         const memi_pd_desc* __restrict__ pds,
-                                                        const double* __restrict__ L,
+                                                        const T* __restrict__ L,
                                                         const double* __restrict__ V, int m,
                                                         const int32_t* __restrict__ jp, const int32_t* __restrict__ active,
                                                         double* __restrict__ W) {
@@ -46,15 +47,15 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restric
   const memi_pd_desc P = pds[pd];
   const int n = P.n;
   if (j >= min(n, m)) return;
-  const double* Lr = L + P.d_off + (int64_t)i * n;
+  const T* Lr = L + P.d_off + (int64_t)i * n;
   const double* v = V + (int64_t)m * P.row0 + (int64_t)j * n;
   double s = 0.0, s1 = 0.0;
   int c = lane;
   for (; c + 64 < n; c += 128) {
-    s += Lr[c] * v[c];
-    s1 += Lr[c + 64] * v[c + 64];
+    s += (double)Lr[c] * v[c];
+    s1 += (double)Lr[c + 64] * v[c + 64];
   }
-  if (c < n) s += Lr[c] * v[c];
+  if (c < n) s += (double)Lr[c] * v[c];
   s += s1;
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) W[P.row0 + i] = s;
@@ -594,9 +595,10 @@ using namespace memi;
 
 extern "C" {
 
-int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const double* L,
-                       double* V, int m, int32_t* j_dev, int nsteps, const int32_t* active,
-                       double* W, double* H, double* ab, int split, void* stream) {
+int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const void* L,
+                       int l_is_f64, double* V, int m, int32_t* j_dev, int nsteps,
+                       const int32_t* active, double* W, double* H, double* ab, int split,
+                       void* stream) {
   MEMI_CHECK_ARG(plan_dev && info && L && V && j_dev && active && W && ab && (!split || H),
                  "memi_lanczos_steps: null pointer");
   MEMI_CHECK_ARG(m >= 2 && m <= 4096 && nsteps >= 0, "memi_lanczos_steps: bad m=%d nsteps=%d",
@@ -611,8 +613,13 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const d
   const unsigned npd = (unsigned)info->n_pd;
   const int32_t* jp = j_dev;
   for (int st = 0; st < nsteps; ++st) {
-    hipLaunchKernelGGL(k_lanczos_matvec, dim3((unsigned)((info->n_rows + 3) / 4)), dim3(256), 0,
-                       s, rows, info->n_rows, pds, L, V, m, jp, active, W);
+    const dim3 gm((unsigned)((info->n_rows + 3) / 4));
+    if (l_is_f64)
+      hipLaunchKernelGGL(k_lanczos_matvec<double>, gm, dim3(256), 0, s, rows, info->n_rows, pds,
+                         (const double*)L, V, m, jp, active, W);
+    else
+      hipLaunchKernelGGL(k_lanczos_matvec<float>, gm, dim3(256), 0, s, rows, info->n_rows, pds,
+                         (const float*)L, V, m, jp, active, W);
     MEMI_CHECK_LAUNCH("k_lanczos_matvec");
     if (!split) {
       hipLaunchKernelGGL(k_lanczos_orth, dim3(npd), dim3(256), (size_t)m * sizeof(double), s,

@@ -439,7 +439,7 @@ def _next_check_gap(j, res, thmax, active, tol, step, hist, slack=0.6, cap=4):
 
 def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202, split=None,
                  check="gpu", graph=True):
-    """Largest-magnitude eigenpairs of every PD's float64 kernel block of L (sembeddingonFly.py
+    """Largest-magnitude eigenpairs of every PD's float32 kernel block of L (sembeddingonFly.py
     semantics: the k eigenpairs of largest |lambda|, sorted by descending lambda) by batched
     Lanczos with full reorthogonalisation on the GPU.
 
@@ -473,7 +473,7 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     K = nat.RITZ_KMAX
     # buffers (and the captured block of steps) persist on the plan between calls with the
     # same L buffer and settings, so repeated embeddings only replay the graph
-    key = (L.data_ptr(), L.numel(), m, bool(split), check_every, bool(graph), check)
+    key = (L.data_ptr(), L.numel(), str(L.dtype), m, bool(split), check_every, bool(graph), check)
     st = getattr(plan, "_lanczos", None)
     if st is None or st["key"] != key:
         st = dict(key=key)
@@ -715,7 +715,7 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
     # the kernel matrix buffer is kept on the plan so that the Lanczos graph stays valid
     L = getattr(plan, "_L", None)
     if L is None or L.numel() < max(info.d_elems, 1) or L.device != dev:
-        L = torch.empty(max(info.d_elems, 1), dtype=torch.float64, device=dev)
+        L = torch.empty(max(info.d_elems, 1), dtype=torch.float32, device=dev)
         plan._L = L
     deg = torch.empty(2 * info.n_rows, dtype=torch.float64, device=dev)
     nat.markov_normalize(plan, D, mask, sigma, L, deg)
@@ -733,7 +733,7 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
             lam, V, r_steps, r_res = eigs[p]
             results[p].lanczos_steps, results[p].lanczos_residual = r_steps, r_res
         else:
-            lam, V = top_eigs_lm(plan.block(L, p), nEigs + 1)
+            lam, V = top_eigs_lm(plan.block(L, p).double(), nEigs + 1)
             lam, V = lam.cpu().numpy(), V.cpu().numpy()
         r = results[p]
         r.lamb = lam

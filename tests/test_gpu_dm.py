@@ -133,12 +133,12 @@ def test_batched_lanczos_matches_dense(cuda, split, check, ns, kk):
                            device=cuda)
     mask = pl.knn_mask(plan, flat)
     sigma = torch.as_tensor([np.sqrt(np.median(d)) for d in D], device=cuda)
-    L = torch.empty(plan.info.d_elems, dtype=torch.float64, device=cuda)
+    L = torch.empty(plan.info.d_elems, dtype=torch.float32, device=cuda)
     deg = torch.empty(2 * plan.info.n_rows, dtype=torch.float64, device=cuda)
     nat.markov_normalize(plan, flat, mask, sigma, L, deg)
     res = pl.lanczos_eigs(plan, L, kk, split=split, check=check)
     for p in range(3):
-        A = plan.block(L, p).cpu().numpy()
+        A = plan.block(L, p).double().cpu().numpy()
         assert np.allclose(A, A.T)
         w, U = np.linalg.eigh(A)
         sel = np.argsort(-np.abs(w), kind="stable")[:kk[p]]
@@ -218,13 +218,13 @@ def test_batched_lanczos_many_pds(cuda, split):
                            device=cuda)
     mask = pl.knn_mask(plan, flat)
     sigma = torch.as_tensor([np.sqrt(np.median(d)) for d in D], device=cuda)
-    L = torch.empty(plan.info.d_elems, dtype=torch.float64, device=cuda)
+    L = torch.empty(plan.info.d_elems, dtype=torch.float32, device=cuda)
     deg = torch.empty(2 * plan.info.n_rows, dtype=torch.float64, device=cuda)
     nat.markov_normalize(plan, flat, mask, sigma, L, deg)
     kk = [9] * len(ns)
     res = pl.lanczos_eigs(plan, L, kk, split=split)
     for p in range(0, len(ns), 7):
-        A = plan.block(L, p).cpu().numpy()
+        A = plan.block(L, p).double().cpu().numpy()
         w, U = np.linalg.eigh(A)
         sel = np.argsort(-np.abs(w), kind="stable")[:9]
         sel = sel[np.argsort(-w[sel], kind="stable")]
