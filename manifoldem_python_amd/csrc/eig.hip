@@ -146,15 +146,18 @@ __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict
   const memi_pd_desc P = pds[pd];
   const int n = P.n;
   if (j >= min(n, m)) return;
-  const int c = blockIdx.x;                       // one block per basis column (m: w itself)
-  const bool norm = with_norm && c == m;
-  if (c > j && !norm) return;
+  // a fixed grid of blocks walks the live columns (entry j + 1 with_norm: w itself, to slot
+  // m); a grid sized for the largest j would launch hundreds of blocks that exit at once
+  const int top = j + 1 + (with_norm ? 1 : 0);
   const double* w = W + P.row0;
-  const double* vc = norm ? w : V + (int64_t)m * P.row0 + (int64_t)c * n;
-  double s = 0.0;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) s += vc[r] * w[r];
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) H[(int64_t)pd * (m + H_STRIDE_EXTRA) + c] = s;
+  for (int c = blockIdx.x; c < top; c += gridDim.x) {   // uniform over the block
+    const bool norm = c == j + 1;
+    const double* vc = norm ? w : V + (int64_t)m * P.row0 + (int64_t)c * n;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) s += vc[r] * w[r];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) H[(int64_t)pd * (m + H_STRIDE_EXTRA) + (norm ? m : c)] = s;
+  }
 }
 
 // Many-PD form of the dot pass: a fixed grid whose waves walk the npd x (j + 1) live
@@ -630,7 +633,7 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const v
       for (int pass = 0; pass < 2; ++pass) {
         const int nrm = pass == 1;
         if (npd < 64) {       // few PDs: a 256-thread block per (column, PD), + w.w in pass 2
-          hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)(m + nrm), npd), dim3(256), 0, s, pds,
+          hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)min(m + nrm, 64), npd), dim3(256), 0, s, pds,
                              V, m, jp, active, W, H + (int64_t)pass * info->n_pd * (m + 1), nrm);
         } else {              // many PDs: waves walk the live (PD, column) pairs
           hipLaunchKernelGGL(k_orth_dot_pairs, dim3(1024), dim3(256), 0, s, pds, info->n_pd, V,
