@@ -1,7 +1,8 @@
-// Mixed-radix Stockham FFTs on images resident in LDS (used by the fused image preparation in
-// prep.hip).  Radices 8, 4, 2, 3, 5, 7; real rows are transformed two at a time as one complex
-// line (x_a + i x_b), columns as complex lines of the half spectrum.  Twiddles come from a table
-// of the N-th roots of unity computed in float64 (fft_twiddles).
+// Mixed-radix FFTs on images resident in LDS (used by the fused image preparation in prep.hip),
+// in place in one buffer per image.  Radices 8, 4, 2, 3, 5, 7; real rows are transformed two at
+// a time as one complex line (x_a + i x_b), columns as complex lines of the half spectrum.
+// Twiddles come from a table of the N-th roots of unity computed in float64 (fft_twiddles).
+// Blocks of FFT_NT = 512 threads.
 #pragma once
 #include "common.h"
 
@@ -119,37 +120,18 @@ __device__ inline int fdivi(int q, int d, float inv_d) {
   return t;
 }
 
-// One Stockham pass of radix R over L lines of length N (line l at base + l*ls, element i at
-// + i*es) from src to dst; Ns = product of the radices of the earlier passes.
-// y[(j/Ns) Ns R + j%Ns + r Ns] = DFT_R( x[j + q N/R] w_N^{q (j%Ns) N/(Ns R)} )_r
-template <int R>
-__device__ inline void stockham_pass(const cpx* src, cpx* dst, int N, int L, int ls, int es,
-                                     int Ns, const cpx* tw, bool inv) {
-  const int m = N / R;
-  const int total = L * m;
-  const int tstep = N / (Ns * R);
-  const float inv_m = 1.0f / m, inv_ns = 1.0f / Ns;
-  for (int q = threadIdx.x; q < total; q += blockDim.x) {
-    const int line = fdivi(q, m, inv_m), j = q - line * m;
-    const cpx* s = src + line * ls;
-    cpx* d = dst + line * ls;
-    const int jg = fdivi(j, Ns, inv_ns);
-    const int k = j - jg * Ns;
-    cpx v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = s[(j + r * m) * es];
-#pragma unroll
-    for (int r = 1; r < R; ++r) {
-      cpx w = tw[r * k * tstep];
-      if (inv) w.y = -w.y;
-      v[r] = cmul(v[r], w);
-    }
-    Dft<R>::run(v, inv);
-    const int idx = jg * Ns * R + k;
-#pragma unroll
-    for (int r = 0; r < R; ++r) d[(idx + r * Ns) * es] = v[r];
-  }
-}
+// Every transform below works in ONE LDS buffer.  The line transforms are in-place
+// mixed-radix FFTs: decimation in frequency forward (each butterfly writes its outputs over its
+// inputs, twiddles after the butterfly), which leaves the frequencies in digit-reversed order,
+// and decimation in time for the inverse, which takes that order back to natural order -- so a
+// forward / inverse pair needs no reordering pass at all.  pos[k] (natural k -> position) and
+// rev[p] (position -> k) translate where a consumer needs natural frequencies.  Only the
+// packing of two real rows into one complex line and its separation stage values in registers
+// (all reads, a barrier, all writes).
+constexpr int FFT_NT = 512;
+
+// items per thread of a step over `total` work items, for the largest image of the template
+__host__ __device__ constexpr int fft_items(int total) { return (total + FFT_NT - 1) / FFT_NT; }
 
 // radix plan of N: up to 8 passes from {8, 4, 2, 3, 5, 7}; returns the count (0: unsupported)
 __host__ __device__ inline int radix_plan(int N, int* rad) {
@@ -163,91 +145,211 @@ __host__ __device__ inline int radix_plan(int N, int* rad) {
   return n == 1 ? c : 0;
 }
 
-// the passes of one batch of line transforms; returns the buffer holding the result
-__device__ inline cpx* fft_lines(cpx* a, cpx* b, int N, int L, int ls, int es, const cpx* tw,
-                                 const int* rad, int nrad, bool inv) {
-  int Ns = 1;
+// position of natural frequency k after the forward passes: with k = d1 + R1 d2 + R1 R2 d3 + ..,
+// p = d1 N/R1 + d2 N/(R1 R2) + ...; fills pos[] and rev[] (LDS, N entries each)
+__device__ inline void fft_digit_tables(int N, const int* rad, int nrad, int* pos, int* rev) {
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    int rem = k, span = N, p = 0;
+    for (int i = 0; i < nrad; ++i) {
+      span /= rad[i];
+      p += (rem % rad[i]) * span;
+      rem /= rad[i];
+    }
+    pos[k] = p;
+    rev[p] = k;
+  }
+}
+
+// one in-place pass of radix R over L lines of length N (line l at l*ls, element i at i*es):
+// sub-transforms of length `span` at multiples of span; butterfly (block, n1) takes
+// idx_r = block*span + n1 + r*span/R.  Forward (DIF): DFT_R, then y_s *= w_span^(n1 s).
+// Inverse (DIT): x_s *= conj w_span^(n1 s), then the conjugate DFT_R.
+// col: lines side by side (ls == 1): consecutive threads take consecutive lines.
+template <int R>
+__device__ inline void fft_pass_ip(cpx* buf, int N, int L, int ls, int es, int span,
+                                   const cpx* tw, bool inv) {
+  const int sub = span / R, per = N / R;
+  const int total = L * per;
+  const int tstep = N / span;
+  const float inv_sub = 1.0f / sub, inv_per = 1.0f / per, inv_l = 1.0f / L;
+  const bool col = ls == 1;
+  for (int q = threadIdx.x; q < total; q += blockDim.x) {
+    int line, t;
+    if (col) {
+      t = fdivi(q, L, inv_l);
+      line = q - t * L;
+    } else {
+      line = fdivi(q, per, inv_per);
+      t = q - line * per;
+    }
+    const int blk = fdivi(t, sub, inv_sub), n1 = t - blk * sub;
+    cpx* x = buf + line * ls + (blk * span + n1) * es;
+    const int st = sub * es;
+    cpx v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = x[r * st];
+    if (!inv) {
+      Dft<R>::run(v, false);
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[n1 * r * tstep]);
+    } else {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cpx w = tw[n1 * r * tstep];
+        w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+      Dft<R>::run(v, true);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r * st] = v[r];
+  }
+  __syncthreads();
+}
+
+// all passes of one batch of line transforms, in place: forward in plan order (spans N,
+// N/R1, ...), inverse in reverse order
+__device__ inline void fft_lines_ip(cpx* buf, int N, int L, int ls, int es, const cpx* tw,
+                                    const int* rad, int nrad, bool inv) {
+  int spans[8];
+  int span = N;
   for (int p = 0; p < nrad; ++p) {
+    spans[p] = span;
+    span /= rad[p];
+  }
+  for (int i = 0; i < nrad; ++i) {
+    const int p = inv ? nrad - 1 - i : i;
     switch (rad[p]) {
-      case 8: stockham_pass<8>(a, b, N, L, ls, es, Ns, tw, inv); break;
-      case 4: stockham_pass<4>(a, b, N, L, ls, es, Ns, tw, inv); break;
-      case 2: stockham_pass<2>(a, b, N, L, ls, es, Ns, tw, inv); break;
-      case 3: stockham_pass<3>(a, b, N, L, ls, es, Ns, tw, inv); break;
-      case 5: stockham_pass<5>(a, b, N, L, ls, es, Ns, tw, inv); break;
-      default: stockham_pass<7>(a, b, N, L, ls, es, Ns, tw, inv); break;
+      case 8: fft_pass_ip<8>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      case 4: fft_pass_ip<4>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      case 2: fft_pass_ip<2>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      case 3: fft_pass_ip<3>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      case 5: fft_pass_ip<5>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      default: fft_pass_ip<7>(buf, N, L, ls, es, spans[p], tw, inv); break;
+    }
+  }
+}
+
+// 2-D real-to-half-spectrum FFT in place: the real image (row stride S floats) in `buf`
+// becomes the half spectrum, kx natural along each row (row stride Nh), rows at ky's
+// digit-reversed position: spectrum (ky, kx) is at [pos[ky] * Nh + kx].
+template <int NMAX>
+__device__ inline cpx* rfft2_ip(float* buf, int N, int S, const cpx* tw, const int* rad,
+                                int nrad, const int* pos) {
+  constexpr int PK = fft_items(((NMAX + 1) / 2) * NMAX);
+  constexpr int UK = fft_items(((NMAX + 1) / 2) * (NMAX / 2 + 1));
+  const int Nh = N / 2 + 1, P = (N + 1) / 2;       // row pairs
+  cpx* X = (cpx*)buf;
+  const float inv_n = 1.0f / N, inv_nh = 1.0f / Nh;
+  {
+    cpx st[PK];
+#pragma unroll
+    for (int b = 0; b < PK; ++b) {
+      const int q = threadIdx.x + b * FFT_NT;
+      if (q < P * N) {
+        const int l = fdivi(q, N, inv_n), c = q - l * N;
+        st[b] = make_float2(buf[(2 * l) * S + c], 2 * l + 1 < N ? buf[(2 * l + 1) * S + c] : 0.f);
+      }
     }
     __syncthreads();
-    Ns *= rad[p];
-    cpx* t = a;
-    a = b;
-    b = t;
-  }
-  return a;
-}
-
-// 2-D real-to-half-spectrum FFT of the real image in `re` (row stride S floats).  `other` is
-// a second buffer of the same size.  The half spectrum [N][Nh] (complex, row stride Nh) is
-// returned; it lives in `re` or `other`.
-__device__ inline cpx* rfft2_lds(float* re, float* other, int N, int S, const cpx* tw,
-                                 const int* rad, int nrad) {
-  const int Nh = N / 2 + 1, P = (N + 1) / 2;       // row pairs
-  cpx* X = (cpx*)other;
-  const float inv_n = 1.0f / N, inv_nh = 1.0f / Nh;
-  for (int q = threadIdx.x; q < P * N; q += blockDim.x) {
-    const int l = fdivi(q, N, inv_n), c = q - l * N;
-    const float a = re[(2 * l) * S + c];
-    const float b = 2 * l + 1 < N ? re[(2 * l + 1) * S + c] : 0.f;
-    X[l * N + c] = make_float2(a, b);
-  }
-  __syncthreads();
-  cpx* Z = fft_lines(X, (cpx*)re, N, P, N, 1, tw, rad, nrad, false);
-  cpx* H = Z == X ? (cpx*)re : X;
-  // unpack: X_a[k] = (Z[k] + conj Z[-k]) / 2, X_b[k] = (Z[k] - conj Z[-k]) / (2i)
-  for (int q = threadIdx.x; q < P * Nh; q += blockDim.x) {
-    const int l = fdivi(q, Nh, inv_nh), k = q - l * Nh;
-    const cpx z = Z[l * N + k], zc = Z[l * N + (k == 0 ? 0 : N - k)];
-    H[(2 * l) * Nh + k] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
-    if (2 * l + 1 < N)
-      H[(2 * l + 1) * Nh + k] = make_float2(0.5f * (z.y + zc.y), 0.5f * (zc.x - z.x));
-  }
-  __syncthreads();
-  return fft_lines(H, Z, N, Nh, 1, Nh, tw, rad, nrad, false);
-}
-
-// 2-D inverse (unnormalised) of the half spectrum H (row stride Nh) into the real image
-// `dst_re` (row stride S); H and `spare` are the two buffers, dst_re is one of them.
-__device__ inline void irfft2_lds(cpx* H, cpx* spare, float* dst_re, int N, int S,
-                                  const cpx* tw, const int* rad, int nrad) {
-  const int Nh = N / 2 + 1, P = (N + 1) / 2;
-  const float inv_n = 1.0f / N;
-  cpx* G = fft_lines(H, spare, N, Nh, 1, Nh, tw, rad, nrad, true);
-  cpx* Zb = G == H ? spare : H;
-  // row pairs: Z[k] = G_a[k] + i G_b[k] with G_r[k] = conj G_r[N-k] for k > N/2
-  for (int q = threadIdx.x; q < P * N; q += blockDim.x) {
-    const int l = fdivi(q, N, inv_n), k = q - l * N;
-    const bool lowk = k < Nh;
-    const int kk = lowk ? k : N - k;
-    cpx ga = G[(2 * l) * Nh + kk];
-    cpx gb = 2 * l + 1 < N ? G[(2 * l + 1) * Nh + kk] : make_float2(0.f, 0.f);
-    if (!lowk) {
-      ga.y = -ga.y;
-      gb.y = -gb.y;
+#pragma unroll
+    for (int b = 0; b < PK; ++b) {
+      const int q = threadIdx.x + b * FFT_NT;
+      if (q < P * N) X[q] = st[b];                 // line l at l * N
     }
-    Zb[l * N + k] = make_float2(ga.x - gb.y, ga.y + gb.x);
+    __syncthreads();
   }
-  __syncthreads();
-  cpx* Z = fft_lines(Zb, G, N, P, N, 1, tw, rad, nrad, true);
-  // Z is in Zb or G; write the real rows into dst_re, which must not be Z's buffer
-  float* tmp = (float*)(Z == Zb ? G : Zb);
-  for (int q = threadIdx.x; q < P * N; q += blockDim.x) {
-    const int l = fdivi(q, N, inv_n), c = q - l * N;
-    const cpx z = Z[l * N + c];
-    tmp[(2 * l) * S + c] = z.x;
-    if (2 * l + 1 < N) tmp[(2 * l + 1) * S + c] = z.y;
+  fft_lines_ip(X, N, P, N, 1, tw, rad, nrad, false);
+  // separate the row pairs: X_a[k] = (Z[k] + conj Z[-k]) / 2, X_b[k] = (Z[k] - conj Z[-k]) / 2i,
+  // Z[k] read at its digit-reversed position
+  {
+    cpx ha[UK], hb[UK];
+#pragma unroll
+    for (int b = 0; b < UK; ++b) {
+      const int q = threadIdx.x + b * FFT_NT;
+      if (q < P * Nh) {
+        const int l = fdivi(q, Nh, inv_nh), k = q - l * Nh;
+        const cpx z = X[l * N + pos[k]], zc = X[l * N + pos[k == 0 ? 0 : N - k]];
+        ha[b] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+        hb[b] = make_float2(0.5f * (z.y + zc.y), 0.5f * (zc.x - z.x));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < UK; ++b) {
+      const int q = threadIdx.x + b * FFT_NT;
+      if (q < P * Nh) {
+        const int l = fdivi(q, Nh, inv_nh), k = q - l * Nh;
+        X[(2 * l) * Nh + k] = ha[b];
+        if (2 * l + 1 < N) X[(2 * l + 1) * Nh + k] = hb[b];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  if (tmp != dst_re) {
-    for (int q = threadIdx.x; q < N * S; q += blockDim.x) dst_re[q] = tmp[q];
+  fft_lines_ip(X, N, Nh, 1, Nh, tw, rad, nrad, false);
+  return X;
+}
+
+// 2-D inverse (unnormalised) of a half spectrum laid out as rfft2_ip leaves it, into the real
+// image (row stride S) in the same buffer.
+template <int NMAX>
+__device__ inline void irfft2_ip(float* buf, int N, int S, const cpx* tw, const int* rad,
+                                 int nrad, const int* pos) {
+  constexpr int PK = fft_items(((NMAX + 1) / 2) * NMAX);
+  const int Nh = N / 2 + 1, P = (N + 1) / 2;
+  cpx* G = (cpx*)buf;
+  const float inv_n = 1.0f / N;
+  fft_lines_ip(G, N, Nh, 1, Nh, tw, rad, nrad, true);     // ky digit-reversed -> y natural
+  // row pairs Z[k] = G_a[k] + i G_b[k] (G_r[k] = conj G_r[N-k] for k > N/2), stored at the
+  // digit-reversed position the row inverse expects
+  {
+    cpx st[PK];
+#pragma unroll
+    for (int b = 0; b < PK; ++b) {
+      const int q = threadIdx.x + b * FFT_NT;
+      if (q < P * N) {
+        const int l = fdivi(q, N, inv_n), k = q - l * N;
+        const bool lowk = k < Nh;
+        const int kk = lowk ? k : N - k;
+        cpx ga = G[(2 * l) * Nh + kk];
+        cpx gb = 2 * l + 1 < N ? G[(2 * l + 1) * Nh + kk] : make_float2(0.f, 0.f);
+        if (!lowk) {
+          ga.y = -ga.y;
+          gb.y = -gb.y;
+        }
+        st[b] = make_float2(ga.x - gb.y, ga.y + gb.x);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PK; ++b) {
+      const int q = threadIdx.x + b * FFT_NT;
+      if (q < P * N) {
+        const int l = fdivi(q, N, inv_n), k = q - l * N;
+        G[l * N + pos[k]] = st[b];
+      }
+    }
+    __syncthreads();
+  }
+  fft_lines_ip(G, N, P, N, 1, tw, rad, nrad, true);
+  // the real rows, row stride S
+  {
+    cpx st[PK];
+#pragma unroll
+    for (int b = 0; b < PK; ++b) {
+      const int q = threadIdx.x + b * FFT_NT;
+      if (q < P * N) st[b] = G[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PK; ++b) {
+      const int q = threadIdx.x + b * FFT_NT;
+      if (q < P * N) {
+        const int l = fdivi(q, N, inv_n), c = q - l * N;
+        buf[(2 * l) * S + c] = st[b].x;
+        if (2 * l + 1 < N) buf[(2 * l + 1) * S + c] = st[b].y;
+      }
+    }
     __syncthreads();
   }
 }

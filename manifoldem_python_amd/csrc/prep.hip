@@ -611,12 +611,14 @@ __global__ __launch_bounds__(512) void k_prep_images(
     const double* __restrict__ tab, double ampc, int prec, int64_t Khp, int64_t row0,
     void* __restrict__ op_hi, void* __restrict__ op_lo, float2* __restrict__ fy,
     float* __restrict__ ctf_half, float* __restrict__ img_out) {
+  // one image buffer: the padded real image ((N+3)^2 floats) is the largest of its layouts
+  // (half spectrum 2 N (N/2+1) floats, row-pair lines 2 ceil(N/2) N floats)
   constexpr int BUF = (NMAX + 3) * (NMAX + 3);
-  __shared__ float lds[2 * BUF];
+  __shared__ float lds[BUF];
   __shared__ cpx tw[NMAX];
+  __shared__ int pos[NMAX], rev[NMAX];              // frequency <-> position after the passes
   __shared__ double red[16];
   float* A = lds;
-  float* B = lds + BUF;
   const int S = N + 3;
   const int Nh = N / 2 + 1;
   const int64_t Kh = (int64_t)N * Nh;
@@ -624,6 +626,7 @@ __global__ __launch_bounds__(512) void k_prep_images(
   int rad[8];
   const int nrad = radix_plan(N, rad);
   fft_twiddles(tw, N);
+  fft_digit_tables(N, rad, nrad, pos, rev);
   // a1: load (conjugates flipped, SPIDER transpose) and background statistics
   const float* raw = stack + src[img] * (int64_t)N * N;
   const bool cj = conj != nullptr && conj[img] != 0;
@@ -660,14 +663,16 @@ __global__ __launch_bounds__(512) void k_prep_images(
     for (int c = lane; c < N; c += 64) A[r * S + c] = (float)(((double)A[r * S + c] - mean) / sd);
   __syncthreads();
   // a2: Butterworth in frequency (the table folds in the 1/N^2 of the inverse)
-  cpx* H = rfft2_lds(A, B, N, S, tw, rad, nrad);
-  for (int64_t q = threadIdx.x; q < Kh; q += blockDim.x) {
-    const float f = (float)tab[q * PT];
+  cpx* H = rfft2_ip<NMAX>(A, N, S, tw, rad, nrad, pos);
+  const float inv_nh = 1.0f / Nh;
+  for (int q = threadIdx.x; q < Kh; q += blockDim.x) {   // H row p holds ky = rev[p]
+    const int py = fdivi(q, Nh, inv_nh), kx = q - py * Nh;
+    const float f = (float)tab[(int64_t)(rev[py] * Nh + kx) * PT];
     H[q] = make_float2(H[q].x * f, H[q].y * f);
   }
   __syncthreads();
-  irfft2_lds(H, H == (cpx*)A ? (cpx*)B : (cpx*)A, A, N, S, tw, rad, nrad);
-  // a4: disk mask, then the two spline rotations (A -> B -> A)
+  irfft2_ip<NMAX>(A, N, S, tw, rad, nrad, pos);
+  // a4: disk mask, then the two spline rotations, each in place
   for (int r = wv; r < N; r += nw)
     for (int c = lane; c < N; c += 64) {
       const int xd = 2 * r - N + 2, yd = 2 * c - N;
@@ -676,18 +681,16 @@ __global__ __launch_bounds__(512) void k_prep_images(
   __syncthreads();
   const double zN = pow(BSPLINE_Z, (double)N);
   bspline_prefilter(A, N, S, zN);
-  spline_sample(A, N, S, rot1[img], B, S);
-  __syncthreads();
-  bspline_prefilter(B, N, S, zN);
-  spline_sample(B, N, S, rot2[img], A, S);
-  __syncthreads();
+  spline_sample_inplace<NMAX>(A, N, S, rot1[img]);
+  bspline_prefilter(A, N, S, zN);
+  spline_sample_inplace<NMAX>(A, N, S, rot2[img]);
   if (img_out) {
     float* o = img_out + img * (int64_t)N * N;
     for (int r = wv; r < N; r += nw)
       for (int c = lane; c < N; c += 64) o[r * N + c] = A[r * S + c];
   }
   // a5/a6: spectrum of the aligned image, CTF, operand rows (as k_ctf_pack)
-  cpx* F = rfft2_lds(A, B, N, S, tw, rad, nrad);
+  cpx* F = rfft2_ip<NMAX>(A, N, S, tw, rad, nrad, pos);
   const double dfi = df[img];
   const double inv2pi = 0.15915494309189535, twopi = 6.283185307179586;
   const int64_t base = (row0 + img) * 4 * Khp;
@@ -700,7 +703,8 @@ __global__ __launch_bounds__(512) void k_prep_images(
       float sf, cf;
       sincosf((float)rd, &sf, &cf);
       const double C = ((double)sf - ampc * (double)cf) * t[3];
-      const cpx X = F[q];
+      const int ky = fdivi((int)q, Nh, inv_nh), kx = (int)q - ky * Nh;
+      const cpx X = F[pos[ky] * Nh + kx];
       ctf_half[img * Kh + q] = (float)C;
       fy[img * Kh + q] = X;
       v[0] = t[4] * ((double)X.x * X.x + (double)X.y * X.y);
