@@ -121,7 +121,7 @@ int memi_spline_rotate(const float* coef, const double* angle_deg, int64_t n, in
  * aligned images, or NULL.  The image stays in LDS from load to operand (mixed-radix FFTs in
  * LDS); 8 <= N <= MEMI_PREP_MAX_N with prime factors 2, 3, 5, 7.  ws: float64 scratch of
  * 6 N(N/2+1) doubles (per-bin tables). */
-#define MEMI_PREP_MAX_N 128
+#define MEMI_PREP_MAX_N 160
 int memi_prep_images(const float* stack, int64_t n_stack, const int64_t* src,
                      const int32_t* conj, const double* rot1_deg, const double* rot2_deg,
                      const double* df, int64_t n, int N, int filter_kind, double Qc,

@@ -210,11 +210,11 @@ def align_images(img, rot1_deg, rot2_deg, N, out, disk=True):
          _p(out), _stream())
 
 
-PREP_MAX_N = 128
+PREP_MAX_N = 160
 
 
 def prep_supported(N):
-    """True when memi_prep_images handles side N (8..128, prime factors 2, 3, 5, 7)."""
+    """True when memi_prep_images handles side N (8..160, prime factors 2, 3, 5, 7)."""
     if not 8 <= N <= PREP_MAX_N:
         return False
     n = N

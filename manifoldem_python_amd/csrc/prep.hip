@@ -863,7 +863,8 @@ int memi_prep_images(const float* stack, int64_t n_stack, const int64_t* src,
                      img_out)
   if (N <= 64) MEMI_PREP(64);
   else if (N <= 96) MEMI_PREP(96);
-  else MEMI_PREP(128);
+  else if (N <= 128) MEMI_PREP(128);
+  else MEMI_PREP(160);
 #undef MEMI_PREP
   MEMI_CHECK_LAUNCH("k_prep_images");
   return MEMI_OK;

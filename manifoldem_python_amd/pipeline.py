@@ -129,7 +129,7 @@ def filter_kind(filterPar):
 
 ALIGN = "auto"      # "fused": memi_align_images; "fft": FFT prefilter + memi_spline_rotate
 PREP = "auto"       # "fused": memi_prep_images (one kernel); "staged": the kernel sequence
-PREP_FUSED_MAX_N = 128  # "auto" uses the fused kernel up to this image side
+PREP_FUSED_MAX_N = 160  # "auto" uses the fused kernel up to this image side
 
 
 def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, operand=None,
@@ -148,8 +148,9 @@ def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, 
     dev = stack.device
     prep = prep or PREP
     if prep == "auto":
-        # measured on MI355X with one LDS image per block (in-place FFTs): 3.97 vs 6.99 ms for
-        # 16384 96x96 images, 0.85 vs 1.37 ms for 2000 128x128 images (fused vs staged)
+        # measured on MI355X with one LDS image per block (in-place FFTs): 3.94 vs 6.99 ms for
+        # 16384 96x96 images, 0.86 vs 1.38 ms for 2000 128x128, 2.60 vs 4.23 ms for 3000
+        # 160x160 (fused vs staged)
         prep = "fused" if (nat.prep_supported(N) and N <= PREP_FUSED_MAX_N and msk2 is None and
                            filterPar.get("type", "Butter") in ("Butter", "Gauss")) else "staged"
     if prep == "fused":

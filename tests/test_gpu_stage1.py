@@ -247,7 +247,7 @@ def _operand_values(op):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("N", [40, 45, 70, 96, 128])
+@pytest.mark.parametrize("N", [40, 45, 70, 96, 128, 150, 160])
 def test_fused_prep_matches_staged(cuda, precision, N):
     """memi_prep_images (load, filter, rotations, FFTs and CTF packing in one kernel with LDS
     FFTs) against the staged path (rocFFT + separate kernels) on the same inputs: aligned
@@ -263,7 +263,11 @@ def test_fused_prep_matches_staged(cuda, precision, N):
     np.testing.assert_allclose(a["ctf"].cpu().numpy(), b["ctf"].cpu().numpy(), atol=1e-6)
     va, vb = _operand_values(opa), _operand_values(opb)
     Kp = opa.Khp
-    for sg in range(4):       # per segment, relative to the segment's scale
+    # per segment, relative to the segment's scale: the split-BF16 pair hi + lo holds a value to
+    # about 2^-17 of itself, so values that differ by float32 rounding between the two paths can
+    # land up to ~2 x 2^-17 apart after the split
+    tol = 1e-5 if precision == 0 else 2e-5
+    for sg in range(4):
         xa = va.reshape(n, 4, Kp)[:, sg]
         xb = vb.reshape(n, 4, Kp)[:, sg]
-        assert np.max(np.abs(xa - xb)) < 1e-5 * np.max(np.abs(xb)), sg
+        assert np.max(np.abs(xa - xb)) < tol * np.max(np.abs(xb)), sg
