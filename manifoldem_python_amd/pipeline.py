@@ -298,7 +298,23 @@ def ferguson_fit(logEps, logSumWij, a0, rand=None):
     return popt, resnorm, R_squared
 
 
-def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_per_round=8):
+_const_cache = {}
+
+
+def _device_const(a, dev):
+    """A device copy of a small constant host array (the eps grid), made once per content."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    key = (str(dev), a.shape, a.tobytes())
+    t = _const_cache.get(key)
+    if t is None:
+        if len(_const_cache) > 16:
+            _const_cache.clear()
+        t = _const_cache[key] = torch.as_tensor(a, device=dev)
+    return t
+
+
+def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_per_round=8,
+                       logS_dev=None):
     """The fit of ``fergusonE.py:45-57`` for many PDs at once on the GPU (memi_tanh_fit):
     fit from a0, pcov = inv(J^T J) ss_res / (m - 4) as scipy's curve_fit forms it, and refit
     from rand(4,1) - 0.5 while sum(sqrt|diag pcov|) > 100.  Returns (popt (npd, 4), R^2).
@@ -307,12 +323,13 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
     starts of every unfinished PD in parallel: its pending start followed by the next draws in
     the order the reference's loop makes them (one after every fit), keeping the first
     acceptable one in that order -- the same result as sequential restarts, in fewer round
-    trips."""
+    trips.  logS_dev: the same curves already on the device (npd, m) float64, if at hand."""
     rand = np.random.rand if rand is None else rand
     dev = device()
     npd, m = logS.shape
-    x = torch.as_tensor(np.ascontiguousarray(logEps, dtype=np.float64), device=dev)
-    y_all = torch.as_tensor(np.ascontiguousarray(logS, dtype=np.float64), device=dev)
+    x = _device_const(logEps, dev)
+    y_all = (logS_dev if logS_dev is not None else
+             torch.as_tensor(np.ascontiguousarray(logS, dtype=np.float64), device=dev))
     pending = np.stack([np.ravel(a) for a in a0s]).astype(np.float64)
     popt = np.zeros((npd, 4))
     R2 = np.zeros(npd)
@@ -331,10 +348,12 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
             starts[:, 1:] = np.reshape(rand(len(idx) * (R - 1) * 4, 1),
                                        (len(idx), R - 1, 4)) - 0.5
         B = len(idx) * R
-        yy = y_all[torch.as_tensor(np.repeat(idx, R), device=dev)].contiguous()
+        if R == 1 and len(idx) == npd:           # every PD once, in order: no gather
+            yy = y_all
+        else:
+            yy = y_all[torch.as_tensor(np.repeat(idx, R), device=dev)].contiguous()
         out = torch.empty(B * 16, dtype=torch.float64, device=dev)
-        act = torch.ones(B, dtype=torch.int32, device=dev)
-        nat.tanh_fit(x, yy, torch.as_tensor(starts.reshape(B, 4), device=dev), out, act)
+        nat.tanh_fit(x, yy, torch.as_tensor(starts.reshape(B, 4), device=dev), out)
         o = out.cpu().numpy().reshape(len(idx), R, 16)
         J = np.zeros((len(idx), R, 4, 4))
         J[:, :, iu[0], iu[1]] = o[:, :, 4:14]
@@ -628,12 +647,13 @@ def uniform_grid(logEps):
     return float(le[0]), float(dle)
 
 
-def ferguson_sums(plan, D, mask, logEps, method=None):
-    """a12 sums: (npd, n_eps) arrays of sum W, d log(sum W)/d log eps and #terms.  method:
-    "moments" (bucket moments, needs a log-uniform grid), "direct" (every entry at every eps)
-    or "auto" (moments when the grid allows)."""
+def ferguson_sums(plan, D, mask, logEps, method=None, want_log_dev=False):
+    """a12 sums: (npd, n_eps) arrays of sum W, d log(sum W)/d log eps and #terms (and, with
+    want_log_dev, log sum W as a device tensor).  method: "moments" (bucket moments, needs a
+    log-uniform grid), "direct" (every entry at every eps) or "auto" (moments when the grid
+    allows)."""
     npd = plan.npd
-    le = torch.as_tensor(np.asarray(logEps, dtype=np.float64), device=D.device)
+    le = _device_const(logEps, D.device)
     n_eps = le.shape[0]
     fer = torch.empty(npd * n_eps * 3 + 2 * npd, dtype=torch.float64, device=D.device)
     method = method or FERGUSON
@@ -645,10 +665,14 @@ def ferguson_sums(plan, D, mask, logEps, method=None):
         nat.ferguson_moments(plan, D, mask, le, grid[0], grid[1], fer, ws)
     else:
         nat.ferguson_sweep(plan, D, mask, le, fer)
+    S_dev = fer[:npd * n_eps * 3].view(npd, n_eps, 3)[:, :, 0]
+    logS_dev = torch.log(S_dev).contiguous() if want_log_dev else None
     fer = fer.cpu().numpy()[:npd * n_eps * 3].reshape(npd, n_eps, 3)
     S = fer[:, :, 0]
     with np.errstate(divide="ignore", invalid="ignore"):
         slope = fer[:, :, 1] / S
+    if want_log_dev:
+        return S, slope, fer[:, :, 2], logS_dev
     return S, slope, fer[:, :, 2]
 
 
@@ -686,7 +710,7 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
         mask = knn_mask(plan, D)
     if timer:
         timer.mark("knn")
-    S, slope, _ = ferguson_sums(plan, D, mask, logEps)
+    S, slope, _, logS_dev = ferguson_sums(plan, D, mask, logEps, want_log_dev=True)
     if timer:
         timer.mark("ferguson")
     rand = np.random.rand if rand is None else rand
@@ -696,7 +720,8 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
     with np.errstate(divide="ignore"):
         logS = np.log(S)
     if fit == "gpu":
-        popt_b, R2_b = ferguson_fit_batch(np.asarray(logEps), logS, a0l, rand)
+        popt_b, R2_b = ferguson_fit_batch(np.asarray(logEps), logS, a0l, rand,
+                                          logS_dev=logS_dev)
     for p in range(npd):
         r = DMResult()
         r.logSumWij = logS[p]
