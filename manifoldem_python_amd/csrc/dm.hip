@@ -604,37 +604,41 @@ __device__ inline void tanh_terms(double x, const double a[4], double& f, double
   J[3] = 1.0;
 }
 
-// block reduction of NV doubles (in place in v) over NW waves: each wave sums by shuffles,
-// thread q < NV adds the NW wave sums of value q in wave order (row NW of sh), and every thread
-// reads the NV totals back.  (Summing all NV x NW partials in every thread had the compiler
-// issue every LDS read at once, holding NV x NW doubles in registers.)
-template <int NV, int NW>
-__device__ inline void block_reduce_vec(double* v, double (*sh)[NV]) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < NV; ++q)
-    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
-  __syncthreads();
-  if (lane == 0)
-#pragma unroll
-    for (int q = 0; q < NV; ++q) sh[wid][q] = v[q];
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) t += sh[w][threadIdx.x];
-    sh[NW][threadIdx.x] = t;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NV; ++q) v[q] = sh[NW][q];
-  __syncthreads();
+// Wave sum of a double by DPP moves (plain VALU operations, no LDS traffic): an inclusive
+// scan within each 16-lane row (row_shr 1, 2, 4, 8; lanes without a source read 0), then
+// row_bcast:15 adds row 0's total into row 1 and row 2's into row 3, and row_bcast:31 adds
+// rows 0 + 1 into rows 2 and 3; lane 63 then holds the wave's total, read back as a uniform
+// value.  Both 32-bit halves of the double move with the same control.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// 1024 threads per PD evaluate the curve (~1.5 points each); the LM bookkeeping (scaling, the
-// damped 4x4 Cholesky solve, accept / reject) runs on thread 0 with the state in LDS, so the
-// evaluation's registers and the solve's are never live together.
-constexpr int TANH_NT = 1024;
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_move<0x111, 0xf>(v);   // row_shr:1
+  v += dpp_move<0x112, 0xf>(v);   // row_shr:2
+  v += dpp_move<0x114, 0xf>(v);   // row_shr:4
+  v += dpp_move<0x118, 0xf>(v);   // row_shr:8
+  v += dpp_move<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// 256 threads per PD evaluate the curve (~6 points each).  Each wave sums its 15 values by
+// DPP, lane 0 stores them, and after one barrier every thread adds the wave sums in wave order,
+// so all threads hold the same totals and run the LM bookkeeping (scaling, the damped 4x4
+// Cholesky solve, accept / reject) themselves, in registers: no serial section on one thread
+// and no state in LDS.  The wave sums alternate between two LDS buffers, so a buffer is
+// rewritten only after the barrier of the next evaluation has passed, and one barrier per
+// evaluation suffices.
+constexpr int TANH_NT = 256;
+constexpr int TANH_NW = TANH_NT / 64;
 
 struct LmState {
   double a[4], at[4], diag[4], d[4];
@@ -643,20 +647,17 @@ struct LmState {
   int it, done;
 };
 
-// cost, J^T J and J^T r at the point pt (LDS), summed over the curve into v; every thread gets
-// them (a plain inlined function: a by-reference lambda here kept its captures in scratch)
+// cost, J^T J and J^T r at the point pt, summed over the curve into v (every thread gets the
+// same totals); sh: this evaluation's buffer of wave sums
 __device__ __forceinline__ void tanh_eval(const double* __restrict__ x,
                                           const double* __restrict__ yp, int m,
-                                          const double* pt, double* v,
-                                          double (*red)[15]) {
-  double aa[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) aa[q] = pt[q];
+                                          const double pt[4], double v[15],
+                                          double (*sh)[15]) {
 #pragma unroll
   for (int q = 0; q < 15; ++q) v[q] = 0.0;
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+  for (int i = threadIdx.x; i < m; i += TANH_NT) {
     double f, J[4];
-    tanh_terms(x[i], aa, f, J);
+    tanh_terms(x[i], pt, f, J);
     const double r = yp[i] - f;
     v[14] += r * r;
     int k = 0;
@@ -667,11 +668,24 @@ __device__ __forceinline__ void tanh_eval(const double* __restrict__ x,
 #pragma unroll
     for (int p = 0; p < 4; ++p) v[10 + p] += J[p] * r;
   }
-  block_reduce_vec<15, TANH_NT / 64>(v, red);
+  const int wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 15; ++q) v[q] = wave_total(v[q]);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < 15; ++q) sh[wid][q] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 15; ++q) {
+    double t = sh[0][q];
+#pragma unroll
+    for (int w = 1; w < TANH_NW; ++w) t += sh[w][q];
+    v[q] = t;
+  }
 }
 
-// thread 0: scaling update and the damped step from the current point into st.at (more
-// damping while the system is not positive definite); st.done when the fit must stop
+// scaling update and the damped step from the current point into st.at (more damping while
+// the system is not positive definite); st.done when the fit must stop
 __device__ inline void lm_propose(LmState& st, int max_iter) {
   const int dg[4] = {0, 4, 7, 9};  // packed (p, p)
   for (;;) {
@@ -743,66 +757,61 @@ __global__ __launch_bounds__(TANH_NT) void k_tanh_fit(const double* __restrict__
                                                       const double* __restrict__ p0,
                                                       const int32_t* __restrict__ active,
                                                       int max_iter, double* __restrict__ out) {
-  __shared__ double red[TANH_NT / 64 + 1][15];
-  __shared__ LmState st;
+  __shared__ double sh[2][TANH_NW][15];
   const int pd = blockIdx.x;
   if (active && !active[pd]) return;
   const double* yp = y + (int64_t)pd * m;
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < 4; ++q) {
-      st.a[q] = p0[pd * 4 + q];
-      st.diag[q] = 0.0;
-    }
-    st.lam = 1e-3;
-    st.it = 0;
-    st.done = 0;
+  LmState st;
+  for (int q = 0; q < 4; ++q) {
+    st.a[q] = p0[pd * 4 + q];
+    st.diag[q] = 0.0;
+    st.d[q] = 0.0;
   }
-  __syncthreads();
+  st.lam = 1e-3;
+  st.it = 0;
+  st.done = 0;
+  int buf = 0;
   {
     double v[15];
-    tanh_eval(x, yp, m, st.a, v, red);
-    if (threadIdx.x == 0) {
-      for (int k = 0; k < 10; ++k) st.jtj[k] = v[k];
-      for (int p = 0; p < 4; ++p) st.jtr[p] = v[10 + p];
-      st.cost = v[14];
-      if (max_iter <= 0) st.done = 1;
-      else lm_propose(st, max_iter);
-    }
+    tanh_eval(x, yp, m, st.a, v, sh[buf]);
+    buf ^= 1;
+    for (int k = 0; k < 10; ++k) st.jtj[k] = v[k];
+    for (int p = 0; p < 4; ++p) st.jtr[p] = v[10 + p];
+    st.cost = v[14];
+    if (max_iter <= 0) st.done = 1;
+    else lm_propose(st, max_iter);
   }
-  __syncthreads();
-  while (!st.done) {                                  // uniform: read after a barrier
+  while (!st.done) {                                  // the same in every thread
     // the trial point's J^T J and J^T r come with its cost: an accepted step then needs no
     // second pass (they are the values a re-evaluation at the new point would give)
     double v[15];
-    tanh_eval(x, yp, m, st.at, v, red);
-    if (threadIdx.x == 0) {
-      const double cost_t = v[14];
-      double pred = 0.0;
-      for (int p = 0; p < 4; ++p)
-        pred += st.d[p] * (st.jtr[p] + st.lam * st.diag[p] * st.diag[p] * st.d[p]);
-      if (cost_t < st.cost) {
-        const double actred = (st.cost - cost_t) / fmax(st.cost, 1e-300);
-        double xn = 0.0, dn = 0.0;
-        for (int p = 0; p < 4; ++p) {
-          xn += (st.diag[p] * st.at[p]) * (st.diag[p] * st.at[p]);
-          dn += (st.diag[p] * st.d[p]) * (st.diag[p] * st.d[p]);
-        }
-        for (int p = 0; p < 4; ++p) st.a[p] = st.at[p];
-        st.cost = cost_t;
-        for (int k = 0; k < 10; ++k) st.jtj[k] = v[k];
-        for (int p = 0; p < 4; ++p) st.jtr[p] = v[10 + p];
-        st.lam = fmax(st.lam * 0.1, 1e-15);
-        const double prered = pred / fmax(cost_t + pred, 1e-300);
-        if ((actred <= 1.49012e-8 && prered <= 1.49012e-8) || sqrt(dn) <= 1.49012e-8 * sqrt(xn))
-          st.done = 1;
-      } else {
-        st.lam *= 10.0;
-        if (st.lam > 1e32) st.done = 1;
+    tanh_eval(x, yp, m, st.at, v, sh[buf]);
+    buf ^= 1;
+    const double cost_t = v[14];
+    double pred = 0.0;
+    for (int p = 0; p < 4; ++p)
+      pred += st.d[p] * (st.jtr[p] + st.lam * st.diag[p] * st.diag[p] * st.d[p]);
+    if (cost_t < st.cost) {
+      const double actred = (st.cost - cost_t) / fmax(st.cost, 1e-300);
+      double xn = 0.0, dn = 0.0;
+      for (int p = 0; p < 4; ++p) {
+        xn += (st.diag[p] * st.at[p]) * (st.diag[p] * st.at[p]);
+        dn += (st.diag[p] * st.d[p]) * (st.diag[p] * st.d[p]);
       }
-      if (!st.done && st.it >= max_iter) st.done = 1;
-      if (!st.done) lm_propose(st, max_iter);
+      for (int p = 0; p < 4; ++p) st.a[p] = st.at[p];
+      st.cost = cost_t;
+      for (int k = 0; k < 10; ++k) st.jtj[k] = v[k];
+      for (int p = 0; p < 4; ++p) st.jtr[p] = v[10 + p];
+      st.lam = fmax(st.lam * 0.1, 1e-15);
+      const double prered = pred / fmax(cost_t + pred, 1e-300);
+      if ((actred <= 1.49012e-8 && prered <= 1.49012e-8) || sqrt(dn) <= 1.49012e-8 * sqrt(xn))
+        st.done = 1;
+    } else {
+      st.lam *= 10.0;
+      if (st.lam > 1e32) st.done = 1;
     }
-    __syncthreads();
+    if (!st.done && st.it >= max_iter) st.done = 1;
+    if (!st.done) lm_propose(st, max_iter);
   }
   if (threadIdx.x == 0) {
     double* o = out + (int64_t)pd * 16;
