@@ -282,11 +282,14 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const v
  * J = n) at every check.  A PD is done when every residual is <= tol * max|theta|, or J = n,
  * or J = m: then active[pd] = 0 and steps[pd] = J.  theta: float64 [npd][16] (every check);
  * S: float64 [npd][m][16] (PD p's J x k row-major
- * block at p*m*16, ready for memi_ritz_vectors); piv: float64 scratch [npd][16][2][m]. */
+ * block at p*m*16, ready for memi_ritz_vectors); piv: float64 scratch [npd][16][2][m].
+ * summary (may be null): float64 [3][npd] = {still active (1 or 0), res, max |theta|} of every
+ * PD after this check (0, 0, 0 for PDs inactive on entry) -- what the caller reads back to
+ * decide and pace the next check, in one copy. */
 int memi_lanczos_ritz(const void* plan_dev, const memi_plan_info* info, const double* ab, int m,
                       int steps_done, const int32_t* ks, double tol, int32_t* active,
                       double* theta, double* S, int32_t* steps, double* res, double* piv,
-                      void* stream);
+                      double* summary, void* stream);
 
 /* Ritz vectors of every PD after Lanczos, the eigenvectors sembeddingonFly.py:27-35 takes
  * from eigsh: Y_p = V_p S_p with V_p the basis of memi_lanczos_step (first J columns) and S_p

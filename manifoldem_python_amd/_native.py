@@ -78,7 +78,7 @@ _SIGS = {
     "memi_ritz_vectors": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_vp, c_vp, c_int, c_vp,
                           c_vp],
     "memi_lanczos_ritz": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_int, c_vp, c_dbl, c_vp,
-                          c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+                          c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_nlsa_condist": [c_vp, c_int, c_vp, c_int, c_int, c_vp, c_vp],
     "memi_nlsa_supervectors": [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_vp, c_vp],
     "memi_tall_gram": [c_vp, c_i64, c_int, c_vp, c_vp],
@@ -485,8 +485,10 @@ def lanczos_steps(plan, L, V, m, j_dev, nsteps, active, W, ab, H=None):
 RITZ_KMAX = 16
 
 
-def lanczos_ritz(plan, ab, m, steps_done, ks, tol, active, theta, S, steps, res, piv):
-    """GPU convergence check of every active PD (memi_lanczos_ritz); buffers as in memi.h."""
+def lanczos_ritz(plan, ab, m, steps_done, ks, tol, active, theta, S, steps, res, piv,
+                 summary=None):
+    """GPU convergence check of every active PD (memi_lanczos_ritz); buffers as in memi.h.
+    summary: optional float64 [3 * npd] device buffer for {active, res, max |theta|}."""
     I = plan.info
     K = RITZ_KMAX
     _need(1 <= steps_done <= m and ab.dtype == torch.float64 and ab.numel() >= I.n_pd * m * 2,
@@ -499,9 +501,11 @@ def lanczos_ritz(plan, ab, m, steps_done, ks, tol, active, theta, S, steps, res,
           res.dtype == torch.float64 and res.numel() >= I.n_pd and
           piv.dtype == torch.float64 and piv.numel() >= I.n_pd * K * 2 * m,
           "lanczos_ritz: theta / S / res / piv sizes")
+    _need(summary is None or (summary.dtype == torch.float64 and summary.numel() >= 3 * I.n_pd),
+          "lanczos_ritz: summary must be float64 [3 * npd]")
     call("memi_lanczos_ritz", _p(plan.dev), ctypes.byref(plan.info), _p(ab), int(m),
          int(steps_done), _p(ks), float(tol), _p(active), _p(theta), _p(S), _p(steps), _p(res),
-         _p(piv), _stream())
+         _p(piv), _p(summary), _stream())
 
 
 def ritz_vectors(plan, V, m, S, meta_host, meta, kmax, Y):

@@ -326,7 +326,8 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
                                                           double* __restrict__ S,
                                                           int32_t* __restrict__ steps,
                                                           double* __restrict__ res_out,
-                                                          double* __restrict__ piv) {
+                                                          double* __restrict__ piv,
+                                                          double* __restrict__ summary) {
   constexpr int KM = MEMI_RITZ_KMAX;
   constexpr bool PLDS = MAXM <= 512;
   __shared__ double2 q2[MAXM];                       // {alpha_i, beta_{i-1}^2}
@@ -338,7 +339,14 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
   __shared__ double gersh[3];
   __shared__ int done_sh;
   const int pd = blockIdx.x;
-  if (!active[pd]) return;                            // uniform over the block
+  if (!active[pd]) {                                  // uniform over the block
+    if (summary && threadIdx.x == 0) {
+      summary[pd] = 0.0;
+      summary[gridDim.x + pd] = 0.0;
+      summary[2 * gridDim.x + pd] = 0.0;
+    }
+    return;
+  }
   const int t = threadIdx.x;
   const int n = pds[pd].n;
   const int J = min(steps_done, min(n, m));
@@ -382,7 +390,10 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
   //    candidate; lane g counts the eigenvalues below X(g + 1), X(j) = l + (h - l) j / (G + 1),
   //    and the bracket becomes [X(f), X(f + 1)] around the first point with more than idx below
   const int nc = J <= 2 * k ? J : 2 * k;
-  int G = 64;
+  // 8 lanes per candidate: 2k <= 32 candidates keep up to 4 waves busy; more lanes per candidate
+  // (fewer rounds, 9-fold narrowing each) cost more issue slots than they save in latency
+  // (measured at J = 32...96: 8 lanes fastest, 32 lanes 15-20% slower)
+  int G = 8;
   while (G > 1 && G * nc > RITZ_NT) G >>= 1;
   const int ci = t / G, g = t % G;
   if (ci < nc) {                                      // whole groups, so ballots stay in-group
@@ -405,25 +416,27 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
     if (g == 0) cand[ci] = 0.5 * (l + h);
   }
   __syncthreads();
-  // 2. selection: k largest |theta|, then descending theta
-  if (t == 0) {
-    uint32_t used = 0u;                       // bit c: candidate c taken (nc <= 32)
-    for (int q = 0; q < k; ++q) {
-      int best = -1;
-      for (int c = 0; c < nc; ++c)
-        if (!((used >> c) & 1u) && (best < 0 || fabs(cand[c]) > fabs(cand[best]))) best = c;
-      used |= 1u << best;
-      sel[q] = cand[best];
+  // 2. selection: the k largest |theta| (of equal magnitudes, the lower candidate index first),
+  //    placed by descending theta.  Thread c ranks candidate c against all others (a serial
+  //    scan on one thread waits on one LDS read after another)
+  __shared__ int taken[2 * KM];
+  double tc = 0.0;
+  if (t < nc) {
+    tc = cand[t];
+    const double ac = fabs(tc);
+    int rank = 0;
+    for (int c = 0; c < nc; ++c) {
+      const double a2 = fabs(cand[c]);
+      rank += (a2 > ac) || (a2 == ac && c < t);
     }
-    for (int q = 1; q < k; ++q) {            // insertion sort, descending
-      const double v = sel[q];
-      int r = q - 1;
-      while (r >= 0 && sel[r] < v) {
-        sel[r + 1] = sel[r];
-        --r;
-      }
-      sel[r + 1] = v;
-    }
+    taken[t] = rank < k;
+  }
+  __syncthreads();
+  if (t < nc && taken[t]) {
+    int pos = 0;
+    for (int c = 0; c < nc; ++c)
+      pos += taken[c] && (cand[c] > tc || (cand[c] == tc && c < t));
+    sel[pos] = tc;
   }
   __syncthreads();
   // 3. twisted factorisation per selected pair, two adjacent threads of the first wave per
@@ -510,6 +523,11 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
     if (done) {
       active[pd] = 0;
       steps[pd] = J;
+    }
+    if (summary) {
+      summary[pd] = done ? 0.0 : 1.0;
+      summary[gridDim.x + pd] = J >= n ? 0.0 : rmax;
+      summary[2 * gridDim.x + pd] = mx;
     }
   }
   __syncthreads();
@@ -669,7 +687,7 @@ int memi_ritz_vectors(const void* plan_dev, const memi_plan_info* info, const do
 int memi_lanczos_ritz(const void* plan_dev, const memi_plan_info* info, const double* ab, int m,
                       int steps_done, const int32_t* ks, double tol, int32_t* active,
                       double* theta, double* S, int32_t* steps, double* res, double* piv,
-                      void* stream) {
+                      double* summary, void* stream) {
   MEMI_CHECK_ARG(plan_dev && info && ab && ks && active && theta && S && steps && res && piv,
                  "memi_lanczos_ritz: null pointer");
   MEMI_CHECK_ARG(m >= 2 && m <= 4096 && steps_done >= 1 && steps_done <= m && tol >= 0,
@@ -680,11 +698,11 @@ int memi_lanczos_ritz(const void* plan_dev, const memi_plan_info* info, const do
   if (m <= 512)
     hipLaunchKernelGGL(k_lanczos_ritz<512>, dim3((unsigned)info->n_pd), dim3(RITZ_NT), 0,
                        (hipStream_t)stream, pds, ab, m, steps_done, ks, tol, active, theta, S,
-                       steps, res, piv);
+                       steps, res, piv, summary);
   else
     hipLaunchKernelGGL(k_lanczos_ritz<4096>, dim3((unsigned)info->n_pd), dim3(RITZ_NT), 0,
                        (hipStream_t)stream, pds, ab, m, steps_done, ks, tol, active, theta, S,
-                       steps, res, piv);
+                       steps, res, piv, summary);
   MEMI_CHECK_LAUNCH("k_lanczos_ritz");
   return MEMI_OK;
 }

@@ -437,15 +437,18 @@ LANCZOS = dict(graph=True, split=None, check_every=None)   # dm_embed's Lanczos 
 
 
 def _next_check_gap(j, res, thmax, active, tol, step, hist, slack=0.6, cap=4):
-    """Steps to the next convergence check.  Each check reports every live PD's largest
-    residual r against its target tol * max|theta|; with two checks, log(r / target) is
-    extrapolated linearly in the step count (Lanczos residuals usually fall faster than that,
-    hence `slack`), and the next check goes to the earliest predicted convergence over the live
-    PDs, in whole blocks of `step`, between 1 and `cap` blocks away."""
+    """Steps to the next convergence check, and the step by which every live PD is predicted
+    to have converged.  Each check reports every live PD's largest residual r against its
+    target tol * max|theta|; with two checks, log(r / target) is extrapolated linearly in the
+    step count.  The next check goes to the earliest predicted convergence over the live PDs
+    (Lanczos residuals usually fall faster than linearly, hence `slack`), in whole blocks of
+    `step`, between 1 and `cap` blocks away.  The last predicted convergence (no slack) is None
+    while any live PD has no falling trend yet.  Returns (gap, history, last)."""
     live = active.astype(bool)
     with np.errstate(divide="ignore", invalid="ignore"):
         lr = np.log(np.maximum(res, 1e-300) / (tol * np.maximum(thmax, 1e-300)))
     gap = step
+    last = None
     if hist is not None and live.any():
         j0, lr0 = hist
         rate = (lr - lr0) / max(j - j0, 1)                # per step; negative while converging
@@ -453,7 +456,8 @@ def _next_check_gap(j, res, thmax, active, tol, step, hist, slack=0.6, cap=4):
         if ok.any() and ok.sum() == live.sum():
             need = np.min(lr[ok] / -rate[ok]) * slack
             gap = int(min(cap, max(1, need // step))) * step
-    return gap, (j, lr)
+            last = j + float(np.max(lr[ok] / -rate[ok]))
+    return gap, (j, lr), last
 
 
 def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202, split=None,
@@ -508,6 +512,9 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
             st["steps"] = torch.zeros(npd, dtype=torch.int32, device=dev)
             st["res"] = torch.zeros(npd, dtype=torch.float64, device=dev)
             st["piv"] = torch.empty(npd * K * 2 * m, dtype=torch.float64, device=dev)
+            st["summary"] = torch.zeros(3 * npd, dtype=torch.float64, device=dev)
+            st["summary_h"] = torch.zeros(3 * npd, dtype=torch.float64, pin_memory=True)
+            st["summary_ev"] = torch.cuda.Event()
         st["graph"] = None
         if graph:
             g = torch.cuda.CUDAGraph()
@@ -537,25 +544,40 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     ks_t = torch.as_tensor(np.asarray(ks, dtype=np.int32), device=dev)
     steps = np.minimum(ns, m)
     out = [None] * npd
-    j = 0
+    def block():
+        if g is not None:
+            g.replay()
+        else:
+            nat.lanczos_steps(plan, L, V, m, j_dev, check_every, active, W, ab, H=H)
+
+    j = 0                            # steps queued on the stream
+    jc = 0                           # step count of the last check
     gap = check_every
     hist = None                      # (j, log of residual / target) at the previous check
+    last = None                      # step by which every live PD is predicted to converge
     while active_h.any():
-        target = first_check if j == 0 else min(m, j + gap)
+        target = first_check if jc == 0 else min(m, jc + gap)
         while j < target:
-            if g is not None:
-                g.replay()
-            else:
-                nat.lanczos_steps(plan, L, V, m, j_dev, check_every, active, W, ab, H=H)
+            block()
             j = min(m, j + check_every)
+        jc = j
         if check == "gpu":
-            nat.lanczos_ritz(plan, ab, m, j, ks_t, tol, active, theta, S_dev, steps_t, res_t,
-                             piv)
-            # one copy: active flags, largest residuals, max |theta| per PD
-            h = torch.stack([active.double(), res_t,
-                             theta.view(npd, K).abs().amax(dim=1)]).cpu().numpy()
+            # the check writes {active, largest residual, max |theta|} per PD; one async copy
+            # into pinned memory and a wait on its event bring them back.  Unless this check
+            # is predicted to end the run, the next block of steps is queued behind it before
+            # the wait, so the GPU works through the read-back (PDs the check finishes are
+            # inactive by then and skipped by those steps)
+            nat.lanczos_ritz(plan, ab, m, jc, ks_t, tol, active, theta, S_dev, steps_t, res_t,
+                             piv, summary=st["summary"])
+            st["summary_h"].copy_(st["summary"], non_blocking=True)
+            st["summary_ev"].record()
+            if jc < m and (last is None or last > jc):
+                block()
+                j = min(m, j + check_every)
+            st["summary_ev"].synchronize()
+            h = st["summary_h"].numpy().reshape(3, npd)
             active_h = h[0].astype(np.int32)
-            gap, hist = _next_check_gap(j, h[1], h[2], active_h, tol, check_every, hist)
+            gap, hist, last = _next_check_gap(jc, h[1], h[2], active_h, tol, check_every, hist)
             continue
         abh = ab.view(npd, m, 2).cpu().numpy()
 

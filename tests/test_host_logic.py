@@ -97,3 +97,26 @@ def test_uniform_grid_detection():
     assert pl.uniform_grid(np.array([0.0, 1.0, 3.0])) is None
     assert pl.uniform_grid(np.array([1.0])) is None
     assert pl.uniform_grid(np.array([2.0, 1.0, 0.0])) is None
+
+
+def test_check_pacing_predicts_last_convergence():
+    """The Lanczos check pacing on a measured residual history (one PD, n = 2000, log10 of
+    residual / target = 7.21, 4.42, 0.94, -2.72 at 32, 40, 48, 56 steps): after two checks the
+    next check is one block on, and the predicted convergence step is the log-linear
+    extrapolation of the last two checks."""
+    from manifoldem_python_amd import pipeline as pl
+    tol, th = 1e-10, np.array([1.0])
+    res = lambda l10: np.array([tol * 10.0 ** l10])
+    act = np.array([1], dtype=np.int32)
+    gap, hist, last = pl._next_check_gap(32, res(7.21), th, act, tol, 8, None)
+    assert gap == 8 and last is None                  # no trend from one check
+    gap, hist, last = pl._next_check_gap(40, res(4.42), th, act, tol, 8, hist)
+    assert gap == 8 and abs(last - (40 + 4.42 / ((7.21 - 4.42) / 8))) < 1e-9
+    assert last > 48                                  # the check at 48 is predicted not final
+    gap, hist, last = pl._next_check_gap(48, res(0.94), th, act, tol, 8, hist)
+    assert gap == 8 and last < 56                     # ... and the one at 56 is
+    # a PD whose residual stopped falling leaves the prediction open
+    two = np.array([1, 1], dtype=np.int32)
+    _, h2, _ = pl._next_check_gap(32, np.r_[res(7.0), res(6.0)], np.r_[th, th], two, tol, 8, None)
+    _, _, last = pl._next_check_gap(40, np.r_[res(5.0), res(6.5)], np.r_[th, th], two, tol, 8, h2)
+    assert last is None
