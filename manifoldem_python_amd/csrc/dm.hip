@@ -604,32 +604,6 @@ __device__ inline void tanh_terms(double x, const double a[4], double& f, double
   J[3] = 1.0;
 }
 
-// Wave sum of a double by DPP moves (plain VALU operations, no LDS traffic): an inclusive
-// scan within each 16-lane row (row_shr 1, 2, 4, 8; lanes without a source read 0), then
-// row_bcast:15 adds row 0's total into row 1 and row 2's into row 3, and row_bcast:31 adds
-// rows 0 + 1 into rows 2 and 3; lane 63 then holds the wave's total, read back as a uniform
-// value.  Both 32-bit halves of the double move with the same control.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_move(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-__device__ __forceinline__ double wave_total(double v) {
-  v += dpp_move<0x111, 0xf>(v);   // row_shr:1
-  v += dpp_move<0x112, 0xf>(v);   // row_shr:2
-  v += dpp_move<0x114, 0xf>(v);   // row_shr:4
-  v += dpp_move<0x118, 0xf>(v);   // row_shr:8
-  v += dpp_move<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
-  v += dpp_move<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, 63);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 // 256 threads per PD evaluate the curve (~6 points each).  Each wave sums its 15 values by
 // DPP, lane 0 stores them, and after one barrier every thread adds the wave sums in wave order,
 // so all threads hold the same totals and run the LM bookkeeping (scaling, the damped 4x4

@@ -30,35 +30,97 @@ __device__ inline void step_arrive(int32_t* jp, int j) {
   }
 }
 
+// w = L v_j: each wave takes `rpw` consecutive rows of the plan's row list (the row index is
+// uniform over the wave, so the row pair, the PD descriptor and the active flag are scalar
+// loads, and one descriptor serves the run of rows of one PD).  Four rows of one PD go
+// together (16 loads of L per lane in flight, each load of v_j shared by the four), others one
+// at a time with eight loads in flight; each row is summed by DPP.  rpw is 1 up to 8192 rows
+// (a wave per row fills the GPU) and 4 beyond: a wave per row spent most of its time on the
+// descriptor loads and the reduction of a short row, and long runs of rows left the waves
+// holding the largest PDs' rows as a tail.
+template <typename T>
+__device__ __forceinline__ void matvec_row(const T* __restrict__ Lr, const double* __restrict__ v,
+                                           int n, int lane, double* __restrict__ out) {
+  double s[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) s[u] = 0.0;
+  int c = lane;
+  for (; c + 7 * 64 < n; c += 8 * 64) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u] += (double)Lr[c + 64 * u] * v[c + 64 * u];
+  }
+#pragma unroll
+  for (int u = 0; u < 7; ++u)
+    if (c + 64 * u < n) s[u] += (double)Lr[c + 64 * u] * v[c + 64 * u];
+  const double t = wave_total(((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])));
+  if (lane == 0) *out = t;
+}
+
+template <typename T>
+__device__ __forceinline__ void matvec_rows4(const T* __restrict__ L0, int64_t stride,
+                                             const double* __restrict__ v, int n, int lane,
+                                             double* __restrict__ out) {
+  double s[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s[r][u] = 0.0;
+  int c = lane;
+  for (; c + 3 * 64 < n; c += 4 * 64) {
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = v[c + 64 * u];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s[r][u] += (double)L0[r * stride + c + 64 * u] * x[u];
+  }
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+    if (c + 64 * u < n) {
+      const double x = v[c + 64 * u];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s[r][u] += (double)L0[r * stride + c + 64 * u] * x;
+    }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double t = wave_total((s[r][0] + s[r][1]) + (s[r][2] + s[r][3]));
+    if (lane == 0) out[r] = t;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restrict__ rows,
                                                         int64_t n_rows,
                                                         const memi_pd_desc* __restrict__ pds,
                                                         const T* __restrict__ L,
                                                         const double* __restrict__ V, int m,
-                                                        const int32_t* __restrict__ jp, const int32_t* __restrict__ active,
-                                                        double* __restrict__ W) {
+                                                        const int32_t* __restrict__ jp,
+                                                        const int32_t* __restrict__ active,
+                                                        double* __restrict__ W, int rpw) {
   const int j = *jp;
   const int lane = threadIdx.x & 63;
-  const int64_t gr = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // one wave per row
-  if (gr >= n_rows) return;
-  const int pd = rows[2 * gr], i = rows[2 * gr + 1];
-  if (!active[pd]) return;
-  const memi_pd_desc P = pds[pd];
-  const int n = P.n;
-  if (j >= min(n, m)) return;
-  const T* Lr = L + P.d_off + (int64_t)i * n;
-  const double* v = V + (int64_t)m * P.row0 + (int64_t)j * n;
-  double s = 0.0, s1 = 0.0;
-  int c = lane;
-  for (; c + 64 < n; c += 128) {
-    s += (double)Lr[c] * v[c];
-    s1 += (double)Lr[c + 64] * v[c + 64];
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t g0 = ((int64_t)blockIdx.x * 4 + wid) * rpw;
+  const int64_t g1 = min(g0 + (int64_t)rpw, n_rows);
+  for (int64_t g = g0; g < g1;) {                     // uniform over the wave
+    const int pd = rows[2 * g], i = rows[2 * g + 1];
+    const memi_pd_desc P = pds[pd];
+    const int n = P.n;
+    const bool live = active[pd] && j < min(n, m);
+    // rows g..g+3 are rows i..i+3 of the same PD when the run and the PD both hold them
+    if (g + 3 < g1 && i + 3 < n) {
+      if (live)
+        matvec_rows4<T>(L + P.d_off + (int64_t)i * n, n, V + (int64_t)m * P.row0 + (int64_t)j * n,
+                        n, lane, W + P.row0 + i);
+      g += 4;
+    } else {
+      if (live)
+        matvec_row<T>(L + P.d_off + (int64_t)i * n, V + (int64_t)m * P.row0 + (int64_t)j * n, n,
+                      lane, W + P.row0 + i);
+      g += 1;
+    }
   }
-  if (c < n) s += (double)Lr[c] * v[c];
-  s += s1;
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) W[P.row0 + i] = s;
 }
 
 // one block per PD: CGS2 against columns 0..j, then normalise into column j+1
@@ -204,15 +266,16 @@ __global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restr
                                                           const int32_t* __restrict__ active,
                                                           double* __restrict__ W,
                                                           const double* __restrict__ H,
-                                                          int64_t npd, int finish,
+                                                          int64_t npd, int finish, int arrive,
                                                           double* __restrict__ ab) {
   __shared__ double part[4][64];
+  __shared__ double qpart[4][64];
   const int j = jp[0];
   const int64_t g = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int cg = threadIdx.x >> 6;
   const int64_t hs = m + H_STRIDE_EXTRA;
   const double* Hp = H + (finish ? npd * hs : 0);
-  double acc = 0.0;
+  double acc = 0.0, q = 0.0;
   bool live = false;
   int pd = 0, i = 0;
   memi_pd_desc P{};
@@ -224,10 +287,15 @@ __global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restr
       live = true;
       const double* Vp = V + (int64_t)m * P.row0 + i;
       const double* h = Hp + (int64_t)pd * hs;
-      for (int c = cg; c <= j; c += 4) acc += Vp[(int64_t)c * P.n] * h[c];
+      for (int c = cg; c <= j; c += 4) {
+        const double hc = h[c];
+        acc += Vp[(int64_t)c * P.n] * hc;
+        q += hc * hc;                  // ||h2||^2 in the same column groups, for beta
+      }
     }
   }
   part[cg][threadIdx.x & 63] = acc;
+  qpart[cg][threadIdx.x & 63] = q;
   __syncthreads();
   if (cg == 0 && live) {
     const int t = threadIdx.x;
@@ -235,10 +303,10 @@ __global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restr
     if (!finish) {
       W[P.row0 + i] = w;
     } else {
+      // every row of the PD sums the same column-group partials in the same order: one beta
       const double* h2 = Hp + (int64_t)pd * hs;
-      double q = 0.0;
-      for (int c = 0; c <= j; ++c) q += h2[c] * h2[c];
-      const double beta = sqrt(fmax(h2[m] - q, 0.0));
+      const double qq = (qpart[0][t] + qpart[1][t]) + (qpart[2][t] + qpart[3][t]);
+      const double beta = sqrt(fmax(h2[m] - qq, 0.0));
       if (j + 1 < m) {
         const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
         V[(int64_t)m * P.row0 + (int64_t)(j + 1) * P.n + i] = w * inv;
@@ -249,7 +317,13 @@ __global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restr
       }
     }
   }
-  if (finish) step_arrive(jp, j);
+  if (finish && arrive) step_arrive(jp, j);
+}
+
+// Advances the step index after a finishing update launched without arrivals (large grids,
+// where thousands of blocks contending for one counter cost more than this launch).
+__global__ void k_step_advance(int32_t* __restrict__ jp) {
+  if (threadIdx.x == 0) jp[0] = jp[0] + 1;
 }
 
 // ---- convergence check: Ritz pairs of the tridiagonal T_p on the GPU -----------------------
@@ -633,14 +707,19 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const v
   const int32_t* rows = (const int32_t*)(plan + info->off_rows);
   const unsigned npd = (unsigned)info->n_pd;
   const int32_t* jp = j_dev;
+  // rows per wave: one for batches up to 8192 rows (a wave per row fills the GPU), else one
+  // group of four (waves of equal row counts but unequal row lengths must stay short, or the
+  // waves holding the largest PDs' rows leave a long tail)
+  const int64_t rpw = info->n_rows > 8192 ? 4 : 1;
+  const int64_t n_waves = (info->n_rows + rpw - 1) / rpw;
+  const dim3 gm((unsigned)((n_waves + 3) / 4));
   for (int st = 0; st < nsteps; ++st) {
-    const dim3 gm((unsigned)((info->n_rows + 3) / 4));
     if (l_is_f64)
       hipLaunchKernelGGL(k_lanczos_matvec<double>, gm, dim3(256), 0, s, rows, info->n_rows, pds,
-                         (const double*)L, V, m, jp, active, W);
+                         (const double*)L, V, m, jp, active, W, (int)rpw);
     else
       hipLaunchKernelGGL(k_lanczos_matvec<float>, gm, dim3(256), 0, s, rows, info->n_rows, pds,
-                         (const float*)L, V, m, jp, active, W);
+                         (const float*)L, V, m, jp, active, W, (int)rpw);
     MEMI_CHECK_LAUNCH("k_lanczos_matvec");
     if (!split) {
       hipLaunchKernelGGL(k_lanczos_orth, dim3(npd), dim3(256), (size_t)m * sizeof(double), s,
@@ -658,10 +737,15 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const v
                              m, jp, active, W, H + (int64_t)pass * info->n_pd * (m + 1), nrm);
         }
         MEMI_CHECK_LAUNCH("k_orth_dot");
-        hipLaunchKernelGGL(k_orth_update_rows, dim3((unsigned)((info->n_rows + 63) / 64)),
-                           dim3(256), 0, s, rows, info->n_rows, pds, V, m, j_dev, active, W, H,
-                           info->n_pd, nrm, ab);
+        const unsigned ub = (unsigned)((info->n_rows + 63) / 64);
+        const int arrive = ub <= 256;   // else one thread advances the step after the update
+        hipLaunchKernelGGL(k_orth_update_rows, dim3(ub), dim3(256), 0, s, rows, info->n_rows, pds,
+                           V, m, j_dev, active, W, H, info->n_pd, nrm, arrive, ab);
         MEMI_CHECK_LAUNCH("k_orth_update_rows");
+        if (nrm && !arrive) {
+          hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, j_dev);
+          MEMI_CHECK_LAUNCH("k_step_advance");
+        }
       }
     }
   }
