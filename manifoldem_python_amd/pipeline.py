@@ -313,6 +313,26 @@ def _device_const(a, dev):
     return t
 
 
+def batched_inv(A):
+    """np.linalg.inv of each matrix of a stack, NaN for the exactly singular ones (those inv
+    rejects: an exact zero pivot, so det == 0), in one batched call instead of one call per
+    matrix once any is singular."""
+    with np.errstate(all="ignore"):
+        ok = np.linalg.det(A) != 0
+        out = np.full_like(A, np.nan)
+        try:
+            if ok.any():
+                out[ok] = np.linalg.inv(A[ok])
+        except np.linalg.LinAlgError:        # a zero pivot beside an infinite one: one at a time
+            out = np.full_like(A, np.nan)
+            for q in range(len(A)):
+                try:
+                    out[q] = np.linalg.inv(A[q])
+                except np.linalg.LinAlgError:
+                    pass
+    return out
+
+
 def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_per_round=8,
                        logS_dev=None):
     """The fit of ``fergusonE.py:45-57`` for many PDs at once on the GPU (memi_tanh_fit):
@@ -360,16 +380,7 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
         J = J + np.triu(J, 1).transpose(0, 1, 3, 2)
         ss_res = o[:, :, 14]
         with np.errstate(all="ignore"):
-            Jf = J.reshape(-1, 4, 4)
-            try:
-                Jinv = np.linalg.inv(Jf)
-            except np.linalg.LinAlgError:        # a singular J^T J somewhere: one at a time
-                Jinv = np.full_like(Jf, np.nan)
-                for q in range(len(Jf)):
-                    try:
-                        Jinv[q] = np.linalg.inv(Jf[q])
-                    except np.linalg.LinAlgError:
-                        pass
+            Jinv = batched_inv(J.reshape(-1, 4, 4))
             diag = np.diagonal(Jinv, axis1=1, axis2=2).reshape(len(idx), R, 4)
             resnorm = np.sum(np.sqrt(np.fabs(diag * (ss_res / (m - 4))[:, :, None])), axis=2)
         resnorm[~np.isfinite(resnorm)] = np.inf

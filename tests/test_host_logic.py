@@ -120,3 +120,24 @@ def test_check_pacing_predicts_last_convergence():
     _, h2, _ = pl._next_check_gap(32, np.r_[res(7.0), res(6.0)], np.r_[th, th], two, tol, 8, None)
     _, _, last = pl._next_check_gap(40, np.r_[res(5.0), res(6.5)], np.r_[th, th], two, tol, 8, h2)
     assert last is None
+
+
+def test_batched_inv_matches_per_matrix_inv():
+    """The fit's batched inverse equals np.linalg.inv matrix by matrix, with NaN exactly where
+    inv rejects the matrix (singular J^T J of a saturated fit), and NaN in, NaN out."""
+    from manifoldem_python_amd import pipeline as pl
+    rng = np.random.default_rng(9)
+    J = rng.normal(size=(40, 6, 4))
+    A = np.einsum("bij,bik->bjk", J, J)
+    A[3] = 0.0                                       # all-zero
+    A[7][:, 2] = A[7][2, :] = 0.0                    # a zero column and row
+    A[11][0, 0] = np.nan
+    got = pl.batched_inv(A)
+    for q in range(len(A)):
+        try:
+            ref = np.linalg.inv(A[q])
+        except np.linalg.LinAlgError:
+            assert np.all(np.isnan(got[q])), q
+            continue
+        np.testing.assert_array_equal(np.isnan(got[q]), np.isnan(ref))
+        np.testing.assert_allclose(got[q][~np.isnan(ref)], ref[~np.isnan(ref)], rtol=1e-12)
