@@ -224,7 +224,9 @@ __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict
 
 // Many-PD form of the dot pass: a fixed grid whose waves walk the npd x (j + 1) live
 // (PD, column) pairs (j + 2 with the w.w entry), one wave per pair (grids sized for the
-// largest j leave most blocks empty when there are hundreds of PDs).
+// largest j leave most blocks empty when there are hundreds of PDs).  The pair index is
+// uniform over the wave (scalar loads of the PD descriptor and flag); four accumulators keep
+// four loads of each operand in flight, and the wave sums by DPP.
 __global__ __launch_bounds__(256) void k_orth_dot_pairs(const memi_pd_desc* __restrict__ pds,
                                                         int64_t npd,
                                                         const double* __restrict__ V, int m,
@@ -237,8 +239,8 @@ __global__ __launch_bounds__(256) void k_orth_dot_pairs(const memi_pd_desc* __re
   const int per = j + 1 + (with_norm ? 1 : 0);
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int64_t total = npd * (int64_t)per;
-  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < total;
-       q += nwaves) {
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int64_t q = (int64_t)blockIdx.x * (blockDim.x >> 6) + wid; q < total; q += nwaves) {
     const int pd = (int)(q / per), c = (int)(q % per);
     if (!active[pd]) continue;                    // uniform over the wave
     const memi_pd_desc P = pds[pd];
@@ -247,9 +249,16 @@ __global__ __launch_bounds__(256) void k_orth_dot_pairs(const memi_pd_desc* __re
     const bool norm = c == j + 1;
     const double* w = W + P.row0;
     const double* vc = norm ? w : V + (int64_t)m * P.row0 + (int64_t)c * n;
-    double s = 0.0;
-    for (int r = lane; r < n; r += 64) s += vc[r] * w[r];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int r = lane;
+    for (; r + 3 * 64 < n; r += 4 * 64) {
+      s0 += vc[r] * w[r];
+      s1 += vc[r + 64] * w[r + 64];
+      s2 += vc[r + 128] * w[r + 128];
+      s3 += vc[r + 192] * w[r + 192];
+    }
+    for (; r < n; r += 64) s0 += vc[r] * w[r];
+    const double s = wave_total((s0 + s1) + (s2 + s3));
     if (lane == 0) H[(int64_t)pd * (m + H_STRIDE_EXTRA) + (norm ? m : c)] = s;
   }
 }
