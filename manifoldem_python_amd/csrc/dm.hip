@@ -105,14 +105,17 @@ __global__ __launch_bounds__(256) void k_knn_select(const int32_t* __restrict__ 
         if ((key & pm) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1);
       }
       __syncthreads();
-      if (threadIdx.x == 0) {
-        int cum = 0, b = 0;
-        for (; b < 256; ++b) {
-          if (cum + hist[b] >= rem) break;
-          cum += hist[b];
+      // the bin holding the rem-th smallest key: thread b owns bin b, an exclusive scan gives
+      // the keys in bins below it, and the one bin with excl < rem <= excl + count is selected
+      // (a serial walk over the 256 bins on one thread waited on one LDS read after another)
+      {
+        const int hb = hist[threadIdx.x];
+        int tot_b;
+        const int excl = block_excl_scan(hb, scan_sh, &tot_b);
+        if (excl < rem && rem <= excl + hb) {
+          sel_bin = threadIdx.x;
+          sel_rem = rem - excl;
         }
-        sel_bin = b;
-        sel_rem = rem - cum;
       }
       __syncthreads();
       prefix |= (uint32_t)sel_bin << shift;

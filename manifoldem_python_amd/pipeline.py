@@ -472,7 +472,7 @@ def _next_check_gap(j, res, thmax, active, tol, step, hist, slack=0.6, cap=4):
 
 
 def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202, split=None,
-                 check="gpu", graph=True):
+                 check="gpu", graph=True, full=False):
     """Largest-magnitude eigenpairs of every PD's float32 kernel block of L (sembeddingonFly.py
     semantics: the k eigenpairs of largest |lambda|, sorted by descending lambda) by batched
     Lanczos with full reorthogonalisation on the GPU.
@@ -487,7 +487,8 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     on) or "host" (scipy tridiagonal
     eigensolver per PD, every 32 steps by default).  graph: the launches of one block of
     check_every steps are captured once in a HIP graph and replayed (the step index lives on
-    the device).  Returns a list of (lam, V (n x k) numpy, steps, max residual)."""
+    the device).  Returns a list of (lam, V (n x k) numpy, steps, max residual); with full,
+    also the (n_rows x kmax) array the V are row blocks of."""
     dev = L.device
     npd = plan.npd
     ns = plan.ns.astype(int)
@@ -657,7 +658,7 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
         lam, _, J, res = out[p]
         k = int(meta_h[p, 1])
         results.append((lam, Yh[rows0[p]:rows0[p] + ns[p], :k], J, res))
-    return results
+    return (results, Yh) if full else results
 
 
 def top_eigs_lm(L, k):
@@ -795,9 +796,13 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
         timer.mark("normalize")
     ks = [min(num_eigs, int(n) - 3) + 1 for n in plan.ns]
     if eig == "lanczos":
-        eigs = lanczos_eigs(plan, L, ks, **LANCZOS)
+        eigs, Yall = lanczos_eigs(plan, L, ks, full=True, **LANCZOS)
         if timer:
             timer.mark("lanczos")
+        # psi = V[:, 1:] / V[:, 0] and mu = V[:, 0]^2 over all rows at once; PDs take row blocks
+        with np.errstate(divide="ignore", invalid="ignore"):
+            psi_all = Yall[:, 1:] / Yall[:, 0:1]
+        mu_all = Yall[:, 0] ** 2
     for p in range(npd):
         n = int(plan.ns[p])
         nEigs = ks[p] - 1
@@ -809,13 +814,15 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
             lam, V = lam.cpu().numpy(), V.cpu().numpy()
         r = results[p]
         r.lamb = lam
-        if V.shape[1] - 1 == nEigs:
-            psi = V[:, 1:] / V[:, 0:1]
+        r0 = int(plan.rows0[p])
+        if eig == "lanczos" and V.shape[1] - 1 == nEigs:
+            r.psi = psi_all[r0:r0 + n, :nEigs]
+            r.mu = mu_all[r0:r0 + n]
         else:                            # fewer vectors than asked (tiny PDs): zero columns
             psi = np.zeros((n, nEigs))
             psi[:, :V.shape[1] - 1] = V[:, 1:] / V[:, 0:1]
-        r.psi = psi
-        r.mu = V[:, 0] ** 2
+            r.psi = psi
+            r.mu = V[:, 0] ** 2
         r.V = V
     if timer:
         timer.mark("eig_post")
