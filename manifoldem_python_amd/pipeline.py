@@ -695,6 +695,22 @@ def uniform_grid(logEps):
     return float(le[0]), float(dle)
 
 
+_PINNED = {}
+
+
+def _to_host(t):
+    """A fresh numpy copy of a device tensor, through a cached pinned buffer (about twice the
+    rate of a copy into pageable memory for the multi-MB arrays of large batches)."""
+    n = t.numel()
+    key = (t.dtype, n)
+    buf = _PINNED.get(key)
+    if buf is None:
+        buf = _PINNED[key] = torch.empty(n, dtype=t.dtype, pin_memory=True)
+    buf.copy_(t.reshape(-1), non_blocking=True)
+    torch.cuda.current_stream().synchronize()
+    return buf.numpy().copy()
+
+
 def ferguson_sums(plan, D, mask, logEps, method=None, want_log_dev=False):
     """a12 sums: (npd, n_eps) arrays of sum W, d log(sum W)/d log eps and #terms (and, with
     want_log_dev, log sum W as a device tensor).  method: "moments" (bucket moments, needs a
@@ -715,7 +731,7 @@ def ferguson_sums(plan, D, mask, logEps, method=None, want_log_dev=False):
         nat.ferguson_sweep(plan, D, mask, le, fer)
     S_dev = fer[:npd * n_eps * 3].view(npd, n_eps, 3)[:, :, 0]
     logS_dev = torch.log(S_dev).contiguous() if want_log_dev else None
-    fer = fer.cpu().numpy()[:npd * n_eps * 3].reshape(npd, n_eps, 3)
+    fer = _to_host(fer[:npd * n_eps * 3]).reshape(npd, n_eps, 3)
     S = fer[:, :, 0]
     with np.errstate(divide="ignore", invalid="ignore"):
         slope = fer[:, :, 1] / S
@@ -765,21 +781,25 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
     results = []
     sig = np.zeros(npd)
     a0l = [((rand(4, 1) - 0.5) if a0s is None else a0s[p]) for p in range(npd)]
-    with np.errstate(divide="ignore"):
-        logS = np.log(S)
+    # log sum W was taken on the device for the fit; the host gets that copy (a host log of
+    # every PD's curve cost more than the transfer)
+    logS = _to_host(logS_dev).reshape(S.shape)
+    logEps_a = np.asarray(logEps)
     if fit == "gpu":
-        popt_b, R2_b = ferguson_fit_batch(np.asarray(logEps), logS, a0l, rand,
-                                          logS_dev=logS_dev)
+        popt_b, R2_b = ferguson_fit_batch(logEps_a, logS, a0l, rand, logS_dev=logS_dev)
+        # DMembeddingII.py:170, for every PD at once
+        sig_b = tune * np.sqrt(2 * np.exp(-popt_b[:, 1] / popt_b[:, 0]))
     for p in range(npd):
         r = DMResult()
         r.logSumWij = logS[p]
         r.slope = slope[p]
-        r.logEps = np.asarray(logEps)
+        r.logEps = logEps_a
         if fit == "gpu":
             r.popt, r.R_squared = popt_b[p], R2_b[p]
+            r.sigma = sig_b[p]
         else:
             r.popt, _, r.R_squared = ferguson_fit(r.logEps, r.logSumWij, a0l[p], rand)
-        r.sigma = tune * np.sqrt(2 * np.exp(-r.popt[1] / r.popt[0]))
+            r.sigma = tune * np.sqrt(2 * np.exp(-r.popt[1] / r.popt[0]))
         sig[p] = r.sigma
         results.append(r)
     if timer:
