@@ -17,6 +17,9 @@
 namespace memi {
 
 constexpr int TB = 128;  // tile side of the full-matrix tiles of the plan
+// blocks per 128 x 128 tile in the Ferguson and normalisation passes, each a quarter of the
+// tile (one block per tile left most of each CU idle for batches of few tiles)
+constexpr int MARKOV_RQ = 4;
 
 __device__ inline uint32_t float_key(float v) {
   const uint32_t u = __float_as_uint(v);
@@ -344,7 +347,9 @@ __global__ __launch_bounds__(256) void k_fer_stats(const int32_t* __restrict__ f
   const bool full = P.k >= P.n;
   double sum = 0.0, cnt = 0.0, zer = 0.0;
   unsigned long long vmin = dbits(INFINITY), vmax = 0ull;
-  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
+  // blockIdx.y: which MARKOV_RQ-th of the tile's entries
+  const int e1 = (blockIdx.y + 1) * (TB * TB / MARKOV_RQ);
+  for (int e = blockIdx.y * (TB * TB / MARKOV_RQ) + threadIdx.x; e < e1; e += blockDim.x) {
     const int i = f[1] * TB + e / TB, j = f[2] * TB + e % TB;
     if (i >= P.n || j >= P.n) continue;
     const double v = entry_value(D, mask, P, W, full, i, j);
@@ -442,7 +447,9 @@ __global__ __launch_bounds__(256) void k_fer_moments(const int32_t* __restrict__
   __syncthreads();
   const int W = (P.n + 31) / 32;
   const bool full = P.k >= P.n;
-  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
+  // blockIdx.y: which MARKOV_RQ-th of the tile's entries
+  const int e1 = (blockIdx.y + 1) * (TB * TB / MARKOV_RQ);
+  for (int e = blockIdx.y * (TB * TB / MARKOV_RQ) + threadIdx.x; e < e1; e += blockDim.x) {
     const int i = f[1] * TB + e / TB, j = f[2] * TB + e % TB;
     if (i >= P.n || j >= P.n) continue;
     const double v = entry_value(D, mask, P, W, full, i, j);
@@ -545,7 +552,10 @@ __global__ __launch_bounds__(256) void k_markov_colsum(const int32_t* __restrict
   const int j = f[2] * TB + cl;
   if (j >= P.n) return;
   double s = 0.0;
-  for (int r = half; r < TB; r += 2) {
+  // blockIdx.y: which MARKOV_RQ-th of the tile's rows (several blocks per tile keep the CUs
+  // busy when a batch has few tiles)
+  const int r1 = (blockIdx.y + 1) * (TB / MARKOV_RQ);
+  for (int r = blockIdx.y * (TB / MARKOV_RQ) + half; r < r1; r += 2) {
     const int i = f[1] * TB + r;
     if (i >= P.n) break;
     const double w = kernel_weight(entry_value(D, mask, P, W, full, i, j), inv_s2);
@@ -572,7 +582,8 @@ __global__ __launch_bounds__(256) void k_markov_write(const int32_t* __restrict_
   if (j >= P.n) return;
   const double dj = deg[P.row0 + j];
   const double dpj = sqrt(deg[n_rows + P.row0 + j] / dj);
-  for (int r = half; r < TB; r += 2) {
+  const int r1 = (blockIdx.y + 1) * (TB / MARKOV_RQ);
+  for (int r = blockIdx.y * (TB / MARKOV_RQ) + half; r < r1; r += 2) {
     const int i = f[1] * TB + r;
     if (i >= P.n) break;
     const double di = deg[P.row0 + i];
@@ -975,12 +986,14 @@ int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, cons
   MEMI_CHECK_LAUNCH("k_zero_f64");
   hipLaunchKernelGGL(k_fer_init, dim3(grid_for(npd)), dim3(256), 0, s, npd, st);
   MEMI_CHECK_LAUNCH("k_fer_init");
-  hipLaunchKernelGGL(k_fer_stats, dim3((unsigned)info->n_ftiles), dim3(256), 0, s, v.ftiles,
+  hipLaunchKernelGGL(k_fer_stats, dim3((unsigned)info->n_ftiles, MARKOV_RQ), dim3(256), 0, s,
+                     v.ftiles,
                      v.pds, D, mask, logEps, n_eps, st);
   MEMI_CHECK_LAUNCH("k_fer_stats");
   hipLaunchKernelGGL(k_fer_grid, dim3(grid_for(npd)), dim3(256), 0, s, npd, st, le0, dle, gp);
   MEMI_CHECK_LAUNCH("k_fer_grid");
-  hipLaunchKernelGGL(k_fer_moments, dim3((unsigned)info->n_ftiles), dim3(256), 0, s, v.ftiles,
+  hipLaunchKernelGGL(k_fer_moments, dim3((unsigned)info->n_ftiles, MARKOV_RQ), dim3(256), 0, s,
+                     v.ftiles,
                      v.pds, D, mask, gp, M);
   MEMI_CHECK_LAUNCH("k_fer_moments");
   hipLaunchKernelGGL(k_fer_sweep_moments, dim3((unsigned)((n_eps + 255) / 256), (unsigned)npd),
@@ -1004,7 +1017,7 @@ int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, cons
   const int64_t nr = info->n_rows;
   hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(2 * nr)), dim3(256), 0, s, deg, 2 * nr);
   MEMI_CHECK_LAUNCH("k_zero_f64");
-  const dim3 g((unsigned)info->n_ftiles);
+  const dim3 g((unsigned)info->n_ftiles, MARKOV_RQ);
   hipLaunchKernelGGL(k_markov_colsum<0>, g, dim3(256), 0, s, v.ftiles, v.pds, D, mask, sigma,
                      deg, nr);
   MEMI_CHECK_LAUNCH("k_markov_colsum<0>");
