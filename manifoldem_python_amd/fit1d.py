@@ -12,6 +12,8 @@ classification is the reference's.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import torch
 
@@ -88,14 +90,27 @@ def initial_guess(psi):
 
 
 def _lstsq_curve(s, nS):
-    """(a, b) from the tau sums, ``fit_1D_open_manifold_3D.py:85-100``."""
-    A11, A12, b1, b2 = s[0:3], s[3:6], s[6:9], s[9:12]
-    coeff = np.zeros((2, 3))
+    """(a, b) from the tau sums, ``fit_1D_open_manifold_3D.py:85-100``: for each coordinate the
+    2x2 normal equations [[A11, A12], [A12, nS]] (a, b) = (b1, b2).  Where a system is well
+    conditioned its least-squares solution is its solution, taken here by Cramer's rule in
+    plain floats (it agrees with lstsq to ~1e-10 relative; numpy's per-call overhead on 2x2
+    arrays cost more than the iteration's other host work); an ill-conditioned or singular one
+    keeps the reference's lstsq (its cutoff picks the minimum-norm solution)."""
+    a, b = np.zeros(3), np.zeros(3)
+    n = float(nS)
     for q in range(3):
-        coeff[:, q] = np.linalg.lstsq(np.array([[A11[q], A12[q]], [A12[q], nS]]),
-                                      np.array([b1[q], b2[q]]), rcond=None)[0]
-    return coeff[0], coeff[1]
-
+        p11, p12, r1, r2 = float(s[q]), float(s[3 + q]), float(s[6 + q]), float(s[9 + q])
+        # singular values of the symmetric system: |mid -+ rad|
+        mid, rad = 0.5 * (p11 + n), math.hypot(0.5 * (p11 - n), p12)
+        lo, hi = sorted((abs(mid - rad), abs(mid + rad)))
+        if lo > 1e-6 * hi:
+            det = p11 * n - p12 * p12
+            a[q] = (r1 * n - p12 * r2) / det
+            b[q] = (p11 * r2 - p12 * r1) / det
+        else:
+            a[q], b[q] = np.linalg.lstsq(np.array([[p11, p12], [p12, n]]), np.array([r1, r2]),
+                                         rcond=None)[0]
+    return a, b
 
 def fit(psi, max_iter=100, delta_a_max=1, delta_b_max=1, delta_tau_max=0.01, device=None):
     """(a, b, tau (nS, 1)).  Like the reference, the tau-change test is identically zero

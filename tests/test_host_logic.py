@@ -141,3 +141,21 @@ def test_batched_inv_matches_per_matrix_inv():
             continue
         np.testing.assert_array_equal(np.isnan(got[q]), np.isnan(ref))
         np.testing.assert_allclose(got[q][~np.isnan(ref)], ref[~np.isnan(ref)], rtol=1e-12)
+
+
+def test_fit1d_curve_solve_matches_lstsq():
+    """The 1-D fit's batched 2x2 solve equals the reference's per-coordinate lstsq, including
+    a singular system (all tau equal), where lstsq's minimum-norm solution is kept."""
+    from manifoldem_python_amd import fit1d
+    rng = np.random.default_rng(5)
+    nS = 300
+    for singular in (False, True):
+        tau = np.full(nS, 0.3) if singular else rng.uniform(0, 1, nS)
+        cj = np.cos(np.pi * tau[:, None] * np.array([[1, 2, 3]]))
+        x = rng.normal(size=(nS, 3))
+        s = np.concatenate([np.sum(cj ** 2, 0), np.sum(cj, 0), np.sum(x * cj, 0), np.sum(x, 0)])
+        a, b = fit1d._lstsq_curve(s, nS)
+        for q in range(3):
+            M = np.array([[s[q], s[3 + q]], [s[3 + q], nS]])
+            ref = np.linalg.lstsq(M, np.array([s[6 + q], s[9 + q]]), rcond=None)[0]
+            np.testing.assert_allclose([a[q], b[q]], ref, rtol=1e-9, atol=1e-12)
