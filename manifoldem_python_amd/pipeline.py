@@ -696,6 +696,7 @@ def uniform_grid(logEps):
 
 
 _PINNED = {}
+_PINNED_MAX = 8         # sizes kept; callers cycling through more sizes reallocate
 
 
 def _to_host(t):
@@ -705,6 +706,8 @@ def _to_host(t):
     key = (t.dtype, n)
     buf = _PINNED.get(key)
     if buf is None:
+        if len(_PINNED) >= _PINNED_MAX:
+            _PINNED.clear()
         buf = _PINNED[key] = torch.empty(n, dtype=t.dtype, pin_memory=True)
     buf.copy_(t.reshape(-1), non_blocking=True)
     torch.cuda.current_stream().synchronize()
