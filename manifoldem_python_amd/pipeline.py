@@ -558,8 +558,12 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     j_dev.zero_()
     V[st["v0_idx"]] = st["v0"]
     active_h = np.ones(npd, dtype=np.int32)
-    active.copy_(torch.as_tensor(active_h, device=dev))
-    ks_t = torch.as_tensor(np.asarray(ks, dtype=np.int32), device=dev)
+    active.fill_(1)
+    ks_key = tuple(int(k) for k in ks)
+    if st.get("ks_key") != ks_key:
+        st["ks_t"] = torch.as_tensor(np.asarray(ks, dtype=np.int32), device=dev)
+        st["ks_key"] = ks_key
+    ks_t = st["ks_t"]
     steps = np.minimum(ns, m)
     out = [None] * npd
     def block():
@@ -623,12 +627,14 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
         active.copy_(torch.as_tensor(active_h, device=dev))
     # Ritz vectors of all PDs in one launch and one copy to the host
     if check == "gpu":
-        J_h = steps_t.cpu().numpy().astype(np.int64)
+        # steps, residuals and Ritz values in one copy
+        fin = _to_host(torch.cat([steps_t.double(), res_t, theta[:npd * K]]))
+        J_h = fin[:npd].astype(np.int64)
+        res_h = fin[npd:2 * npd]
+        lam_h = fin[2 * npd:].reshape(npd, K)
         k_h = np.minimum(np.asarray(ks, dtype=np.int64), J_h)
         meta_h = np.stack([J_h, k_h, np.arange(npd, dtype=np.int64) * m * K], axis=1)
         S_all = S_dev
-        lam_h = theta.view(npd, K).cpu().numpy()
-        res_h = res_t.cpu().numpy()
         for p in range(npd):
             out[p] = (lam_h[p, :k_h[p]].copy(), None, int(J_h[p]), float(res_h[p]))
     else:
