@@ -373,8 +373,8 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
         else:
             yy = y_all[torch.as_tensor(np.repeat(idx, R), device=dev)].contiguous()
         out = torch.empty(B * 16, dtype=torch.float64, device=dev)
-        nat.tanh_fit(x, yy, torch.as_tensor(starts.reshape(B, 4), device=dev), out)
-        o = out.cpu().numpy().reshape(len(idx), R, 16)
+        nat.tanh_fit(x, yy, _to_device(starts.reshape(B, 4), dev), out)
+        o = _to_host(out).reshape(len(idx), R, 16)
         J = np.zeros((len(idx), R, 4, 4))
         J[:, :, iu[0], iu[1]] = o[:, :, 4:14]
         J = J + np.triu(J, 1).transpose(0, 1, 3, 2)
@@ -701,23 +701,39 @@ def uniform_grid(logEps):
     return float(le[0]), float(dle)
 
 
+# Pinned staging buffers, one per dtype and direction, grown to the largest size seen (a
+# pinned allocation costs far more than a copy, so sizes that change from call to call, like
+# the fit's restart rounds, reuse one buffer).
 _PINNED = {}
-_PINNED_MAX = 8         # sizes kept; callers cycling through more sizes reallocate
+_PINNED_UP = {}
+
+
+def _pinned(cache, dtype, n):
+    buf = cache.get(dtype)
+    if buf is None or buf.numel() < n:
+        cap = max(n, 2 * (buf.numel() if buf is not None else 0), 1024)
+        buf = cache[dtype] = torch.empty(cap, dtype=dtype, pin_memory=True)
+    return buf[:n]
 
 
 def _to_host(t):
-    """A fresh numpy copy of a device tensor, through a cached pinned buffer (about twice the
-    rate of a copy into pageable memory for the multi-MB arrays of large batches)."""
-    n = t.numel()
-    key = (t.dtype, n)
-    buf = _PINNED.get(key)
-    if buf is None:
-        if len(_PINNED) >= _PINNED_MAX:
-            _PINNED.clear()
-        buf = _PINNED[key] = torch.empty(n, dtype=t.dtype, pin_memory=True)
+    """A fresh numpy copy of a device tensor, through a pinned buffer (about twice the rate of a
+    copy into pageable memory for the multi-MB arrays of large batches)."""
+    buf = _pinned(_PINNED, t.dtype, t.numel())
     buf.copy_(t.reshape(-1), non_blocking=True)
     torch.cuda.current_stream().synchronize()
     return buf.numpy().copy()
+
+
+def _to_device(a, dev):
+    """A device copy of a small host array, staged through a pinned buffer and copied without
+    waiting (a copy from pageable memory blocks the host).  The staging buffer is reused by the
+    next call, which in every caller here comes after a synchronising read-back."""
+    a = np.ascontiguousarray(a)
+    dtype = torch.from_numpy(a[:0].reshape(0)).dtype
+    buf = _pinned(_PINNED_UP, dtype, a.size)
+    buf.numpy()[...] = a.reshape(-1)
+    return buf.to(dev, non_blocking=True).view(a.shape)
 
 
 def ferguson_sums(plan, D, mask, logEps, method=None, want_log_dev=False):
@@ -813,7 +829,7 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
         results.append(r)
     if timer:
         timer.mark("tanh_fit_host")
-    sigma = torch.as_tensor(sig, device=dev)
+    sigma = _to_device(sig, dev)
     # the kernel matrix buffer is kept on the plan so that the Lanczos graph stays valid
     L = getattr(plan, "_L", None)
     if L is None or L.numel() < max(info.d_elems, 1) or L.device != dev:
