@@ -480,12 +480,12 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     Converged when every selected Ritz pair has residual |beta_J s_J| <= tol * max|theta| or
     the Krylov space is the whole space.  split: orthogonalise with the whole grid (default
     for batches of fewer than 128 PDs or with a PD above 1024 rows) instead of one workgroup
-    per PD.  check: "gpu" (memi_lanczos_ritz: Ritz pairs and the decision on the device, one
-    int per PD read back per check; the first check at the first step count at which the
-    selected pairs can have converged, later ones every 8 steps or, once two checks show how
-    fast every live PD's residual falls, at the extrapolated convergence step, up to 32 steps
-    on) or "host" (scipy tridiagonal
-    eigensolver per PD, every 32 steps by default).  graph: the launches of one block of
+    per PD.  check: "gpu" (memi_lanczos_ritz: Ritz pairs and the decision on the device, a
+    summary per PD read back per check; the first check after 2k + 8 steps, later ones every 8
+    steps or, once two checks show how fast every live PD's residual falls, at the extrapolated
+    convergence step, up to 32 steps on; the next block is queued behind every check that is
+    not predicted to end the run) or "host" (scipy tridiagonal eigensolver per PD, every 32
+    steps by default).  graph: the launches of one block of
     check_every steps are captured once in a HIP graph and replayed (the step index lives on
     the device).  Returns a list of (lam, V (n x k) numpy, steps, max residual); with full,
     also the (n_rows x kmax) array the V are row blocks of."""
@@ -502,6 +502,8 @@ def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202
     if check_every is None:
         check_every = 8 if check == "gpu" else 32
     # no Ritz pair can have converged before the Krylov space holds both ends of the spectrum
+    # (a first check at 2k + 16 instead saved nothing measurable on one large PD and made the
+    # NLSA's second embedding run 5 steps longer on average)
     first_check = min(m, max(check_every, 2 * max(ks) + 8)) if check == "gpu" else check_every
     if max(ks) > nat.RITZ_KMAX:
         raise ValueError(f"lanczos_eigs: at most {nat.RITZ_KMAX} eigenpairs per PD")
