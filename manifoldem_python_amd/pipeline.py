@@ -758,13 +758,14 @@ def ferguson_sums(plan, D, mask, logEps, method=None, want_log_dev=False):
         nat.ferguson_sweep(plan, D, mask, le, fer)
     S_dev = fer[:npd * n_eps * 3].view(npd, n_eps, 3)[:, :, 0]
     logS_dev = torch.log(S_dev).contiguous() if want_log_dev else None
-    fer = _to_host(fer[:npd * n_eps * 3]).reshape(npd, n_eps, 3)
-    S = fer[:, :, 0]
-    with np.errstate(divide="ignore", invalid="ignore"):
-        slope = fer[:, :, 1] / S
+    # the slope (d sum W / d log eps over sum W) divided on the device (IEEE float64 division
+    # there and in numpy give the same values), then S, slope and the counts in one copy
+    f3 = fer[:npd * n_eps * 3].view(npd, n_eps, 3)
+    h = _to_host(torch.stack([f3[:, :, 0], f3[:, :, 1] / f3[:, :, 0], f3[:, :, 2]]))
+    h = h.reshape(3, npd, n_eps)
     if want_log_dev:
-        return S, slope, fer[:, :, 2], logS_dev
-    return S, slope, fer[:, :, 2]
+        return h[0], h[1], h[2], logS_dev
+    return h[0], h[1], h[2]
 
 
 class StageTimer:
