@@ -458,8 +458,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); PDs are "
-              "independent, so a multi-GPU run would partition the PD list by cost with no "
-              "collective (see DESIGN.md).", file=sys.stderr)
+              "independent, so a multi-GPU run would give each GPU a share of the PD list, "
+              "balanced by cost (n^2 K_h + n N^2 log N), with no collective.", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
