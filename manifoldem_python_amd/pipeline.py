@@ -14,7 +14,6 @@ psi/mu arithmetic on the (n x nEigs+1) eigenvectors.
 """
 from __future__ import annotations
 
-import math
 import warnings
 
 import numpy as np
