@@ -219,7 +219,10 @@ int memi_ferguson_sweep(const void* plan_dev, const memi_plan_info* info, const 
  * moments: positive entry values go into log-uniform buckets whose edges contain every
  * eps's threshold cut, each bucket keeps the power sums of its values' offsets from its
  * centre, and each eps sums a 17-term series per bucket below its cut (~1e-14 relative to
- * the direct sums; cost O(n_eps * buckets) per PD instead of O(n_eps * entries)).
+ * the direct sums; cost O(n_eps * buckets) per PD instead of O(n_eps * entries)).  A PD whose
+ * values span more than 544 buckets at the grid's step, or whose step is too coarse for the
+ * 17-term series (thr (e^{dle/2} - 1) > 1), is summed by the direct sweep of
+ * memi_ferguson_sweep instead, within the same call.
  * out as memi_ferguson_sweep; ws: float64 scratch of npd * (6 + 8 + 544 * 18) doubles. */
 int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, const float* D,
                           const uint32_t* mask, const double* logEps, int n_eps, double le0,
@@ -236,11 +239,12 @@ int memi_tanh_fit(const double* x, const double* y, int m, int npd, const double
 
 /* Markov normalisation, slaplacianonFly.py:42-81 with alpha = 1: W = exp(-v/sigma^2) on the
  * kernel's entries, d = column sums, W/(d_i d_j), d' = sqrt(column sums), L = |L + L^T| / 2.
- * Sums in float64; L is stored as float32 (its entries come from float32 distances).
- * sigma: float64 [npd]; L: float32 [info->d_elems]; deg: float64 scratch [2 * n_rows]. */
+ * Sums in float64.  sigma: float64 [npd]; L: [info->d_elems] float64 (l_is_f64 = 1, the
+ * float64 kernel of the reference) or float32 (l_is_f64 = 0: each entry rounded once, half the
+ * bytes for the Lanczos matvecs); deg: float64 scratch [2 * n_rows]. */
 int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, const float* D,
-                          const uint32_t* mask, const double* sigma, float* L, double* deg,
-                          void* stream);
+                          const uint32_t* mask, const double* sigma, void* L, int l_is_f64,
+                          double* deg, void* stream);
 
 /* fergusonE.py:25-43 on a flat value list (the reference's D argument; squared here):
  * out float64 [n_eps][3] = {sum exp(-x), sum x exp(-x), #terms} for x = D^2/(2 eps) < thr,

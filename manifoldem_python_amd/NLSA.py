@@ -37,7 +37,8 @@ def op(NLSAPar, DD, posPath, posPsi1, imgAll, msk2, CTF, ExtPar):
                       torch.as_tensor(idx, device=dev),
                       torch.as_tensor(imgAll, dtype=torch.float32, device=dev),
                       torch.as_tensor(_ctf_half(CTF, N), device=dev), N, ConOrder, psiTrunc,
-                      nS=nS, tune=tune, num_eigs=p.num_eigs, msk2=m2)
+                      nS=nS, tune=tune, num_eigs=p.num_eigs, msk2=m2,
+                      k=NLSAPar.get("k", num - ConOrder))
     r.IMGT = r.IMGT_dev.cpu().numpy()
     if NLSAPar.get("save", False):
         myio.fout1(ExtPar["filename"], ["psirec", "tau", "a", "b"], [r.psirec, r.tau, r.a, r.b])

@@ -69,7 +69,8 @@ _SIGS = {
     "memi_ferguson_sweep": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_ferguson_moments": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_int, c_dbl,
                               c_dbl, c_vp, c_vp, c_i64, c_vp],
-    "memi_markov_normalize": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "memi_markov_normalize": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_int,
+                              c_vp, c_vp],
     "memi_tanh_fit": [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_ferguson_values": [c_vp, c_i64, c_vp, c_int, c_vp, c_vp],
     "memi_markov_coo": [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_vp],
@@ -433,11 +434,11 @@ def markov_normalize(plan, D, mask, sigma, L, deg):
     I = plan.info
     _need(D.numel() >= I.d_elems and mask.numel() >= I.mask_words, "markov_normalize: D/mask")
     _need(sigma.dtype == torch.float64 and sigma.numel() >= I.n_pd, "markov_normalize: sigma")
-    _need(L.dtype == torch.float32 and L.numel() >= I.d_elems,
-          "markov_normalize: L must be float32 with d_elems elements")
+    _need(L.dtype in (torch.float32, torch.float64) and L.numel() >= I.d_elems,
+          "markov_normalize: L must be float32 or float64 with d_elems elements")
     _need(deg.dtype == torch.float64 and deg.numel() >= 2 * I.n_rows, "markov_normalize: deg")
     call("memi_markov_normalize", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask),
-         _p(sigma), _p(L), _p(deg), _stream())
+         _p(sigma), _p(L), int(L.dtype == torch.float64), _p(deg), _stream())
 
 
 def tanh_fit(x, y, p0, out, active=None, max_iter=400):

@@ -20,6 +20,7 @@ constexpr int TB = 128;  // tile side of the full-matrix tiles of the plan
 // blocks per 128 x 128 tile in the Ferguson and normalisation passes, each a quarter of the
 // tile (one block per tile left most of each CU idle for batches of few tiles)
 constexpr int MARKOV_RQ = 4;
+constexpr int FM_GP = 8;  // grid parameters per PD of the Ferguson moments (k_fer_grid)
 
 __device__ inline uint32_t float_key(float v) {
   const uint32_t u = __float_as_uint(v);
@@ -234,19 +235,24 @@ __global__ __launch_bounds__(256) void k_ferguson_sweep(const int32_t* __restric
                                                         const double* __restrict__ logEps,
                                                         int n_eps,
                                                         const double* __restrict__ thr_acc,
+                                                        int thr_stride,
+                                                        const double* __restrict__ gate,
                                                         double* __restrict__ out) {
   constexpr int SUB = 2048;
   __shared__ float vals[SUB];
   __shared__ int scan_sh[8];
   const int32_t* f = ftiles + 4 * (int64_t)blockIdx.y;
   const int pd = f[0];
+  // gate (the moments path's grid parameters): only PDs marked for the direct sweep
+  if (gate != nullptr && gate[FM_GP * (int64_t)pd + 7] == 0.0) return;
   const memi_pd_desc P = pds[pd];
   const int W = (P.n + 31) / 32;
   const bool full = P.k >= P.n;
   const int e_idx = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = e_idx < n_eps;
   // fergusonE.py:25-31: thr = max(-log(0.01 * ss / len), 10)
-  const double thr = fmax(-log(0.01 * thr_acc[2 * pd + 1] / thr_acc[2 * pd]), 10.0);
+  const double thr = fmax(-log(0.01 * thr_acc[(int64_t)thr_stride * pd + 1] /
+                              thr_acc[(int64_t)thr_stride * pd]), 10.0);
   const double c = active ? 1.0 / (2.0 * exp(logEps[e_idx])) : 0.0;
   const double L2E = 1.4426950408889634;
   double S = 0.0, SX = 0.0, CN = 0.0;
@@ -303,12 +309,15 @@ __global__ __launch_bounds__(256) void k_ferguson_sweep(const int32_t* __restric
 // u = (v - c_k) / h_k (m < FM_NM) give, for x = v / (2 eps) and t = h_k / (2 eps),
 //   sum exp(-x)   = exp(-c_k / (2 eps)) sum_m (-t)^m M_m / m!,
 //   sum x exp(-x) = exp(-c_k / (2 eps)) sum_m (-t)^m (c_k/(2 eps) M_m + t M_{m+1}) / m!,
-// truncated at m = 16: t <= thr * (e^{w/2} - 1) <= 0.8 leaves < 1e-14 relative.  s is the
-// largest of 1..8 that keeps the PD's value range within FM_BMAX buckets.  Zero entries
-// (v = 0, x = 0) count 1 each.  Results equal the direct sweep to ~1e-14 relative.
+// truncated at m = 16: the terms fall as t^m / m!, so t <= thr * (e^{w/2} - 1) <= FM_TMAX = 1
+// leaves < 3e-15 relative.  s is the largest of 1..8 that keeps the PD's value range within
+// FM_BMAX buckets.  Zero entries (v = 0, x = 0) count 1 each.  Results equal the direct
+// sweep to ~1e-14 relative.  A PD whose values span more than FM_BMAX buckets of w = dle (a
+// fine grid over many decades) or whose w gives t > FM_TMAX (a coarse grid) is marked in its
+// grid parameters and summed by the direct sweep instead, in the same launch sequence.
 constexpr int FM_BMAX = 544;  // buckets per PD: ln(3.4e38 / 1e-6) / 0.2 + 2 < 544
 constexpr int FM_NM = 18;     // power sums M_0 .. M_17
-constexpr int FM_GP = 8;      // grid parameters per PD
+constexpr double FM_TMAX = 1.0;
 
 __device__ inline unsigned long long dbits(double v) { return (unsigned long long)__double_as_longlong(v); }
 
@@ -408,13 +417,19 @@ __global__ void k_fer_grid(int64_t npd, const double* __restrict__ st, double le
     const double vmin = sp[3], vmax = sp[4];
     int B = 0, s = 1, b0 = 0;
     double w = dle, base = log(2.0 * thr) + le0;
+    bool direct = false;
     if (vmax > 0.0 && vmin <= vmax) {
       const double range = fmax(log(vmax) - log(vmin), 1e-300);
       s = (int)fmin(8.0, fmax(1.0, floor((FM_BMAX - 32) * dle / range)));
       w = dle / s;
-      b0 = fm_bucket(vmin, base, w);
-      B = fm_bucket(vmax, base, w) - b0 + 1;
-      B = min(max(B, 0), FM_BMAX);      // bounds the tables for any input (NaN, inf)
+      // buckets needed, in float64 before any int conversion (the +2 covers the edge buckets)
+      const double need = range / w + 2.0;
+      direct = !(need <= (double)FM_BMAX) || !(thr * (exp(0.5 * w) - 1.0) <= FM_TMAX);
+      if (!direct) {
+        b0 = fm_bucket(vmin, base, w);
+        B = fm_bucket(vmax, base, w) - b0 + 1;
+        B = min(max(B, 0), FM_BMAX);    // bounds the tables for any input (NaN, inf)
+      }
     }
     g[0] = thr;
     g[1] = w;
@@ -423,7 +438,7 @@ __global__ void k_fer_grid(int64_t npd, const double* __restrict__ st, double le
     g[4] = B;
     g[5] = s;
     g[6] = sp[2];
-    g[7] = 0.0;
+    g[7] = direct ? 1.0 : 0.0;          // 1: summed by the direct sweep
   }
 }
 
@@ -498,6 +513,11 @@ __global__ __launch_bounds__(256) void k_fer_sweep_moments(const double* __restr
   __syncthreads();
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_eps) return;
+  if (g[7] != 0.0) {                  // direct-sweep PD: cleared for its accumulation
+    double* o = out + ((int64_t)pd * n_eps + e) * 3;
+    o[0] = o[1] = o[2] = 0.0;
+    return;
+  }
   constexpr double invf[FM_NM - 1] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
                                       1.0 / 5040, 1.0 / 40320, 1.0 / 362880, 1.0 / 3628800,
                                       1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0,
@@ -564,13 +584,14 @@ __global__ __launch_bounds__(256) void k_markov_colsum(const int32_t* __restrict
   atomicAdd(&deg[(PASS == 0 ? 0 : n_rows) + P.row0 + j], s);
 }
 
+template <typename LT>
 __global__ __launch_bounds__(256) void k_markov_write(const int32_t* __restrict__ ftiles,
                                                       const memi_pd_desc* __restrict__ pds,
                                                       const float* __restrict__ D,
                                                       const uint32_t* __restrict__ mask,
                                                       const double* __restrict__ sigma,
                                                       const double* __restrict__ deg,
-                                                      int64_t n_rows, float* __restrict__ L) {
+                                                      int64_t n_rows, LT* __restrict__ L) {
   const int32_t* f = ftiles + 4 * (int64_t)blockIdx.x;
   const int pd = f[0];
   const memi_pd_desc P = pds[pd];
@@ -596,9 +617,9 @@ __global__ __launch_bounds__(256) void k_markov_write(const int32_t* __restrict_
     // slaplacianonFly.py:63-78: W/(d_i d_j), then /(d'_i d'_j), then |l + l^T| / 2
     const double lij = wij / (di * dj) / (dpi * dpj);
     const double lji = wji / (dj * di) / (dpj * dpi);
-    // stored in float32: the entries come from float32 distances, so a float64 copy would carry
-    // no more of the data and cost the Lanczos matvecs twice the bytes
-    L[P.d_off + (int64_t)i * P.n + j] = (float)(fabs(lij + lji) / 2.0);
+    // LT = double keeps the float64 kernel the eigensolve is specified on; LT = float halves
+    // the bytes of every Lanczos matvec at ~6e-8 relative rounding of each entry
+    L[P.d_off + (int64_t)i * P.n + j] = (LT)(fabs(lij + lji) / 2.0);
   }
 }
 
@@ -944,6 +965,8 @@ int memi_ferguson_sweep(const void* plan_dev, const memi_plan_info* info, const 
   // out holds npd*n_eps*3 results followed by 2*npd threshold sums (count, sum exp)
   MEMI_CHECK_ARG(plan_dev && info && D && mask && logEps && out && n_eps > 0,
                  "memi_ferguson_sweep: bad arguments");
+  MEMI_CHECK_ARG(info->n_ftiles < 65536, "memi_ferguson_sweep: too many tiles (%lld)",
+                 (long long)info->n_ftiles);
   if (info->n_pd == 0) return MEMI_OK;
   hipStream_t s = (hipStream_t)stream;
   PlanView v = view(plan_dev, info);
@@ -957,7 +980,7 @@ int memi_ferguson_sweep(const void* plan_dev, const memi_plan_info* info, const 
   MEMI_CHECK_LAUNCH("k_ferguson_thr");
   dim3 grid((unsigned)((n_eps + 255) / 256), (unsigned)info->n_ftiles);
   hipLaunchKernelGGL(k_ferguson_sweep, grid, dim3(256), 0, s, v.ftiles, v.pds, D, mask, logEps,
-                     n_eps, thr_acc, out);
+                     n_eps, thr_acc, 2, (const double*)nullptr, out);
   MEMI_CHECK_LAUNCH("k_ferguson_sweep");
   return MEMI_OK;
 }
@@ -974,7 +997,8 @@ int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, cons
   const int64_t need = npd * (6 + FM_GP + (int64_t)FM_BMAX * FM_NM);
   MEMI_CHECK_ARG(ws_doubles >= need, "memi_ferguson_moments: workspace %lld < %lld doubles",
                  (long long)ws_doubles, (long long)need);
-  MEMI_CHECK_ARG(npd < 65536 && n_eps <= (1 << 24), "memi_ferguson_moments: too many PDs/eps");
+  MEMI_CHECK_ARG(npd < 65536 && n_eps <= (1 << 24) && info->n_ftiles < 65536,
+                 "memi_ferguson_moments: too many PDs/eps/tiles");
   if (npd == 0) return MEMI_OK;
   hipStream_t s = (hipStream_t)stream;
   PlanView v = view(plan_dev, info);
@@ -999,6 +1023,13 @@ int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, cons
   hipLaunchKernelGGL(k_fer_sweep_moments, dim3((unsigned)((n_eps + 255) / 256), (unsigned)npd),
                      dim3(256), 0, s, logEps, n_eps, gp, M, out);
   MEMI_CHECK_LAUNCH("k_fer_sweep_moments");
+  // PDs whose value range or grid step the moments cannot carry (marked by k_fer_grid): the
+  // direct sweep, its blocks of every other PD exiting at once
+  hipLaunchKernelGGL(k_ferguson_sweep, dim3((unsigned)((n_eps + 255) / 256),
+                                            (unsigned)info->n_ftiles),
+                     dim3(256), 0, s, v.ftiles, v.pds, D, mask, logEps, n_eps, st, 6,
+                     (const double*)gp, out);
+  MEMI_CHECK_LAUNCH("k_ferguson_sweep");
   // the threshold sums after the results, as memi_ferguson_sweep leaves them
   hipLaunchKernelGGL(k_fer_thr_copy, dim3(grid_for(npd)), dim3(256), 0, s, npd, st,
                      out + npd * (int64_t)n_eps * 3);
@@ -1007,8 +1038,8 @@ int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, cons
 }
 
 int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, const float* D,
-                          const uint32_t* mask, const double* sigma, float* L, double* deg,
-                          void* stream) {
+                          const uint32_t* mask, const double* sigma, void* L, int l_is_f64,
+                          double* deg, void* stream) {
   MEMI_CHECK_ARG(plan_dev && info && D && mask && sigma && L && deg,
                  "memi_markov_normalize: null pointer");
   if (info->n_pd == 0) return MEMI_OK;
@@ -1024,8 +1055,12 @@ int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, cons
   hipLaunchKernelGGL(k_markov_colsum<1>, g, dim3(256), 0, s, v.ftiles, v.pds, D, mask, sigma,
                      deg, nr);
   MEMI_CHECK_LAUNCH("k_markov_colsum<1>");
-  hipLaunchKernelGGL(k_markov_write, g, dim3(256), 0, s, v.ftiles, v.pds, D, mask, sigma, deg,
-                     nr, L);
+  if (l_is_f64)
+    hipLaunchKernelGGL(k_markov_write<double>, g, dim3(256), 0, s, v.ftiles, v.pds, D, mask,
+                       sigma, deg, nr, (double*)L);
+  else
+    hipLaunchKernelGGL(k_markov_write<float>, g, dim3(256), 0, s, v.ftiles, v.pds, D, mask,
+                       sigma, deg, nr, (float*)L);
   MEMI_CHECK_LAUNCH("k_markov_write");
   return MEMI_OK;
 }

@@ -444,6 +444,7 @@ def lm_ritz(alpha, beta, k):
 
 
 LANCZOS = dict(graph=True, split=None, check_every=None)   # dm_embed's Lanczos settings
+LANCZOS_TOL = 1e-10         # residual target of lanczos_eigs, relative to max |theta|
 
 
 def _next_check_gap(j, res, thmax, active, tol, step, hist, slack=0.6, cap=4):
@@ -470,11 +471,11 @@ def _next_check_gap(j, res, thmax, active, tol, step, hist, slack=0.6, cap=4):
     return gap, (j, lr), last
 
 
-def lanczos_eigs(plan, L, ks, m=None, tol=1e-10, check_every=None, seed=20261202, split=None,
-                 check="gpu", graph=True, full=False):
-    """Largest-magnitude eigenpairs of every PD's float32 kernel block of L (sembeddingonFly.py
-    semantics: the k eigenpairs of largest |lambda|, sorted by descending lambda) by batched
-    Lanczos with full reorthogonalisation on the GPU.
+def lanczos_eigs(plan, L, ks, m=None, tol=LANCZOS_TOL, check_every=None, seed=20261202,
+                 split=None, check="gpu", graph=True, full=False):
+    """Largest-magnitude eigenpairs of every PD's kernel block of L (float64 or float32) with
+    sembeddingonFly.py semantics (the k eigenpairs of largest |lambda|, sorted by descending
+    lambda) by batched Lanczos with full reorthogonalisation on the GPU.
 
     Converged when every selected Ritz pair has residual |beta_J s_J| <= tol * max|theta| or
     the Krylov space is the whole space.  split: orthogonalise with the whole grid (default
@@ -787,6 +788,13 @@ class StageTimer:
         self._last = now
 
 
+# Storage type of the normalised kernel L that the Lanczos matvecs read.  float64 is the
+# reference's kernel (slaplacianonFly.py builds it in float64); float32 rounds each entry once
+# (~6e-8 relative), which moves the eigenvectors by about that over the eigengap, for half the
+# matvec bytes.
+KERNEL_DTYPE = torch.float64
+
+
 def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, mask=None,
              eig="lanczos", timer=None, fit="gpu"):
     """a10-a16 for every PD of the plan.  D: flat float32 device buffer.  Returns a list of
@@ -834,8 +842,9 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
     sigma = _to_device(sig, dev)
     # the kernel matrix buffer is kept on the plan so that the Lanczos graph stays valid
     L = getattr(plan, "_L", None)
-    if L is None or L.numel() < max(info.d_elems, 1) or L.device != dev:
-        L = torch.empty(max(info.d_elems, 1), dtype=torch.float32, device=dev)
+    if (L is None or L.numel() < max(info.d_elems, 1) or L.device != dev or
+            L.dtype != KERNEL_DTYPE):
+        L = torch.empty(max(info.d_elems, 1), dtype=KERNEL_DTYPE, device=dev)
         plan._L = L
     deg = torch.empty(2 * info.n_rows, dtype=torch.float64, device=dev)
     nat.markov_normalize(plan, D, mask, sigma, L, deg)
@@ -856,6 +865,12 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
         if eig == "lanczos":
             lam, V, r_steps, r_res = eigs[p]
             results[p].lanczos_steps, results[p].lanczos_residual = r_steps, r_res
+            if r_steps < n and r_res > LANCZOS_TOL * max(np.max(np.abs(lam)), 1e-300):
+                # the reference reports ARPACK's non-convergence and keeps the partial
+                # results (sembeddingonFly.py:28-32); so does this path
+                warnings.warn(f"Lanczos did not converge for PD {p} (n={n}) in {r_steps} "
+                              f"steps: largest Ritz residual {r_res:.3e}; keeping the partial "
+                              "results", RuntimeWarning, stacklevel=2)
         else:
             lam, V = top_eigs_lm(plan.block(L, p).double(), nEigs + 1)
             lam, V = lam.cpu().numpy(), V.cpu().numpy()

@@ -52,13 +52,14 @@ class NLSAResult:
 
 def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tune=3,
          num_eigs=15, msk2=None, logEps=pl.LOG_EPS, rand=None, snr=5.0, want_imgt=True,
-         keep=False, timer=None):
+         keep=False, timer=None, k=None):
     """One psi of psiAnalysisParS2 / NLSA.op.
 
     D: (n, n) float32/float64 device distances of the PD (posPath order); ord_d: int64 device
     sort order along psi (PosPsi1), so DD = D[ord][:, ord]; img_idx: int64 device indices of
     the images in psi order into imgAll / ctf_half (posPath[PosPsi1]).  timer: an optional
     pipeline.StageTimer that gets one mark per stage (it synchronises the device at each).
+    k: neighbours kept in the embedding of ConD (NLSAPar['k'], NLSA.py:45); None = all m.
     """
     mark = timer.mark if timer is not None else (lambda name: None)
     if timer is not None:
@@ -75,7 +76,7 @@ def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tu
     nat.nlsa_condist(D.reshape(-1), ord_d, num, ConOrder, ConD)
     mark("condist")
     # a20
-    plan = _plan(m, m, dev)
+    plan = _plan(m, m if k is None else max(1, min(int(k), m)), dev)
     sub = pl.StageTimer() if timer is not None else None
     dm1 = pl.dm_embed(plan, ConD.float(), tune, num_eigs, logEps, rand=rand, timer=sub)[0]
     psiC, mu = dm1.psi, dm1.mu

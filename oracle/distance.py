@@ -156,6 +156,117 @@ def direct_distance(fy, CTF):
     return D
 
 
+def pd_geometry(q):
+    """``getDistanceCTF...:297-301`` and ``:206-213``: per-image PD vectors, the normalised
+    PD sum and the in-plane reference angle psi_p (degrees)."""
+    nS = q.shape[1]
+    PDs = calc_avg_pd(q, nS)
+    PD = np.sum(PDs, 1)
+    PD = PD / np.linalg.norm(PD)
+    return PDs, PD, psi_ang(PD)
+
+
+def prep_rows(input_data, filterPar, raw, nStot, params, rows, geometry=None, msk2=1):
+    """a1-a6 of ``getDistanceCTF...:244-349`` for the images ``rows`` of one PD only (every
+    image is processed on its own in the reference; only PD and psi_p use all of q).
+
+    Returns a dict of per-row arrays: imgAll, fy, CTF (N x N, fftshift-free), imgAllFlip,
+    Psis, Nom, Dnom, imgLabels."""
+    ind, q, df = input_data[0], input_data[1], input_data[2]
+    N = params["nPix"]
+    N2 = N / 2.0
+    PDs, PD, psi_p = pd_geometry(q) if geometry is None else geometry
+    msk = annular_mask(0, N2, N, N)
+    Q = create_grid(N, N2)
+    G = ifftshift(create_filter(filterPar["type"], filterPar["N"], filterPar["Qc"], Q))
+    rows = np.asarray(rows, dtype=np.int64)
+    m = len(rows)
+    out = dict(imgAll=np.zeros((m, N, N)), fy=np.zeros((m, N, N), dtype=complex),
+               CTF=np.zeros((m, N, N)), imgAllFlip=np.zeros((m, N, N)),
+               Psis=np.zeros(m), Nom=np.zeros(m), Dnom=np.zeros(m),
+               imgLabels=np.zeros(m, dtype=int))
+    for r, iS in enumerate(rows):
+        conj = not (ind[iS] < nStot / 2)
+        src = int(ind[iS] - nStot / 2) if conj else int(ind[iS])
+        out["imgLabels"][r] = -1 if conj else 1
+        y = load_normalized(raw, src, conj, msk)
+        y = ifft2(fft2(y) * G).real
+        Psi, s, c = get_psi(q, PD, iS)
+        if np.isnan(Psi):
+            Psi = 0
+        out["Psis"][r], out["Nom"][r], out["Dnom"][r] = Psi, s, c
+        img = y * msk
+        img = rotatefill(img, -(180 / math.pi) * Psi)
+        img = rotatefill(img, -psi_p)
+        C = ifftshift(ctf(Q / (2 * params["pix_size"]),
+                          [params["Cs"], df[iS], params["EkV"], params["gaussEnv"],
+                           params["AmpContrast"]]))
+        out["CTF"][r] = C
+        out["fy"][r] = fft2(img * msk2)
+        out["imgAllFlip"][r] = phase_flip(out["fy"][r], C)
+        out["imgAll"][r] = img
+    return out
+
+
+def prep_rows_parallel(input_data, filterPar, raw, nStot, params, workers=8, msk2=1):
+    """``prep_rows`` over all images of one PD, split across `workers` spawned processes, each
+    given only the stack images its rows read.  Same values as the serial loop."""
+    import multiprocessing as mp
+    ind, q, df = input_data[0], input_data[1], input_data[2]
+    n = len(ind)
+    geometry = pd_geometry(q)
+    chunks = [c for c in np.array_split(np.arange(n), max(1, workers)) if len(c)]
+    jobs = []
+    for c in chunks:
+        conj = ~(ind[c] < nStot / 2)
+        src = np.where(conj, ind[c] - nStot // 2, ind[c]).astype(np.int64)
+        uniq, inv = np.unique(src, return_inverse=True)
+        sub = np.ascontiguousarray(raw[uniq])
+        m = len(uniq)
+        # local stack of m images, local nStot = 2m: conjugates index m + local
+        loc = np.where(conj, inv + m, inv)
+        ind_l = np.zeros(n, dtype=np.int64)
+        ind_l[c] = loc
+        jobs.append((ind_l, q, df, filterPar, sub, 2 * m, params, c, geometry, msk2))
+    # spawned workers (the caller may hold a GPU context, which a fork would copy); an
+    # executor, not a Pool, so a worker that dies raises instead of being replaced forever
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(len(jobs), mp_context=mp.get_context("spawn")) as ex:
+        parts = list(ex.map(_prep_job, jobs))
+    out = {}
+    for k in parts[0]:
+        out[k] = np.concatenate([p[k] for p in parts])
+    return out, geometry
+
+
+def _prep_job(args):
+    ind_l, q, df, filterPar, sub, nStot_l, params, rows, geometry, msk2 = args
+    return prep_rows([ind_l, q, df], filterPar, sub, nStot_l, params, rows, geometry, msk2)
+
+
+def averages(imgAll, imgAllFlip, CTF, msk2=1, snr=5.0):
+    """a6-a7 outputs (``getDistanceCTF...:348``, ``:352-367``, ``:400``, ``:422-430``):
+    the Wiener average imgAvg with W = -(sum_i C_i^2 + 1/SNR) (SNR = 5, note the sign), the
+    phase-flipped average imgAvgFlip and imgAllIntensity = mean(imgAllFlip^2)."""
+    nS, N = imgAll.shape[0], imgAll.shape[1]
+    imgAvgFlip = np.zeros((N, N))
+    for iS in range(nS):
+        imgAvgFlip = imgAvgFlip + imgAllFlip[iS]
+    wiener_dom = -(np.sum(CTF ** 2, axis=0) + 1.0 / snr)
+    imgAvg = 0
+    for iS in range(nS):
+        imgAvg = imgAvg + ifft2(fft2(imgAll[iS]) * (CTF[iS] / wiener_dom)).real
+    imgAvg = imgAvg * msk2 / nS
+    imgAvgFlip = imgAvgFlip.real * msk2 / nS
+    imgAllIntensity = np.mean(imgAllFlip ** 2, axis=0)
+    return imgAvg, imgAvgFlip, imgAllIntensity
+
+
+def phase_flip(fy, C):
+    """``getDistanceCTF...:346``: real(ifft2(sign(C) fy))."""
+    return ifft2(np.sign(C) * fy).real
+
+
 def op(input_data, filterPar, raw, nStot, options, params, msk2=1):
     """``getDistanceCTF_local_Conj9combinedS2.op`` (``:216-420``) returning the 19-key dict
     it pickles instead of writing it.
@@ -166,66 +277,16 @@ def op(input_data, filterPar, raw, nStot, options, params, msk2=1):
     ind, q, df = input_data[0], input_data[1], input_data[2]
     nS = ind.shape[0]
     N = params["nPix"]
-    N2 = N / 2.0
-    Psis = np.nan * np.ones((nS, 1))
-    Nom = np.nan * np.ones((nS, 1))
-    Dnom = np.nan * np.ones((nS, 1))
-    imgAvgFlip = np.zeros((N, N))
-    imgAllFlip = np.zeros((nS, N, N))
-    imgAll = np.zeros((nS, N, N))
-    fy = np.zeros((nS, N, N), dtype=complex)
-    CTF = np.zeros((nS, N, N))
-    msk = annular_mask(0, N2, N, N)
-
-    imgLabels = np.zeros(nS, dtype=int)
-    y = np.zeros((nS, N, N))
-    for iS in range(nS):
-        conj = not (ind[iS] < nStot / 2)
-        src = int(ind[iS] - nStot / 2) if conj else int(ind[iS])
-        imgLabels[iS] = -1 if conj else 1
-        y[iS] = load_normalized(raw, src, conj, msk)
-
-    Q = create_grid(N, N2)
-    G = ifftshift(create_filter(filterPar["type"], filterPar["N"], filterPar["Qc"], Q))
-    for iS in range(nS):
-        y[iS] = ifft2(fft2(y[iS]) * G).real
-
-    PDs = calc_avg_pd(q, nS)
-    PD = np.sum(PDs, 1)
-    PD = PD / np.linalg.norm(PD)
-    psi_p = psi_ang(PD)
-    for iS in range(nS):
-        Psi, s, c = get_psi(q, PD, iS)
-        if np.isnan(Psi):
-            Psi = 0
-        Psis[iS] = Psi
-        Dnom[iS] = c
-        Nom[iS] = s
-        img = y[iS] * msk
-        img = rotatefill(img, -(180 / math.pi) * Psi)
-        img = rotatefill(img, -psi_p)
-        CTF[iS] = ifftshift(ctf(Q / (2 * params["pix_size"]),
-                                [params["Cs"], df[iS], params["EkV"], params["gaussEnv"],
-                                 params["AmpContrast"]]))
-        fy[iS] = fft2(img * msk2)
-        imgFlip = ifft2(np.sign(CTF[iS]) * fy[iS])
-        imgAllFlip[iS] = imgFlip.real
-        imgAvgFlip = imgAvgFlip + imgFlip.real
-        imgAll[iS] = img
-
-    wiener_dom = -(np.sum(CTF ** 2, axis=0) + 1.0 / 5)
-    imgAvg = 0
-    for iS in range(nS):
-        imgAvg = imgAvg + ifft2(fft2(imgAll[iS]) * (CTF[iS] / wiener_dom)).real
-    imgAvg = imgAvg * msk2 / nS
-    imgAvgFlip = imgAvgFlip.real * msk2 / nS
-
+    PDs, PD, psi_p = geometry = pd_geometry(q)
+    pr = prep_rows(input_data, filterPar, raw, nStot, params, np.arange(nS), geometry, msk2)
+    imgAll, fy, CTF, imgAllFlip = pr["imgAll"], pr["fy"], pr["CTF"], pr["imgAllFlip"]
+    imgAvg, imgAvgFlip, imgAllIntensity = averages(imgAll, imgAllFlip, CTF, msk2)
     D = np.zeros((nS, nS))
     if not options.get("avgOnly", False):
         D = gram_distance(fy, CTF)
-    imgAllIntensity = np.mean(imgAllFlip ** 2, axis=0)
     return dict(D=D, ind=ind, q=q, df=df, CTF=CTF.reshape(nS, N * N), imgAll=imgAll, msk2=msk2,
-                PD=PD, PDs=PDs, Psis=Psis, imgAvg=imgAvg, imgAvgFlip=imgAvgFlip,
-                imgAllFlip=imgAllFlip, imgLabels=imgLabels, Dnom=Dnom, Nom=Nom,
+                PD=PD, PDs=PDs, Psis=pr["Psis"].reshape(nS, 1), imgAvg=imgAvg,
+                imgAvgFlip=imgAvgFlip, imgAllFlip=imgAllFlip, imgLabels=pr["imgLabels"],
+                Dnom=pr["Dnom"].reshape(nS, 1), Nom=pr["Nom"].reshape(nS, 1),
                 imgAllIntensity=imgAllIntensity, version="getDistanceCTF_local9, V 1.0",
                 options=options, fy=fy, psi_p=psi_p)

@@ -178,8 +178,10 @@ def test_gpu_tanh_fit_matches_curve_fit(cuda):
 def test_ferguson_moments_match_direct_sweep(cuda):
     """Bucket-moment sums (memi_ferguson_moments) against the direct sweep over every entry at
     every eps, for a batch of PDs with kNN and full kernels, duplicate points (zero entries)
-    and value spreads from one to eight decades, on the default 1501-point grid and a coarse
-    one."""
+    and value spreads from one to eight decades, on the default 1501-point grid, a coarser
+    one, a fine one (dle = 0.01: eight decades need more buckets than the moments carry) and a
+    very coarse one (dle = 2: too coarse for the series); the last two go to the direct sweep
+    inside memi_ferguson_moments."""
     import torch
     from manifoldem_python_amd import _native as nat, pipeline as pl
     ns, ks = [150, 400, 257, 90], [150, 60, 257, 30]
@@ -191,7 +193,8 @@ def test_ferguson_moments_match_direct_sweep(cuda):
     flat = torch.as_tensor(np.concatenate([d.astype(np.float32).ravel() for d in D]),
                            device=cuda)
     mask = pl.knn_mask(plan, flat)
-    for le in (pl.LOG_EPS, np.arange(-40, 40.5, 0.5)):
+    for le in (pl.LOG_EPS, np.arange(-40, 40.5, 0.5), np.arange(-60, 60.005, 0.01),
+               np.arange(-150, 151, 2.0)):
         S1, sl1, c1 = pl.ferguson_sums(plan, flat, mask, le, method="moments")
         S0, sl0, c0 = pl.ferguson_sums(plan, flat, mask, le, method="direct")
         np.testing.assert_array_equal(c1, c0)
