@@ -160,18 +160,25 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False
 
 
 def issued_gram_flops(plan, Khp, products):
-    """MFMA flops the Gram kernel issues: every upper tile's 16-row/16-column blocks that hold
-    data (edge blocks beyond a PD's size are skipped), 4 segments of Khp, `products` MFMA
-    products per block (3 for split-BF16, 1 for FP32)."""
+    """MFMA flops the Gram kernel issues: for every upper tile of the plan's tile side, the
+    16 x 16 blocks that hold data (edge blocks beyond a PD's size and, on diagonal tiles,
+    blocks wholly below the diagonal are skipped), 4 segments of Khp, `products` MFMA products
+    per block (3 for split-BF16, 1 for FP32)."""
+    bt = int(plan.gram_tile)
     tot = 0
     for n in plan.ns:
-        T = -(-int(n) // 128)
-        m = [min(128, int(n) - 128 * a) for a in range(T)]
-        b = [-(-x // 16) * 16 for x in m]
+        nb = -(-int(n) // 16)                    # 16-blocks per side holding data
+        per = bt // 16
+        T = -(-int(n) // bt)
         for a in range(T):
+            ra = range(a * per, min((a + 1) * per, nb))
             for c in range(a, T):
-                tot += b[a] * b[c]
-    return float(tot) * 2.0 * 4 * Khp * products
+                rc = range(c * per, min((c + 1) * per, nb))
+                if a == c:
+                    tot += sum(1 for i in ra for j in rc if i <= j)
+                else:
+                    tot += len(ra) * len(rc)
+    return float(tot) * 256 * 2.0 * 4 * Khp * products
 
 
 def fp64_error(batch, wsp, dist):

@@ -59,8 +59,8 @@ typedef struct memi_pd_desc {
 /* Sizes and table offsets of a batch plan (memi_plan). */
 typedef struct memi_plan_info {
   int64_t n_pd;
-  int64_t n_items;     /* Gram work items: (PD, tile i, tile j, segment, chunk)          */
-  int64_t n_utiles;    /* 128x128 tiles of the upper triangles                          */
+  int64_t n_items;     /* Gram work items: (PD, tile i, tile j, half, chunk)             */
+  int64_t n_utiles;    /* gram_tile^2 tiles of the upper triangles                      */
   int64_t n_ftiles;    /* 128x128 tiles of the full matrices                            */
   int64_t n_rows;      /* sum of n                                                      */
   int64_t off_pds, off_items, off_utiles, off_ftiles, off_rows;  /* byte offsets         */
@@ -70,8 +70,10 @@ typedef struct memi_plan_info {
   int64_t list_elems;  /* sum min(k,n)*n: kNN lists                                     */
   int64_t ws_floats;   /* Gram partial-sum workspace (float32)                          */
   int64_t flag_cap;    /* sum n(n-1)/2: capacity of the recompute list (int32 triplets) */
-  int32_t chunks;      /* K chunks per operand segment                                  */
+  int32_t chunks;      /* K chunks per half (P or R: two operand segments)               */
   int32_t max_n;
+  int32_t gram_tile;   /* side of the Gram's output tiles (128 or 256)                  */
+  int32_t reserved;
 } memi_plan_info;
 
 typedef struct memi_ctf_params {
@@ -165,28 +167,31 @@ int memi_image_moments(const float* img, int64_t n, int64_t npix, double* sum, d
 /* ---------------- Stage 1: distance Gram (getDistanceCTF...:391-397) --------------------- */
 
 /* Batch plan (host only).  From pds_in (row0, n, k per PD; rows packed contiguously, i.e.
- * row0 of PD p = sum of n over PDs before p) compute the table sizes into
- * *info; when plan_host is non-NULL (capacity plan_cap bytes >= info->plan_bytes) also write
- * the tables: PD descriptors with their offsets filled in, Gram work items over the 128x128
- * upper-triangle tiles of every PD x 4 operand segments x `chunks`, the upper tiles, the full
- * tiles and the row list.  The caller copies the plan to device memory once and passes it to
- * every batched entry point below (plan_dev). */
-int memi_plan(const memi_pd_desc* pds_in_host, int npd, int chunks, void* plan_host,
-              int64_t plan_cap, memi_plan_info* info);
+ * row0 of PD p = sum of n over PDs before p) compute the table sizes into *info; when
+ * plan_host is non-NULL (capacity plan_cap bytes >= info->plan_bytes) also write the tables:
+ * PD descriptors with their offsets filled in, Gram work items over the gram_tile x gram_tile
+ * upper-triangle tiles of every PD x 2 halves (P, R) x `chunks`, those upper tiles, the
+ * 128 x 128 full tiles of the Stage-2 kernels and the row list.  gram_tile: 128 or 256.  The
+ * caller copies the plan to device memory once and passes it to every batched entry point
+ * below (plan_dev). */
+int memi_plan(const memi_pd_desc* pds_in_host, int npd, int chunks, int gram_tile,
+              void* plan_host, int64_t plan_cap, memi_plan_info* info);
 
 /* CTF-weighted distances D_ij = sum_k |C_j X_ik - C_i X_jk|^2 for every PD of the plan,
  * getDistanceCTF_local_Conj9combinedS2.py:391-397 (dead direct form :106-122).  MFMA Gram of
- * the packed operand into per-segment partials (ws, info->ws_floats floats), float64
- * reduction D = P - 2R with P = S + S^T the |X|^2 C^2 terms and R the cross term, exact zero
- * diagonal, and float32 direct-form recomputation of entries whose cancellation ratio
- * (P + 2|R|)/D exceeds `kappa` (flags: 3*info->flag_cap int32; *flag_count: int32 on the
- * device, zeroed here, gets the number flagged).  D: float32 [info->d_elems].  fy/ctf_half:
- * the spectra and CTFs that produced the operand (rows indexed like the operand).
- * The K range of each segment (Khp elements, 64-byte steps) is split into info->chunks
- * chunks of ceil(steps / chunks) steps, the last one shorter; a chunk count that leaves an
- * empty chunk is rejected.
- * variant: Gram kernel schedule, 0 = default, 1 = prefetch one K step (two workgroups per
- * CU), 2 = prefetch two K steps (two workgroups per CU as well).
+ * the packed operand into partial tiles of its two halves (ws, info->ws_floats floats),
+ * P = [A B].[B A]^T (the |X|^2 C^2 terms) and R = [Yr Yi].[Yr Yi]^T (the cross term), float64
+ * reduction D = P - 2R, exact zero diagonal, and float32 direct-form recomputation of entries
+ * whose cancellation ratio (P + 2|R|)/D exceeds `kappa` (flags: 3*info->flag_cap int32;
+ * *flag_count: int32 on the device, zeroed here, gets the number flagged).  D: float32
+ * [info->d_elems].  fy/ctf_half: the spectra and CTFs that produced the operand (rows indexed
+ * like the operand).
+ * Each half's K range (two segments of Khp elements, 64-byte steps) is split into
+ * info->chunks chunks of ceil(2 steps / chunks) steps, the last one shorter; a chunk count
+ * that leaves an empty chunk is rejected.
+ * variant: Gram kernel schedule, 0 = default; for 128-wide tiles (two workgroups per CU)
+ * 1 = prefetch one K step, 2 = prefetch two (the default); 256-wide tiles (one workgroup per
+ * CU, 128 x 128 per wave) prefetch one (variant 0 or 1).
  * ev_start / ev_stop: optional hipEvent_t recorded on `stream` right before and after the
  * MFMA Gram kernel (NULL: not recorded), for timing that kernel alone. */
 int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const void* op_hi,

@@ -39,7 +39,8 @@ class PlanInfo(ctypes.Structure):
                 ("off_utiles", c_i64), ("off_ftiles", c_i64), ("off_rows", c_i64),
                 ("plan_bytes", c_i64), ("d_elems", c_i64), ("mask_words", c_i64),
                 ("list_elems", c_i64), ("ws_floats", c_i64), ("flag_cap", c_i64),
-                ("chunks", c_i32), ("max_n", c_i32)]
+                ("chunks", c_i32), ("max_n", c_i32), ("gram_tile", c_i32),
+                ("reserved", c_i32)]
 
 
 class CtfParams(ctypes.Structure):
@@ -62,7 +63,7 @@ _SIGS = {
     "memi_phase_flip": [c_vp, c_vp, c_i64, c_int, c_vp, c_vp],
     "memi_wiener_average": [c_vp, c_vp, c_i64, c_int, c_dbl, c_vp, c_vp],
     "memi_image_moments": [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp],
-    "memi_plan": [c_vp, c_int, c_int, c_vp, c_i64, ctypes.POINTER(PlanInfo)],
+    "memi_plan": [c_vp, c_int, c_int, c_int, c_vp, c_i64, ctypes.POINTER(PlanInfo)],
     "memi_ctf_distance": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_i64, c_int, c_dbl, c_vp,
                           c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp],
     "memi_knn_select": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp],
@@ -305,7 +306,7 @@ def image_moments(img, s, s2):
 class Plan:
     """Host-built batch plan (memi_plan) uploaded once to the device."""
 
-    def __init__(self, rows0, ns, ks, chunks=1, device="cuda"):
+    def __init__(self, rows0, ns, ks, chunks=1, device="cuda", gram_tile=128):
         npd = len(ns)
         descs = (PdDesc * max(npd, 1))()
         for p in range(npd):
@@ -313,9 +314,10 @@ class Plan:
             descs[p].n = int(ns[p])
             descs[p].k = int(ks[p])
         info = PlanInfo()
-        call("memi_plan", ctypes.cast(descs, c_vp), npd, chunks, None, 0, ctypes.byref(info))
+        call("memi_plan", ctypes.cast(descs, c_vp), npd, chunks, int(gram_tile), None, 0,
+             ctypes.byref(info))
         host = np.zeros(max(info.plan_bytes, 16), dtype=np.uint8)
-        call("memi_plan", ctypes.cast(descs, c_vp), npd, chunks,
+        call("memi_plan", ctypes.cast(descs, c_vp), npd, chunks, int(gram_tile),
              c_vp(host.ctypes.data), host.nbytes, ctypes.byref(info))
         self.info = info
         self.host = host
@@ -329,6 +331,7 @@ class Plan:
         self.ks = np.asarray(ks, dtype=np.int64)
         self.rows0 = np.asarray(rows0, dtype=np.int64)
         self.chunks = chunks
+        self.gram_tile = int(gram_tile)
 
     @property
     def npd(self):

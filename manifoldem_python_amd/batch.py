@@ -69,11 +69,12 @@ def make_batch(stack, pds, N, ctf, filterPar, device):
 class Workspace:
     """Buffers reused across steps of the same batch and precision."""
 
-    def __init__(self, batch, precision, chunk_rows=16384):
+    def __init__(self, batch, precision, chunk_rows=16384, gram_tile=None, chunks=None):
         dev = batch.stack.device
         self.precision = precision
-        self.plan = nat.Plan(batch.rows0, batch.ns, batch.ks, chunks=pl.CHUNKS[precision],
-                             device=dev)
+        self.plan = nat.Plan(batch.rows0, batch.ns, batch.ks,
+                             chunks=pl.CHUNKS[precision] if chunks is None else chunks,
+                             device=dev, gram_tile=gram_tile or pl.GRAM_TILE)
         self.operand = pl.Operand(batch.rows, batch.N, precision, dev)
         Nh = batch.N // 2 + 1
         self.fy = torch.empty((batch.rows, batch.N, Nh), dtype=torch.complex64, device=dev)

@@ -6,35 +6,37 @@
 //
 // Formulation.  With the packed operand row of image i
 //     L_i = [ A_i | B_i | Yr_i | Yi_i ],  A = w|X|^2, B = C^2, Y = sqrt(w) C X
-// (half spectrum, column weights w folded in), the four segment products
-//     S1 = A B^T,  S2 = B A^T,  R1 = Yr Yr^T,  R2 = Yi Yi^T
-// give P = S1 + S2 and R = R1 + R2, and D = P - 2R.  Each (tile, segment, chunk) is an
-// independent work item: an MFMA GEMM over one K range into its own float32 partial tile,
-// so every accumulator chain is short and the float64 finalize sees P and R separately,
-// which also yields the cancellation ratio kappa = (P + 2|R|)/D per entry.
+// (half spectrum, column weights w folded in), the two halves of the contraction
+//     P = [A B] . [B A]^T      (segments 0, 1 against 1, 0)
+//     R = [Yr Yi] . [Yr Yi]^T  (segments 2, 3 against themselves)
+// give D = P - 2R.  A work item is one (tile, half, K chunk): an MFMA GEMM over its K range
+// into its own float32 partial tile, so every accumulator chain is short and the float64
+// finalize sees P and R separately, which also yields the cancellation ratio
+// kappa = (P + 2|R|)/D per entry.
 //
-// Tiles: 128x128 outputs per 256-thread workgroup (4 waves, 2x2, 64x64 each as 4x4 blocks of
-// 16x16 MFMA outputs).  K steps of 64 bytes per operand row are staged through LDS, double
-// buffered, 16-byte chunks XOR-swizzled per row.  Split-BF16 issues hi*hi, hi*lo and lo*hi
-// v_mfma_f32_16x16x32_bf16 per block; FP32 issues four v_mfma_f32_16x16x4_f32.  Both
-// operands of an MFMA read the same k slots of their lanes, so the sum over a block's k range
-// is exact whatever order the hardware uses inside an instruction.
+// Tiles: BT x BT outputs per workgroup (BT = 128: 4 waves of 64 x 64, two workgroups per CU;
+// BT = 256: 8 waves of 128 x 64, one workgroup per CU, 3/4 of the operand bytes staged per
+// MFMA).  K steps of 64 bytes per operand row are staged through LDS, double buffered,
+// 16-byte chunks XOR-swizzled per row.  Split-BF16 issues hi*hi, hi*lo and lo*hi
+// v_mfma_f32_16x16x32_bf16 per block; FP32 issues four v_mfma_f32_16x16x4_f32.  Both operands
+// of an MFMA read the same k slots of their lanes, so the sum over a block's k range is exact
+// whatever order the hardware uses inside an instruction.  On a diagonal tile the blocks
+// wholly below the diagonal are skipped (the finalize reads the upper triangle only).
 #include "common.h"
 
 namespace memi {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));   // 16 staged bytes
 
-constexpr int BM = 128;
-constexpr int BN = 128;
+constexpr int TB_FIN = 2048;           // tile entries per finalize workgroup
 constexpr int ROW_STEP = 64;           // bytes of one operand row per K step
 constexpr int LDS_ROW = 64;            // LDS row stride (bytes); 16-byte chunks swizzled
-constexpr int LDS_TILE = BM * LDS_ROW; // 10240 bytes per operand tile
 
 // LDS placement of 16-byte chunk c of tile row r: chunk c ^ swz(r), with swz's two bits
-// row bit 2 and row bits 1 ^ 3 -- conflict-free for the 16-row fragment reads and the 4-lane
-// row stores (measured 2-5% faster than rows padded to 80 bytes)
+// row bit 2 and row bits 1 ^ 3 -- conflict-free for the 16-row fragment reads (measured 2-5%
+// faster than rows padded to 80 bytes).  Rows r and r + 64q get the same swizzle.
 __device__ inline int swz(int r) { return ((r >> 2) & 1) | ((((r >> 1) ^ (r >> 3)) & 1) << 1); }
 
 // segment of the right operand that pairs with left segment s: A<->B, Yr, Yi
@@ -50,209 +52,276 @@ struct GramCfg<MEMI_PREC_FP32> {
 template <>
 struct GramCfg<MEMI_PREC_SPLIT_BF16> {
   static constexpr int ESIZE = 2;
-  static constexpr int NARR = 4;  // A hi, A lo, B hi, B lo
+  static constexpr int NARR = 4;  // A hi, B hi, A lo, B lo
 };
 
-// workgroups per CU every variant is built for (depth-2 split-BF16 fits in 253 VGPRs)
-template <int PREC, int DEPTH>
-struct GramOcc {
-  static constexpr int value = 2;
+// Workgroup shapes: WAVES_M x WAVES_N waves, each computing WM x WN outputs as
+// (WM/16) x (WN/16) blocks of 16x16 MFMA outputs; the workgroup tile is BT x BT.
+//   Shape128: 2 x 2 waves of 64 x 64 (256 threads, two workgroups per CU)
+//   Shape256: 2 x 4 waves of 128 x 64 (512 threads, one workgroup per CU, two waves per SIMD)
+template <int WM_, int WN_, int WAVES_M_, int WAVES_N_, int OCC_>
+struct GramShape {
+  static constexpr int WM = WM_, WN = WN_, WAVES_M = WAVES_M_, WAVES_N = WAVES_N_;
+  static constexpr int NT = 64 * WAVES_M * WAVES_N;       // threads
+  static constexpr int BT = WAVES_M * WM;                 // tile side
+  static constexpr int NBI = WM / 16, NBJ = WN / 16;      // MFMA blocks per wave
+  static constexpr int RPT = BT / (NT / 4);               // staged rows per thread per array
+  static constexpr int OCC = OCC_;                        // workgroups per CU
+  static_assert(WAVES_N * WN == BT, "square workgroup tile");
+};
+using Shape128 = GramShape<64, 64, 2, 2, 2>;
+using Shape256 = GramShape<128, 64, 2, 4, 1>;
+
+// register staging of one K step: NA arrays x RPT rows of 16 bytes per thread
+template <int NA, int RPT>
+struct GramStage {
+  u32x4_t v[NA][RPT];
 };
 
-template <int PREC, int DEPTH>
-__global__ __launch_bounds__(256, (GramOcc<PREC, DEPTH>::value)) void k_gram_tiles(
+template <int NA, int RPT>
+__device__ __forceinline__ void gram_gload(GramStage<NA, RPT>& R, const char* gAh,
+                                           const char* gBh, const char* gAl, const char* gBl,
+                                           const uint32_t (&oA)[RPT], const uint32_t (&oB)[RPT],
+                                           uint32_t ao, uint32_t bo) {
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    R.v[0][q] = *(const u32x4_t*)(gAh + (oA[q] + ao));
+    R.v[1][q] = *(const u32x4_t*)(gBh + (oB[q] + bo));
+    if constexpr (NA == 4) {
+      R.v[2][q] = *(const u32x4_t*)(gAl + (oA[q] + ao));
+      R.v[3][q] = *(const u32x4_t*)(gBl + (oB[q] + bo));
+    }
+  }
+}
+
+// LDS array order: A hi, B hi (FP32: A, B) then A lo, B lo; row q of a thread is
+// ld_row0 + q * RSTRIDE
+template <int NA, int RPT, int TILE, int RSTRIDE>
+__device__ __forceinline__ void gram_lstore(const GramStage<NA, RPT>& R, char* b) {
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+      *(u32x4_t*)(b + a * TILE + q * RSTRIDE * LDS_ROW) = R.v[a][q];
+}
+
+// MFMAs of one K step for this wave's WM x WN outputs.  G: guarded blocks (inside the PD:
+// nbi x nbj of the wave's blocks hold data; on a diagonal tile, not wholly below the
+// diagonal); full off-diagonal tiles run unguarded.  Split-BF16 keeps the B hi / lo fragments
+// of JB column blocks resident and streams A hi / lo one 16-row block at a time; each
+// accumulator gets hi*hi, hi*lo, lo*hi.
+template <int PREC, class S, int TILE, bool G>
+__device__ __forceinline__ void gram_compute(floatx4_t (&acc)[S::NBI][S::NBJ],
+                                             const char* base, int frag_off, int wr, int wc,
+                                             int nbi, int nbj, bool diag) {
+  constexpr int NBI = S::NBI, NBJ = S::NBJ, WM = S::WM, WN = S::WN;
+  if constexpr (PREC == MEMI_PREC_SPLIT_BF16) {
+    constexpr int JB = NBJ > 4 ? 4 : NBJ;    // column blocks per pass (bounds live fragments)
+#pragma unroll
+    for (int j0 = 0; j0 < NBJ; j0 += JB) {
+      bf16x8_t fb[JB], fc[JB];
+#pragma unroll
+      for (int j = 0; j < JB; ++j) {
+        const int c = (wc * WN + (j0 + j) * 16) * LDS_ROW + frag_off;
+        fb[j] = *(const bf16x8_t*)(base + 1 * TILE + c);
+        fc[j] = *(const bf16x8_t*)(base + 3 * TILE + c);
+      }
+#pragma unroll
+      for (int i = 0; i < NBI; ++i) {
+        const int r = (wr * WM + i * 16) * LDS_ROW + frag_off;
+        const bf16x8_t ah = *(const bf16x8_t*)(base + 0 * TILE + r);
+        const bf16x8_t al = *(const bf16x8_t*)(base + 2 * TILE + r);
+#pragma unroll
+        for (int j = 0; j < JB; ++j) {
+          const int jj = j0 + j;
+          if (!G || (i < nbi && jj < nbj && !(diag && wr * NBI + i > wc * NBJ + jj))) {
+            acc[i][jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, fb[j], acc[i][jj], 0, 0, 0);
+            acc[i][jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, fc[j], acc[i][jj], 0, 0, 0);
+            acc[i][jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, fb[j], acc[i][jj], 0, 0, 0);
+          }
+        }
+      }
+    }
+  } else {
+    floatx4_t a[NBI], b[NBJ];
+#pragma unroll
+    for (int i = 0; i < NBI; ++i)
+      a[i] = *(const floatx4_t*)(base + 0 * TILE + (wr * WM + i * 16) * LDS_ROW + frag_off);
+#pragma unroll
+    for (int j = 0; j < NBJ; ++j)
+      b[j] = *(const floatx4_t*)(base + 1 * TILE + (wc * WN + j * 16) * LDS_ROW + frag_off);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int i = 0; i < NBI; ++i)
+#pragma unroll
+        for (int j = 0; j < NBJ; ++j)
+          if (!G || (i < nbi && j < nbj && !(diag && wr * NBI + i > wc * NBJ + j)))
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][e], b[j][e], acc[i][j], 0, 0, 0);
+  }
+}
+
+// The K loop over nks steps of 64 bytes per row: prefetch distance DEPTH (one or two register
+// stages), two LDS buffers, one barrier per step.
+template <int PREC, class S, int DEPTH, bool G>
+__device__ __forceinline__ void gram_kloop(floatx4_t (&acc)[S::NBI][S::NBJ], char* lds, int nks,
+                                           int s0, int steps, int half, uint32_t seg_bytes,
+                                           const char* gAh, const char* gBh, const char* gAl,
+                                           const char* gBl, const uint32_t (&oA)[S::RPT],
+                                           const uint32_t (&oB)[S::RPT], int lds_st,
+                                           int frag_off, int wr, int wc, int nbi, int nbj,
+                                           bool diag) {
+  using Cfg = GramCfg<PREC>;
+  constexpr int NA = Cfg::NARR;
+  constexpr int RPT = S::RPT;
+  constexpr int TILE = S::BT * LDS_ROW;
+  constexpr int RS = S::NT / 4;
+  // byte offsets of K step ks (within the chunk) in the left and right operand rows
+#define MEMI_GLOAD(R, ks)                                                                 \
+  do {                                                                                    \
+    const int gs_ = s0 + (ks);                                                            \
+    const int sec_ = gs_ >= steps;                                                        \
+    const int sl_ = 2 * half + sec_;                                                      \
+    const uint32_t kk_ = (uint32_t)(gs_ - sec_ * steps) * ROW_STEP;                       \
+    gram_gload<NA, RPT>(R, gAh, gBh, gAl, gBl, oA, oB, sl_ * seg_bytes + kk_,             \
+                        seg_pair(sl_) * seg_bytes + kk_);                                 \
+  } while (0)
+  GramStage<NA, RPT> X;
+  if constexpr (DEPTH == 1) {
+    MEMI_GLOAD(X, 0);
+    gram_lstore<NA, RPT, TILE, RS>(X, lds + lds_st);
+    __syncthreads();
+    for (int ks = 0; ks < nks; ++ks) {
+      const int cur = ks & 1;
+      if (ks + 1 < nks) MEMI_GLOAD(X, ks + 1);
+      gram_compute<PREC, S, TILE, G>(acc, lds + cur * NA * TILE, frag_off, wr, wc, nbi, nbj,
+                                     diag);
+      if (ks + 1 < nks) gram_lstore<NA, RPT, TILE, RS>(X, lds + (cur ^ 1) * NA * TILE + lds_st);
+      __syncthreads();
+    }
+  } else {
+    // prologue: step 0 into LDS buffer 0, step 1 in flight in Y; steady state unrolled by two
+    // so the register stages alternate by name
+    GramStage<NA, RPT> Y;
+    MEMI_GLOAD(X, 0);
+    if (nks > 1) MEMI_GLOAD(Y, 1);
+    gram_lstore<NA, RPT, TILE, RS>(X, lds + lds_st);
+    __syncthreads();
+    int ks = 0;
+    while (ks < nks) {
+      if (ks + 2 < nks) MEMI_GLOAD(X, ks + 2);
+      gram_compute<PREC, S, TILE, G>(acc, lds, frag_off, wr, wc, nbi, nbj, diag);
+      if (ks + 1 < nks) gram_lstore<NA, RPT, TILE, RS>(Y, lds + NA * TILE + lds_st);
+      __syncthreads();
+      ++ks;
+      if (ks >= nks) break;
+      if (ks + 2 < nks) MEMI_GLOAD(Y, ks + 2);
+      gram_compute<PREC, S, TILE, G>(acc, lds + NA * TILE, frag_off, wr, wc, nbi, nbj, diag);
+      if (ks + 1 < nks) gram_lstore<NA, RPT, TILE, RS>(X, lds + lds_st);
+      __syncthreads();
+      ++ks;
+    }
+  }
+#undef MEMI_GLOAD
+}
+
+template <int PREC, class S, int DEPTH>
+__global__ __launch_bounds__(S::NT, S::OCC) void k_gram_tiles(
     const char* __restrict__ op_hi, const char* __restrict__ op_lo, int64_t Khp, int spc,
     int steps, const int32_t* __restrict__ items, int64_t n_items,
     const memi_pd_desc* __restrict__ pds, float* __restrict__ ws) {
   using Cfg = GramCfg<PREC>;
   constexpr int ES = Cfg::ESIZE;
   constexpr int NA = Cfg::NARR;
-  __shared__ __attribute__((aligned(16))) char lds[2 * NA * LDS_TILE];
+  constexpr int BT = S::BT;
+  constexpr int NBI = S::NBI, NBJ = S::NBJ;
+  constexpr int RPT = S::RPT;
+  constexpr int TILE = BT * LDS_ROW;         // LDS bytes of one array's K step
+  static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth 1 or 2");
+  __shared__ __attribute__((aligned(16))) char lds[2 * NA * TILE];
 
   // XCD-aware order: workgroup w runs on XCD w % 8; give each XCD a contiguous range of
-  // items (consecutive items share an operand panel).  gridDim.x is a multiple of 8.
-  const int64_t q = gridDim.x / 8;
-  const int64_t it = (int64_t)(blockIdx.x % 8) * q + blockIdx.x / 8;
+  // items (consecutive items share operand panels and K ranges).  gridDim.x % 8 == 0.
+  const int64_t q8 = gridDim.x / 8;
+  const int64_t it = (int64_t)(blockIdx.x % 8) * q8 + blockIdx.x / 8;
   if (it >= n_items) return;
   const int32_t* itm = items + it * 6;
-  const int pd = itm[0], ti = itm[1], tj = itm[2], seg = itm[3], chunk = itm[4], slot = itm[5];
+  const int pd = itm[0], ti = itm[1], tj = itm[2], half = itm[3], chunk = itm[4];
+  const int64_t slot = itm[5];
   const memi_pd_desc P = pds[pd];
-  const int mi = min(BM, P.n - ti * BM);
-  const int mj = min(BN, P.n - tj * BN);
+  const int mi = min(BT, P.n - ti * BT);
+  const int mj = min(BT, P.n - tj * BT);
   const int64_t row_bytes = 4 * Khp * ES;
-  // chunk c covers K steps [c*spc, min((c+1)*spc, steps)); the host guarantees none is empty
-  const int64_t a_k0 = (int64_t)seg * Khp * ES + (int64_t)chunk * spc * ROW_STEP;
-  const int64_t b_k0 = (int64_t)seg_pair(seg) * Khp * ES + (int64_t)chunk * spc * ROW_STEP;
-  const int nks = min(spc, steps - chunk * spc);
+  // chunk c covers steps [c*spc, min((c+1)*spc, 2*steps)) of the half's two segments; the
+  // host guarantees none is empty
+  const int s0 = chunk * spc;
+  const int nks = min(spc, 2 * steps - s0);
 
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
-  const int wr = wv >> 1, wc = wv & 1;
+  const int wr = wv / S::WAVES_N, wc = wv % S::WAVES_N;
 
-  // global -> LDS copy assignment: 2 rows x 16 bytes per thread per operand tile
+  // global -> LDS copy assignment: RPT rows x 16 bytes per thread per array, four lanes per
+  // 64-byte row (coalesced).  Tile bases are uniform 64-bit pointers; per-thread offsets
+  // within a tile fit 32 bits.
   const int ld_row0 = t >> 2, ld_part = t & 3;
-  int64_t oA0, oA1, oB0, oB1;
-  {
-    const int r0 = ld_row0, r1 = ld_row0 + 64;
-    oA0 = (P.row0 + (int64_t)ti * BM + min(r0, mi - 1)) * row_bytes + a_k0 + ld_part * 16;
-    oA1 = (P.row0 + (int64_t)ti * BM + min(r1, mi - 1)) * row_bytes + a_k0 + ld_part * 16;
-    oB0 = (P.row0 + (int64_t)tj * BN + min(r0, mj - 1)) * row_bytes + b_k0 + ld_part * 16;
-    oB1 = (P.row0 + (int64_t)tj * BN + min(r1, mj - 1)) * row_bytes + b_k0 + ld_part * 16;
+  const int64_t tA = (P.row0 + (int64_t)ti * BT) * row_bytes;
+  const int64_t tB = (P.row0 + (int64_t)tj * BT) * row_bytes;
+  const char* const gAh = op_hi + tA;
+  const char* const gBh = op_hi + tB;
+  const char* const gAl = NA == 4 ? op_lo + tA : nullptr;
+  const char* const gBl = NA == 4 ? op_lo + tB : nullptr;
+  uint32_t oA[RPT], oB[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = ld_row0 + (S::NT / 4) * q;
+    oA[q] = (uint32_t)(min(r, mi - 1) * row_bytes + ld_part * 16);
+    oB[q] = (uint32_t)(min(r, mj - 1) * row_bytes + ld_part * 16);
   }
   const int lds_st = ld_row0 * LDS_ROW + ((ld_part ^ swz(ld_row0)) * 16);
-  constexpr int HALF = 64 * LDS_ROW;
-
-  // Two register staging sets (X, Y) give a prefetch distance of two K steps: the loads for
-  // step k+2 are issued before step k's MFMAs and land in LDS only after step k+1's.
-  uint4 X0h, X1h, X2h, X3h, X0l, X1l, X2l, X3l;
-  uint4 Y0h, Y1h, Y2h, Y3h, Y0l, Y1l, Y2l, Y3l;
-#define MEMI_GLOAD(R, ks)                                                     \
-  do {                                                                        \
-    const int64_t ko_ = (int64_t)(ks) * ROW_STEP;                             \
-    R##0h = *(const uint4*)(op_hi + oA0 + ko_);                               \
-    R##1h = *(const uint4*)(op_hi + oA1 + ko_);                               \
-    R##2h = *(const uint4*)(op_hi + oB0 + ko_);                               \
-    R##3h = *(const uint4*)(op_hi + oB1 + ko_);                               \
-    if (NA == 4) {                                                            \
-      R##0l = *(const uint4*)(op_lo + oA0 + ko_);                             \
-      R##1l = *(const uint4*)(op_lo + oA1 + ko_);                             \
-      R##2l = *(const uint4*)(op_lo + oB0 + ko_);                             \
-      R##3l = *(const uint4*)(op_lo + oB1 + ko_);                             \
-    }                                                                         \
-  } while (0)
-  // LDS tile order: A hi, B hi (FP32: A, B) then A lo, B lo
-#define MEMI_LSTORE(R, buf)                                                   \
-  do {                                                                        \
-    char* b_ = lds + (buf) * NA * LDS_TILE + lds_st;                          \
-    *(uint4*)(b_ + 0 * LDS_TILE) = R##0h;                                     \
-    *(uint4*)(b_ + 0 * LDS_TILE + HALF) = R##1h;                              \
-    *(uint4*)(b_ + 1 * LDS_TILE) = R##2h;                                     \
-    *(uint4*)(b_ + 1 * LDS_TILE + HALF) = R##3h;                              \
-    if (NA == 4) {                                                            \
-      *(uint4*)(b_ + 2 * LDS_TILE) = R##0l;                                   \
-      *(uint4*)(b_ + 2 * LDS_TILE + HALF) = R##1l;                            \
-      *(uint4*)(b_ + 3 * LDS_TILE) = R##2l;                                   \
-      *(uint4*)(b_ + 3 * LDS_TILE + HALF) = R##3l;                            \
-    }                                                                         \
-  } while (0)
-
-  floatx4_t acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
-
   const int frag_off = (lane & 15) * LDS_ROW + (((lane >> 4) ^ swz(lane & 15)) * 16);
+  const int nbi = max(0, min(NBI, (mi - wr * S::WM + 15) / 16));
+  const int nbj = max(0, min(NBJ, (mj - wc * S::WN + 15) / 16));
+  const bool diag = ti == tj;
 
-#define MEMI_COMPUTE(cur, G)                                                                 \
-  do {                                                                                      \
-    const char* base = lds + (cur) * NA * LDS_TILE;                                         \
-    if constexpr (PREC == MEMI_PREC_SPLIT_BF16) {                                           \
-      /* B hi / lo fragments stay resident; A hi / lo of one 16-row block at a time keeps    \
-         the live fragments at 40 VGPRs.  Each accumulator still gets hi*hi, hi*lo, lo*hi. */ \
-      bf16x8_t fb[4], fc[4];                                                                \
-      _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                       \
-        fb[j] = *(const bf16x8_t*)(base + 1 * LDS_TILE + (wc * 64 + j * 16) * LDS_ROW + frag_off); \
-        fc[j] = *(const bf16x8_t*)(base + 3 * LDS_TILE + (wc * 64 + j * 16) * LDS_ROW + frag_off); \
-      }                                                                                     \
-      _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                       \
-        const bf16x8_t ah = *(const bf16x8_t*)(base + 0 * LDS_TILE + (wr * 64 + i * 16) * LDS_ROW + frag_off); \
-        const bf16x8_t al = *(const bf16x8_t*)(base + 2 * LDS_TILE + (wr * 64 + i * 16) * LDS_ROW + frag_off); \
-        _Pragma("unroll") for (int j = 0; j < 4; ++j)                                       \
-          if (!(G) || (i < nbi && j < nbj)) {                                               \
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, fb[j], acc[i][j], 0, 0, 0); \
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, fc[j], acc[i][j], 0, 0, 0); \
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, fb[j], acc[i][j], 0, 0, 0); \
-          }                                                                                 \
-      }                                                                                     \
-    } else {                                                                                \
-      floatx4_t a[4], b[4];                                                                 \
-      _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                       \
-        a[i] = *(const floatx4_t*)(base + 0 * LDS_TILE + (wr * 64 + i * 16) * LDS_ROW + frag_off); \
-        b[i] = *(const floatx4_t*)(base + 1 * LDS_TILE + (wc * 64 + i * 16) * LDS_ROW + frag_off); \
-      }                                                                                     \
-      _Pragma("unroll") for (int e = 0; e < 4; ++e) _Pragma("unroll") for (int i = 0; i < 4; ++i) \
-        _Pragma("unroll") for (int j = 0; j < 4; ++j)                                       \
-          if (!(G) || (i < nbi && j < nbj))                                                 \
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][e], b[j][e], acc[i][j], 0, 0, 0); \
-    }                                                                                       \
-  } while (0)
+  floatx4_t acc[NBI][NBJ];
+#pragma unroll
+  for (int i = 0; i < NBI; ++i)
+#pragma unroll
+    for (int j = 0; j < NBJ; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
 
-  // The K loop, G = true for edge tiles: MFMA blocks outside the tile's rows/columns are
-  // skipped (nbi x nbj of this wave's 4x4 blocks hold data), full tiles run unguarded.
-#define MEMI_KLOOP(G)\
-  do {                                                                                      \
-  if constexpr (DEPTH == 1) {                                                               \
-    /* prefetch distance one K step: loads for ks+1 are issued before step ks's MFMAs */    \
-    MEMI_GLOAD(X, 0);                                                                       \
-    MEMI_LSTORE(X, 0);                                                                      \
-    __syncthreads();                                                                        \
-    for (int ks = 0; ks < nks; ++ks) {                                                      \
-      const int cur = ks & 1;                                                               \
-      if (ks + 1 < nks) MEMI_GLOAD(X, ks + 1);                                              \
-      MEMI_COMPUTE(cur, G);                                                                 \
-      if (ks + 1 < nks) MEMI_LSTORE(X, cur ^ 1);                                            \
-      __syncthreads();                                                                      \
-    }                                                                                       \
-  } else {                                                                                  \
-    /* prologue: step 0 into LDS buffer 0, step 1 in flight in Y */                         \
-    MEMI_GLOAD(X, 0);                                                                       \
-    if (nks > 1) MEMI_GLOAD(Y, 1);                                                          \
-    MEMI_LSTORE(X, 0);                                                                      \
-    __syncthreads();                                                                        \
-    /* steady state, unrolled by two so the register sets alternate by name */              \
-    int ks = 0;                                                                             \
-    while (ks < nks) {                                                                      \
-      /* even step: compute buffer 0; X is free -> fetch ks+2; Y (step ks+1) -> buffer 1 */ \
-      if (ks + 2 < nks) MEMI_GLOAD(X, ks + 2);                                              \
-      MEMI_COMPUTE(0, G);                                                                   \
-      if (ks + 1 < nks) MEMI_LSTORE(Y, 1);                                                  \
-      __syncthreads();                                                                      \
-      ++ks;                                                                                 \
-      if (ks >= nks) break;                                                                 \
-      /* odd step: compute buffer 1; Y is free -> fetch ks+2; X (step ks+1) -> buffer 0 */  \
-      if (ks + 2 < nks) MEMI_GLOAD(Y, ks + 2);                                              \
-      MEMI_COMPUTE(1, G);                                                                   \
-      if (ks + 1 < nks) MEMI_LSTORE(X, 0);                                                  \
-      __syncthreads();                                                                      \
-      ++ks;                                                                                 \
-    }                                                                                       \
-  }                                                                                         \
-  } while (0)
-  const int nbi = max(0, min(4, (mi - wr * 64 + 15) / 16));
-  const int nbj = max(0, min(4, (mj - wc * 64 + 15) / 16));
-  if (mi == BM && mj == BN) {
-    MEMI_KLOOP(false);
-  } else {
-    MEMI_KLOOP(true);
-  }
-#undef MEMI_KLOOP
-#undef MEMI_GLOAD
-#undef MEMI_LSTORE
-#undef MEMI_COMPUTE
+  const uint32_t seg_bytes = (uint32_t)(Khp * ES);
+  if (mi == BT && mj == BT && !diag)
+    gram_kloop<PREC, S, DEPTH, false>(acc, lds, nks, s0, steps, half, seg_bytes, gAh, gBh, gAl,
+                                      gBl, oA, oB, lds_st, frag_off, wr, wc, nbi, nbj, diag);
+  else
+    gram_kloop<PREC, S, DEPTH, true>(acc, lds, nks, s0, steps, half, seg_bytes, gAh, gBh, gAl,
+                                     gBl, oA, oB, lds_st, frag_off, wr, wc, nbi, nbj, diag);
 
   // 16x16 output block: lane holds rows 4*(lane/16) + r, column lane % 16
-  float* out = ws + (int64_t)slot * BM * BN;
+  float* out = ws + slot * BT * BT;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < NBI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < NBJ; ++j) {
+      if (diag && wr * NBI + i > wc * NBJ + j) continue;   // never read by the finalize
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = wr * 64 + i * 16 + (lane >> 4) * 4 + r;
-        const int col = wc * 64 + j * 16 + (lane & 15);
-        out[row * BN + col] = acc[i][j][r];
+        const int row = wr * S::WM + i * 16 + (lane >> 4) * 4 + r;
+        const int col = wc * S::WN + j * 16 + (lane & 15);
+        out[row * BT + col] = acc[i][j][r];
       }
+    }
 }
 
-constexpr int FIN_SPLIT = 8;   // workgroups per upper tile in the finalize
-
-// Reduce the segment partials of one upper tile in float64, write D and its mirror,
-// and queue entries with a large cancellation ratio for direct recomputation.
+// Reduce the P and R partials of one upper tile in float64, write D and its mirror, and queue
+// entries with a large cancellation ratio for direct recomputation.  blockIdx.y: which
+// TB_FIN-entry part of the BT x BT tile this block reduces.
 __global__ __launch_bounds__(256) void k_gram_finalize(const int32_t* __restrict__ utiles,
                                                        const memi_pd_desc* __restrict__ pds,
                                                        const float* __restrict__ ws, int chunks,
-                                                       double kappa, float* __restrict__ D,
+                                                       int BT, double kappa,
+                                                       float* __restrict__ D,
                                                        int32_t* __restrict__ flags,
                                                        int64_t flag_cap,
                                                        int32_t* __restrict__ flag_count) {
@@ -261,22 +330,19 @@ __global__ __launch_bounds__(256) void k_gram_finalize(const int32_t* __restrict
   const int64_t slot0 = u[3];
   const memi_pd_desc P = pds[pd];
   const int n = P.n;
-  const int64_t tile = (int64_t)BM * BN;
-  // blockIdx.y: which FIN_SPLIT-th of the tile's entries this block reduces
-  const int e0 = blockIdx.y * (BM * BN / FIN_SPLIT), e1 = e0 + BM * BN / FIN_SPLIT;
+  const int64_t tile = (int64_t)BT * BT;
+  const int e0 = blockIdx.y * TB_FIN, e1 = e0 + TB_FIN;
   for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const int r = e / BN, c = e % BN;
-    const int i = ti * BM + r, j = tj * BN + c;
+    const int r = e / BT, c = e % BT;
+    const int i = ti * BT + r, j = tj * BT + c;
     if (i >= n || j >= n) continue;
     if (ti == tj && c < r) continue;
     double d = 0.0;
     if (i != j) {
       double Pp = 0.0, R = 0.0;
       for (int ch = 0; ch < chunks; ++ch) {
-        Pp += (double)ws[(slot0 + 0 * chunks + ch) * tile + e];
-        Pp += (double)ws[(slot0 + 1 * chunks + ch) * tile + e];
-        R += (double)ws[(slot0 + 2 * chunks + ch) * tile + e];
-        R += (double)ws[(slot0 + 3 * chunks + ch) * tile + e];
+        Pp += (double)ws[(slot0 + ch) * tile + e];
+        R += (double)ws[(slot0 + chunks + ch) * tile + e];
       }
       d = Pp - 2.0 * R;
       if (Pp + 2.0 * fabs(R) > kappa * d) {
@@ -339,13 +405,18 @@ using namespace memi;
 
 extern "C" {
 
-int memi_plan(const memi_pd_desc* pds_in, int npd, int chunks, void* plan_host,
-              int64_t plan_cap, memi_plan_info* info) {
+int memi_plan(const memi_pd_desc* pds_in, int npd, int chunks, int gram_tile,
+              void* plan_host, int64_t plan_cap, memi_plan_info* info) {
   MEMI_CHECK_ARG(info != nullptr && (npd == 0 || pds_in != nullptr), "memi_plan: null pointer");
   MEMI_CHECK_ARG(npd >= 0 && chunks >= 1 && chunks <= 64, "memi_plan: bad npd/chunks");
+  MEMI_CHECK_ARG(gram_tile == 128 || gram_tile == 256,
+                 "memi_plan: gram_tile must be 128 or 256 (got %d)", gram_tile);
+  constexpr int FT = 128;             // side of the full-matrix tiles of the Stage-2 kernels
+  const int BT = gram_tile;
   memi_plan_info I{};
   I.n_pd = npd;
   I.chunks = chunks;
+  I.gram_tile = BT;
   int64_t d = 0, mw = 0, le = 0, rows = 0, items = 0, ut = 0, ft = 0, fc = 0;
   int max_n = 0;
   for (int p = 0; p < npd; ++p) {
@@ -355,17 +426,19 @@ int memi_plan(const memi_pd_desc* pds_in, int npd, int chunks, void* plan_host,
     MEMI_CHECK_ARG(pds_in[p].row0 == rows,
                    "memi_plan: PD %d starts at row %lld; PD rows must be packed contiguously "
                    "in order (expected %lld)", p, (long long)pds_in[p].row0, (long long)rows);
-    const int64_t T = (n + BM - 1) / BM;
+    const int64_t T = (n + BT - 1) / BT;
+    const int64_t TF = (n + FT - 1) / FT;
     d += n * n;
     mw += n * ((n + 31) / 32);
     le += (int64_t)std::min<int64_t>(pds_in[p].k, n) * n;
     rows += n;
     ut += T * (T + 1) / 2;
-    ft += T * T;
+    ft += TF * TF;
     fc += n * (n - 1) / 2;
     if (n > max_n) max_n = (int)n;
   }
-  items = ut * 4 * chunks;
+  items = ut * 2 * chunks;
+  MEMI_CHECK_ARG(items < (1LL << 31), "memi_plan: too many Gram work items");
   I.n_items = items;
   I.n_utiles = ut;
   I.n_ftiles = ft;
@@ -373,7 +446,7 @@ int memi_plan(const memi_pd_desc* pds_in, int npd, int chunks, void* plan_host,
   I.d_elems = d;
   I.mask_words = mw;
   I.list_elems = le;
-  I.ws_floats = ut * 4 * chunks * (int64_t)BM * BN;
+  I.ws_floats = ut * 2 * chunks * (int64_t)BT * BT;
   I.flag_cap = std::max<int64_t>(fc, 1);
   I.max_n = max_n;
   int64_t off = 0;
@@ -411,28 +484,30 @@ int memi_plan(const memi_pd_desc* pds_in, int npd, int chunks, void* plan_host,
     d += (int64_t)n * n;
     mw += (int64_t)n * ((n + 31) / 32);
     le += (int64_t)std::min(P.k, n) * n;
-    const int T = (n + BM - 1) / BM;
+    const int T = (n + BT - 1) / BT;
     const int64_t u0 = uidx;
     for (int a = 0; a < T; ++a)
       for (int b = a; b < T; ++b) {
         int32_t* u = uts + 4 * uidx;
-        u[0] = p; u[1] = a; u[2] = b; u[3] = (int32_t)(uidx * 4 * chunks);
+        u[0] = p; u[1] = a; u[2] = b; u[3] = (int32_t)(uidx * 2 * chunks);
         ++uidx;
       }
-    // items: segment-major, then chunk, then row panel, so neighbours share operand panels
-    for (int s = 0; s < 4; ++s)
+    // items: half-major, then chunk, then row panel, so neighbours share operand panels and
+    // K ranges; slot = utile * 2 chunks + half * chunks + chunk
+    for (int h = 0; h < 2; ++h)
       for (int c = 0; c < chunks; ++c) {
         int64_t uu = u0;
         for (int a = 0; a < T; ++a)
           for (int b = a; b < T; ++b) {
             int32_t* x = it + 6 * nit++;
-            x[0] = p; x[1] = a; x[2] = b; x[3] = s; x[4] = c;
-            x[5] = (int32_t)(uu * 4 * chunks + s * chunks + c);
+            x[0] = p; x[1] = a; x[2] = b; x[3] = h; x[4] = c;
+            x[5] = (int32_t)(uu * 2 * chunks + h * chunks + c);
             ++uu;
           }
       }
-    for (int a = 0; a < T; ++a)
-      for (int b = 0; b < T; ++b) {
+    const int TF = (n + FT - 1) / FT;
+    for (int a = 0; a < TF; ++a)
+      for (int b = 0; b < TF; ++b) {
         int32_t* f = fts + 4 * nft++;
         f[0] = p; f[1] = a; f[2] = b; f[3] = 0;
       }
@@ -456,14 +531,19 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
                  "memi_ctf_distance: bad precision %d", precision);
   MEMI_CHECK_ARG(Khp > 0 && Khp % 64 == 0 && Khp >= (int64_t)N * (N / 2 + 1),
                  "memi_ctf_distance: bad Khp %lld", (long long)Khp);
+  const int BT = info->gram_tile;
+  MEMI_CHECK_ARG(BT == 128 || BT == 256, "memi_ctf_distance: plan has Gram tile %d", BT);
+  MEMI_CHECK_ARG(variant >= 0 && variant <= 2 && !(BT == 256 && variant == 2),
+                 "memi_ctf_distance: variant %d is not available for Gram tile %d", variant, BT);
   const int chunks = info->chunks;
   const int kstep = ROW_STEP / (precision == MEMI_PREC_FP32 ? 4 : 2);
-  // K steps per segment, split into chunks of spc steps (the last one may be shorter)
+  // K steps per segment; a half spans two segments, split into chunks of spc steps (the last
+  // one may be shorter)
   const int64_t steps = Khp / kstep;
-  const int64_t spc = (steps + chunks - 1) / chunks;
-  MEMI_CHECK_ARG(steps < (1LL << 30) && (int64_t)(chunks - 1) * spc < steps,
+  const int64_t spc = (2 * steps + chunks - 1) / chunks;
+  MEMI_CHECK_ARG(steps < (1LL << 29) && (int64_t)(chunks - 1) * spc < 2 * steps,
                  "memi_ctf_distance: %d chunks leave an empty chunk of the %lld K steps",
-                 chunks, (long long)steps);
+                 chunks, (long long)(2 * steps));
   MEMI_CHECK_ARG(kappa > 0, "memi_ctf_distance: kappa must be positive");
   if (info->n_pd == 0) return MEMI_OK;
   hipStream_t s = (hipStream_t)stream;
@@ -479,18 +559,21 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
     set_error("memi_ctf_distance: hipEventRecord failed");
     return MEMI_ELAUNCH;
   }
-  MEMI_CHECK_ARG(variant >= 0 && variant <= 2, "memi_ctf_distance: unknown variant %d", variant);
-  const int v = variant == 0 ? 1 : variant;
+  // default schedule: prefetch two K steps for 128-wide tiles, one for 256-wide
+  const int depth = variant == 0 ? (BT == 128 ? 2 : 1) : variant;
   const char* lo = precision == MEMI_PREC_FP32 ? nullptr : (const char*)op_lo;
-#define MEMI_LAUNCH_GRAM(P, V)                                                                \
-  hipLaunchKernelGGL((k_gram_tiles<P, V>), dim3((unsigned)grid), dim3(256), 0, s,              \
+#define MEMI_LAUNCH_GRAM(P, SH, V)                                                            \
+  hipLaunchKernelGGL((k_gram_tiles<P, SH, V>), dim3((unsigned)grid), dim3(SH::NT), 0, s,       \
                      (const char*)op_hi, lo, Khp, (int)spc, (int)steps, items, info->n_items, \
                      pds, ws)
   if (precision == MEMI_PREC_FP32) {
-    if (v == 1) MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, 1); else MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, 2);
+    if (BT == 256) MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, Shape256, 1);
+    else if (depth == 1) MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, Shape128, 1);
+    else MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, Shape128, 2);
   } else {
-    if (v == 1) MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, 1);
-    else MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, 2);
+    if (BT == 256) MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, Shape256, 1);
+    else if (depth == 1) MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, Shape128, 1);
+    else MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, Shape128, 2);
   }
 #undef MEMI_LAUNCH_GRAM
   MEMI_CHECK_LAUNCH("k_gram_tiles");
@@ -498,8 +581,9 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
     set_error("memi_ctf_distance: hipEventRecord failed");
     return MEMI_ELAUNCH;
   }
-  hipLaunchKernelGGL(k_gram_finalize, dim3((unsigned)info->n_utiles, FIN_SPLIT), dim3(256), 0, s,
-                     ut, pds, ws, chunks, kappa, D, flags, info->flag_cap, flag_count);
+  hipLaunchKernelGGL(k_gram_finalize, dim3((unsigned)info->n_utiles, BT * BT / TB_FIN),
+                     dim3(256), 0, s, ut, pds, ws, chunks, BT, kappa, D, flags, info->flag_cap,
+                     flag_count);
   MEMI_CHECK_LAUNCH("k_gram_finalize");
   hipLaunchKernelGGL(k_gram_recompute, dim3(1024), dim3(256), 0, s, flags, flag_count,
                      info->flag_cap, pds, (const float2*)fy, ctf_half, N, D);

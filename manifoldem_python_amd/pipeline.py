@@ -27,7 +27,8 @@ warnings.simplefilter("ignore", OptimizeWarning)
 
 LOG_EPS = np.arange(-150, 150.2, 0.2)        # DMembeddingII.py:146 (1501 points)
 KAPPA = {nat.PREC_FP32: 1.0e4, nat.PREC_SPLIT_BF16: 8.0}
-CHUNKS = {nat.PREC_FP32: 4, nat.PREC_SPLIT_BF16: 1}
+CHUNKS = {nat.PREC_FP32: 8, nat.PREC_SPLIT_BF16: 2}     # default K chunks per Gram half
+GRAM_TILE = 128                                          # default Gram output tile side
 
 
 def round_up(x, m):

@@ -26,16 +26,20 @@ def test_library_exports_every_declared_symbol():
     assert lib.memi_version() >= 10000
 
 
-def test_plan_tables():
+@pytest.mark.parametrize("tile", [128, 256])
+def test_plan_tables(tile):
     from manifoldem_python_amd import _native as nat
     ns, ks = [300, 129, 1], [150, 200, 1]
     rows0 = np.concatenate(([0], np.cumsum(ns)[:-1]))
-    pl = nat.Plan(rows0, ns, ks, chunks=2, device="cpu")
+    pl = nat.Plan(rows0, ns, ks, chunks=2, device="cpu", gram_tile=tile)
     I = pl.info
-    T = [(n + 127) // 128 for n in ns]
+    T = [(n + tile - 1) // tile for n in ns]
+    TF = [(n + 127) // 128 for n in ns]
+    assert I.gram_tile == tile
     assert I.n_utiles == sum(t * (t + 1) // 2 for t in T)
-    assert I.n_ftiles == sum(t * t for t in T)
-    assert I.n_items == I.n_utiles * 4 * 2
+    assert I.n_ftiles == sum(t * t for t in TF)
+    assert I.n_items == I.n_utiles * 2 * 2
+    assert I.ws_floats == I.n_items * tile * tile
     assert I.n_rows == sum(ns)
     assert I.d_elems == sum(n * n for n in ns)
     assert I.mask_words == sum(n * ((n + 31) // 32) for n in ns)
@@ -43,12 +47,17 @@ def test_plan_tables():
     assert I.flag_cap == sum(n * (n - 1) // 2 for n in ns)
     np.testing.assert_array_equal(pl.pds["d_off"], [0, 90000, 90000 + 129 * 129])
     items = pl.host[I.off_items:I.off_items + I.n_items * 24].view(np.int32).reshape(-1, 6)
-    # every (tile, segment, chunk) slot appears exactly once
+    # every (tile, half, chunk) slot appears exactly once, slot = utile * 2 chunks + half *
+    # chunks + chunk
     assert len(set(items[:, 5].tolist())) == I.n_items
-    assert items[:, 5].max() == I.n_utiles * 4 * 2 - 1
-    assert set(items[:, 3].tolist()) == {0, 1, 2, 3}
+    assert items[:, 5].max() == I.n_utiles * 2 * 2 - 1
+    assert set(items[:, 3].tolist()) == {0, 1}
+    assert set(items[:, 4].tolist()) == {0, 1}
     ut = pl.host[I.off_utiles:I.off_utiles + I.n_utiles * 16].view(np.int32).reshape(-1, 4)
     assert np.all(ut[:, 1] <= ut[:, 2])
+    for x in items:
+        u = ut[np.nonzero((ut[:, 0] == x[0]) & (ut[:, 1] == x[1]) & (ut[:, 2] == x[2]))[0][0]]
+        assert x[5] == u[3] + x[3] * 2 + x[4]
 
 
 def test_plan_rejects_bad_input():
@@ -57,6 +66,8 @@ def test_plan_rejects_bad_input():
         nat.Plan([0, 5], [10, 10], [5, 5], device="cpu")
     with pytest.raises(RuntimeError, match="n=0"):
         nat.Plan([0], [0], [5], device="cpu")
+    with pytest.raises(RuntimeError, match="gram_tile"):
+        nat.Plan([0], [10], [5], device="cpu", gram_tile=64)
 
 
 def test_wrappers_refuse_host_tensors():

@@ -54,10 +54,11 @@ def oracle_D():
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_batched_distances_match_oracle_per_pd(cuda, oracle_D, precision):
+@pytest.mark.parametrize("tile", [128, 256])
+def test_batched_distances_match_oracle_per_pd(cuda, oracle_D, precision, tile):
     from manifoldem_python_amd import batch as B
     batch = _batch(_pds(), cuda)
-    wsp = B.Workspace(batch, precision, chunk_rows=200)     # several row chunks
+    wsp = B.Workspace(batch, precision, chunk_rows=200, gram_tile=tile)   # several row chunks
     dist = B.distances(batch, wsp)
     for p, ref in enumerate(oracle_D):
         D = dist.block(p).cpu().numpy()

@@ -210,16 +210,17 @@ def test_gram_schedule_variants_agree(cuda, precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("chunks", [5, 7])
-def test_gram_uneven_chunks_vs_float64(cuda, precision, chunks):
-    """K splits whose last chunk is shorter (N = 64: 66 split-BF16 / 132 FP32 steps per
-    segment) against the float64 Gram of the same spectra, and against the default split."""
+@pytest.mark.parametrize("chunks,tile", [(5, 128), (7, 128), (1, 256), (3, 256), (7, 256)])
+def test_gram_uneven_chunks_vs_float64(cuda, precision, chunks, tile):
+    """K splits whose last chunk is shorter (N = 64: 132 split-BF16 / 264 FP32 steps per half)
+    with 128- and 256-wide tiles (n = 300: full, edge and diagonal tiles of both) against the
+    float64 Gram of the same spectra, and against the default split."""
     from manifoldem_python_amd import _native as nat, pipeline as pl
     from oracle import distance as od
     n, N = 300, 64
     s = _setup(n, N, seed=4)
     geo, out, op = _gpu_prep(s, cuda, precision)
-    plan = nat.Plan([0], [n], [n], chunks=chunks)
+    plan = nat.Plan([0], [n], [n], chunks=chunks, gram_tile=tile)
     D = pl.ctf_distances(plan, op, out["fy"], out["ctf"], N).block(0).cpu().numpy()
     base = nat.Plan([0], [n], [n], chunks=pl.CHUNKS[precision])
     D0 = pl.ctf_distances(base, op, out["fy"], out["ctf"], N).block(0).cpu().numpy()
