@@ -131,6 +131,21 @@ int memi_prep_images(const float* stack, int64_t n_stack, const int64_t* src,
                      int64_t row0, void* op_hi, void* op_lo, void* fy, float* ctf_half,
                      float* img_out, double* ws, int64_t ws_doubles, void* stream);
 
+/* memi_prep_images without a workspace argument: the per-bin tables live in a static device
+ * array of 6 * 160 * 81 doubles (622 KB, enough for N <= MEMI_PREP_MAX_N) that each GPU holds
+ * until the process ends.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops).  It calls no
+ * allocation routine; its first call on a device looks up that device's copy of the array.
+ * Every call on one GPU shares that array (each call rewrites it in full before use), so calls
+ * that overlap on one GPU (other streams, other host threads) race.  N beyond the array's
+ * size is rejected with MEMI_EINVAL.  Other arguments and results as memi_prep_images. */
+int memi_prep_images_noalloc(const float* stack, int64_t n_stack, const int64_t* src,
+                             const int32_t* conj, const double* rot1_deg, const double* rot2_deg,
+                             const double* df, int64_t n, int N, int filter_kind, double Qc,
+                             int order, const memi_ctf_params* ctf, int precision, int64_t Khp,
+                             int64_t row0, void* op_hi, void* op_lo, void* fy, float* ctf_half,
+                             float* img_out, void* stream);
+
 /* Fused in-plane alignment, getDistanceCTF_local_Conj9combinedS2.py:315-330 with
  * rotatefill.py:21-28 and annularMask.py:19-32: optionally the disk mask (disk != 0), then
  * the periodic cubic-spline rotation by rot1_deg[i] followed by the one by rot2_deg[i], each

@@ -58,6 +58,9 @@ _SIGS = {
     "memi_prep_images": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dbl,
                          c_int, ctypes.POINTER(CtfParams), c_int, c_i64, c_i64, c_vp, c_vp, c_vp,
                          c_vp, c_vp, c_vp, c_i64, c_vp],
+    "memi_prep_images_noalloc": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int,
+                                 c_dbl, c_int, ctypes.POINTER(CtfParams), c_int, c_i64, c_i64,
+                                 c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_ctf_pack": [c_vp, c_vp, c_i64, c_int, ctypes.POINTER(CtfParams), c_int, c_i64, c_i64,
                       c_vp, c_vp, c_vp, c_vp],
     "memi_phase_flip": [c_vp, c_vp, c_i64, c_int, c_vp, c_vp],
@@ -257,6 +260,39 @@ def prep_images(stack, src, conj, rot1, rot2, df, N, filter_kind, Qc, order, ctf
          ctypes.byref(ctf), int(precision), int(Khp), int(row0), _p(op_hi),
          _p(op_lo) if op_lo is not None else None, _p(fy), _p(ctf_half),
          None if img_out is None else _p(img_out), _p(ws), ws.numel(), _stream())
+
+
+def prep_images_noalloc(stack, src, conj, rot1, rot2, df, N, filter_kind, Qc, order, ctf,
+                        precision, Khp, row0, op_hi, op_lo, fy, ctf_half, img_out=None):
+    """memi_prep_images_noalloc: prep_images with the tables in the library's static device
+    scratch instead of a workspace (calls that overlap on one GPU race; see include/memi.h)."""
+    n = int(src.shape[0])
+    Kh = N * (N // 2 + 1)
+    _need(prep_supported(N), f"prep_images_noalloc: N={N} not supported by the fused kernel")
+    _need(stack.dtype == torch.float32 and tuple(stack.shape[1:]) == (N, N),
+          "prep_images_noalloc: stack must be float32 (n_stack, N, N)")
+    _need(src.dtype == torch.int64 and (conj is None or (conj.dtype == torch.int32 and
+                                                        conj.numel() == n)),
+          "prep_images_noalloc: src int64 / conj int32")
+    for t, name in ((rot1, "rot1"), (rot2, "rot2"), (df, "df")):
+        _need(t.dtype == torch.float64 and t.numel() == n,
+              f"prep_images_noalloc: {name} float64 (n,)")
+    if n:
+        _need(int(src.min()) >= 0 and int(src.max()) < stack.shape[0],
+              "prep_images_noalloc: image index outside the stack")
+    esz = 4 if precision == PREC_FP32 else 2
+    _need(_bytes(op_hi) >= (row0 + n) * 4 * Khp * esz and
+          (precision == PREC_FP32 or _bytes(op_lo) >= (row0 + n) * 4 * Khp * esz),
+          "prep_images_noalloc: operand smaller than row0 + n rows")
+    _need(fy.dtype == torch.complex64 and fy.numel() >= n * Kh and ctf_half.numel() >= n * Kh,
+          "prep_images_noalloc: fy / ctf sizes")
+    _need(img_out is None or (img_out.dtype == torch.float32 and img_out.numel() >= n * N * N),
+          "prep_images_noalloc: img_out size")
+    call("memi_prep_images_noalloc", _p(stack), stack.shape[0], _p(src),
+         None if conj is None else _p(conj), _p(rot1), _p(rot2), _p(df), n, N, int(filter_kind),
+         float(Qc), int(order), ctypes.byref(ctf), int(precision), int(Khp), int(row0),
+         _p(op_hi), _p(op_lo) if op_lo is not None else None, _p(fy), _p(ctf_half),
+         None if img_out is None else _p(img_out), _stream())
 
 
 def ctf_params(pix_size, cs, ekv, ampc, gauss_env):

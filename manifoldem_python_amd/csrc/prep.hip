@@ -4,6 +4,8 @@
 #include "common.h"
 #include "fft_lds.h"
 
+#include <mutex>
+
 namespace memi {
 
 // annularMask.op(0, N/2, N, N): r^2 < (N/2)^2 about row N/2-1, column N/2.
@@ -730,6 +732,40 @@ __global__ __launch_bounds__(512) void k_prep_images(
   }
 }
 
+// Device scratch of memi_prep_images_noalloc: the per-bin tables (PT doubles per half-spectrum
+// bin) of the largest supported image side.  One copy per GPU, held until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress returns for the current
+// device (the name itself is a host-side stand-in).
+constexpr int64_t PREP_SCRATCH_DOUBLES =
+    (int64_t)PT * MEMI_PREP_MAX_N * (MEMI_PREP_MAX_N / 2 + 1);
+static __device__ __align__(256) double scratch[PREP_SCRATCH_DOUBLES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// The current device's address of `scratch`, looked up on the first call on that device.
+static int scratch_for_current_device(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+    set_error("memi_prep_images_noalloc: no usable current device (%d)", dev);
+    return MEMI_ELAUNCH;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("memi_prep_images_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s",
+                dev, hipGetErrorString(e));
+      return MEMI_ELAUNCH;
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MEMI_OK;
+}
+
 }  // namespace memi
 
 using namespace memi;
@@ -868,6 +904,26 @@ int memi_prep_images(const float* stack, int64_t n_stack, const int64_t* src,
 #undef MEMI_PREP
   MEMI_CHECK_LAUNCH("k_prep_images");
   return MEMI_OK;
+}
+
+int memi_prep_images_noalloc(const float* stack, int64_t n_stack, const int64_t* src,
+                             const int32_t* conj, const double* rot1_deg, const double* rot2_deg,
+                             const double* df, int64_t n, int N, int filter_kind, double Qc,
+                             int order, const memi_ctf_params* ctf, int precision, int64_t Khp,
+                             int64_t row0, void* op_hi, void* op_lo, void* fy, float* ctf_half,
+                             float* img_out, void* stream) {
+  const int64_t need = (int64_t)PT * N * (N / 2 + 1);
+  MEMI_CHECK_ARG(N >= 8 && N <= MEMI_PREP_MAX_N && need <= PREP_SCRATCH_DOUBLES,
+                 "memi_prep_images_noalloc: N=%d needs %lld doubles of tables, beyond the %lld of "
+                 "the static scratch (at most N = %d)", N, (long long)need,
+                 (long long)PREP_SCRATCH_DOUBLES, MEMI_PREP_MAX_N);
+  double* ws = nullptr;
+  const int st = scratch_for_current_device(&ws);
+  if (st != MEMI_OK) return st;
+  // the tables are rewritten in full by every call (k_prep_tables), so nothing is cleared
+  return memi_prep_images(stack, n_stack, src, conj, rot1_deg, rot2_deg, df, n, N, filter_kind,
+                          Qc, order, ctf, precision, Khp, row0, op_hi, op_lo, fy, ctf_half,
+                          img_out, ws, PREP_SCRATCH_DOUBLES, stream);
 }
 
 int memi_ctf_pack(const void* fy, const double* df, int64_t n, int N, const memi_ctf_params* ctf,

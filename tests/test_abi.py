@@ -150,3 +150,15 @@ def test_ritz_vectors_checks_meta_on_the_host():
     for mh in bad:
         with pytest.raises(RuntimeError, match="ritz_vectors"):
             nat.ritz_vectors(plan, V, m, S, mh, meta, kmax, Y)
+
+
+def test_noalloc_prep_rejects_images_beyond_its_scratch():
+    """memi_prep_images_noalloc checks N against its static scratch before any device call."""
+    from manifoldem_python_amd import _native as nat
+    lib = nat.load()
+    ctf = nat.ctf_params(1.5, 2.7, 300.0, 0.1, np.inf)
+    rc = lib.memi_prep_images_noalloc(None, 0, None, None, None, None, None, 0, 170, 1, 0.5, 8,
+                                      ctypes.byref(ctf), 0, 64 * 1024, 0, None, None, None, None,
+                                      None, None)
+    assert rc == -1
+    assert b"static scratch" in lib.memi_last_error()
