@@ -7,11 +7,16 @@ distance Gram, and the diffusion-map embedding (kNN, 1501-point Ferguson sweep, 
 Markov normalisation, float64 eigensolve).  Prints ONE JSON line.
 
 Workload (default, --workload c2): one PD of n=2000 synthetic 128x128 particles, defocus
-U(0.5, 3.5) um, kNN k=200, 10 eigenvectors.  --workload c3: 500 PDs, lognormal sizes
-(median 400, sigma 0.6, clipped to [100, 3000]), 96x96, k=min(n,150), 8 eigenvectors.
+U(0.5, 3.5) um, kNN k=200, 10 eigenvectors; the same line carries a nested "c3" record of the
+batched workload (500 PDs, lognormal sizes (median 400, sigma 0.6, clipped to [100, 3000]),
+96x96, k=min(n,150), 8 eigenvectors, prep + Gram replayed from a HIP graph), split-BF16 against
+FP32.  --workload c3 measures that workload alone; --workload c1: the plain method on one PD
+(n=200, 70x70, full kernel, 101 eps, 8 eigenvectors), whose CPU baseline runs whole.
 
 value = image-pair distances per second over the whole step (split-BF16 Gram);
-fp32 = the same with the plain FP32 MFMA Gram.
+fp32 = the same with the plain FP32 MFMA Gram.  cpu_baseline: the float64 oracle on this
+host's cores in the reference's two modes (one process with all BLAS threads; one
+single-threaded process per core on separate PDs).
 
 Secondary workloads (their own JSON line, same contract):
   --workload c4: the differentiable distance -> Gaussian kernel -> log-sum op at n=1000,
@@ -44,7 +49,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--no-c3", action="store_true",
+                    help="c2 only: skip the nested 500-PD batched measurement")
+    ap.add_argument("--gram-config", default=None,
+                    help="tile,variant,chunks: use this Gram configuration for the split-BF16 "
+                         "path instead of autotuning (for counter runs of one configuration)")
     ap.add_argument("--c5-pds", type=int, default=2,
                     help="PDs of the NLSA workload timed end to end (c5 only)")
     ap.add_argument("--lanczos-graph", type=int, default=1,
@@ -80,7 +90,15 @@ def build_workload(name, dev):
     import torch
     from manifoldem_python_amd import synth, batch as B, _native as nat
     ctf = nat.ctf_params(synth.PIX, synth.CS, synth.EKV, synth.AMPC, np.inf)
-    if name == "c2":
+    if name == "c1":
+        n, N = 200, 70
+        s = synth.make_pd(synth.seeds(6)[0], n, N, device=dev, as_torch=True)
+        pds = [dict(src=np.arange(n), conj=np.zeros(n, np.int32), q=s.q, df=s.df, k=n)]
+        stack = s.raw
+        meta = dict(workload="c1_plain_one_pd", n_pd=1, particles=n, image=N, knn="n (full)",
+                    n_eigs=8, n_eps=101)
+        num_eigs = 8
+    elif name == "c2":
         n, N = 2000, 128
         s = synth.make_pd(synth.seeds(3)[1], n, N, df_range_um=(0.5, 3.5), device=dev,
                           as_torch=True)
@@ -108,10 +126,12 @@ def build_workload(name, dev):
                     n_eigs=8, n_eps=1501)
         num_eigs = 8
     batch = B.make_batch(stack, pds, N, ctf, FILTER, dev)
+    batch.q_host = [d["q"] for d in pds]            # per-PD quaternions (CPU baseline)
     return batch, meta, num_eigs
 
 
-def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False):
+def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False,
+                  logEps=None, gram_config=None):
     """Tune the Gram on the prepared operand, warm up, then time exactly `steps` full steps;
     returns timings and the last results.  graph: the prep + Gram part of each step is one
     replay of a captured HIP graph; the Gram kernel is then timed with HIP events in `steps`
@@ -121,16 +141,23 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False
     from manifoldem_python_amd import autotune
     from manifoldem_python_amd import pipeline as pl
     wsp = B.Workspace(batch, precision)
+    le = pl.LOG_EPS if logEps is None else logEps
     rng = np.random.default_rng(0)
     rand = lambda *sh: rng.random(sh)
     B.distances(batch, wsp)
-    tuned = autotune.tune_gram(wsp, batch.N)
+    if gram_config is not None:
+        t, v, c = (int(x) for x in gram_config.split(","))
+        autotune._install(wsp, c, t)
+        pl.GRAM_VARIANT[precision] = v
+        tuned = {"best": v, "chunks": c, "tile": t, "ms": {}, "forced": True}
+    else:
+        tuned = autotune.tune_gram(wsp, batch.N)
     g = B.GraphedDistances(batch, wsp) if graph else None
     dist_fn = (lambda ev=None: g.run()) if graph else (
         lambda ev=None: B.distances(batch, wsp, events=ev))
     for _ in range(warmup):
         dist = dist_fn()
-        B.embed(wsp, dist, num_eigs=num_eigs, rand=rand)
+        B.embed(wsp, dist, num_eigs=num_eigs, rand=rand, logEps=le)
     torch.cuda.synchronize()
     pairs = [events.new_pair() for _ in range(steps)]
     t_dist = 0.0
@@ -140,7 +167,7 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False
         dist = dist_fn(pairs[k])
         torch.cuda.synchronize()
         t_dist += time.perf_counter() - a
-        res = B.embed(wsp, dist, num_eigs=num_eigs, rand=rand)
+        res = B.embed(wsp, dist, num_eigs=num_eigs, rand=rand, logEps=le)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     if graph:
@@ -149,7 +176,7 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False
                              events=pairs[k], flags=wsp.flags)
         torch.cuda.synchronize()
     gram_ms = [events.elapsed_ms(p) for p in pairs]
-    stages = B.breakdown(batch, wsp, num_eigs, rand)
+    stages = B.breakdown(batch, wsp, num_eigs, rand, logEps=le)
     J = np.array([getattr(r, "lanczos_steps", 0) for r in res])
     lanczos = dict(mean_steps=float(J.mean()), max_steps=int(J.max()),
                    p90_steps=float(np.percentile(J, 90)),
@@ -207,63 +234,143 @@ def fp64_error(batch, wsp, dist):
     return dict(max=float(e.max()), p99=float(np.percentile(e, 99)), median=float(np.median(e)))
 
 
-def cpu_baseline(meta, batch):
-    """Float64 oracle (the numpy/scipy restatement of the reference) timed on a bounded sample
-    on this host, extrapolated to the workload with the stated cost model."""
-    import torch
-    from oracle import distance as od, dm as odm
-    from threadpoolctl import threadpool_info
-    N = batch.N
-    ns = batch.ns
-    n_img = 24
-    stack = batch.stack[:n_img].cpu().numpy()
-    q = batch.geometry[0]
-    p = dict(nPix=N, pix_size=1.5, Cs=2.7, EkV=300.0, gaussEnv=np.inf, AmpContrast=0.1)
-    df = batch.df[:n_img].cpu().numpy()
-    ind = np.arange(n_img)
-    qq = np.tile(np.array([[1.0], [0], [0], [0]]), (1, n_img))
-    t0 = time.perf_counter()
-    od.op([ind, qq, df, None, 0], FILTER, stack, 2 * n_img, dict(avgOnly=True), p)
-    t_img = (time.perf_counter() - t0) / n_img
-    # Gram: float64 np.dot pair at n_g images, cost ~ n^2
-    n_g = 1000
-    rng = np.random.default_rng(0)
-    fy = (rng.normal(size=(n_g, N * N)) + 1j * rng.normal(size=(n_g, N * N)))
-    C = rng.normal(size=(n_g, N * N))
-    t0 = time.perf_counter()
-    od.gram_distance(fy, C)
-    t_gram_pair = (time.perf_counter() - t0) / (n_g * n_g)
-    del fy, C
-    # Ferguson: a 20-value sample of the eps sweep on a kernel-sized value list, scaled to 1501
-    n_ref = int(ns.max())
-    k_ref = int(batch.ks.max())
-    nnz = int(2 * n_ref * k_ref)
-    vals = rng.uniform(1e6, 2e6, nnz)
-    le = np.linspace(-150, 150, 20)
-    t0 = time.perf_counter()
-    odm.ferguson_sums(np.sqrt(vals), le)
-    t_fer_per_eps_value = (time.perf_counter() - t0) / (20 * nnz)
-    # eigsh on a sparse normalised kernel of the reference size
+def _blas_limit(threads):
+    from threadpoolctl import threadpool_limits
+    return threadpool_limits(limits=threads)
+
+
+def _cpu_unit_costs(stack, df, N, n_img, n_g, nnz, n_ref, k_ref, threads, seed=0):
+    """One bounded sample of the oracle's per-part costs on this host (float64 numpy/scipy, the
+    reference's operations): per-image prep (load, filter, two 3x3-tile spline rotations, CTF,
+    FFTs), the float64 Gram per image pair, the Ferguson sweep per (eps, kernel value) with
+    Python's builtin sum as the reference sums, and one eigsh on a kernel-sized sparse matrix.
+    Runs with `threads` BLAS threads."""
     import scipy.sparse as sp
     from scipy.sparse.linalg import eigsh
-    A = sp.random(n_ref, n_ref, density=min(1.0, 2 * k_ref / n_ref), random_state=1)
-    A = (A + A.T) / 2
+    from oracle import distance as od, dm as odm
+    with _blas_limit(threads):
+        p = dict(nPix=N, pix_size=1.5, Cs=2.7, EkV=300.0, gaussEnv=np.inf, AmpContrast=0.1)
+        ind = np.arange(n_img)
+        qq = np.tile(np.array([[1.0], [0], [0], [0]]), (1, n_img))
+        t0 = time.perf_counter()
+        od.op([ind, qq, df[:n_img], None, 0], FILTER, stack[:n_img], 2 * n_img,
+              dict(avgOnly=True), p)
+        t_img = (time.perf_counter() - t0) / n_img
+        rng = np.random.default_rng(seed)
+        fy = rng.normal(size=(n_g, N * N)) + 1j * rng.normal(size=(n_g, N * N))
+        C = rng.normal(size=(n_g, N * N))
+        t0 = time.perf_counter()
+        od.gram_distance(fy, C)
+        t_pair = (time.perf_counter() - t0) / (n_g * n_g)
+        del fy, C
+        vals = rng.uniform(1e6, 2e6, nnz)
+        le = np.linspace(-150, 150, 1501)[700:705]       # eps where every value is summed
+        t0 = time.perf_counter()
+        odm.ferguson_sums(np.sqrt(vals), le, builtin=True)
+        t_eps = (time.perf_counter() - t0) / (len(le) * nnz)
+        A = sp.random(n_ref, n_ref, density=min(1.0, 2 * k_ref / n_ref), random_state=1)
+        A = (A + A.T) / 2
+        t0 = time.perf_counter()
+        eigsh(A, k=11, maxiter=300)
+        t_eig = time.perf_counter() - t0
+    return dict(t_img=t_img, t_pair=t_pair, t_eps=t_eps, t_eig=t_eig)
+
+
+def _cpu_unit_costs_job(args):
+    return _cpu_unit_costs(*args)
+
+
+def _cpu_full_pd(raw, q, df, ind, nStot, N, k, n_eps, num_eigs, threads):
+    """The oracle's whole per-PD path (getDistanceCTF...op, then DMembeddingII.op with the
+    builtin-sum Ferguson sweep) on one PD with `threads` BLAS threads; returns seconds."""
+    from oracle import distance as od, dm as odm
+    with _blas_limit(threads):
+        p = dict(nPix=N, pix_size=1.5, Cs=2.7, EkV=300.0, gaussEnv=np.inf, AmpContrast=0.1)
+        t0 = time.perf_counter()
+        out = od.op([ind, q, df, None, 0], FILTER, raw, nStot, dict(avgOnly=False), p)
+        odm.op(out["D"], k, 3, 60000, num_eigs=num_eigs, rng=np.random.default_rng(0),
+               logEps=np.linspace(-150, 150, n_eps), builtin=True)
+        return time.perf_counter() - t0
+
+
+def _cpu_full_pd_job(args):
+    return _cpu_full_pd(*args)
+
+
+def _pool_run(fn, jobs):
+    """Run jobs in spawned worker processes (one per job) and return (results, wall seconds)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
     t0 = time.perf_counter()
-    eigsh(A, k=11, maxiter=300)
-    t_eig_ref = time.perf_counter() - t0
-    t_total = 0.0
-    for n, k in zip(ns, batch.ks):
-        nnz_p = 2 * n * min(k, n)
-        t_total += (n * t_img + n * n * t_gram_pair + 1501 * nnz_p * t_fer_per_eps_value +
-                    t_eig_ref * (n / n_ref) ** 2)
+    with ProcessPoolExecutor(len(jobs), mp_context=mp.get_context("spawn")) as ex:
+        res = list(ex.map(fn, jobs))
+    return res, time.perf_counter() - t0
+
+
+def cpu_baseline(meta, batch, workload):
+    """The float64 oracle (numpy/scipy restatement of the reference path) timed on this host's
+    cores in the reference's two execution modes: A, one process with every BLAS thread
+    (GetDistancesS2.py:102-108, ncpu = 1), and B, one single-threaded process per core working
+    on separate PDs (GetDistancesS2.py:110-120, the multiprocessing pool).  The one-PD
+    plain-method workload runs whole; the larger ones time a bounded sample of each part and
+    extrapolate with the cost model stated in `sample`."""
+    from threadpoolctl import threadpool_info
+    N = batch.N
+    ns = np.asarray(batch.ns)
     pairs = float(np.sum(ns * (ns - 1) / 2))
+    cores = os.cpu_count() or 1
     threads = sum(x.get("num_threads", 0) for x in threadpool_info()) or 1
-    sample = (f"oracle timed on host: per-image prep ({n_img} images, single thread), float64 "
-              f"Gram {n_g}^2 pairs (BLAS {threads} threads), Ferguson sweep 20 of 1501 eps on "
-              f"{nnz} values, eigsh k=11 on n={n_ref}; extrapolated per PD as "
-              f"n*t_img + n^2*t_pair + 1501*nnz*t_eps + t_eig*(n/{n_ref})^2")
-    return dict(value=pairs / t_total, unit="pairs/s", cores=int(threads), kind="port",
-                sample=sample, seconds_per_step=t_total)
+    procs = max(1, min(16, cores))
+    stack = batch.stack.cpu().numpy()
+    df = batch.df.cpu().numpy()
+    if workload == "c1":
+        n = int(ns[0])
+        ind = np.arange(n)
+        job = (stack[:n], batch.q_host[0], df[:n], ind, 2 * n, N, n, 101, 8)
+        tA = _cpu_full_pd(*job, threads)
+        tB, _ = _pool_run(_cpu_full_pd_job, [job + (1,)] * procs)
+        wB = max(tB)           # the processes run together; each reports its own PD's time
+        per_pd = pairs
+        return dict(value=per_pd / tA, unit="pairs/s", cores=int(threads), kind="port",
+                    sample=f"the whole one-PD path on the host, float64 oracle: stage 1 "
+                           f"(n={n}, {N}x{N}) and DMembeddingII with 101 eps (builtin-sum "
+                           f"Ferguson sweep), mode A ({threads} BLAS threads, {tA:.2f} s)",
+                    seconds_per_step=tA,
+                    mode_b=dict(value=procs * per_pd / wB, unit="pairs/s", processes=procs,
+                                wall_s=wB, note=f"{procs} single-threaded processes, one PD "
+                                                f"each, run together (no extrapolation; "
+                                                f"the slowest process's time)"))
+    n_ref, k_ref = int(ns.max()), int(max(batch.ks))
+    nnz = int(2 * n_ref * min(k_ref, n_ref))
+    n_img, n_g = 16, 600
+    args = (stack, df, N, n_img, n_g, nnz, n_ref, k_ref)
+    uA = _cpu_unit_costs(*args, threads)
+    resB, wB = _pool_run(_cpu_unit_costs_job, [args + (1, p) for p in range(procs)])
+    uB = {k: float(np.mean([r[k] for r in resB])) for k in resB[0]}
+
+    def per_step(u):
+        t = 0.0
+        for n, k in zip(ns, batch.ks):
+            n = int(n)
+            nnz_p = 2 * n * min(int(k), n)
+            t += (n * u["t_img"] + n * n * u["t_pair"] + 1501 * nnz_p * u["t_eps"] +
+                  u["t_eig"] * (n / n_ref) ** 2)
+        return t
+
+    tA, tB1 = per_step(uA), per_step(uB)
+    sample = (f"float64 oracle on this host, bounded sample extrapolated per PD as "
+              f"n t_img + n^2 t_pair + 1501 nnz t_eps + t_eig (n/{n_ref})^2: t_img from "
+              f"{n_img} images at {N}x{N} (load, filter, two 3x3-tile spline rotations, CTF, "
+              f"FFTs), t_pair from a {n_g}^2 float64 Gram, t_eps from 5 eps x {nnz} values "
+              f"summed with builtin sum (fergusonE.py:43), t_eig from eigsh k=11 at n={n_ref}; "
+              f"mode A {threads} BLAS threads")
+    return dict(value=pairs / tA, unit="pairs/s", cores=int(threads), kind="port",
+                sample=sample, seconds_per_step=tA, unit_costs_mode_a=uA,
+                mode_b=dict(value=procs * pairs / tB1, unit="pairs/s", processes=procs,
+                            seconds_per_step_one_process=tB1, unit_costs=uB,
+                            note=f"{procs} single-threaded processes timed together on the "
+                                 f"same sample, each working on its own PDs (throughput "
+                                 f"= {procs} x one process's, extrapolated as mode A)"))
 
 
 def _ctf():
@@ -481,46 +588,60 @@ def main():
         log("timing", args.workload)
         print(json.dumps((run_c4 if args.workload == "c4" else run_c5)(args, dev)), flush=True)
         return
-    log("building workload", args.workload)
-    batch, meta, num_eigs = build_workload(args.workload, dev)
-    torch.cuda.synchronize()
     ev = nat.HipEvents()
-    log("timing split-BF16")
-    graph = bool(args.graph) if args.graph is not None else args.workload == "c3"
-    bf = run_precision(batch, nat.PREC_SPLIT_BF16, args.steps, args.warmup, num_eigs, ev,
-                       graph=graph)
+    out = measure(args.workload, args, dev, ev, args.steps, args.warmup,
+                  with_fp32=not args.no_fp32, with_check=not args.no_check,
+                  with_cpu=not args.no_cpu)
+    if args.workload == "c2" and not args.no_c3:
+        # the batched 500-PD workload, measured in the same run (split-BF16 against FP32)
+        torch.cuda.empty_cache()
+        c3 = measure("c3", args, dev, ev, max(2, min(args.steps, 3)), 1, with_fp32=True,
+                     with_check=not args.no_check, with_cpu=False)
+        keep = ("value", "unit", "pds_per_s", "ms_per_step", "steps", "config", "gram_kernel_ms",
+                "stage_ms", "gram_tuned", "kappa_flagged", "roofline", "d_rel_err_vs_fp64",
+                "speedup_vs_fp32", "lanczos")
+        out["c3"] = {k: c3[k] for k in keep if k in c3}
+        if "fp32" in c3:
+            out["c3"]["fp32"] = {k: c3["fp32"][k] for k in ("value", "pds_per_s", "ms_per_step",
+                                                            "gram_kernel_ms", "roofline")}
+    ev.close()
+    print(json.dumps(out), flush=True)
+
+
+def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=True,
+            with_cpu=True):
+    """One JSON record for a distance + diffusion-map workload (c1, c2, c3)."""
+    import torch
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    log("building workload", workload)
+    batch, meta, num_eigs = build_workload(workload, dev)
+    torch.cuda.synchronize()
+    logEps = np.linspace(-150, 150, 101) if workload == "c1" else None
+    log("timing split-BF16", workload)
+    graph = bool(args.graph) if args.graph is not None else workload == "c3"
+    bf = run_precision(batch, nat.PREC_SPLIT_BF16, steps, warmup, num_eigs, ev, graph=graph,
+                       logEps=logEps, gram_config=args.gram_config)
     ns = batch.ns
     pairs = float(np.sum(ns * (ns - 1) / 2))
     Kh = pl.half_len(batch.N)
     flops = 8.0 * Kh * pairs                       # algorithmic: 8 K_h per unordered pair
     exec_flops = issued_gram_flops(bf["wsp"].plan, bf["wsp"].operand.Khp, 3)
-    traffic, traffic_note = None, None
-    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                         "k_gram_tiles_c2_traffic.json")
-    if args.workload == "c2" and os.path.exists(tfile):
-        with open(tfile) as f:
-            tr = json.load(f)
-        cfg = tr["configs"].get(f"v{bf['tuned']['best']}c{bf['tuned']['chunks']}")
-        if cfg is not None:        # the counters of the (variant, chunks) pair this run tuned to
-            traffic = cfg["bytes_per_launch"]
-            traffic_note = (f"rocprofv3 FETCH_SIZE x {tr['fetch_scale']:.3f} + WRITE_SIZE x "
-                            f"{tr['write_scale']:.3f} (copy-calibrated) per launch of variant "
-                            f"{cfg['variant']} with {cfg['chunks']} K chunks; {tr['source']}")
+    traffic, traffic_note = gram_traffic_lookup(workload, bf["tuned"])
     gram_s = bf["gram_ms"] / 1e3
     achieved = flops / gram_s / 1e12
     out = {
         "metric": "image-pair distances/s & PDs/s, split-BF16 vs FP32 at <=1e-5 rel err vs FP64",
-        "value": pairs * args.steps / bf["t"],
+        "value": pairs * steps / bf["t"],
         "unit": "pairs/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * bf["t"] / args.steps,
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * bf["t"] / steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3 (split) Gram, fp32 prep, fp64 eigensolve",
+        "dtype": "bf16x3 (split) Gram, fp32 prep, fp64 kernel + eigensolve",
         "data": "synthetic",
         "config": dict(meta, precision="split-bf16", hip_graph=graph,
-                       lanczos=dict(pl.LANCZOS)),
-        "pds_per_s": batch.ns.size * args.steps / bf["t"],
-        "distance_ms_per_step": 1e3 * bf["t_dist"] / args.steps,
+                       lanczos=dict(pl.LANCZOS), kernel_dtype=str(pl.KERNEL_DTYPE)),
+        "pds_per_s": batch.ns.size * steps / bf["t"],
+        "distance_ms_per_step": 1e3 * bf["t_dist"] / steps,
         "gram_kernel_ms": bf["gram_ms"],
         "stage_ms": bf["stages"],
         "gram_tuned": bf["tuned"],
@@ -530,19 +651,20 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
                      "traffic_note": traffic_note,
                      "kernel": "k_gram_tiles<split-bf16>",
+                     "flop_per_launch": flops,
                      "mfma_util": exec_flops / gram_s / 1e12 / PEAK_BF16_TFLOPS},
     }
-    if not args.no_check:
+    if with_check:
         out["d_rel_err_vs_fp64"] = fp64_error(batch, bf["wsp"], bf["dist"])
     del bf
-    if not args.no_fp32:
-        log("timing FP32")
-        f32 = run_precision(batch, nat.PREC_FP32, args.steps, args.warmup, num_eigs, ev,
-                            graph=graph)
+    if with_fp32:
+        log("timing FP32", workload)
+        f32 = run_precision(batch, nat.PREC_FP32, steps, warmup, num_eigs, ev, graph=graph,
+                            logEps=logEps)
         a32 = flops / (f32["gram_ms"] / 1e3) / 1e12
-        out["fp32"] = {"value": pairs * args.steps / f32["t"],
-                       "ms_per_step": 1e3 * f32["t"] / args.steps,
-                       "pds_per_s": batch.ns.size * args.steps / f32["t"],
+        out["fp32"] = {"value": pairs * steps / f32["t"],
+                       "ms_per_step": 1e3 * f32["t"] / steps,
+                       "pds_per_s": batch.ns.size * steps / f32["t"],
                        "gram_kernel_ms": f32["gram_ms"],
                        "stage_ms": f32["stages"],
                        "gram_tuned": f32["tuned"],
@@ -551,15 +673,34 @@ def main():
                                     "mfma_util": issued_gram_flops(
                                         f32["wsp"].plan, f32["wsp"].operand.Khp, 1)
                                     / (f32["gram_ms"] / 1e3) / 1e12 / PEAK_FP32_TFLOPS}}
-        if not args.no_check:
+        if with_check:
             out["fp32"]["d_rel_err_vs_fp64"] = fp64_error(batch, f32["wsp"], f32["dist"])
         out["speedup_vs_fp32"] = out["value"] / out["fp32"]["value"]
         del f32
-    ev.close()
-    if not args.no_cpu:
-        log("timing CPU baseline sample")
-        out["cpu_baseline"] = cpu_baseline(meta, batch)
-    print(json.dumps(out), flush=True)
+    if with_cpu:
+        log("timing CPU baseline", workload)
+        out["cpu_baseline"] = cpu_baseline(meta, batch, workload)
+    return out
+
+
+def gram_traffic_lookup(workload, tuned):
+    """HBM bytes per Gram launch for the (tile, variant, chunks) this run tuned to, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE table (a lookup, not a counter read of this
+    run; null when the table has no row for the configuration)."""
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "k_gram_tiles_traffic.json")
+    if not os.path.exists(tfile):
+        return None, None
+    with open(tfile) as f:
+        tr = json.load(f)
+    key = f"{workload}:t{tuned.get('tile', 128)}v{tuned['best']}c{tuned['chunks']}"
+    cfg = tr["configs"].get(key)
+    if cfg is None:
+        return None, f"no counter row for {key} in profiles/k_gram_tiles_traffic.json"
+    return cfg["bytes_per_launch"], (
+        f"lookup: rocprofv3 FETCH_SIZE x {tr['fetch_scale']:.3f} + WRITE_SIZE x "
+        f"{tr['write_scale']:.3f} (copy-calibrated) per launch of {key}, measured by "
+        f"{tr['source']}")
 
 
 if __name__ == "__main__":

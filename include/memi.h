@@ -96,6 +96,30 @@ const char* memi_last_error(void);
 int memi_load_normalize(const float* stack, int64_t n_stack, const int64_t* src,
                         const int32_t* conj, int64_t n, int N, float* out, void* stream);
 
+/* Sub-pixel shift of RELION particle images, getDistanceCTF_local_Conj9combinedS2.py:259-264:
+ * out[i] = scipy.ndimage.shift(in[i], (shift0[i], shift1[i]), order=3, mode='wrap') (the
+ * reference passes (sh[1] - 0.5, sh[0] - 0.5)): mirror-boundary cubic B-spline coefficients,
+ * coordinates outside [0, N-1] wrapped with period N-1, mirrored taps; float64 arithmetic.
+ * in: float32 [n][N][N] (mrcfile's image layout); out: float32 [n][N][N], transposed when
+ * transpose_out != 0 (the SPIDER stack layout the loaders above read); in and out must not
+ * alias.  lu: float64 [2N] LU factors of the mirror spline system, {m[0..N-1], 1/pivot[0..N-1]}
+ * (diagonal 4, off-diagonals 1, first row [4 2], last row [2 4]); ws: float64 scratch of
+ * n N^2 doubles. */
+int memi_shift_images(const float* in, int64_t n, int N, const double* shift0,
+                      const double* shift1, const double* lu, int transpose_out, float* out,
+                      double* ws, int64_t ws_doubles, void* stream);
+
+/* Volume-mask projection of one PD, CC/projectMask.py:118-140 (getDistanceCTF...:304-310):
+ * proj[j][i] = sum_k f(T (i, j, k) + t) with f scipy's affine_transform interpolant of the
+ * (already axis-swapped, CC/projectMask.py:120) volume vol float32 [L][L][L] -- order-3 spline,
+ * mode 'nearest': the volume padded by 12 edge samples per side, half-sample-reflection spline
+ * coefficients of the padded array, taps clamped to it -- and mask = proj > 1 as 1.0 / 0.0.
+ * T: float64 [12] = {the 3x3 matrix row-major, the offset t}; lu: float64 [2(L+24)] LU factors
+ * of the reflection spline system of length L + 24 (first / last rows [5 1] / [1 5]);
+ * proj: float64 [L][L]; mask: float32 [L][L]; ws: float64 scratch of (L+24)^3 doubles. */
+int memi_project_mask(const float* vol, int L, const double* T, const double* lu, double* proj,
+                      float* mask, double* ws, int64_t ws_doubles, void* stream);
+
 /* Spectral multiply on rfft2 spectra, getDistanceCTF_local_Conj9combinedS2.py:285-293
  * (create_grid/create_filter :139-167) and the cubic-spline prefilter of rotatefill.py:24
  * (scipy ndimage.rotate order 3): kind is an OR of MEMI_FILTER_* flags. */

@@ -58,6 +58,8 @@ _SIGS = {
     "memi_prep_images": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dbl,
                          c_int, ctypes.POINTER(CtfParams), c_int, c_i64, c_i64, c_vp, c_vp, c_vp,
                          c_vp, c_vp, c_vp, c_i64, c_vp],
+    "memi_shift_images": [c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_i64, c_vp],
+    "memi_project_mask": [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp],
     "memi_prep_images_noalloc": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int,
                                  c_dbl, c_int, ctypes.POINTER(CtfParams), c_int, c_i64, c_i64,
                                  c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
@@ -293,6 +295,78 @@ def prep_images_noalloc(stack, src, conj, rot1, rot2, df, N, filter_kind, Qc, or
          float(Qc), int(order), ctypes.byref(ctf), int(precision), int(Khp), int(row0),
          _p(op_hi), _p(op_lo) if op_lo is not None else None, _p(fy), _p(ctf_half),
          None if img_out is None else _p(img_out), _stream())
+
+
+_SPLINE_LU = {}
+
+
+def spline_lu(N, boundary="mirror"):
+    """LU factors {m, 1/pivot} (float64 [2N]) of the cubic B-spline coefficient system
+    (c[k-1] + 4 c[k] + c[k+1]) = 6 x[k] with the whole-sample mirror extension (c[-1] = c[1],
+    c[N] = c[N-2]; boundary rows [4 2] / [2 4]) or the half-sample reflection (c[-1] = c[0],
+    c[N] = c[N-1]; rows [5 1] / [1 5])."""
+    key = (N, boundary)
+    lu = _SPLINE_LU.get(key)
+    if lu is None:
+        if boundary == "mirror":
+            d0, up0, low_last, dlast = 4.0, 2.0, 2.0, 4.0
+        elif boundary == "reflect":
+            d0, up0, low_last, dlast = 5.0, 1.0, 1.0, 5.0
+        else:
+            raise ValueError(boundary)
+        m = np.zeros(N)
+        piv = np.zeros(N)
+        piv[0] = d0
+        for k in range(1, N):
+            a = low_last if k == N - 1 else 1.0
+            cprev = up0 if k == 1 else 1.0
+            m[k] = a / piv[k - 1]
+            piv[k] = (dlast if k == N - 1 else 4.0) - m[k] * cprev
+        lu = _SPLINE_LU[key] = np.concatenate([m, 1.0 / piv])
+    return lu
+
+
+MASK_NPAD = 12
+
+
+def project_mask(vol_swapped, T):
+    """memi_project_mask: (proj float64 (L, L), mask float32 (L, L)) of an axis-swapped volume
+    (float32 (L, L, L) device tensor) under the affine map T (float64 (12,) host array)."""
+    L = int(vol_swapped.shape[0])
+    _need(vol_swapped.dtype == torch.float32 and tuple(vol_swapped.shape) == (L, L, L) and
+          vol_swapped.is_contiguous(), "project_mask: volume must be contiguous float32 (L,L,L)")
+    dev = vol_swapped.device
+    Lp = L + 2 * MASK_NPAD
+    lu = torch.as_tensor(spline_lu(Lp, "reflect"), device=dev)
+    Td = torch.as_tensor(np.asarray(T, dtype=np.float64).reshape(12), device=dev)
+    proj = torch.empty((L, L), dtype=torch.float64, device=dev)
+    mask = torch.empty((L, L), dtype=torch.float32, device=dev)
+    ws = torch.empty(Lp ** 3, dtype=torch.float64, device=dev)
+    call("memi_project_mask", _p(vol_swapped), L, _p(Td), _p(lu), _p(proj), _p(mask), _p(ws),
+         ws.numel(), _stream())
+    return proj, mask
+
+
+def shift_images(img, shift0, shift1, out, transpose_out=True, chunk=8192):
+    """memi_shift_images in chunks of images (bounded float64 scratch): out[i] =
+    ndimage.shift(img[i], (shift0[i], shift1[i]), order=3, mode='wrap'), transposed when
+    transpose_out.  img, out: float32 (n, N, N) device tensors; shift0/1: float64 (n,)."""
+    n, N = int(img.shape[0]), int(img.shape[1])
+    _need(img.dtype == torch.float32 and tuple(img.shape[1:]) == (N, N) and img.is_contiguous(),
+          "shift_images: img must be contiguous float32 (n, N, N)")
+    _need(out.dtype == torch.float32 and out.shape == img.shape and out.is_contiguous(),
+          "shift_images: out must match img")
+    _need(out.data_ptr() != img.data_ptr(), "shift_images: in and out alias")
+    for t in (shift0, shift1):
+        _need(t.dtype == torch.float64 and t.numel() == n, "shift_images: shifts float64 (n,)")
+    lu = torch.as_tensor(spline_lu(N), device=img.device)
+    step = max(1, min(n, chunk))
+    ws = torch.empty(step * N * N, dtype=torch.float64, device=img.device)
+    for i0 in range(0, n, step):
+        i1 = min(n, i0 + step)
+        call("memi_shift_images", _p(img[i0:i1]), i1 - i0, N, _p(shift0[i0:i1]),
+             _p(shift1[i0:i1]), _p(lu), int(bool(transpose_out)), _p(out[i0:i1]), _p(ws),
+             ws.numel(), _stream())
 
 
 def ctf_params(pix_size, cs, ekv, ampc, gauss_env):

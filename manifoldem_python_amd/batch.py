@@ -136,7 +136,7 @@ def embed(wsp, dist, tune=3, num_eigs=15, logEps=pl.LOG_EPS, rand=None, timer=No
     return pl.dm_embed(wsp.plan, dist.D, tune, num_eigs, logEps, rand=rand, timer=timer)
 
 
-def breakdown(batch, wsp, num_eigs, rand):
+def breakdown(batch, wsp, num_eigs, rand, logEps=pl.LOG_EPS):
     """Wall time per stage of one step (synchronising between stages), in ms."""
     tm = pl.StageTimer()
     tm.start()
@@ -145,5 +145,5 @@ def breakdown(batch, wsp, num_eigs, rand):
     dist = pl.ctf_distances(wsp.plan, wsp.operand, wsp.fy, wsp.ctf, batch.N, ws=wsp.ws,
                             flags=wsp.flags)
     tm.mark("gram")
-    embed(wsp, dist, num_eigs=num_eigs, rand=rand, timer=tm)
+    embed(wsp, dist, num_eigs=num_eigs, rand=rand, timer=tm, logEps=logEps)
     return {k: 1e3 * v for k, v in tm.t.items()}

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libmemi.so")
-SOURCES = ["common.cpp", "prep.hip", "gram.hip", "dm.hip", "eig.hip", "nlsa.hip", "grad.hip"]
+SOURCES = ["common.cpp", "prep.hip", "gram.hip", "dm.hip", "eig.hip", "nlsa.hip", "grad.hip",
+           "ingest.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -6,9 +6,9 @@ progress marker (``:409-419``).  The work runs on the GPU: image preparation as 
 transforms with HIP kernels between them, the distance Gram on the MFMA units
 (``p.precision``: split-BF16 or FP32), averages as HIP reductions.
 
-Not supported (raises NotImplementedError): ``options['relion_data']`` (MRCS input with the
-``ndimage.shift`` recentring, ``:259-264``; mrcfile is not available) and a volume mask
-``p.mask_vol_file`` (``projectMask``, ``:304-310``).
+RELION input (``options['relion_data']``, ``:259-264``) is read from the MRCS stack with numpy
+(``mrc.py``) and shifted on the GPU (``memi_shift_images``); a volume mask ``p.mask_vol_file``
+(``:304-310``) is projected along the PD on the GPU (``memi_project_mask``, ``projectMask``).
 """
 from __future__ import annotations
 
@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from . import myio, p
+from . import mrc, myio, p
 from . import pipeline as pl
 
 version = "getDistanceCTF_local9, V 1.0"
@@ -70,8 +70,6 @@ def compute(ind, q, df, filterPar, stack, nStot, options, N=None, precision=None
     device tensor) indexed by the reference's conjugate convention."""
     dev = pl.device()
     N = p.nPix if N is None else N
-    if p.mask_vol_file:
-        raise NotImplementedError("volume masks (p.mask_vol_file) are not supported")
     ind = np.asarray(ind)
     nS = ind.shape[0]
     conj = ~(ind < nStot / 2)
@@ -83,6 +81,11 @@ def compute(ind, q, df, filterPar, stack, nStot, options, N=None, precision=None
         stack_d = torch.as_tensor(np.ascontiguousarray(stack[uniq]), device=dev)
         src = inv.astype(np.int64)
     geo = pl.pd_geometry(q)
+    # volume mask projected along PD (:304-310), or none
+    msk2_d, msk2 = None, 1
+    if p.mask_vol_file:
+        msk2_d, _ = pl.volume_mask(mrc.read_volume(p.mask_vol_file), geo["PD"])
+        msk2 = msk2_d.cpu().numpy() > 0
     ctf = nat.ctf_params(p.pix_size, p.Cs, p.EkV, p.AmpContrast, p.gaussEnv)
     prec = precision_code(precision)
     op = pl.Operand(nS, N, prec, dev)
@@ -91,8 +94,8 @@ def compute(ind, q, df, filterPar, stack, nStot, options, N=None, precision=None
                             torch.as_tensor(-(180 / math.pi) * geo["Psi"], device=dev),
                             torch.full((nS,), -geo["psi_p"], dtype=torch.float64, device=dev),
                             torch.as_tensor(np.asarray(df, dtype=np.float64), device=dev), N,
-                            filterPar, ctf, operand=op)
-    avg = pl.averages(out["imgAll"], out["fy"], out["ctf"], N)
+                            filterPar, ctf, operand=op, msk2=msk2_d)
+    avg = pl.averages(out["imgAll"], out["fy"], out["ctf"], N, msk2=msk2_d)
     if options.get("avgOnly", False):
         D = np.zeros((nS, nS))
     else:
@@ -101,7 +104,7 @@ def compute(ind, q, df, filterPar, stack, nStot, options, N=None, precision=None
         D = D.double().cpu().numpy()
     return dict(D=D, ind=ind, q=q, df=df,
                 CTF=pl.full_ctf(out["ctf"], N).double().cpu().numpy(),
-                imgAll=out["imgAll"].double().cpu().numpy(), msk2=1, PD=geo["PD"],
+                imgAll=out["imgAll"].double().cpu().numpy(), msk2=msk2, PD=geo["PD"],
                 PDs=geo["PDs"], Psis=geo["Psi"].reshape(-1, 1),
                 imgAvg=avg["imgAvg"].cpu().numpy(), imgAvgFlip=avg["imgAvgFlip"].cpu().numpy(),
                 imgAllFlip=avg["imgAllFlip"].double().cpu().numpy(),
@@ -122,16 +125,27 @@ def write(outFile, res, prD):
     open(os.path.join(p.dist_prog, "%s" % (prD,)), "a").close()
 
 
+def read_relion(imgFileName, src, sh):
+    """RELION particles ``src`` of an MRCS stack (``:259-264``): read, then shifted by
+    (sh[1] - 0.5, sh[0] - 0.5) with the order-3 'wrap' spline on the GPU; returned as a
+    SPIDER-layout device stack."""
+    img = mrc.read_images(imgFileName, src)
+    s0 = np.asarray(sh[1], dtype=np.float64)[src] - 0.5
+    s1 = np.asarray(sh[0], dtype=np.float64)[src] - 0.5
+    return pl.relion_stack(img, s0, s1)
+
+
 def op(input_data, filterPar, imgFileName, sh, nStot, options):
     ind, q, df, outFile, prD = input_data[:5]
-    if options.get("relion_data", False):
-        raise NotImplementedError("RELION/MRCS input (mrcfile + ndimage.shift) is not supported")
     N = p.nPix
     ind = np.asarray(ind)
     conj = ~(ind < nStot / 2)
     src = np.where(conj, ind - nStot // 2, ind).astype(np.int64)
     uniq, inv = np.unique(src, return_inverse=True)
-    stack = read_spider(imgFileName, uniq, N)
+    if options.get("relion_data", False):
+        stack = read_relion(imgFileName, uniq, sh)
+    else:
+        stack = read_spider(imgFileName, uniq, N)
     # compute() re-derives src from ind; give it the compact stack with remapped indices
     ind_c = np.where(conj, inv + nStot // 2, inv)
     res = compute(ind_c, q, df, filterPar, stack, nStot, options, N=N)

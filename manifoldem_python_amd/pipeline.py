@@ -79,6 +79,13 @@ def _qmult(q, s):
 def psi_ang(PD):
     """``getDistanceCTF...:206-213`` with ``q2Spider.py:19-60``: ZYZ angles by the same TRF
     least-squares fit from x0 = 0 (ftol 1e-12); returns psi in degrees, mod 360."""
+    return np.mod(spider_angles(PD)[2], 2 * np.pi) * (180 / np.pi)
+
+
+def spider_angles(PD):
+    """(phi, theta, psi) in radians of the rotation taking the z axis to PD: ``q2Spider.op`` of
+    the normalised Qr = [1 + PD_2, PD_1, -PD_0, 0] (``getDistanceCTF...:206-213``,
+    ``CC/projectMask.py:76-79``), the TRF least-squares fit from x0 = 0 (ftol 1e-12)."""
     Qr = np.array([1 + PD[2], PD[1], -PD[0], 0.0])
     q = Qr / np.sqrt(np.sum(Qr ** 2))
 
@@ -89,7 +96,38 @@ def psi_ang(PD):
         return q - _qmult(q3, _qmult(q2, q1))
 
     res = optimize.least_squares(dev1, np.array([0, 0, 0]), bounds=(-np.inf, np.inf), ftol=1e-12)
-    return np.mod(res.x[2], 2 * np.pi) * (180 / np.pi)
+    return res.x[0], res.x[1], res.x[2]
+
+
+def spider_rotation(Phi, Theta, Psi):
+    """``CC/projectMask.py:24-37`` (eulerRotMatrix3DSpider, radians)."""
+    cf, sf, ct, st, cp, sp = (np.cos(Phi), np.sin(Phi), np.cos(Theta), np.sin(Theta),
+                              np.cos(Psi), np.sin(Psi))
+    return np.array([[cf * cp * ct - sf * sp, cp * ct * sf + cf * sp, -cp * st],
+                     [-cp * sf - cf * ct * sp, cf * cp - ct * sf * sp, sp * st],
+                     [cf * st, sf * st, ct]])
+
+
+def mask_transform(PD, L):
+    """The affine map of ``CC/projectMask.py:40-53,116-127`` for PD: the Euler angles of PD with
+    psi set to 0 and negated, R = eulerRotMatrix3DSpider(0, -theta, -phi), offset = c - R c with
+    c = L/2 on every axis; returned as float64 [12] = {R row-major, offset}."""
+    phi, theta, _ = spider_angles(PD)
+    R = spider_rotation(0.0, -theta, -phi)
+    c = 0.5 * np.array([L, L, L], dtype=np.float64)
+    return np.concatenate([R.ravel(), c - R @ c])
+
+
+def volume_mask(vol, PD):
+    """msk2 of ``getDistanceCTF...:304-310`` on the GPU: ``projectMask.op(vol, PD)`` for a
+    cubic mask volume vol (numpy, mrcfile's (z, y, x) orientation).  Returns (mask float32
+    (L, L) device tensor of 1.0 / 0.0, projection sums float64 device tensor)."""
+    L = vol.shape[0]
+    if vol.shape != (L, L, L):
+        raise ValueError(f"volume_mask: the mask volume must be cubic, got {vol.shape}")
+    sw = np.ascontiguousarray(np.swapaxes(np.asarray(vol, dtype=np.float32), 0, 2))
+    proj, mask = nat.project_mask(torch.as_tensor(sw, device=device()), mask_transform(PD, L))
+    return mask, proj
 
 
 def pd_geometry(q):
@@ -206,6 +244,21 @@ def prepare_images(stack, src, conj, rot1_deg, rot2_deg, df, N, filterPar, ctf, 
         tmp = Operand(n, N, nat.PREC_FP32, dev)
         nat.ctf_pack(fy, df, N, ctf, nat.PREC_FP32, tmp.Khp, 0, tmp.hi, tmp.hi, ctf_half)
     return dict(imgAll=y if want_images else None, fy=fy, ctf=ctf_half)
+
+
+def relion_stack(images, shift0, shift1):
+    """RELION particle images (mrcfile's layout, float32 (m, N, N), numpy or device tensor)
+    shifted as getDistanceCTF_local_Conj9combinedS2.py:259-264 does (ndimage.shift by
+    (shift0, shift1) = (sh[1] - 0.5, sh[0] - 0.5), order 3, mode 'wrap') on the GPU, returned
+    transposed: the SPIDER stack layout the loaders read (they transpose it back)."""
+    dev = device()
+    img = torch.as_tensor(np.ascontiguousarray(images, dtype=np.float32) if
+                          isinstance(images, np.ndarray) else images, device=dev).contiguous()
+    out = torch.empty_like(img)
+    s0 = torch.as_tensor(np.asarray(shift0, dtype=np.float64), device=dev)
+    s1 = torch.as_tensor(np.asarray(shift1, dtype=np.float64), device=dev)
+    nat.shift_images(img, s0, s1, out, transpose_out=True)
+    return out
 
 
 def averages(imgAll, fy, ctf_half, N, msk2=None, snr=5.0):

@@ -103,6 +103,37 @@ def psi_ang(PD):
     return np.mod(psi, 2 * np.pi) * (180 / np.pi)
 
 
+def spider_rotation(Phi, Theta, Psi):
+    """``CC/projectMask.py:24-37`` (eulerRotMatrix3DSpider, radians)."""
+    from math import cos, sin
+    return np.array([[cos(Phi) * cos(Psi) * cos(Theta) + (-1) * sin(Phi) * sin(Psi),
+                      cos(Psi) * cos(Theta) * sin(Phi) + cos(Phi) * sin(Psi),
+                      (-1) * cos(Psi) * sin(Theta)],
+                     [(-1) * cos(Psi) * sin(Phi) + (-1) * cos(Phi) * cos(Theta) * sin(Psi),
+                      cos(Phi) * cos(Psi) + (-1) * cos(Theta) * sin(Phi) * sin(Psi),
+                      sin(Psi) * sin(Theta)],
+                     [cos(Phi) * sin(Theta), sin(Phi) * sin(Theta), cos(Theta)]])
+
+
+def project_mask(vol, PD):
+    """``CC/projectMask.py:118-140`` (projectMask.op) in float64: swap axes 0 and 2, rotate by
+    the inverse of PD's Euler angles with psi = 0 (scipy affine_transform, order 3, mode
+    'nearest', about the centre dims/2), sum along the last axis, transpose, threshold > 1.
+    Returns (projection sums, mask)."""
+    from scipy.ndimage import affine_transform
+    vol = np.swapaxes(np.asarray(vol, dtype=np.float64), 0, 2)
+    nPix = vol.shape[0]
+    Qr = np.array([1 + PD[2], PD[1], -PD[0], 0])
+    phi, theta, psi = q2spider(Qr / np.sqrt(np.sum(Qr ** 2)))
+    sym = -np.array([phi, theta, 0.0])
+    T = spider_rotation(sym[2], sym[1], sym[0])
+    c = 0.5 * np.array(vol.shape)
+    rho = affine_transform(input=vol, matrix=T, offset=c - T @ c, output_shape=vol.shape,
+                           mode="nearest")
+    proj = np.sum(rho, axis=2).reshape(nPix, nPix).T
+    return proj, proj > 1
+
+
 def rotatefill(img, angle):
     """``rotatefill.py:21-28``: order-3 spline rotation of the 3x3-tiled image, centre crop."""
     n = img.shape[0]
@@ -129,6 +160,20 @@ def load_normalized(raw, src, conj, msk):
     """``getDistanceCTF...:246-281`` for one SPIDER image: ``.T``, flipud, background norm
     with mean/std over the full N^2 array (ddof 0)."""
     tmp = np.asarray(raw[src], dtype=np.float64).T
+    if conj:
+        tmp = np.flipud(tmp)
+    backg = tmp * (1 - msk)
+    return (tmp - backg.mean()) / backg.std()
+
+
+def load_normalized_relion(data, src, conj, msk, sh):
+    """``getDistanceCTF...:259-281`` for one RELION image: ``mrcfile`` data[src] (no transpose),
+    ``ndimage.shift`` by (sh[1][src] - 0.5, sh[0][src] - 0.5) with order 3 and mode 'wrap',
+    flipud for a conjugate, background normalisation as ``load_normalized``."""
+    from scipy.ndimage import shift
+    tmp = np.asarray(data[src])
+    tmp = shift(tmp, (sh[1][src] - 0.5, sh[0][src] - 0.5), order=3, mode="wrap")
+    tmp = np.asarray(tmp, dtype=np.float64)
     if conj:
         tmp = np.flipud(tmp)
     backg = tmp * (1 - msk)
@@ -166,12 +211,14 @@ def pd_geometry(q):
     return PDs, PD, psi_ang(PD)
 
 
-def prep_rows(input_data, filterPar, raw, nStot, params, rows, geometry=None, msk2=1):
+def prep_rows(input_data, filterPar, raw, nStot, params, rows, geometry=None, msk2=1,
+              relion_sh=None):
     """a1-a6 of ``getDistanceCTF...:244-349`` for the images ``rows`` of one PD only (every
     image is processed on its own in the reference; only PD and psi_p use all of q).
 
-    Returns a dict of per-row arrays: imgAll, fy, CTF (N x N, fftshift-free), imgAllFlip,
-    Psis, Nom, Dnom, imgLabels."""
+    raw: the SPIDER stack, or with relion_sh = (shx, shy) the RELION stack in mrcfile's layout
+    (``:259-264``).  Returns a dict of per-row arrays: imgAll, fy, CTF (N x N, fftshift-free),
+    imgAllFlip, Psis, Nom, Dnom, imgLabels."""
     ind, q, df = input_data[0], input_data[1], input_data[2]
     N = params["nPix"]
     N2 = N / 2.0
@@ -189,7 +236,10 @@ def prep_rows(input_data, filterPar, raw, nStot, params, rows, geometry=None, ms
         conj = not (ind[iS] < nStot / 2)
         src = int(ind[iS] - nStot / 2) if conj else int(ind[iS])
         out["imgLabels"][r] = -1 if conj else 1
-        y = load_normalized(raw, src, conj, msk)
+        if relion_sh is None:
+            y = load_normalized(raw, src, conj, msk)
+        else:
+            y = load_normalized_relion(raw, src, conj, msk, relion_sh)
         y = ifft2(fft2(y) * G).real
         Psi, s, c = get_psi(q, PD, iS)
         if np.isnan(Psi):
@@ -267,7 +317,7 @@ def phase_flip(fy, C):
     return ifft2(np.sign(C) * fy).real
 
 
-def op(input_data, filterPar, raw, nStot, options, params, msk2=1):
+def op(input_data, filterPar, raw, nStot, options, params, msk2=1, sh=None):
     """``getDistanceCTF_local_Conj9combinedS2.op`` (``:216-420``) returning the 19-key dict
     it pickles instead of writing it.
 
@@ -278,7 +328,9 @@ def op(input_data, filterPar, raw, nStot, options, params, msk2=1):
     nS = ind.shape[0]
     N = params["nPix"]
     PDs, PD, psi_p = geometry = pd_geometry(q)
-    pr = prep_rows(input_data, filterPar, raw, nStot, params, np.arange(nS), geometry, msk2)
+    relion = sh if options.get("relion_data", False) else None
+    pr = prep_rows(input_data, filterPar, raw, nStot, params, np.arange(nS), geometry, msk2,
+                   relion)
     imgAll, fy, CTF, imgAllFlip = pr["imgAll"], pr["fy"], pr["CTF"], pr["imgAllFlip"]
     imgAvg, imgAvgFlip, imgAllIntensity = averages(imgAll, imgAllFlip, CTF, msk2)
     D = np.zeros((nS, nS))

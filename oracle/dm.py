@@ -6,8 +6,9 @@ Restates ``DMembeddingII.py:16-185``, ``fergusonE.py:21-59``, ``slaplacianonFly.
 ``sembeddingonFly.py:17-39`` and the trimming loop of ``manifoldTrimmingAuto.py:38-95``.
 
 Deviations (numerical only, documented):
-  * ``fergusonE`` sums with ``np.sum`` (pairwise) where the reference uses the Python builtin
-    ``sum`` (sequential); both are float64, difference <= 1e-13 relative.
+  * ``fergusonE`` sums with ``np.sum`` (pairwise) by default where the reference uses the
+    Python builtin ``sum`` (sequential); both are float64, difference <= 1e-13 relative.
+    ``builtin=True`` sums as the reference does (the CPU baseline times that form).
   * the unseeded ``np.random.rand`` restarts of the reference (``DMembeddingII.py:142``,
     ``fergusonE.py:52``) take an explicit ``rng`` here so tests are reproducible.
   * ``eigsh`` may be replaced by a dense ``eigh`` (``dense=True``) with the same
@@ -95,14 +96,17 @@ def find_thres(logEps, D2):
     return max(-np.log(0.01 * ss / len(D2)), 10)
 
 
-def ferguson_sums(Dv, logEps):
-    """``fergusonE.py:33-43``: logSumWij[k] = log sum_{d < thr} exp(-d), d = D^2/(2 eps_k)."""
+def ferguson_sums(Dv, logEps, builtin=False):
+    """``fergusonE.py:33-43``: logSumWij[k] = log sum_{d < thr} exp(-d), d = D^2/(2 eps_k).
+    builtin: sum with Python's builtin ``sum`` over the array as the reference does (``:43``;
+    sequential float64 adds, slower) instead of ``np.sum`` (pairwise)."""
     D2 = Dv * Dv
     thr = find_thres(logEps, D2)
     out = np.zeros(len(logEps))
     for k in range(len(logEps)):
         d = 1.0 / (2.0 * np.exp(logEps[k])) * D2
-        out[k] = np.log(np.sum(np.exp(-d[d < thr])))
+        w = np.exp(-d[d < thr])
+        out[k] = np.log(sum(w) if builtin else np.sum(w))
     return out, thr
 
 
@@ -120,9 +124,9 @@ def ferguson_fit(logEps, logSumWij, a0, rng):
     return popt, resnorm, R_squared
 
 
-def fergusonE(Dv, logEps, a0, rng):
+def fergusonE(Dv, logEps, a0, rng, builtin=False):
     """``fergusonE.op`` -> (popt, logSumWij, resnorm, R_squared)."""
-    logSumWij, _ = ferguson_sums(Dv, logEps)
+    logSumWij, _ = ferguson_sums(Dv, logEps, builtin)
     popt, resnorm, R2 = ferguson_fit(logEps, logSumWij, a0, rng)
     return popt, logSumWij, resnorm, R2
 
@@ -158,7 +162,7 @@ def top_eigs(l, k, dense=False):
     return lam[ix], V[:, ix]
 
 
-def op(D, k, tune, prefsigma, num_eigs=15, rng=None, dense=False, logEps=None):
+def op(D, k, tune, prefsigma, num_eigs=15, rng=None, dense=False, logEps=None, builtin=False):
     """``DMembeddingII.op`` -> (lamb, psi, sigma, mu, logEps, logSumWij, popt, R_squared).
 
     D is copied (the reference mutates the caller's diagonal in place)."""
@@ -168,7 +172,7 @@ def op(D, k, tune, prefsigma, num_eigs=15, rng=None, dense=False, logEps=None):
     yRow, yCol, yVal = assemble(D, k)
     a0 = rng.random((4, 1)) - 0.5
     logEps = LOG_EPS if logEps is None else logEps
-    popt, logSumWij, resnorm, R2 = fergusonE(np.sqrt(yVal), logEps, a0, rng)
+    popt, logSumWij, resnorm, R2 = fergusonE(np.sqrt(yVal), logEps, a0, rng, builtin)
     nEigs = min(num_eigs, nS - 3)
     sigma = tune * np.sqrt(2 * np.exp(-popt[1] / popt[0]))
     l = slaplacian(yVal, yCol, yRow, nS, sigma)
