@@ -296,6 +296,15 @@ int memi_markov_normalize(const void* plan_dev, const memi_plan_info* info, cons
 int memi_ferguson_values(const double* vals, int64_t nv, const double* logEps, int n_eps,
                          double* out, void* stream);
 
+/* Sub-matrices for trimming rounds (manifoldTrimmingAuto.py:63-65, D1 = D[posPath1][:, posPath1]
+ * for every PD of a batch at once): block p of dst (the plan's n_p x n_p block at d_off) gets
+ * src[src_off[p] + sel[row0_p + a] * src_n[p] + sel[row0_p + b]] at (a, b).  src_off int64
+ * [npd], src_n int32 [npd] and sel int32 [n_rows] are device arrays; the caller guarantees
+ * 0 <= sel < src_n[p] and that each source block lies inside src. */
+int memi_gather_blocks(const void* plan_dev, const memi_plan_info* info, const float* src,
+                       const int64_t* src_off, const int32_t* src_n, const int32_t* sel,
+                       float* dst, void* stream);
+
 /* slaplacianonFly.py:42-78 on COO triplets (rows, cols, squared distances): out[e] =
  * W_e / (d_r d_c) / (d'_r d'_c) with W = exp(-val/sigma^2), d = column sums (^alpha),
  * d' = sqrt(column sums of W/(d d)); duplicates behave as the reference's csc sums.

@@ -3,8 +3,8 @@
 ``op(NLSAPar, DD, posPath, posPsi1, imgAll, msk2, CTF, ExtPar)`` keeps the reference signature
 and returns ``(IMGT, Topo_mean, psirec, psiC1, sdiag, VX, mu, tau)``.  DD is the distance matrix
 already sorted along psi; imgAll / CTF are the full per-PD stacks indexed by posPath[posPsi1].
-Only the reference's 'prD' branch (Wiener-filtered supervectors) is implemented; the 'cuti'
-branch raises NotImplementedError.
+With 'prD' in ExtPar the supervectors are the Wiener-filtered, masked images posPath[posPsi1]
+(``NLSA.py:52-53``); with 'cuti' they are the raw images imgAll[posPsi1] (``NLSA.py:54-55``).
 """
 from __future__ import annotations
 
@@ -22,15 +22,20 @@ def _ctf_half(CTF, N):
 
 
 def op(NLSAPar, DD, posPath, posPsi1, imgAll, msk2, CTF, ExtPar):
-    if "prD" not in ExtPar:
-        raise NotImplementedError("only the Wiener-filtered ('prD') branch of NLSA is supported")
+    if "prD" in ExtPar:
+        wiener = True
+    elif "cuti" in ExtPar:
+        wiener = False
+    else:
+        raise ValueError("NLSA.op: ExtPar names neither 'prD' nor 'cuti'")
     dev = pl.device()
     num, ConOrder = NLSAPar["num"], NLSAPar["ConOrder"]
     nS, psiTrunc, tune = NLSAPar["nS"], NLSAPar["psiTrunc"], NLSAPar["tune"]
     imgAll = np.asarray(imgAll)
     N = imgAll.shape[1]
-    idx = np.asarray(posPath)[np.asarray(posPsi1)].astype(np.int64)
-    m2 = None if np.isscalar(msk2) or np.ndim(msk2) == 0 else torch.as_tensor(
+    idx = (np.asarray(posPath)[np.asarray(posPsi1)] if wiener else
+           np.asarray(posPsi1)).astype(np.int64)
+    m2 = None if not wiener or np.ndim(msk2) == 0 else torch.as_tensor(
         np.asarray(msk2, dtype=np.float32), device=dev)
     r = spectral.nlsa(torch.as_tensor(np.ascontiguousarray(DD, dtype=np.float64), device=dev),
                       torch.arange(num, dtype=torch.int64, device=dev),
@@ -38,7 +43,7 @@ def op(NLSAPar, DD, posPath, posPsi1, imgAll, msk2, CTF, ExtPar):
                       torch.as_tensor(imgAll, dtype=torch.float32, device=dev),
                       torch.as_tensor(_ctf_half(CTF, N), device=dev), N, ConOrder, psiTrunc,
                       nS=nS, tune=tune, num_eigs=p.num_eigs, msk2=m2,
-                      k=NLSAPar.get("k", num - ConOrder))
+                      k=NLSAPar.get("k", num - ConOrder), wiener=wiener)
     r.IMGT = r.IMGT_dev.cpu().numpy()
     if NLSAPar.get("save", False):
         myio.fout1(ExtPar["filename"], ["psirec", "tau", "a", "b"], [r.psirec, r.tau, r.a, r.b])

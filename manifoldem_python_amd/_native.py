@@ -78,6 +78,7 @@ _SIGS = {
     "memi_markov_normalize": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_int,
                               c_vp, c_vp],
     "memi_tanh_fit": [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp],
+    "memi_gather_blocks": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_ferguson_values": [c_vp, c_i64, c_vp, c_int, c_vp, c_vp],
     "memi_markov_coo": [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_vp],
     "memi_lanczos_steps": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_vp, c_int, c_vp, c_int,
@@ -516,6 +517,32 @@ def knn_select(plan, D, mask, idx_out=None, val_out=None):
     _need(val_out is None or val_out.numel() >= I.list_elems, "knn_select: val_out too small")
     call("memi_knn_select", _p(plan.dev), ctypes.byref(plan.info), _p(D), _p(mask), _p(idx_out),
          _p(val_out), _stream())
+
+
+def gather_blocks(plan, src, src_off, src_n, sel, dst):
+    """dst block p = src block (src_off[p], src_n[p]) at rows/cols sel[row0_p : row0_p + n_p]
+    (host int arrays, checked here and uploaded)."""
+    I = plan.info
+    src_off = np.ascontiguousarray(src_off, dtype=np.int64)
+    src_n = np.ascontiguousarray(src_n, dtype=np.int32)
+    sel = np.ascontiguousarray(sel, dtype=np.int32)
+    _need(src.dtype == torch.float32 and dst.dtype == torch.float32 and
+          dst.numel() >= I.d_elems, "gather_blocks: float32 src / dst of d_elems")
+    _need(len(src_off) == plan.npd and len(src_n) == plan.npd and len(sel) == I.n_rows,
+          "gather_blocks: one source block per PD and one index per row")
+    _need(np.all(src_off >= 0) and np.all(src_off + src_n.astype(np.int64) ** 2 <= src.numel()),
+          "gather_blocks: source block outside src")
+    for p in range(plan.npd):
+        r0, n = int(plan.rows0[p]), int(plan.ns[p])
+        s = sel[r0:r0 + n]
+        _need(n == 0 or (s.min() >= 0 and s.max() < src_n[p]),
+              "gather_blocks: index outside source block %d" % p)
+    dev = dst.device
+    t_off = torch.from_numpy(src_off).to(dev)
+    t_n = torch.from_numpy(src_n).to(dev)
+    t_sel = torch.from_numpy(sel).to(dev)
+    call("memi_gather_blocks", _p(plan.dev), ctypes.byref(plan.info), _p(src), _p(t_off),
+         _p(t_n), _p(t_sel), _p(dst), _stream())
 
 
 def ferguson_sweep(plan, D, mask, logEps, out):

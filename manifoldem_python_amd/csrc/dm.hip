@@ -922,6 +922,24 @@ __global__ void k_zero_f64(double* p, int64_t n) {
     p[e] = 0.0;
 }
 
+// Trimming rounds (manifoldTrimmingAuto.py:63-75): PD p of the destination plan is the block of
+// source PD src_off[p] / src_n[p] restricted to rows and columns sel[row0_p .. row0_p + n_p).
+// One block per destination row; sel is ascending, so a row's reads are near-contiguous.
+__global__ __launch_bounds__(256) void k_gather_blocks(const int32_t* __restrict__ rows,
+                                                       const memi_pd_desc* __restrict__ pds,
+                                                       const float* __restrict__ src,
+                                                       const int64_t* __restrict__ src_off,
+                                                       const int32_t* __restrict__ src_n,
+                                                       const int32_t* __restrict__ sel,
+                                                       float* __restrict__ dst) {
+  const int p = rows[2 * blockIdx.x], i = rows[2 * blockIdx.x + 1];
+  const memi_pd_desc P = pds[p];
+  const int32_t* s = sel + P.row0;
+  const float* srow = src + src_off[p] + (int64_t)s[i] * src_n[p];
+  float* drow = dst + P.d_off + (int64_t)i * P.n;
+  for (int j = threadIdx.x; j < P.n; j += blockDim.x) drow[j] = srow[s[j]];
+}
+
 }  // namespace memi
 
 using namespace memi;
@@ -1104,6 +1122,21 @@ int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, i
   hipLaunchKernelGGL(k_coo_norm<2>, g, dim3(256), 0, s, row, col, val, nnz, inv_s2, alpha, deg,
                      nS, out);
   MEMI_CHECK_LAUNCH("k_coo_norm<2>");
+  return MEMI_OK;
+}
+
+int memi_gather_blocks(const void* plan_dev, const memi_plan_info* info, const float* src,
+                       const int64_t* src_off, const int32_t* src_n, const int32_t* sel,
+                       float* dst, void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && src && src_off && src_n && sel && dst,
+                 "memi_gather_blocks: null pointer");
+  if (info->n_rows == 0) return MEMI_OK;
+  MEMI_CHECK_ARG(info->n_rows < 0x7fffffffLL, "memi_gather_blocks: %lld rows",
+                 (long long)info->n_rows);
+  PlanView v = view(plan_dev, info);
+  hipLaunchKernelGGL(k_gather_blocks, dim3((unsigned)info->n_rows), dim3(256), 0,
+                     (hipStream_t)stream, v.rows, v.pds, src, src_off, src_n, sel, dst);
+  MEMI_CHECK_LAUNCH("k_gather_blocks");
   return MEMI_OK;
 }
 

@@ -1,19 +1,41 @@
 """Drop-in for ``manifoldTrimmingAuto`` (``manifoldTrimmingAuto.py:18-95``): diffusion map with
 k = n, then re-embedding while points fall outside radius ``rad`` on psi_1..psi_3; writes the
-``gC_trimmed_psi`` pickle, the progress marker and ``eig_spec.txt``."""
+``gC_trimmed_psi`` pickle, the progress marker and ``eig_spec.txt``.  The rounds run on the
+GPU with the PD's distances resident on the device (``trimming.trim_batch``)."""
 from __future__ import annotations
 
 import os
 
 import numpy as np
+import torch
 
 from . import DMembeddingII, myio, p
+from . import pipeline as pl
+from . import trimming
+from .trimming import get_psiPath  # noqa: F401  (reference name, manifoldTrimmingAuto.py:18)
+
+OUT_KEYS = ["lamb", "psi", "sigma", "mu", "posPath", "ind", "logEps", "logSumWij", "popt",
+            "R_squared"]
 
 
-def get_psiPath(psi, rad, plotNum):
-    """``:18-28``."""
-    d = np.sqrt(psi[:, plotNum] ** 2 + psi[:, plotNum + 1] ** 2 + psi[:, plotNum + 2] ** 2)
-    return (d < rad).nonzero()[0]
+def upload_kernel_distances(Ds, dev):
+    """Host distance matrices -> (plan, flat float32 device buffer) of their kernel distances
+    (``DMembeddingII.kernel_distances``), k = n for every PD."""
+    plan = trimming.packed_plan([d.shape[0] for d in Ds], dev)
+    flat = torch.empty(max(plan.info.d_elems, 1), dtype=torch.float32, device=dev)
+    for j, d in enumerate(Ds):
+        plan.block(flat, j).copy_(torch.from_numpy(DMembeddingII.kernel_distances(d)))
+    return plan, flat
+
+
+def embed_trimmed_batch(Ds, tune, rad, logEps=None):
+    """The loop of ``:48-77`` for several in-memory D (already restricted to their posPath);
+    returns per PD (posPath1 into D's rows, DM tuple)."""
+    dev = pl.device()
+    plan, flat = upload_kernel_distances(Ds, dev)
+    keep, res, _ = trimming.trim_batch(plan, flat, tune, rad, p.num_eigs,
+                                       pl.LOG_EPS if logEps is None else logEps)
+    return [(keep[j], res[j].as_tuple()) for j in range(len(Ds))]
 
 
 def embed_trimmed(D, tune, rad, posPath=None):
@@ -21,19 +43,23 @@ def embed_trimmed(D, tune, rad, posPath=None):
     nS = D.shape[1]
     if posPath is None:
         posPath = np.arange(nS)
-    D = np.array(D[posPath][:, posPath], dtype=np.float64)
-    nS = D.shape[1]
-    res = DMembeddingII.op(D, nS, tune, 60000)
-    posPath1 = get_psiPath(res[1], rad, 0)
-    while len(posPath1) < nS:
-        nS = len(posPath1)
-        D1 = D[posPath1][:, posPath1]
-        res = DMembeddingII.op(D1, D1.shape[0], tune, 600000)
-        lamb = res[0]
-        res = (lamb[lamb > 0],) + tuple(res[1:])
-        posPathInt = get_psiPath(res[1], rad, 0)
-        posPath1 = posPath1[posPathInt]
-    return posPath[posPath1], res
+    Dp = np.asarray(D, dtype=np.float64)[posPath][:, posPath]
+    keep, res = embed_trimmed_batch([Dp], tune, rad)[0]
+    return posPath[keep], res
+
+
+def write_outputs(psi_file, eig_file, prD, posPath, ind, res, doSave):
+    """``:79-95``: the gC_trimmed_psi pickle, its progress marker and eig_spec.txt."""
+    lamb, psi, sigma, mu, logEps, logSumWij, popt, R_squared = res
+    if doSave["Is"]:
+        myio.fout1(psi_file, OUT_KEYS,
+                   [lamb, psi, sigma, mu, posPath, ind, logEps, logSumWij, popt, R_squared])
+        open(os.path.join(p.psi_prog, "%s" % (prD,)), "a").close()
+    if os.path.exists(eig_file):
+        os.remove(eig_file)
+    with open(eig_file, "a") as f:
+        for i in range(len(lamb) - 1):
+            f.write("%d\t%.5f\n" % (i + 1, lamb[i + 1]))
 
 
 def op(input_data, posPath, tune, rad, visual, doSave):
@@ -42,15 +68,5 @@ def op(input_data, posPath, tune, rad, visual, doSave):
     D, ind = data["D"], data["ind"]
     pp = None if (np.isscalar(posPath) and posPath == 0) else np.asarray(posPath)
     posPath, res = embed_trimmed(D, tune, rad, pp)
-    lamb, psi, sigma, mu, logEps, logSumWij, popt, R_squared = res
-    if doSave["Is"]:
-        myio.fout1(psi_file, ["lamb", "psi", "sigma", "mu", "posPath", "ind", "logEps",
-                              "logSumWij", "popt", "R_squared"],
-                   [lamb, psi, sigma, mu, posPath, ind, logEps, logSumWij, popt, R_squared])
-        open(os.path.join(p.psi_prog, "%s" % (prD,)), "a").close()
-    if os.path.exists(eig_file):
-        os.remove(eig_file)
-    with open(eig_file, "a") as f:
-        for i in range(len(lamb) - 1):
-            f.write("%d\t%.5f\n" % (i + 1, lamb[i + 1]))
+    write_outputs(psi_file, eig_file, prD, posPath, ind, res, doSave)
     return None

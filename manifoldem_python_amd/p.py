@@ -50,6 +50,9 @@ psi2_file = ""
 psi2_prog = ""
 EL_file = ""
 EL_prog = ""
+out_dir = ""
+CC_file = ""
+trash_list = np.zeros(0, dtype=int)     # 1 = PD excluded by the user (EL1D.py:61)
 
 # --- GPU options -------------------------------------------------------------------------
 precision = "split_bf16"      # "split_bf16" or "fp32" distance Gram
@@ -70,8 +73,11 @@ def create_dir():
     for the stages this package writes."""
     import os
     global dist_dir, dist_prog, psi_dir, psi_prog, psi2_dir, psi2_prog, EL_dir, EL_prog
-    global dist_file, psi_file, psi2_file, EL_file, tess_file
+    global dist_file, psi_file, psi2_file, EL_file, tess_file, out_dir, CC_file
     out = os.path.join(user_dir, "outputs_{}".format(proj_name))
+    out_dir = out + "/"
+    os.makedirs(os.path.join(out, "topos"), exist_ok=True)
+    os.makedirs(os.path.join(out, "CC"), exist_ok=True)
     dist_dir = os.path.join(out, "distances/")
     psi_dir = os.path.join(out, "diff_maps/")
     psi2_dir = os.path.join(out, "psi_analysis/")
@@ -87,4 +93,5 @@ def create_dir():
     psi_file = "{}/gC_trimmed_psi_".format(psi_dir)
     psi2_file = "{}/S2_".format(psi2_dir)
     EL_file = "{}/S2_".format(EL_dir)
+    CC_file = os.path.join(out, "CC", "CC_file")
     return None

@@ -52,7 +52,7 @@ class NLSAResult:
 
 def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tune=3,
          num_eigs=15, msk2=None, logEps=pl.LOG_EPS, rand=None, snr=5.0, want_imgt=True,
-         keep=False, timer=None, k=None):
+         keep=False, timer=None, k=None, wiener=True):
     """One psi of psiAnalysisParS2 / NLSA.op.
 
     D: (n, n) float32/float64 device distances of the PD (posPath order); ord_d: int64 device
@@ -60,6 +60,9 @@ def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tu
     the images in psi order into imgAll / ctf_half (posPath[PosPsi1]).  timer: an optional
     pipeline.StageTimer that gets one mark per stage (it synchronises the device at each).
     k: neighbours kept in the embedding of ConD (NLSAPar['k'], NLSA.py:45); None = all m.
+    wiener: True for the 'prD' supervectors (Wiener-filtered, masked images, NLSA.py:52-53,
+    66-78); False for the 'cuti' branch (NLSA.py:54-55), which contracts the images themselves
+    (ctf_half and msk2 unused).
     """
     mark = timer.mark if timer is not None else (lambda name: None)
     if timer is not None:
@@ -86,19 +89,30 @@ def nlsa(D, ord_d, img_idx, imgAll, ctf_half, N, ConOrder, psiTrunc, nS=None, tu
         timer.t.update({"dm1." + k: v for k, v in sub.t.items()})
     # a21
     IMG1 = imgAll.index_select(0, img_idx)
-    CTF1 = ctf_half.index_select(0, img_idx).contiguous()
-    spec = torch.fft.rfft2(IMG1).contiguous()
     ell = psiTrunc - 1
     psiCx = np.hstack((np.ones((m, 1)), psiC[:, 0:ell]))
     mu_psi = torch.as_tensor(mu.reshape(-1, 1) * psiCx, device=dev).contiguous()
-    sv = torch.empty((ConOrder, psiTrunc, N, Nh), dtype=torch.complex64, device=dev)
-    nat.nlsa_supervectors(spec, CTF1, num, N, ConOrder, mu_psi, snr, sv)
-    imgs = torch.fft.irfft2(sv, s=(N, N))
-    if msk2 is not None:
-        imgs = imgs * msk2
-    # column-major flatten of each image (NLSA.py:80: img.T.reshape(-1))
-    A = imgs.permute(0, 3, 2, 1).reshape(ConOrder * Npix, psiTrunc).double().contiguous()
-    del sv, imgs, spec, IMG1
+    if wiener:
+        CTF1 = ctf_half.index_select(0, img_idx).contiguous()
+        spec = torch.fft.rfft2(IMG1).contiguous()
+        sv = torch.empty((ConOrder, psiTrunc, N, Nh), dtype=torch.complex64, device=dev)
+        nat.nlsa_supervectors(spec, CTF1, num, N, ConOrder, mu_psi, snr, sv)
+        imgs = torch.fft.irfft2(sv, s=(N, N))
+        if msk2 is not None:
+            imgs = imgs * msk2
+        # column-major flatten of each image (NLSA.py:80: img.T.reshape(-1))
+        A = imgs.permute(0, 3, 2, 1).reshape(ConOrder * Npix, psiTrunc).double().contiguous()
+        del sv, imgs, spec
+    else:
+        # block ii = X[ConOrder-ii-1 : ConOrder-ii-1+m]^T @ mu_psi, X the column-major
+        # flattened images in psi order (NLSA.py:63-86 without the filter)
+        X = IMG1.double().transpose(1, 2).reshape(num, Npix)
+        A = torch.empty((ConOrder * Npix, psiTrunc), dtype=torch.float64, device=dev)
+        for ii in range(ConOrder):
+            s0 = ConOrder - ii - 1
+            torch.mm(X[s0:s0 + m].t(), mu_psi, out=A[ii * Npix:(ii + 1) * Npix])
+        del X
+    del IMG1
     mark("supervectors")
     # a22
     G = torch.empty((psiTrunc, psiTrunc), dtype=torch.float64, device=dev)

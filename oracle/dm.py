@@ -201,6 +201,7 @@ def trim(D, tune, rad, num_eigs=15, rng=None, dense=False):
         nS = len(posPath1)
         D1 = D[posPath1][:, posPath1]
         res = op(D1, D1.shape[0], tune, 600000, num_eigs, rng, dense)
+        res = (res[0][res[0] > 0],) + tuple(res[1:])          # manifoldTrimmingAuto.py:68
         psi = res[1]
         posPathInt = np.nonzero(np.sqrt(np.sum(psi[:, 0:3] ** 2, axis=1)) < rad)[0]
         posPath1 = posPath1[posPathInt]

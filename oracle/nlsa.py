@@ -10,7 +10,8 @@ window (``get_wiener.py:16``: CTF index ConOrder-ii+i) and the image it divides
 
 Pinned by (``tests/test_oracle_nlsa.py``):
   condist   - brute-force triple loop of the commented-out form at ``NLSA.py:36-41``;
-  supervec  - the per-(ii, i) FFT loop equals the linearity-restructured spectral sum;
+  supervec  - the per-(ii, i) FFT loop equals the linearity-restructured spectral sum, and
+              with CTF = W = 1 it equals the raw ('cuti') window sum;
   svd       - ``np.linalg.svd`` (singular values; vectors up to sign);
   l2        - brute-force pairwise norms;
   binning   - every class index lands in its tau bin.
@@ -57,6 +58,19 @@ def supervectors(IMG1, CTF1, wiener_dom, mu_psi, ConOrder, num, msk2=1):
             img_f = fft2(IMG1[ind3])
             img = ifft2(img_f * (CTF1[ind3] / wiener_dom[i])).real * msk2
             tmp[:, i] = img.T.reshape(-1)
+        A[ii * dim * dim:(ii + 1) * dim * dim, :] = np.matmul(tmp, mu_psi)
+    return A
+
+
+def supervectors_raw(IMG1, mu_psi, ConOrder, num):
+    """``NLSA.py:54-55,63-86`` 'cuti' branch: the same window sums as ``supervectors`` over the
+    images themselves (no Wiener filter, no mask), flattened in column-major order."""
+    dim = IMG1.shape[1]
+    A = np.zeros((ConOrder * dim * dim, mu_psi.shape[1]))
+    tmp = np.zeros((dim * dim, num - ConOrder))
+    for ii in range(ConOrder):
+        for i in range(num - ConOrder):
+            tmp[:, i] = IMG1[ConOrder - ii + i - 1].T.reshape(-1)
         A[ii * dim * dim:(ii + 1) * dim * dim, :] = np.matmul(tmp, mu_psi)
     return A
 
@@ -115,8 +129,9 @@ def reconstruct(U, S, V, psiC, ConOrder, num, nS, Npixel):
 
 
 def op(NLSAPar, DD, posPath, posPsi1, imgAll, msk2, CTF, tune=3, num_eigs=15, rng=None,
-       dense=False, logEps=None):
-    """``NLSA.op`` -> (IMGT, Topo_mean, psirec, psiC1, sdiag, VX, mu, tau) (prD branch)."""
+       dense=False, logEps=None, branch="prD"):
+    """``NLSA.op`` -> (IMGT, Topo_mean, psirec, psiC1, sdiag, VX, mu, tau); branch 'prD'
+    (Wiener-filtered images posPath[posPsi1]) or 'cuti' (raw images posPsi1, ``NLSA.py:54-55``)."""
     rng = np.random.default_rng(0) if rng is None else rng
     num, ConOrder = NLSAPar["num"], NLSAPar["ConOrder"]
     k, nS, psiTrunc = NLSAPar["k"], NLSAPar["nS"], NLSAPar["psiTrunc"]
@@ -124,14 +139,20 @@ def op(NLSAPar, DD, posPath, posPsi1, imgAll, msk2, CTF, tune=3, num_eigs=15, rn
     lambdaC, psiC, sigmaC, mu, _, _, _, _ = odm.op(ConD, k, tune, 600000, num_eigs, rng, dense,
                                                    logEps)
     psiC1 = np.copy(psiC)
-    IMG1 = imgAll[posPath[posPsi1], :, :]
-    wiener_dom, CTF1 = get_wiener(CTF, posPath, posPsi1, ConOrder, num)
+    if branch == "prD":
+        IMG1 = imgAll[posPath[posPsi1], :, :]
+        wiener_dom, CTF1 = get_wiener(CTF, posPath, posPsi1, ConOrder, num)
+    else:
+        IMG1 = imgAll[posPsi1, :, :]
     dim = CTF.shape[1]
     ell = psiTrunc - 1
     m = psiC.shape[0]
     psiC = np.hstack((np.ones((m, 1)), psiC[:, 0:ell]))
     mu_psi = mu.reshape((-1, 1)) * psiC
-    A = supervectors(IMG1, CTF1, wiener_dom, mu_psi, ConOrder, num, msk2)
+    if branch == "prD":
+        A = supervectors(IMG1, CTF1, wiener_dom, mu_psi, ConOrder, num, msk2)
+    else:
+        A = supervectors_raw(IMG1, mu_psi, ConOrder, num)
     U, S, V = svdRF(A)
     VX = np.matmul(V.T, psiC.T)
     IMGT, Topo_mean, sdiag = reconstruct(U, S, V, psiC, ConOrder, num, nS, dim * dim)

@@ -32,3 +32,15 @@ def cuda():
     if not torch.cuda.is_available():
         pytest.skip("no ROCm GPU")
     return torch.device("cuda")
+
+
+def curve_with_outlier(n=400, gap=0.13, seed=0):
+    """Squared distances (x 1e4) of n points on a planar curve plus one point `gap` beyond its
+    end: the diffusion map puts that point at ||psi[:, 0:3]|| ~ 6.6 and every curve point
+    below 2.1, so trimming at rad = 5 removes exactly the outlier and then stops."""
+    rng = np.random.default_rng(seed)
+    t = np.sort(rng.uniform(0, 1, n))
+    X = np.stack([t, 0.1 * np.sin(6 * t)], 1) + 0.002 * rng.normal(size=(n, 2))
+    out = np.array([[1 + gap, 0.1 * np.sin(6)]]) + 0.002 * rng.normal(size=(1, 2))
+    X = np.concatenate([X, out])
+    return np.sum((X[:, None] - X[None]) ** 2, -1) * 1e4

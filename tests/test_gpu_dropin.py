@@ -60,6 +60,14 @@ def test_small_helpers(cuda):
     Ur, Sr, Vr = nlsa.svdRF(A)
     np.testing.assert_allclose(S, Sr, rtol=1e-12)
     np.testing.assert_allclose(np.abs(U), np.abs(Ur), atol=1e-10)
+    # wide, square and a short side above the tall-Gram kernels' 8 columns
+    for shape in ((6, 300), (12, 12), (400, 20), (30, 900)):
+        A = rng.normal(size=shape)
+        U, S, V = svdRF.op(A)
+        Ur, Sr, Vr = nlsa.svdRF(A)
+        assert U.shape == Ur.shape and V.shape == Vr.shape
+        np.testing.assert_allclose(S, Sr, rtol=1e-10)
+        np.testing.assert_allclose(U @ S @ V.T, A, atol=1e-9)
     a = rng.normal(size=(30, 12))
     np.testing.assert_allclose(L2_distance.op(a, a), nlsa.l2_distance(a, a), atol=1e-7)
 
@@ -94,15 +102,16 @@ def test_getdistance_op_writes_reference_pickle(cuda, tmp_path):
 
 
 def test_stage_drivers_end_to_end(cuda, tmp_path):
-    """Tessellation pickle -> GetDistancesS2 -> manifoldAnalysis -> psiAnalysis, with the
-    resume-by-marker rule: a PD with a marker is skipped."""
+    """Tessellation pickle -> GetDistancesS2 -> manifoldAnalysis (rad = 5) -> psiAnalysis ->
+    EL1D, with the resume-by-marker rule (a PD with a marker is skipped); the trimmed-psi and
+    psi-analysis pickles against the oracle, and the isFull pass's tau sense rule."""
     from manifoldem_python_amd import (GetDistancesS2, manifoldAnalysis, psiAnalysis, myio,
-                                       synth)
+                                       synth, EL1D)
     N = 32
     p = _set_params(N)
     p.user_dir, p.proj_name = str(tmp_path), "t"
     p.create_dir()
-    p.num_psis, p.conOrderRange, p.num_eigs, p.rad = 2, 20, 8, 50
+    p.num_psis, p.conOrderRange, p.num_eigs, p.rad = 2, 20, 8, 5
     sizes = [80, 70, 90]
     raws, CG, qs, dfs, off = [], [], [], [], 0
     for i, n in enumerate(sizes):
@@ -132,13 +141,68 @@ def test_stage_drivers_end_to_end(cuda, tmp_path):
         d = myio.fin1("{}prD_{}".format(p.dist_file, i))
         assert d["D"].shape == (sizes[i], sizes[i])
     manifoldAnalysis.op()
+    from oracle import dm, nlsa
     for i in range(3):
         d = myio.fin1("{}prD_{}".format(p.psi_file, i))
         assert d["psi"].shape[1] == 8 and len(d["posPath"]) <= sizes[i]
+        # the gC_trimmed_psi pickle against the oracle's trimming of the stage-1 D
+        D = myio.fin1("{}prD_{}".format(p.dist_file, i))["D"]
+        pos_r, res_r = dm.trim(D, p.tune, p.rad, num_eigs=8, rng=np.random.default_rng(i))
+        np.testing.assert_array_equal(d["posPath"], pos_r)
+        np.testing.assert_allclose(d["lamb"], res_r[0], atol=1e-6)
+        assert d["sigma"] == pytest.approx(res_r[2], rel=1e-5)
+        assert os.path.exists(os.path.join(p.out_dir, "topos", "PrD_%d" % (i + 1),
+                                           "eig_spec.txt"))
     psiAnalysis.op()
+    first = {}
     for i in range(3):
+        img = myio.fin1("{}prD_{}".format(p.dist_file, i))
+        psd = myio.fin1("{}prD_{}".format(p.psi_file, i))
+        pos = psd["posPath"]
+        nS = len(pos)
+        C = nS // p.conOrderRange
+        Dp = img["D"][pos][:, pos]
         for k in range(2):
             d = myio.fin1("{}prD_{}_psi_{}".format(p.psi2_file, i, k))
+            first[i, k] = d
             assert d["IMG1"].shape[0] == N * N
-            assert np.all(np.isfinite(d["tau"]))
             assert os.path.exists(os.path.join(p.psi2_prog, "%d_%d" % (i, k)))
+            # the S2_prD_psi pickle against the oracle's NLSA of the same psi ordering
+            order = np.argsort(psd["psi"][:, k])
+            par = dict(num=nS, ConOrder=C, k=nS - C, tune=p.tune, nS=nS, save=False,
+                       psiTrunc=p.num_psiTrunc)
+            ref = nlsa.op(par, Dp[order][:, order], pos, order, img["imgAll"], img["msk2"],
+                          img["CTF"].reshape(-1, N, N), num_eigs=8,
+                          rng=np.random.default_rng(k))
+            np.testing.assert_allclose(d["sdiag"], ref[4], rtol=1e-4)
+            for c in range(ref[3].shape[1]):
+                a, b = d["psiC1"][:, c], ref[3][:, c]
+                assert np.max(np.abs(np.sign(a @ b) * a - b)) < 1e-3 * np.max(np.abs(b))
+            # tau up to direction; the 1-D fit stops once its updates fall below 1 %
+            # (get_fit_1D_open_manifold_3D_param.py:17-21), so two runs whose inputs differ in
+            # the last digits agree to about that
+            tau_r = (ref[7] - ref[7].min()) / (ref[7].max() - ref[7].min())
+            tau = np.ravel(d["tau"])
+            assert min(np.max(np.abs(tau - tau_r.ravel())),
+                       np.max(np.abs(tau - (1 - tau_r.ravel())))) < 2e-2
+    # second pass (EL1D, psiAnalysisParS2 isFull = 1): PD 0 keeps its sense, PD 1 flips, PD 2
+    # is removed by the user
+    p.trash_list = np.array([0, 0, 1])
+    myio.fout1(p.CC_file, ["psinums", "senses"], [np.array([[1, 0, 1]]), np.array([[1, -1, 1]])])
+    open(os.path.join(p.EL_prog, "0"), "a").close()       # PD 0 already done: skipped
+    EL1D.op()
+    assert not os.path.exists("{}prD_0_{}_1".format(p.EL_file, p.trajName))
+    assert not os.path.exists("{}prD_2_{}_1".format(p.EL_file, p.trajName))
+    os.remove(os.path.join(p.EL_prog, "0"))
+    EL1D.op()
+    for i, k, sense in ((0, 1, 1), (1, 0, -1)):
+        e = myio.fin1("{}prD_{}_{}_1".format(p.EL_file, i, p.trajName))
+        assert sorted(e) == sorted(["IMG1", "IMGT", "posPath", "PosPsi1", "psirec", "tau",
+                                    "psiC1", "mu", "VX", "sdiag", "Topo_mean", "tauinds"])
+        a = first[i, k]
+        # the re-run reproduces the first pass's frames, so diff_corr > 0: sense -1 flips tau
+        expect = np.ravel(a["tau"]) if sense == 1 else 1 - np.ravel(a["tau"])
+        assert np.max(np.abs(np.ravel(e["tau"]) - expect)) < 1e-4
+        np.testing.assert_allclose(e["sdiag"], a["sdiag"], rtol=1e-5)
+        assert e["IMGT"].shape[0] == N * N
+        assert os.path.exists(os.path.join(p.EL_prog, "%d" % i))

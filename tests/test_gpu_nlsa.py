@@ -195,3 +195,54 @@ def test_fit1d_matches_oracle(cuda, noise):
     np.testing.assert_allclose(b, br, rtol=1e-9, atol=1e-12)
     assert tau.shape == tr.shape
     assert np.max(np.abs(tau - tr)) < 1e-9
+
+
+def test_nlsa_cuti_branch_matches_oracle(cuda, pd_case):
+    """'cuti' supervectors (NLSA.py:54-55): the raw images imgAll[posPsi1], no Wiener filter,
+    no mask; singular values and the reconstructed frames against the oracle."""
+    import torch
+    from manifoldem_python_amd import spectral
+    from oracle import nlsa
+    n, N = pd_case["n"], pd_case["N"]
+    out = pd_case["out"]
+    C = 6
+    order = np.argsort(pd_case["psi"][:, 0])
+    CTF = out["CTF"].reshape(n, N, N)
+    DD = out["D"][order][:, order]
+    par = dict(num=n, ConOrder=C, k=n - C, tune=3, nS=n, save=False, psiTrunc=8)
+    ref = nlsa.op(par, DD, np.arange(n), order, out["imgAll"], 1, CTF,
+                  rng=np.random.default_rng(9), branch="cuti")
+    IMGT_r, sdiag_r, extra = ref[0], ref[4], ref[8]
+    imgAll = torch.as_tensor(out["imgAll"], dtype=torch.float32, device=cuda)
+    ctf_half = torch.zeros((n, N * (N // 2 + 1)), dtype=torch.float32, device=cuda)
+    rng = np.random.default_rng(9)
+    r = spectral.nlsa(torch.as_tensor(out["D"], device=cuda), torch.as_tensor(order, device=cuda),
+                      torch.as_tensor(order, device=cuda), imgAll, ctf_half, N, C, 8, nS=n,
+                      tune=3, num_eigs=15, rand=lambda *sh: rng.random(sh), keep=True,
+                      wiener=False)
+    A = r.A.cpu().numpy()
+    Ar = extra["A"]
+    # columns 1.. carry eigenvector signs: compare column by column up to sign
+    for c in range(A.shape[1]):
+        s = np.sign(A[:, c] @ Ar[:, c]) or 1.0
+        assert np.max(np.abs(s * A[:, c] - Ar[:, c])) < 1e-4 * np.max(np.abs(Ar[:, c])), c
+    np.testing.assert_allclose(r.sdiag, sdiag_r, rtol=1e-5)
+    IMGT = r.IMGT_dev.cpu().numpy()
+    assert np.max(np.abs(IMGT - IMGT_r)) < 1e-4 * np.max(np.abs(IMGT_r))
+
+
+def test_nlsa_dropin_cuti_indexes_imgall_by_pospsi1(cuda, pd_case):
+    """NLSA.op with 'cuti' reads imgAll[posPsi1] (not posPath[posPsi1]): a posPath that is a
+    permutation leaves the result unchanged."""
+    from manifoldem_python_amd import NLSA
+    n, N = pd_case["n"], pd_case["N"]
+    out = pd_case["out"]
+    C = 6
+    order = np.argsort(pd_case["psi"][:, 0])
+    CTF = out["CTF"].reshape(n, N, N)
+    DD = out["D"][order][:, order]
+    par = dict(num=n, ConOrder=C, k=n - C, tune=3, nS=n, save=False, psiTrunc=8)
+    perm = np.random.default_rng(3).permutation(n)
+    s1 = NLSA.op(par, DD, np.arange(n), order, out["imgAll"], 1, CTF, {"cuti": 1})[4]
+    s2 = NLSA.op(par, DD, perm, order, out["imgAll"], 1, CTF, {"cuti": 1})[4]
+    np.testing.assert_allclose(s1, s2, rtol=1e-4)

@@ -159,3 +159,23 @@ def test_fit1d_curve_solve_matches_lstsq():
             M = np.array([[s[q], s[3 + q]], [s[3 + q], nS]])
             ref = np.linalg.lstsq(M, np.array([s[6 + q], s[9 + q]]), rcond=None)[0]
             np.testing.assert_allclose([a[q], b[q]], ref, rtol=1e-9, atol=1e-12)
+
+
+def test_embedd_kept_points_selects_the_surviving_points():
+    """embedd.py:32-38: the GUI's new points are copies of the original points outside the
+    user's polygon, so the selection is exactly their indices (in the original order)."""
+    from manifoldem_python_amd import embedd
+    rng = np.random.default_rng(7)
+    x, y = rng.normal(size=50), rng.normal(size=50)
+    keep = np.sort(rng.choice(50, 31, replace=False))
+    got = embedd.kept_points(zip(x, y), zip(x[keep], y[keep]))
+    np.testing.assert_array_equal(got, keep)
+    # the reference compares bit patterns: x equal to a kept point's y coordinate also counts
+    x2 = x.copy()
+    drop = np.setdiff1d(np.arange(50), keep)[0]
+    x2[drop] = y[keep[0]]
+    got = embedd.kept_points(zip(x2, y), zip(x2[keep], y[keep]))
+    np.testing.assert_array_equal(got, np.sort(np.append(keep, drop)))
+    # -0.0 and 0.0 differ as bit patterns
+    got = embedd.kept_points([(-0.0, 1.0), (0.5, 2.0)], [(0.0, 1.0), (0.5, 2.0)])
+    np.testing.assert_array_equal(got, [1])

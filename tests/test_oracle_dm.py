@@ -127,3 +127,22 @@ def test_trim_keeps_points_inside_radius():
     psi = res[1]
     assert np.all(np.sqrt(np.sum(psi[:, :3] ** 2, 1)) < 5)
     assert len(pos) == psi.shape[0]
+
+
+def test_trim_removes_the_outlier_at_rad5():
+    """manifoldTrimmingAuto.py:48-77 at the default rad = 5: the point off the end of the
+    curve lies outside the radius (||psi_1..3|| ~ 6.6, curve points < 2.1), so one re-embedding
+    runs on the curve alone and removes nothing; its eigenvalues are the positive ones."""
+    from conftest import curve_with_outlier
+    D = curve_with_outlier()
+    n = D.shape[0]
+    res0 = dm.op(D, n, 3, 60000, 15, np.random.default_rng(1))
+    r0 = np.sqrt(np.sum(res0[1][:, :3] ** 2, 1))
+    assert r0[-1] > 6 and np.max(r0[:-1]) < 2.5
+    pos, res = dm.trim(D, 3, 5, num_eigs=15, rng=np.random.default_rng(1))
+    np.testing.assert_array_equal(pos, np.arange(n - 1))
+    assert res[1].shape[0] == n - 1 and np.all(res[0] > 0)
+    ref = dm.op(D[:n - 1, :n - 1], n - 1, 3, 600000, 15, np.random.default_rng(1))
+    assert res[2] == pytest.approx(ref[2], rel=1e-6)            # sigma (fit restarts differ)
+    np.testing.assert_allclose(res[0], ref[0][ref[0] > 0], atol=1e-6)
+

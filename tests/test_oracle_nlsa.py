@@ -52,6 +52,27 @@ def test_supervectors_by_linearity():
     np.testing.assert_allclose(A, B, rtol=1e-10, atol=1e-12)
 
 
+def test_raw_supervectors_window():
+    """'cuti' branch: with image j constant at j + 1, block ii column c is the constant
+    sum_i (ConOrder - ii + i) mu_psi[i, c] (image index ConOrder - ii + i - 1); and the raw sum
+    equals the Wiener form with CTF = W = 1 (an FFT round trip of each image)."""
+    rng = np.random.default_rng(13)
+    n, C, N, ell = 12, 4, 5, 3
+    m = n - C
+    mu_psi = rng.normal(size=(m, ell))
+    IMG = np.arange(1, n + 1, dtype=float)[:, None, None] * np.ones((n, N, N))
+    A = nlsa.supervectors_raw(IMG, mu_psi, C, n)
+    for ii in range(C):
+        expect = (C - ii + np.arange(m)) @ mu_psi
+        np.testing.assert_allclose(A[ii * N * N:(ii + 1) * N * N], np.tile(expect, (N * N, 1)),
+                                   rtol=1e-13)
+    IMG = rng.normal(size=(n, N, N))
+    ones = np.ones((n, N, N))
+    np.testing.assert_allclose(nlsa.supervectors_raw(IMG, mu_psi, C, n),
+                               nlsa.supervectors(IMG, ones, ones[:m], mu_psi, C, n),
+                               rtol=1e-10, atol=1e-12)
+
+
 def test_svdrf_and_l2():
     rng = np.random.default_rng(4)
     A = rng.normal(size=(200, 6))
@@ -59,6 +80,11 @@ def test_svdrf_and_l2():
     s = np.linalg.svd(A, compute_uv=False)
     np.testing.assert_allclose(np.diag(S), s, rtol=1e-12)
     np.testing.assert_allclose(U @ S @ V.T, A, atol=1e-10)
+    for shape in ((7, 40), (12, 12)):               # wide and square: eigh(A A^T) branch
+        A = rng.normal(size=shape)
+        U, S, V = nlsa.svdRF(A)
+        np.testing.assert_allclose(np.diag(S), np.linalg.svd(A, compute_uv=False), rtol=1e-10)
+        np.testing.assert_allclose(U @ S @ V.T, A, atol=1e-9)
     a = rng.normal(size=(5, 7))
     d = nlsa.l2_distance(a, a)
     brute = np.array([[np.linalg.norm(a[:, i] - a[:, j]) for j in range(7)] for i in range(7)])
