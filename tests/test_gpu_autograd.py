@@ -25,10 +25,10 @@ def _ref_loss(img, C, w, eps):
     return torch.log(K.sum(2)).sum(1), D
 
 
-def test_backward_matches_float64_autograd_and_fd(cuda):
+@pytest.mark.parametrize("n,N", [(48, 32), (160, 64)])
+def test_backward_matches_float64_autograd_and_fd(cuda, n, N):
     import torch
     from manifoldem_python_amd import _native as nat, autograd, synth
-    n, N = 48, 32
     s = synth.make_pd(synth.seeds(7)[2], n, N)
     rng = np.random.default_rng(0)
     img64 = torch.as_tensor(rng.normal(size=(n, N, N)), dtype=torch.float64)
@@ -59,11 +59,18 @@ def test_backward_matches_float64_autograd_and_fd(cuda):
                                atol=1e-5)
     ge = eps.grad.cpu().numpy()
     ger = e.grad.numpy()
-    assert np.max(np.abs(ge - ger) / (np.abs(ger) + 1e-30 + 1e-6 * np.max(np.abs(ger)))) < 1e-3
+    err_e = np.max(np.abs(ge - ger) / (np.abs(ger) + 1e-30 + 1e-6 * np.max(np.abs(ger))))
     gx = img.grad.cpu().numpy().astype(np.float64)
     gxr = x.grad.numpy()
     rel = np.linalg.norm(gx - gxr) / np.linalg.norm(gxr)
-    assert rel < 1e-3, rel
+    err_x = np.max(np.abs(gx - gxr)) / np.max(np.abs(gxr))
+    print(f"[autograd n={n} N={N}] dL/deps max rel {err_e:.3e}; dL/dX norm rel {rel:.3e}, "
+          f"max/max {err_x:.3e}")
+    # measured on the MI355X: 1.2e-6 / 1.4e-6 (eps), 2.1e-7 / 2.6e-7 (X, norm), 2.5e-7 /
+    # 3.3e-7 (X, max) at n = 48 / 160: float32 rounding of the images and spectra
+    assert err_e < 1e-5, err_e
+    assert rel < 2e-6, rel
+    assert err_x < 3e-6, err_x
     # the float64 reference itself against central differences (step 1e-3 relative)
     for e_idx in (10, 20, 30):
         h = 1e-3 * float(e.detach()[e_idx])
