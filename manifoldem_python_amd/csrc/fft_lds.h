@@ -2,7 +2,7 @@
 // in place in one buffer per image.  Radices 8, 4, 2, 3, 5, 7; real rows are transformed two at
 // a time as one complex line (x_a + i x_b), columns as complex lines of the half spectrum.
 // Twiddles come from a table of the N-th roots of unity computed in float64 (fft_twiddles).
-// Blocks of FFT_NT = 512 threads.
+// Blocks of NT threads (a template parameter of the 2-D transforms).
 #pragma once
 #include "common.h"
 
@@ -115,9 +115,19 @@ struct Dft<7> : DftOdd<7> {};
 // q / d for 0 <= q < 2^22, d >= 1, from a float reciprocal and one correction each way
 __device__ inline int fdivi(int q, int d, float inv_d) {
   int t = (int)((float)q * inv_d);
-  t += (t + 1) * d <= q;
-  t -= t * d > q;
+  t += (int)__umul24(t + 1, d) <= q;   // 24-bit products: full-rate multiplies
+  t -= (int)__umul24(t, d) > q;
   return t;
+}
+
+// q / d for the compile-time image side NF > 0 (d then folds to a constant after inlining,
+// and the division becomes a multiply-high) or by fdivi for a run-time side (NF = 0)
+template <int NF>
+__device__ inline int qdiv(int q, int d, float inv_d) {
+  if constexpr (NF > 0)
+    return q / d;
+  else
+    return fdivi(q, d, inv_d);
 }
 
 // Every transform below works in ONE LDS buffer.  The line transforms are in-place
@@ -128,10 +138,35 @@ __device__ inline int fdivi(int q, int d, float inv_d) {
 // rev[p] (position -> k) translate where a consumer needs natural frequencies.  Only the
 // packing of two real rows into one complex line and its separation stage values in registers
 // (all reads, a barrier, all writes).
-constexpr int FFT_NT = 512;
+
 
 // items per thread of a step over `total` work items, for the largest image of the template
-__host__ __device__ constexpr int fft_items(int total) { return (total + FFT_NT - 1) / FFT_NT; }
+__host__ __device__ constexpr int fft_items(int total, int nt) { return (total + nt - 1) / nt; }
+
+// the same plan at compile time (fixed-size kernels)
+struct RadixPlan {
+  int n;
+  int r[8];
+};
+constexpr RadixPlan radix_plan_c(int N) {
+  RadixPlan P{0, {0, 0, 0, 0, 0, 0, 0, 0}};
+  int n = N;
+  const int order[6] = {8, 4, 2, 3, 5, 7};
+  for (int o = 0; o < 6; ++o)
+    while (n % order[o] == 0 && P.n < 8) {
+      P.r[P.n++] = order[o];
+      n /= order[o];
+    }
+  if (n != 1) P.n = 0;
+  return P;
+}
+// sub-transform length at pass I of N's plan: N / (r_0 ... r_{I-1})
+constexpr int plan_span(int N, int I) {
+  const RadixPlan P = radix_plan_c(N);
+  int s = N;
+  for (int p = 0; p < I; ++p) s /= P.r[p];
+  return s;
+}
 
 // radix plan of N: up to 8 passes from {8, 4, 2, 3, 5, 7}; returns the count (0: unsupported)
 __host__ __device__ inline int radix_plan(int N, int* rad) {
@@ -160,13 +195,30 @@ __device__ inline void fft_digit_tables(int N, const int* rad, int nrad, int* po
   }
 }
 
+// fft_digit_tables for a compile-time size (no run-time plan array)
+template <int NF>
+__device__ __forceinline__ void fft_digit_tables_c(int* pos, int* rev) {
+  constexpr RadixPlan P = radix_plan_c(NF);
+  for (int k = threadIdx.x; k < NF; k += blockDim.x) {
+    int rem = k, span = NF, p = 0;
+#pragma unroll
+    for (int i = 0; i < P.n; ++i) {
+      span /= P.r[i];
+      p += (rem % P.r[i]) * span;
+      rem /= P.r[i];
+    }
+    pos[k] = p;
+    rev[p] = k;
+  }
+}
+
 // one in-place pass of radix R over L lines of length N (line l at l*ls, element i at i*es):
 // sub-transforms of length `span` at multiples of span; butterfly (block, n1) takes
 // idx_r = block*span + n1 + r*span/R.  Forward (DIF): DFT_R, then y_s *= w_span^(n1 s).
 // Inverse (DIT): x_s *= conj w_span^(n1 s), then the conjugate DFT_R.
 // col: lines side by side (ls == 1): consecutive threads take consecutive lines.
-template <int R>
-__device__ inline void fft_pass_ip(cpx* buf, int N, int L, int ls, int es, int span,
+template <int R, int NF>
+__device__ __forceinline__ void fft_pass_ip(cpx* buf, int N, int L, int ls, int es, int span,
                                    const cpx* tw, bool inv) {
   const int sub = span / R, per = N / R;
   const int total = L * per;
@@ -176,13 +228,13 @@ __device__ inline void fft_pass_ip(cpx* buf, int N, int L, int ls, int es, int s
   for (int q = threadIdx.x; q < total; q += blockDim.x) {
     int line, t;
     if (col) {
-      t = fdivi(q, L, inv_l);
+      t = qdiv<NF>(q, L, inv_l);
       line = q - t * L;
     } else {
-      line = fdivi(q, per, inv_per);
+      line = qdiv<NF>(q, per, inv_per);
       t = q - line * per;
     }
-    const int blk = fdivi(t, sub, inv_sub), n1 = t - blk * sub;
+    const int blk = qdiv<NF>(t, sub, inv_sub), n1 = t - blk * sub;
     cpx* x = buf + line * ls + (blk * span + n1) * es;
     const int st = sub * es;
     cpx v[R];
@@ -209,7 +261,7 @@ __device__ inline void fft_pass_ip(cpx* buf, int N, int L, int ls, int es, int s
 
 // all passes of one batch of line transforms, in place: forward in plan order (spans N,
 // N/R1, ...), inverse in reverse order
-__device__ inline void fft_lines_ip(cpx* buf, int N, int L, int ls, int es, const cpx* tw,
+__device__ __forceinline__ void fft_lines_ip(cpx* buf, int N, int L, int ls, int es, const cpx* tw,
                                     const int* rad, int nrad, bool inv) {
   int spans[8];
   int span = N;
@@ -220,24 +272,50 @@ __device__ inline void fft_lines_ip(cpx* buf, int N, int L, int ls, int es, cons
   for (int i = 0; i < nrad; ++i) {
     const int p = inv ? nrad - 1 - i : i;
     switch (rad[p]) {
-      case 8: fft_pass_ip<8>(buf, N, L, ls, es, spans[p], tw, inv); break;
-      case 4: fft_pass_ip<4>(buf, N, L, ls, es, spans[p], tw, inv); break;
-      case 2: fft_pass_ip<2>(buf, N, L, ls, es, spans[p], tw, inv); break;
-      case 3: fft_pass_ip<3>(buf, N, L, ls, es, spans[p], tw, inv); break;
-      case 5: fft_pass_ip<5>(buf, N, L, ls, es, spans[p], tw, inv); break;
-      default: fft_pass_ip<7>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      case 8: fft_pass_ip<8, 0>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      case 4: fft_pass_ip<4, 0>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      case 2: fft_pass_ip<2, 0>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      case 3: fft_pass_ip<3, 0>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      case 5: fft_pass_ip<5, 0>(buf, N, L, ls, es, spans[p], tw, inv); break;
+      default: fft_pass_ip<7, 0>(buf, N, L, ls, es, spans[p], tw, inv); break;
     }
+  }
+}
+
+// the passes of a compile-time size NF, each with its radix and span as constants
+template <int NF, int I, bool INV>
+__device__ __forceinline__ void fixed_passes(cpx* buf, int L, int ls, int es, const cpx* tw) {
+  constexpr RadixPlan P = radix_plan_c(NF);
+  if constexpr (I < P.n) {
+    constexpr int p = INV ? P.n - 1 - I : I;
+    constexpr int span = plan_span(NF, p);
+    fft_pass_ip<P.r[p], NF>(buf, NF, L, ls, es, span, tw, INV);
+    fixed_passes<NF, I + 1, INV>(buf, L, ls, es, tw);
+  }
+}
+
+template <int NF>
+__device__ __forceinline__ void fft_lines(cpx* buf, int N, int L, int ls, int es, const cpx* tw,
+                                 const int* rad, int nrad, bool inv) {
+  if constexpr (NF > 0) {
+    static_assert(radix_plan_c(NF).n > 0, "image side without a radix plan");
+    if (inv)
+      fixed_passes<NF, 0, true>(buf, L, ls, es, tw);
+    else
+      fixed_passes<NF, 0, false>(buf, L, ls, es, tw);
+  } else {
+    fft_lines_ip(buf, N, L, ls, es, tw, rad, nrad, inv);
   }
 }
 
 // 2-D real-to-half-spectrum FFT in place: the real image (row stride S floats) in `buf`
 // becomes the half spectrum, kx natural along each row (row stride Nh), rows at ky's
 // digit-reversed position: spectrum (ky, kx) is at [pos[ky] * Nh + kx].
-template <int NMAX>
+template <int NMAX, int NF, int NT>
 __device__ inline cpx* rfft2_ip(float* buf, int N, int S, const cpx* tw, const int* rad,
                                 int nrad, const int* pos) {
-  constexpr int PK = fft_items(((NMAX + 1) / 2) * NMAX);
-  constexpr int UK = fft_items(((NMAX + 1) / 2) * (NMAX / 2 + 1));
+  constexpr int PK = fft_items(((NMAX + 1) / 2) * NMAX, NT);
+  constexpr int UK = fft_items(((NMAX + 1) / 2) * (NMAX / 2 + 1), NT);
   const int Nh = N / 2 + 1, P = (N + 1) / 2;       // row pairs
   cpx* X = (cpx*)buf;
   const float inv_n = 1.0f / N, inv_nh = 1.0f / Nh;
@@ -245,30 +323,30 @@ __device__ inline cpx* rfft2_ip(float* buf, int N, int S, const cpx* tw, const i
     cpx st[PK];
 #pragma unroll
     for (int b = 0; b < PK; ++b) {
-      const int q = threadIdx.x + b * FFT_NT;
+      const int q = threadIdx.x + b * NT;
       if (q < P * N) {
-        const int l = fdivi(q, N, inv_n), c = q - l * N;
+        const int l = qdiv<NF>(q, N, inv_n), c = q - l * N;
         st[b] = make_float2(buf[(2 * l) * S + c], 2 * l + 1 < N ? buf[(2 * l + 1) * S + c] : 0.f);
       }
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < PK; ++b) {
-      const int q = threadIdx.x + b * FFT_NT;
+      const int q = threadIdx.x + b * NT;
       if (q < P * N) X[q] = st[b];                 // line l at l * N
     }
     __syncthreads();
   }
-  fft_lines_ip(X, N, P, N, 1, tw, rad, nrad, false);
+  fft_lines<NF>(X, N, P, N, 1, tw, rad, nrad, false);
   // separate the row pairs: X_a[k] = (Z[k] + conj Z[-k]) / 2, X_b[k] = (Z[k] - conj Z[-k]) / 2i,
   // Z[k] read at its digit-reversed position
   {
     cpx ha[UK], hb[UK];
 #pragma unroll
     for (int b = 0; b < UK; ++b) {
-      const int q = threadIdx.x + b * FFT_NT;
+      const int q = threadIdx.x + b * NT;
       if (q < P * Nh) {
-        const int l = fdivi(q, Nh, inv_nh), k = q - l * Nh;
+        const int l = qdiv<NF>(q, Nh, inv_nh), k = q - l * Nh;
         const cpx z = X[l * N + pos[k]], zc = X[l * N + pos[k == 0 ? 0 : N - k]];
         ha[b] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
         hb[b] = make_float2(0.5f * (z.y + zc.y), 0.5f * (zc.x - z.x));
@@ -277,38 +355,38 @@ __device__ inline cpx* rfft2_ip(float* buf, int N, int S, const cpx* tw, const i
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < UK; ++b) {
-      const int q = threadIdx.x + b * FFT_NT;
+      const int q = threadIdx.x + b * NT;
       if (q < P * Nh) {
-        const int l = fdivi(q, Nh, inv_nh), k = q - l * Nh;
+        const int l = qdiv<NF>(q, Nh, inv_nh), k = q - l * Nh;
         X[(2 * l) * Nh + k] = ha[b];
         if (2 * l + 1 < N) X[(2 * l + 1) * Nh + k] = hb[b];
       }
     }
     __syncthreads();
   }
-  fft_lines_ip(X, N, Nh, 1, Nh, tw, rad, nrad, false);
+  fft_lines<NF>(X, N, Nh, 1, Nh, tw, rad, nrad, false);
   return X;
 }
 
 // 2-D inverse (unnormalised) of a half spectrum laid out as rfft2_ip leaves it, into the real
 // image (row stride S) in the same buffer.
-template <int NMAX>
+template <int NMAX, int NF, int NT>
 __device__ inline void irfft2_ip(float* buf, int N, int S, const cpx* tw, const int* rad,
                                  int nrad, const int* pos) {
-  constexpr int PK = fft_items(((NMAX + 1) / 2) * NMAX);
+  constexpr int PK = fft_items(((NMAX + 1) / 2) * NMAX, NT);
   const int Nh = N / 2 + 1, P = (N + 1) / 2;
   cpx* G = (cpx*)buf;
   const float inv_n = 1.0f / N;
-  fft_lines_ip(G, N, Nh, 1, Nh, tw, rad, nrad, true);     // ky digit-reversed -> y natural
+  fft_lines<NF>(G, N, Nh, 1, Nh, tw, rad, nrad, true);     // ky digit-reversed -> y natural
   // row pairs Z[k] = G_a[k] + i G_b[k] (G_r[k] = conj G_r[N-k] for k > N/2), stored at the
   // digit-reversed position the row inverse expects
   {
     cpx st[PK];
 #pragma unroll
     for (int b = 0; b < PK; ++b) {
-      const int q = threadIdx.x + b * FFT_NT;
+      const int q = threadIdx.x + b * NT;
       if (q < P * N) {
-        const int l = fdivi(q, N, inv_n), k = q - l * N;
+        const int l = qdiv<NF>(q, N, inv_n), k = q - l * N;
         const bool lowk = k < Nh;
         const int kk = lowk ? k : N - k;
         cpx ga = G[(2 * l) * Nh + kk];
@@ -323,29 +401,29 @@ __device__ inline void irfft2_ip(float* buf, int N, int S, const cpx* tw, const 
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < PK; ++b) {
-      const int q = threadIdx.x + b * FFT_NT;
+      const int q = threadIdx.x + b * NT;
       if (q < P * N) {
-        const int l = fdivi(q, N, inv_n), k = q - l * N;
+        const int l = qdiv<NF>(q, N, inv_n), k = q - l * N;
         G[l * N + pos[k]] = st[b];
       }
     }
     __syncthreads();
   }
-  fft_lines_ip(G, N, P, N, 1, tw, rad, nrad, true);
+  fft_lines<NF>(G, N, P, N, 1, tw, rad, nrad, true);
   // the real rows, row stride S
   {
     cpx st[PK];
 #pragma unroll
     for (int b = 0; b < PK; ++b) {
-      const int q = threadIdx.x + b * FFT_NT;
+      const int q = threadIdx.x + b * NT;
       if (q < P * N) st[b] = G[q];
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < PK; ++b) {
-      const int q = threadIdx.x + b * FFT_NT;
+      const int q = threadIdx.x + b * NT;
       if (q < P * N) {
-        const int l = fdivi(q, N, inv_n), c = q - l * N;
+        const int l = qdiv<NF>(q, N, inv_n), c = q - l * N;
         buf[(2 * l) * S + c] = st[b].x;
         if (2 * l + 1 < N) buf[(2 * l + 1) * S + c] = st[b].y;
       }

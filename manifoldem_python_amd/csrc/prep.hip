@@ -165,6 +165,16 @@ __device__ inline void bspline_w(float t, float w[4]) {
   w[3] = t3 * (1.f / 6.f);
 }
 
+// 6 x bspline_w: {u^3, 4 - 6 t^2 + 3 t^3, 4 - 6 u^2 + 3 u^3, t^3}, u = 1 - t
+__device__ __forceinline__ void bspline_w6(float t, float w[4]) {
+  const float u = 1.f - t;
+  const float t2 = t * t, u2 = u * u;
+  w[0] = u2 * u;
+  w[1] = fmaf(t2, fmaf(3.f, t, -6.f), 4.f);
+  w[2] = fmaf(u2, fmaf(3.f, u, -6.f), 4.f);
+  w[3] = t2 * t;
+}
+
 __global__ __launch_bounds__(256) void k_spline_rotate(const float* __restrict__ coef,
                                                        const double* __restrict__ angle_deg,
                                                        int N, float* __restrict__ out) {
@@ -212,20 +222,21 @@ __global__ __launch_bounds__(256) void k_spline_rotate(const float* __restrict__
 //   c+[0] = sum_j z^j s[-j mod N] / (1 - z^N),       c+[k] = s[k] + z c+[k-1],
 //   c-[N-1] = -z / (1 - z^N) sum_j z^j c+[(N-1+j) mod N],  c-[k] = z (c-[k+1] - c+[k]),
 //   c = 6 c-,
-// the periodic sums truncated after 24 terms (z^24 < 1e-13).  Rows then columns.
+// the periodic sums truncated after BSPLINE_NT terms.  Rows then columns.
 constexpr double BSPLINE_Z = -0.2679491924311227;   // sqrt(3) - 2
-constexpr int BSPLINE_NT = 24;
+constexpr int BSPLINE_NT = 16;   // |z|^16 / (1 - |z|) < 1e-9: below float32 rounding
 
-// Each line is cut into nseg segments handled by one thread each (N lines x nseg <= block):
-// every segment starts from its own truncated periodic sum, so the recursions are short and
-// independent.  Phases: causal starts (read only), causal recursion, anticausal starts (read
-// the finished c+), anticausal recursion; the block synchronises between phases.  The start
-// sums run in two pieces (before and after the line's wrap) so no term needs a modulo.
+// Cubic B-spline prefilter (the interpolation coefficients of scipy's spline_filter with
+// periodic lines) of every row (ROWS) or column of the image, in place, in float32: the
+// causal recursion c+[k] = x[k] + z c+[k-1] and the anticausal c[k] = z (c[k+1] - c+[k]),
+// times 6.  Each line is cut into nseg segments run by consecutive threads; a segment's first
+// value comes from the truncated periodic sum over the BSPLINE_NT preceding samples (causal)
+// or following coefficients (anticausal), exact to float rounding since |z|^16 < 1e-9.
 template <bool ROWS>
-__device__ inline void bspline_lines(float* buf, int N, int S, double zN) {
+__device__ __forceinline__ void bspline_lines(float* buf, int N, int S, float zN) {
   const int es = ROWS ? 1 : S;
   const int nt = min(N, BSPLINE_NT);
-  const double z = BSPLINE_Z;
+  const float z = (float)BSPLINE_Z;
   const int nseg = max(1, min((int)blockDim.x / N, (N + 7) / 8));
   const int Ls = (N + nseg - 1) / nseg;
   const int t = threadIdx.x;
@@ -233,60 +244,60 @@ __device__ inline void bspline_lines(float* buf, int N, int S, double zN) {
   const int a = sg * Ls, e = min(a + Ls, N);
   const bool act = t < N * nseg && a < N;
   float* x = buf + (ROWS ? l * S : l);
-  double c0 = 0.0;
+  float c0 = 0.f;
   if (act) {
     // sum_{j < nt} z^j x[a - j mod N]: j <= a directly, then from the line's end
-    double zj = 1.0;
+    float zj = 1.f;
     const int j1 = min(a + 1, nt);
     for (int j = 0; j < j1; ++j) {
-      c0 += zj * (double)x[(a - j) * es];
+      c0 = fmaf(zj, x[(a - j) * es], c0);
       zj *= z;
     }
     for (int j = j1; j < nt; ++j) {
-      c0 += zj * (double)x[(N + a - j) * es];
+      c0 = fmaf(zj, x[(N + a - j) * es], c0);
       zj *= z;
     }
-    c0 /= 1.0 - zN;
+    c0 /= 1.f - zN;
   }
   __syncthreads();
   if (act) {
-    double prev = c0;
-    x[a * es] = (float)prev;
+    float prev = c0;
+    x[a * es] = prev;
     for (int k = a + 1; k < e; ++k) {
-      prev = (double)x[k * es] + z * prev;
-      x[k * es] = (float)prev;
+      prev = fmaf(z, prev, x[k * es]);
+      x[k * es] = prev;
     }
   }
   __syncthreads();
-  double cm = 0.0;
+  float cm = 0.f;
   if (act) {
     // sum_{j < nt} z^j c+[e - 1 + j mod N]: up to the line's end, then from its start
-    double zj = 1.0;
+    float zj = 1.f;
     const int j1 = min(N - e + 1, nt);
     for (int j = 0; j < j1; ++j) {
-      cm += zj * (double)x[(e - 1 + j) * es];
+      cm = fmaf(zj, x[(e - 1 + j) * es], cm);
       zj *= z;
     }
     for (int j = j1; j < nt; ++j) {
-      cm += zj * (double)x[(e - 1 + j - N) * es];
+      cm = fmaf(zj, x[(e - 1 + j - N) * es], cm);
       zj *= z;
     }
-    cm *= -z / (1.0 - zN);
+    cm *= -z / (1.f - zN);
   }
   __syncthreads();
   if (act) {
-    double nxt = cm;
-    x[(e - 1) * es] = (float)(6.0 * nxt);
+    float nxt = cm;
+    x[(e - 1) * es] = 6.f * nxt;
     for (int k = e - 2; k >= a; --k) {
-      nxt = z * (nxt - (double)x[k * es]);
-      x[k * es] = (float)(6.0 * nxt);
+      nxt = z * (nxt - x[k * es]);
+      x[k * es] = 6.f * nxt;
     }
   }
 }
 
 // The image is stored with row stride S = N + 3 and three extra rows and columns that repeat
 // rows / columns 0..2 (the periodic continuation), so the 4x4 spline taps never wrap.
-__device__ inline void refresh_pad(float* buf, int N, int S) {
+__device__ __forceinline__ void refresh_pad(float* buf, int N, int S) {
   for (int q = threadIdx.x; q < N * 3; q += blockDim.x) {
     const int r = q / 3, c = q % 3;
     buf[r * S + N + c] = buf[r * S + c];
@@ -299,7 +310,7 @@ __device__ inline void refresh_pad(float* buf, int N, int S) {
   __syncthreads();
 }
 
-__device__ inline void bspline_prefilter(float* buf, int N, int S, double zN) {
+__device__ __forceinline__ void bspline_prefilter(float* buf, int N, int S, float zN) {
   bspline_lines<true>(buf, N, S, zN);
   __syncthreads();
   bspline_lines<false>(buf, N, S, zN);
@@ -347,59 +358,99 @@ __device__ inline void spline_sample(const float* src, int N, int S, double angl
   }
 }
 
-// spline_sample into the same LDS image: every thread keeps its outputs in registers (a
-// 512-thread block gives each thread at most ceil(NMAX/8) rows x ceil(NMAX/64) columns), the
-// block synchronises, then writes them back -- one image of LDS per rotation instead of two.
-template <int NMAX>
-__device__ inline void spline_sample_inplace(float* buf, int N, int S, double angle_deg) {
-  constexpr int NW = 8, RI = (NMAX + NW - 1) / NW, CI = (NMAX + 63) / 64;
+// spline_sample into the same LDS image: thread t evaluates pixels p = t + NT k (row p / N,
+// column p % N) into registers, the block synchronises, then writes them back -- one image of
+// LDS per rotation instead of two.  Source coordinates in float64 (as the reference's), the
+// tap indices clamped as in spline_sample.  Inlined into its kernel, so the image is addressed
+// as LDS and, for a compile-time side NF, p / N is a multiply-high.
+// x = hi + lo with hi on the 2^-12 grid (at most 13 significant bits for |x| <= 1, 21 for
+// |x| < 2^9) and lo = x - hi rounded to float32
+__device__ __forceinline__ void split12(double x, float* hi, float* lo) {
+  const double h = rint(x * 4096.0) * (1.0 / 4096.0);
+  *hi = (float)h;
+  *lo = (float)(x - h);
+}
+
+// floor and fractional part of hi + lo, hi exact: hi - floor(hi) is exact, adding lo rounds once
+__device__ __forceinline__ float floor_frac(float hi, float lo, int* fl) {
+  float f0 = floorf(hi);
+  float t = (hi - f0) + lo;
+  if (t < 0.f) {
+    t += 1.f;
+    f0 -= 1.f;
+  } else if (t >= 1.f) {
+    t -= 1.f;
+    f0 += 1.f;
+  }
+  *fl = (int)f0;
+  return t;
+}
+
+template <int NMAX, int NF, int NT>
+__device__ __forceinline__ void spline_sample_inplace(float* buf, int N, int S,
+                                                      double angle_deg) {
+  constexpr int PIX = (NMAX * NMAX + NT - 1) / NT;
+  static_assert(NMAX < 256, "pixel indices must stay below 2^8 for the exact products");
   double sn, cs;
   sincospi(angle_deg / 180.0, &sn, &cs);
   const double h = 0.5 * (N - 1);
-  const double lo = 2 - N, hi = 2 * N - 3;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  float v[RI][CI];
+  // src_r = ar + cs r + sn c and src_c = ac - sn r + cs c, evaluated in float32 from split
+  // coefficients: with r, c < 2^8 the hi products and their sums are exact, the lo terms sum
+  // to less than 2^-4 with rounding below 2^-28, so the fractional parts carry the float64
+  // coordinates' values to float32 rounding
+  float cs_h, cs_l, sn_h, sn_l, ar_h, ar_l, ac_h, ac_l;
+  split12(cs, &cs_h, &cs_l);
+  split12(sn, &sn_h, &sn_l);
+  split12(h - cs * h - sn * h, &ar_h, &ar_l);
+  split12(h + sn * h - cs * h, &ac_h, &ac_l);
+  const int NN = N * N;
+  const float inv_n = 1.0f / N;
+  float v[PIX];
 #pragma unroll
-  for (int i = 0; i < RI; ++i) {
-    const int r = wv + i * NW;
-    const double dr = r - h;
-    const double br = h + cs * dr, bc = h - sn * dr;
+  for (int k = 0; k < PIX; ++k) {
+    const int p = threadIdx.x + k * NT;
+    float acc = 0.f;
+    if (p < NN) {
+      const int r = qdiv<NF>(p, N, inv_n), c = p - __umul24(r, N);
+      const float rf = (float)r, cf = (float)c;
+      const float sr_h = fmaf(sn_h, cf, fmaf(cs_h, rf, ar_h));
+      const float sr_l = fmaf(sn_l, cf, fmaf(cs_l, rf, ar_l));
+      const float sc_h = fmaf(cs_h, cf, fmaf(-sn_h, rf, ac_h));
+      const float sc_l = fmaf(cs_l, cf, fmaf(-sn_l, rf, ac_l));
+      int ir, ic;
+      float wr[4], wc[4];
+      bspline_w6(floor_frac(sr_h, sr_l, &ir), wr);
+      bspline_w6(floor_frac(sc_h, sc_l, &ic), wc);
+      // |src - h| <= sqrt(2) h for |sin|, |cos| <= 1 (a NaN angle converts to 0), so the first
+      // tap lies in [-N, 2N) and one wrap brings it into [0, N); the three after it come from
+      // the padding
+      ir -= 1;
+      ic -= 1;
+      ir += ir < 0 ? N : 0;
+      ir -= ir >= N ? N : 0;
+      ic += ic < 0 ? N : 0;
+      ic -= ic >= N ? N : 0;
+      const float* q = buf + __umul24(ir, S) + ic;
 #pragma unroll
-    for (int q = 0; q < CI; ++q) {
-      const int c = lane + q * 64;
-      float acc = 0.f;
-      if (r < N && c < N) {
-        const double dc = c - h;
-        const double sr = br + sn * dc;
-        const double sc = bc + cs * dc;
-        const double fr = floor(sr), fc = floor(sc);
-        float wr[4], wc[4];
-        bspline_w((float)(sr - fr), wr);
-        bspline_w((float)(sc - fc), wc);
-        int ir = (int)fmin(fmax(fr, lo), hi) - 1;
-        int ic = (int)fmin(fmax(fc, lo), hi) - 1;
-        ir += ir < 0 ? N : 0;
-        ir -= ir >= N ? N : 0;
-        ic += ic < 0 ? N : 0;
-        ic -= ic >= N ? N : 0;
-        const float* p = buf + ir * S + ic;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float* rp = p + u * S;
-          acc += wr[u] * (wc[0] * rp[0] + wc[1] * rp[1] + wc[2] * rp[2] + wc[3] * rp[3]);
-        }
+      for (int u = 0; u < 4; ++u) {
+        const float* rp = q + u * S;
+        acc += wr[u] * (wc[0] * rp[0] + wc[1] * rp[1] + wc[2] * rp[2] + wc[3] * rp[3]);
       }
-      v[i][q] = acc;
+      acc *= 1.f / 36.f;          // the two sets of weights were 6x the B-spline's
     }
+    v[k] = acc;
+    // one pixel at a time: without this the scheduler hoists every pixel's taps at once
+    __builtin_amdgcn_sched_barrier(0);
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < RI; ++i)
-#pragma unroll
-    for (int q = 0; q < CI; ++q) {
-      const int r = wv + i * NW, c = lane + q * 64;
-      if (r < N && c < N) buf[r * S + c] = v[i][q];
+  for (int k = 0; k < PIX; ++k) {
+    const int p = threadIdx.x + k * NT;
+    if (p < NN) {
+      const int r = qdiv<NF>(p, N, inv_n), c = p - __umul24(r, N);
+      buf[__umul24(r, S) + c] = v[k];
     }
+  }
   __syncthreads();
 }
 
@@ -427,11 +478,11 @@ __global__ __launch_bounds__(512) void k_align_images(const float* in, const dou
       A[r * S + c] = v;
     }
   __syncthreads();
-  const double zN = pow(BSPLINE_Z, (double)N);
+  const float zN = (float)pow(BSPLINE_Z, (double)N);
   bspline_prefilter(A, N, S, zN);
   float* o = out + img * N * N;
   if constexpr (NBUF == 2) {
-    spline_sample_inplace<NMAX>(A, N, S, ang1[img]);
+    spline_sample_inplace<NMAX, 0, 512>(A, N, S, ang1[img]);
     bspline_prefilter(A, N, S, zN);
     spline_sample(A, N, S, ang2[img], o, N);
   } else {
@@ -450,6 +501,55 @@ __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t
 // wr = (w1/2 k^2 - w2) k^2 = a_k - c_k df is formed in float64, reduced to [-pi, pi] in float64,
 // and only then passed to the float32 sincos (|C| error ~1e-7).
 constexpr int CTF_IMG = 4;
+// sin and cos of a CTF phase x (float64, |x| < 2^40): q = rint(x / (pi/2)) and
+// r = x - q pi/2 in float64 (|r| <= pi/4), then the Taylor series of sin r (to r^9) and cos r
+// (to r^10) in float32, whose truncation errors at |r| = pi/4 are 1.7e-9 and 1.1e-10, and
+// the quadrant q mod 4 picks the signs and the swap.
+__device__ inline void sincos_phase(double x, float* s, float* c) {
+  const double q = rint(x * 0.63661977236758134);
+  const float r = (float)fma(-q, 1.5707963267948966, x);
+  const float r2 = r * r;
+  const float sr =
+      r * fmaf(r2, fmaf(r2, fmaf(r2, fmaf(r2, 2.7557319e-6f, -1.9841270e-4f), 8.3333333e-3f),
+                        -1.6666667e-1f), 1.f);
+  const float cr = fmaf(
+      r2, fmaf(r2, fmaf(r2, fmaf(r2, fmaf(r2, -2.7557319e-7f, 2.4801587e-5f), -1.3888889e-3f),
+                        4.1666667e-2f), -0.5f), 1.f);
+  const int k = (int)((long long)q & 3);
+  *s = k == 0 ? sr : k == 1 ? cr : k == 2 ? -sr : -cr;
+  *c = k == 0 ? cr : k == 1 ? -sr : k == 2 ? -cr : sr;
+}
+
+// Operand entries of one half-spectrum bin from its spectrum value X, CTF value C and the
+// bin's half-spectrum weight w (1 or 2): {w |X|^2, C^2, sqrt(w) C Re X, sqrt(w) C Im X}.
+__device__ inline void operand_bin(float2 X, float C, float w, float v[4]) {
+  const float swc = (w == 1.f ? 1.f : 1.41421356f) * C;
+  v[0] = w * fmaf(X.x, X.x, X.y * X.y);
+  v[1] = C * C;
+  v[2] = swc * X.x;
+  v[3] = swc * X.y;
+}
+
+// The four segment entries of one operand row at element offset base (segment stride Khp):
+// float32, or the bf16 pair hi = rne(v), lo = rne(v - hi) (v - hi is exact in float32).
+__device__ inline void store_operand(int prec, void* op_hi, void* op_lo, int64_t base,
+                                     int64_t Khp, const float v[4]) {
+  if (prec == MEMI_PREC_FP32) {
+    float* o = (float*)op_hi;
+#pragma unroll
+    for (int sg = 0; sg < 4; ++sg) o[base + sg * Khp] = v[sg];
+  } else {
+    uint16_t* oh = (uint16_t*)op_hi;
+    uint16_t* ol = (uint16_t*)op_lo;
+#pragma unroll
+    for (int sg = 0; sg < 4; ++sg) {
+      const uint16_t h = bf16_rne(v[sg]);
+      oh[base + sg * Khp] = h;
+      ol[base + sg * Khp] = bf16_rne(v[sg] - bf16_to_f(h));
+    }
+  }
+}
+
 __global__ void k_ctf_pack(const float2* __restrict__ fy, const double* __restrict__ df,
                            int64_t n, int N, memi_ctf_params cp, int prec, int64_t Khp,
                            int64_t row0, void* __restrict__ op_hi, void* __restrict__ op_lo,
@@ -471,9 +571,7 @@ __global__ void k_ctf_pack(const float2* __restrict__ fy, const double* __restri
     const double sigm = cp.gauss_env / sqrt(2.0 * log(2.0));
     env = exp(-k2 / (2.0 * sigm * sigm));
   }
-  const double w = live ? half_weight(col, N) : 0.0;
-  const double sw = sqrt(w);
-  const double inv2pi = 0.15915494309189535, twopi = 6.283185307179586;
+  const float w = live ? half_weight(col, N) : 0.f;
   // all loads of the chunk are issued before any store (CTF_IMG images per thread)
   const int64_t i0 = (int64_t)blockIdx.y * CTF_IMG;
   double dfv[CTF_IMG];
@@ -488,36 +586,15 @@ __global__ void k_ctf_pack(const float2* __restrict__ fy, const double* __restri
   for (int u = 0; u < CTF_IMG; ++u) {
     const int64_t i = i0 + u;
     if (i >= n) break;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
     if (live) {
-      const double wr = a_k - c_k * dfv[u];
-      const double red = wr - twopi * rint(wr * inv2pi);
       float sf, cf;
-      sincosf((float)red, &sf, &cf);
-      const double C = ((double)sf - cp.ampc * (double)cf) * env;
-      ctf_half[i * Kh + k] = (float)C;
-      const float2 X = Xv[u];
-      v[0] = w * ((double)X.x * X.x + (double)X.y * X.y);
-      v[1] = C * C;
-      v[2] = sw * C * X.x;
-      v[3] = sw * C * X.y;
+      sincos_phase(a_k - c_k * dfv[u], &sf, &cf);
+      const float C = (sf - (float)cp.ampc * cf) * (float)env;
+      ctf_half[i * Kh + k] = C;
+      operand_bin(Xv[u], C, w, v);
     }
-    const int64_t base = (row0 + i) * 4 * Khp + k;
-    if (prec == MEMI_PREC_FP32) {
-      float* o = (float*)op_hi;
-#pragma unroll
-      for (int sg = 0; sg < 4; ++sg) o[base + sg * Khp] = (float)v[sg];
-    } else {
-      uint16_t* oh = (uint16_t*)op_hi;
-      uint16_t* ol = (uint16_t*)op_lo;
-#pragma unroll
-      for (int sg = 0; sg < 4; ++sg) {
-        const float f = (float)v[sg];
-        const uint16_t h = bf16_rne(f);
-        oh[base + sg * Khp] = h;
-        ol[base + sg * Khp] = bf16_rne((float)((double)f - (double)bf16_to_f(h)));
-      }
-    }
+    store_operand(prec, op_hi, op_lo, (row0 + i) * 4 * Khp + k, Khp, v);
   }
 }
 
@@ -568,8 +645,9 @@ __global__ void k_image_moments(const float* __restrict__ img, int64_t n, int64_
 }
 
 // ---- fused image preparation (a1-a6 in one kernel) ------------------------------------
-// Per half-spectrum bin k (row-major [ky][kx]): {Butterworth factor / N^2, a_k, c_k,
-// envelope, w_k, sqrt(w_k)} with the CTF phase wr = a_k - c_k df (ctemh_cryoFrank.py:24-44).
+// Tables over the half-spectrum bins k (row-major [ky][kx]), as arrays: a_k and c_k (float64,
+// the CTF phase wr = a_k - c_k df of ctemh_cryoFrank.py:24-44), then the Butterworth factor
+// / N^2 and the envelope (float32) -- 3 Kh doubles of the PT Kh the workspace holds.
 constexpr int PT = 6;
 
 __global__ void k_prep_tables(int N, int kind, double Qc, int order, memi_ctf_params cp,
@@ -594,27 +672,33 @@ __global__ void k_prep_tables(int N, int kind, double Qc, int order, memi_ctf_pa
       const double sigm = cp.gauss_env / sqrt(2.0 * log(2.0));
       env = exp(-k2 / (2.0 * sigm * sigm));
     }
-    const double w = half_weight(col, N);
-    double* t = tab + k * PT;
-    t[0] = f / ((double)N * N);
-    t[1] = 0.5 * (M_PI * cs * wav * wav * wav) * k2 * k2;
-    t[2] = M_PI * wav * k2;
-    t[3] = env;
-    t[4] = w;
-    t[5] = sqrt(w);
+    // structure of arrays, so that the kernels' loads of one field are contiguous
+    tab[k] = 0.5 * (M_PI * cs * wav * wav * wav) * k2 * k2;
+    tab[Kh + k] = M_PI * wav * k2;
+    float* tf = (float*)(tab + 2 * Kh);
+    tf[k] = (float)(f / ((double)N * N));
+    tf[Kh + k] = (float)env;
   }
 }
 
-template <int NMAX>
-__global__ __launch_bounds__(512) void k_prep_images(
+// NF > 0: a kernel for image side NF only (every size, count and divisor a constant);
+// NF = 0: any side up to NMAX, given at run time.  NT threads per image: 256 where four
+// images fit in one CU's LDS (sides up to 96: four blocks of four waves interleave their
+// barriers and serial spline recursions), else 512.  The 256-thread and the 128 kernels are
+// held to 128 registers: 4 waves per SIMD, i.e. four (two) images per CU, as their LDS allows.
+constexpr int prep_threads(int nmax) { return nmax <= 96 ? 256 : 512; }
+template <int NMAX, int NF, int NT = prep_threads(NMAX)>
+__global__ __launch_bounds__(NT, (NMAX == 128 || NT == 256) ? 4 : 1) void k_prep_images(
     const float* __restrict__ stack, const int64_t* __restrict__ src,
     const int32_t* __restrict__ conj, const double* __restrict__ rot1,
-    const double* __restrict__ rot2, const double* __restrict__ df, int N,
+    const double* __restrict__ rot2, const double* __restrict__ df, int N_arg,
     const double* __restrict__ tab, double ampc, int prec, int64_t Khp, int64_t row0,
     void* __restrict__ op_hi, void* __restrict__ op_lo, float2* __restrict__ fy,
     float* __restrict__ ctf_half, float* __restrict__ img_out) {
   // one image buffer: the padded real image ((N+3)^2 floats) is the largest of its layouts
   // (half spectrum 2 N (N/2+1) floats, row-pair lines 2 ceil(N/2) N floats)
+  static_assert(NF == 0 || NF == NMAX, "a fixed-size kernel is sized for its own side");
+  const int N = NF > 0 ? NF : N_arg;
   constexpr int BUF = (NMAX + 3) * (NMAX + 3);
   __shared__ float lds[BUF];
   __shared__ cpx tw[NMAX];
@@ -624,31 +708,37 @@ __global__ __launch_bounds__(512) void k_prep_images(
   const int S = N + 3;
   const int Nh = N / 2 + 1;
   const int64_t Kh = (int64_t)N * Nh;
+  const float* tabf = (const float*)(tab + 2 * Kh);    // filter and envelope (k_prep_tables)
   const int64_t img = blockIdx.x;
-  int rad[8];
-  const int nrad = radix_plan(N, rad);
+  int rad[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int nrad = 0;
+  if constexpr (NF > 0) {
+    fft_digit_tables_c<NF>(pos, rev);
+  } else {
+    nrad = radix_plan(N, rad);
+    fft_digit_tables(N, rad, nrad, pos, rev);
+  }
   fft_twiddles(tw, N);
-  fft_digit_tables(N, rad, nrad, pos, rev);
   // a1: load (conjugates flipped, SPIDER transpose) and background statistics
   const float* raw = stack + src[img] * (int64_t)N * N;
   const bool cj = conj != nullptr && conj[img] != 0;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double s1 = 0.0, s2 = 0.0;
   // every load of this thread in flight at once, then the LDS stores and the statistics
-  constexpr int PER = (NMAX * NMAX + 511) / 512;
+  constexpr int PER = (NMAX * NMAX + NT - 1) / NT;
   const int NNi = N * N;
   float vv[PER];
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
-    const int p = threadIdx.x + u * 512;
+    const int p = threadIdx.x + u * NT;
     vv[u] = p < NNi ? raw[p] : 0.f;
   }
   const float inv_n = 1.0f / N;
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
-    const int p = threadIdx.x + u * 512;
+    const int p = threadIdx.x + u * NT;
     if (p < NNi) {
-      const int a = fdivi(p, N, inv_n), b = p - a * N;
+      const int a = qdiv<NF>(p, N, inv_n), b = p - a * N;
       const int r = cj ? N - 1 - b : b;           // raw[a][b] lands at (r, a)
       A[r * S + a] = vv[u];
       const int xd = 2 * r - N + 2, yd = 2 * a - N;
@@ -661,19 +751,21 @@ __global__ __launch_bounds__(512) void k_prep_images(
   const double NN = (double)N * N;
   const double mean = block_sum(s1, red) / NN;
   const double sd = sqrt(block_sum(s2, red + 8) / NN - mean * mean);
+  const double inv_sd = 1.0 / sd;
   for (int r = wv; r < N; r += nw)
-    for (int c = lane; c < N; c += 64) A[r * S + c] = (float)(((double)A[r * S + c] - mean) / sd);
+    for (int c = lane; c < N; c += 64)
+      A[r * S + c] = (float)(((double)A[r * S + c] - mean) * inv_sd);
   __syncthreads();
   // a2: Butterworth in frequency (the table folds in the 1/N^2 of the inverse)
-  cpx* H = rfft2_ip<NMAX>(A, N, S, tw, rad, nrad, pos);
+  cpx* H = rfft2_ip<NMAX, NF, NT>(A, N, S, tw, rad, nrad, pos);
   const float inv_nh = 1.0f / Nh;
   for (int q = threadIdx.x; q < Kh; q += blockDim.x) {   // H row p holds ky = rev[p]
-    const int py = fdivi(q, Nh, inv_nh), kx = q - py * Nh;
-    const float f = (float)tab[(int64_t)(rev[py] * Nh + kx) * PT];
+    const int py = qdiv<NF>(q, Nh, inv_nh), kx = q - py * Nh;
+    const float f = tabf[rev[py] * Nh + kx];
     H[q] = make_float2(H[q].x * f, H[q].y * f);
   }
   __syncthreads();
-  irfft2_ip<NMAX>(A, N, S, tw, rad, nrad, pos);
+  irfft2_ip<NMAX, NF, NT>(A, N, S, tw, rad, nrad, pos);
   // a4: disk mask, then the two spline rotations, each in place
   for (int r = wv; r < N; r += nw)
     for (int c = lane; c < N; c += 64) {
@@ -681,54 +773,38 @@ __global__ __launch_bounds__(512) void k_prep_images(
       if (xd * xd + yd * yd >= N * N) A[r * S + c] = 0.f;
     }
   __syncthreads();
-  const double zN = pow(BSPLINE_Z, (double)N);
+  const float zN = (float)pow(BSPLINE_Z, (double)N);
   bspline_prefilter(A, N, S, zN);
-  spline_sample_inplace<NMAX>(A, N, S, rot1[img]);
+  // the rotations run with the side as a run-time value: with every pixel's coordinates
+  // constant-folded the compiler computes them all ahead and spills
+  int n_rt = N;
+  asm volatile("" : "+s"(n_rt));
+  spline_sample_inplace<NMAX, 0, NT>(A, n_rt, n_rt + 3, rot1[img]);
   bspline_prefilter(A, N, S, zN);
-  spline_sample_inplace<NMAX>(A, N, S, rot2[img]);
+  spline_sample_inplace<NMAX, 0, NT>(A, n_rt, n_rt + 3, rot2[img]);
   if (img_out) {
     float* o = img_out + img * (int64_t)N * N;
     for (int r = wv; r < N; r += nw)
       for (int c = lane; c < N; c += 64) o[r * N + c] = A[r * S + c];
   }
   // a5/a6: spectrum of the aligned image, CTF, operand rows (as k_ctf_pack)
-  cpx* F = rfft2_ip<NMAX>(A, N, S, tw, rad, nrad, pos);
+  cpx* F = rfft2_ip<NMAX, NF, NT>(A, N, S, tw, rad, nrad, pos);
   const double dfi = df[img];
-  const double inv2pi = 0.15915494309189535, twopi = 6.283185307179586;
+  const float ampf = (float)ampc;
   const int64_t base = (row0 + img) * 4 * Khp;
-  for (int64_t q = threadIdx.x; q < Khp; q += blockDim.x) {
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int q = threadIdx.x; q < Khp; q += blockDim.x) {
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
     if (q < Kh) {
-      const double* t = tab + q * PT;
-      const double wr = t[1] - t[2] * dfi;
-      const double rd = wr - twopi * rint(wr * inv2pi);
       float sf, cf;
-      sincosf((float)rd, &sf, &cf);
-      const double C = ((double)sf - ampc * (double)cf) * t[3];
-      const int ky = fdivi((int)q, Nh, inv_nh), kx = (int)q - ky * Nh;
+      sincos_phase(tab[q] - tab[Kh + q] * dfi, &sf, &cf);
+      const float C = (sf - ampf * cf) * tabf[Kh + q];
+      const int ky = qdiv<NF>(q, Nh, inv_nh), kx = q - ky * Nh;
       const cpx X = F[pos[ky] * Nh + kx];
-      ctf_half[img * Kh + q] = (float)C;
+      ctf_half[img * Kh + q] = C;
       fy[img * Kh + q] = X;
-      v[0] = t[4] * ((double)X.x * X.x + (double)X.y * X.y);
-      v[1] = C * C;
-      v[2] = t[5] * C * X.x;
-      v[3] = t[5] * C * X.y;
+      operand_bin(X, C, half_weight(kx, N), v);
     }
-    if (prec == MEMI_PREC_FP32) {
-      float* o = (float*)op_hi;
-#pragma unroll
-      for (int sg = 0; sg < 4; ++sg) o[base + sg * Khp + q] = (float)v[sg];
-    } else {
-      uint16_t* oh = (uint16_t*)op_hi;
-      uint16_t* ol = (uint16_t*)op_lo;
-#pragma unroll
-      for (int sg = 0; sg < 4; ++sg) {
-        const float f = (float)v[sg];
-        const uint16_t h = bf16_rne(f);
-        oh[base + sg * Khp + q] = h;
-        ol[base + sg * Khp + q] = bf16_rne((float)((double)f - (double)bf16_to_f(h)));
-      }
-    }
+    store_operand(prec, op_hi, op_lo, base + q, Khp, v);
   }
 }
 
@@ -892,15 +968,25 @@ int memi_prep_images(const float* stack, int64_t n_stack, const int64_t* src,
   hipLaunchKernelGGL(k_prep_tables, dim3(grid_for(Kh)), dim3(256), 0, s, N, filter_kind, Qc,
                      order, *ctf, ws);
   MEMI_CHECK_LAUNCH("k_prep_tables");
-  const dim3 g((unsigned)n), b(512);
-#define MEMI_PREP(NM)                                                                          \
-  hipLaunchKernelGGL(k_prep_images<NM>, g, b, 0, s, stack, src, conj, rot1_deg, rot2_deg, df, N, \
-                     ws, ctf->ampc, precision, Khp, row0, op_hi, op_lo, (float2*)fy, ctf_half,  \
-                     img_out)
-  if (N <= 64) MEMI_PREP(64);
-  else if (N <= 96) MEMI_PREP(96);
-  else if (N <= 128) MEMI_PREP(128);
-  else MEMI_PREP(160);
+  const dim3 g((unsigned)n);
+#define MEMI_PREP(NM, NF)                                                                      \
+  hipLaunchKernelGGL((k_prep_images<NM, NF>), g, dim3(prep_threads(NM)), 0, s, stack, src, conj, \
+                     rot1_deg, rot2_deg, df, N, ws, ctf->ampc, precision, Khp, row0, op_hi,     \
+                     op_lo, (float2*)fy, ctf_half, img_out)
+  // the power-of-two-rich sides of the standard workloads get their own kernels (a fixed
+  // 70 = 2 x 5 x 7 kernel needs more registers than the generic one); any other side runs on
+  // the kernel of the next larger size class
+  switch (N) {
+    case 64: MEMI_PREP(64, 64); break;
+    case 96: MEMI_PREP(96, 96); break;
+    case 128: MEMI_PREP(128, 128); break;
+    case 160: MEMI_PREP(160, 160); break;
+    default:
+      if (N <= 64) MEMI_PREP(64, 0);
+      else if (N <= 96) MEMI_PREP(96, 0);
+      else if (N <= 128) MEMI_PREP(128, 0);
+      else MEMI_PREP(160, 0);
+  }
 #undef MEMI_PREP
   MEMI_CHECK_LAUNCH("k_prep_images");
   return MEMI_OK;
