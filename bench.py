@@ -68,7 +68,44 @@ def parse():
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device copies of 512 MiB (PMC byte-counter calibration)")
+    ap.add_argument("--prep-images", default=None, metavar="N,R",
+                    help="only time the fused image prep on R random N x N images (split-BF16 "
+                         "operand; for counter runs of that kernel)")
     return ap.parse_args()
+
+
+def prep_images_only(spec, steps):
+    """steps launches of the fused image prep (memi_prep_images) on R random N x N images with
+    random angles and defoci; prints the per-launch times (HIP events) as one JSON line."""
+    import torch
+    from manifoldem_python_amd import _native as nat, pipeline as pl, synth
+    N, R = (int(x) for x in spec.split(","))
+    dev = torch.device("cuda")
+    nat.load()
+    g = torch.Generator(device=dev).manual_seed(1)
+    stack = torch.randn((R, N, N), generator=g, device=dev)
+    rng = np.random.default_rng(0)
+    src = torch.arange(R, device=dev)
+    conj = torch.zeros(R, dtype=torch.int32, device=dev)
+    rot1, rot2 = (torch.as_tensor(rng.uniform(0, 360, R), device=dev) for _ in range(2))
+    df = torch.as_tensor(rng.uniform(5000, 30000, R), device=dev)
+    ctf = nat.ctf_params(synth.PIX, synth.CS, synth.EKV, synth.AMPC, np.inf)
+    op = pl.Operand(R, N, nat.PREC_SPLIT_BF16, dev)
+    fy = torch.empty((R, N, N // 2 + 1), dtype=torch.complex64, device=dev)
+    ct = torch.empty((R, N * (N // 2 + 1)), dtype=torch.float32, device=dev)
+    ms = []
+    for _ in range(steps):
+        # the prep kernel runs on torch's current stream, which these events time
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        pl.prepare_images(stack, src, conj, rot1, rot2, df, N, FILTER, ctf, operand=op, row0=0,
+                          fy_out=fy, ctf_out=ct, want_images=False, src_max=R - 1,
+                          prep="fused")
+        b.record()
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    print(json.dumps({"kernel": "k_prep_images", "image": N, "images": R, "launch_ms": ms,
+                      "us_per_image": 1e3 * min(ms[1:] or ms) / R}), flush=True)
 
 
 def calibrate_copy():
@@ -577,6 +614,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.prep_images:
+        prep_images_only(args.prep_images, max(args.steps, 2))
         return
     import torch
     from manifoldem_python_amd import _native as nat, pipeline as pl
