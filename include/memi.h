@@ -123,6 +123,12 @@ int memi_project_mask(const float* vol, int L, const double* T, const double* lu
 /* Spectral multiply on rfft2 spectra, getDistanceCTF_local_Conj9combinedS2.py:285-293
  * (create_grid/create_filter :139-167) and the cubic-spline prefilter of rotatefill.py:24
  * (scipy ndimage.rotate order 3): kind is an OR of MEMI_FILTER_* flags. */
+/* S2tessellation.py:61-65 (classS2): for each particle direction pts[i] (float64 [n][3]), the
+ * index of the nearest bin centre (float64 [m][3]) by Euclidean distance, float64 arithmetic,
+ * ties to the lower index; idx int32 [n].  Replaces the reference's sklearn ball-tree 1-NN. */
+int memi_nearest_bins(const double* pts, int64_t n, const double* centres, int m, int32_t* idx,
+                      void* stream);
+
 int memi_spectrum_scale(void* spec, int64_t n, int N, int kind, double Qc, int order,
                         void* stream);
 

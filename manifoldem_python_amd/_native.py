@@ -79,6 +79,7 @@ _SIGS = {
                               c_vp, c_vp],
     "memi_tanh_fit": [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp],
     "memi_gather_blocks": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "memi_nearest_bins": [c_vp, c_i64, c_vp, c_int, c_vp, c_vp],
     "memi_ferguson_values": [c_vp, c_i64, c_vp, c_int, c_vp, c_vp],
     "memi_markov_coo": [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_vp],
     "memi_lanczos_steps": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_int, c_vp, c_int, c_vp, c_int,
@@ -543,6 +544,18 @@ def gather_blocks(plan, src, src_off, src_n, sel, dst):
     t_sel = torch.from_numpy(sel).to(dev)
     call("memi_gather_blocks", _p(plan.dev), ctypes.byref(plan.info), _p(src), _p(t_off),
          _p(t_n), _p(t_sel), _p(dst), _stream())
+
+
+def nearest_bins(pts, centres, idx):
+    """memi_nearest_bins: idx[i] = argmin_j ||pts[i] - centres[j]|| (float64 device tensors
+    (n, 3) and (m, 3); idx int32 (n,))."""
+    _need(pts.dtype == torch.float64 and centres.dtype == torch.float64 and pts.dim() == 2 and
+          centres.dim() == 2 and pts.shape[1] == 3 and centres.shape[1] == 3 and
+          pts.is_contiguous() and centres.is_contiguous(), "nearest_bins: float64 (n,3), (m,3)")
+    _need(idx.dtype == torch.int32 and idx.numel() >= pts.shape[0], "nearest_bins: idx int32 (n,)")
+    _need(centres.shape[0] >= 1, "nearest_bins: no centres")
+    call("memi_nearest_bins", _p(pts), pts.shape[0], _p(centres), centres.shape[0], _p(idx),
+         _stream())
 
 
 def ferguson_sweep(plan, D, mask, logEps, out):

@@ -204,6 +204,38 @@ __global__ void k_mask_project(const double* __restrict__ coef, int L, const dou
   proj[(int64_t)j * L + i] = s;
 }
 
+// S^2 tessellation (S2tessellation.py:61-65, classS2): the index of the nearest bin centre of
+// every particle direction, by squared Euclidean distance in float64 (the same ordering as the
+// distances of the reference's 1-NN search); equal distances go to the lower bin index.  One
+// thread per particle; the centres stream through LDS in tiles of NB_TILE.
+constexpr int NB_TILE = 1024;
+__global__ __launch_bounds__(256) void k_nearest_bins(const double* __restrict__ pts, int64_t n,
+                                                      const double* __restrict__ ctr, int m,
+                                                      int32_t* __restrict__ idx) {
+  __shared__ double c[3 * NB_TILE];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  const double px = live ? pts[3 * i] : 0.0, py = live ? pts[3 * i + 1] : 0.0,
+               pz = live ? pts[3 * i + 2] : 0.0;
+  double best = INFINITY;
+  int arg = 0;
+  for (int t0 = 0; t0 < m; t0 += NB_TILE) {
+    const int nt = min(NB_TILE, m - t0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 3 * nt; e += blockDim.x) c[e] = ctr[3 * (int64_t)t0 + e];
+    __syncthreads();
+    for (int j = 0; j < nt; ++j) {
+      const double dx = px - c[3 * j], dy = py - c[3 * j + 1], dz = pz - c[3 * j + 2];
+      const double d = fma(dx, dx, fma(dy, dy, dz * dz));
+      if (d < best) {
+        best = d;
+        arg = t0 + j;
+      }
+    }
+  }
+  if (live) idx[i] = arg;
+}
+
 }  // namespace memi
 
 using namespace memi;
@@ -255,6 +287,18 @@ int memi_project_mask(const float* vol, int L, const double* T, const double* lu
   hipLaunchKernelGGL(k_mask_threshold, dim3(grid_for((int64_t)L * L)), dim3(256), 0, s, proj,
                      (int64_t)L * L, mask);
   MEMI_CHECK_LAUNCH("k_mask_threshold");
+  return MEMI_OK;
+}
+
+int memi_nearest_bins(const double* pts, int64_t n, const double* centres, int m, int32_t* idx,
+                      void* stream) {
+  MEMI_CHECK_ARG(pts && centres && idx, "memi_nearest_bins: null pointer");
+  MEMI_CHECK_ARG(n >= 0 && m >= 1, "memi_nearest_bins: bad sizes n=%lld m=%d", (long long)n, m);
+  if (n == 0) return MEMI_OK;
+  MEMI_CHECK_ARG((n + 255) / 256 < 0x7fffffffLL, "memi_nearest_bins: too many points");
+  hipLaunchKernelGGL(k_nearest_bins, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, pts, n, centres, m, idx);
+  MEMI_CHECK_LAUNCH("k_nearest_bins");
   return MEMI_OK;
 }
 

@@ -24,6 +24,7 @@ machinefile = False
 eps = 1e-10
 
 # tessellation
+ang_width = 0.0               # bin width on S^2 in radians (set by the caller, as the GUI does)
 PDsizeThL = 100
 PDsizeThH = 2000
 numberofJobs = 0
