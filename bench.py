@@ -696,6 +696,13 @@ def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=T
     }
     if with_check:
         out["d_rel_err_vs_fp64"] = fp64_error(batch, bf["wsp"], bf["dist"])
+    d_sample = None
+    if with_cpu:
+        # a block of the timed run's D for the end-to-end check in the CPU baseline leg
+        n0 = int(batch.ns[0])
+        rows = np.unique(np.linspace(0, n0 - 1, min(n0, 24)).astype(np.int64))
+        blk = bf["dist"].block(0).double().cpu().numpy()
+        d_sample = (rows, blk[np.ix_(rows, rows)])
     del bf
     if with_fp32:
         log("timing FP32", workload)
@@ -720,7 +727,33 @@ def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=T
     if with_cpu:
         log("timing CPU baseline", workload)
         out["cpu_baseline"] = cpu_baseline(meta, batch, workload)
+        if d_sample is not None:
+            out["cpu_baseline"]["e2e_check"] = e2e_check(batch, *d_sample)
     return out
+
+
+def e2e_check(batch, rows, D_gpu):
+    """Relative error of the timed run's D on a sample of PD 0's images against the float64
+    oracle run from the same raw images (its prep a1-a6 with the PD geometry of all of PD 0's
+    particles, then its Gram, getDistanceCTF...:244-397): the whole path, not the Gram alone."""
+    from oracle import distance as od
+    N = batch.N
+    n0 = int(batch.ns[0])
+    src = batch.src[:n0].cpu().numpy()
+    conj = batch.conj[:n0].cpu().numpy().astype(bool)
+    nStot = 2 * int(batch.stack.shape[0])
+    ind = np.where(conj, src + nStot // 2, src)
+    stack = batch.stack[:int(src.max()) + 1].cpu().numpy()
+    p = dict(nPix=N, pix_size=1.5, Cs=2.7, EkV=300.0, gaussEnv=np.inf, AmpContrast=0.1)
+    pr = od.prep_rows([ind, batch.q_host[0], batch.df[:n0].cpu().numpy()], FILTER, stack,
+                      nStot, p, rows)
+    D64 = od.gram_distance(pr["fy"], pr["CTF"])
+    off = ~np.eye(len(rows), dtype=bool)
+    e = np.abs(D_gpu - D64)[off] / np.abs(D64)[off]
+    return {"images": int(len(rows)), "rel_err_max": float(e.max()),
+            "rel_err_median": float(np.median(e)),
+            "note": "D of a sample of PD 0's images: this run's split-BF16 path from raw images "
+                    "vs the float64 oracle's prep + Gram of the same raw images"}
 
 
 def gram_traffic_lookup(workload, tuned):
