@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--prep-images", default=None, metavar="N,R",
                     help="only time the fused image prep on R random N x N images (split-BF16 "
                          "operand; for counter runs of that kernel)")
+    ap.add_argument("--optflow", default=None, metavar="M,F,DIM",
+                    help="only time the optical flow of M synthetic F-frame DIM x DIM movies "
+                         "(Horn-Schunck, 200 iterations per frame pair)")
     return ap.parse_args()
 
 
@@ -106,6 +109,70 @@ def prep_images_only(spec, steps):
         ms.append(a.elapsed_time(b))
     print(json.dumps({"kernel": "k_prep_images", "image": N, "images": R, "launch_ms": ms,
                       "us_per_image": 1e3 * min(ms[1:] or ms) / R}), flush=True)
+
+
+def optflow_only(spec, steps, warmup, with_cpu):
+    """steps batched optical-flow passes (OpticalFlowMovie.op, 'HS', blockSize_avg 5) over M
+    movies of F frames of DIM x DIM; the Horn-Schunck kernel is also timed alone with HIP events
+    on the stream it runs on.  One JSON line."""
+    import torch
+    from manifoldem_python_amd import _native as nat, optflow
+    M, F, dim = (int(x) for x in spec.split(","))
+    dev = torch.device("cuda")
+    nat.load()
+    g = torch.Generator(device=dev).manual_seed(3)
+    movies = 50.0 * torch.randn((M, F, dim, dim), generator=g, device=dev)
+    movies = optflow.gaussian(movies.reshape(M * F, dim, dim), 2.0,
+                              torch.float32).reshape(M, F, dim, dim)
+    for _ in range(max(1, warmup)):
+        optflow.movie_flows(movies, 5, "HS")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        optflow.movie_flows(movies, 5, "HS")
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / steps
+    # the Horn-Schunck kernel alone: 200 iterations = 50 launches of 4 fused iterations
+    A = -(-F // 5)
+    nP = M * A
+    fr = movies[:, ::5].reshape(nP, dim, dim).contiguous()
+    pairs = optflow.frame_pairs(M, A, dev)
+    fx, fy, ft = optflow.derivatives(fr, pairs)
+    U = torch.zeros((nP, dim, dim), dtype=torch.float64, device=dev)
+    V = torch.zeros_like(U)
+    tu, tv = torch.empty_like(U), torch.empty_like(V)
+    nat.hs_iterate(fx, fy, ft, 2.0, 200, U, V, tu, tv)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()  # hs_iterate launches on torch's current stream, which these events time
+    for _ in range(steps):
+        nat.hs_iterate(fx, fy, ft, 2.0, 200, U, V, tu, tv)
+    b.record()
+    torch.cuda.synchronize()
+    launches = steps * 50
+    us_launch = 1e3 * a.elapsed_time(b) / launches
+    # algorithmic bytes of one launch (4 fused iterations): U, V read and written once in
+    # float64 and fx, fy, ft read once in float32 -- 44 B per pixel of every frame pair
+    alg = nP * dim * dim * 44.0
+    ach = alg / (us_launch * 1e-6) / 1e9
+    line = {"metric": "optical_flow_movies_per_s", "value": M / t_step, "unit": "movies/s",
+            "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * t_step,
+            "higher_is_better": True, "dtype": "f64 (flow), f32 (images)", "data": "synthetic",
+            "config": {"workload": "optflow_hs", "movies": M, "frames": F, "image": dim,
+                       "frame_pairs": nP, "hs_iterations": 200, "block_size_avg": 5},
+            "roofline": {"kernel": "k_hs_iterate<double,32,4,256>", "bound": "hbm",
+                         "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": ach / PEAK_HBM_GBS, "traffic": None,
+                         "us_per_launch": us_launch, "alg_bytes_per_launch": alg}}
+    if with_cpu:
+        from oracle import optflow as oof
+        mv = movies[0].reshape(F, dim * dim).cpu().numpy()
+        t0 = time.perf_counter()
+        oof.movie_flow(mv, 5, "HS")
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": 1.0 / tc, "unit": "movies/s", "cores": 1,
+                                "kind": "port", "sample": "1 movie, oracle.optflow.movie_flow"}
+    print(json.dumps(line), flush=True)
 
 
 def calibrate_copy():
@@ -617,6 +684,9 @@ def main():
         return
     if args.prep_images:
         prep_images_only(args.prep_images, max(args.steps, 2))
+        return
+    if args.optflow:
+        optflow_only(args.optflow, max(args.steps, 1), args.warmup, not args.no_cpu)
         return
     import torch
     from manifoldem_python_amd import _native as nat, pipeline as pl

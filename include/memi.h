@@ -120,15 +120,15 @@ int memi_shift_images(const float* in, int64_t n, int N, const double* shift0,
 int memi_project_mask(const float* vol, int L, const double* T, const double* lu, double* proj,
                       float* mask, double* ws, int64_t ws_doubles, void* stream);
 
-/* Spectral multiply on rfft2 spectra, getDistanceCTF_local_Conj9combinedS2.py:285-293
- * (create_grid/create_filter :139-167) and the cubic-spline prefilter of rotatefill.py:24
- * (scipy ndimage.rotate order 3): kind is an OR of MEMI_FILTER_* flags. */
 /* S2tessellation.py:61-65 (classS2): for each particle direction pts[i] (float64 [n][3]), the
  * index of the nearest bin centre (float64 [m][3]) by Euclidean distance, float64 arithmetic,
  * ties to the lower index; idx int32 [n].  Replaces the reference's sklearn ball-tree 1-NN. */
 int memi_nearest_bins(const double* pts, int64_t n, const double* centres, int m, int32_t* idx,
                       void* stream);
 
+/* Spectral multiply on rfft2 spectra, getDistanceCTF_local_Conj9combinedS2.py:285-293
+ * (create_grid/create_filter :139-167) and the cubic-spline prefilter of rotatefill.py:24
+ * (scipy ndimage.rotate order 3): kind is an OR of MEMI_FILTER_* flags. */
 int memi_spectrum_scale(void* spec, int64_t n, int N, int kind, double Qc, int order,
                         void* stream);
 
@@ -431,6 +431,51 @@ int memi_kernel_rowsums(const float* D, int n, const double* eps, int n_eps, dou
  * K recomputed from D (float32 [n][n]); Wt float32 [n][n]. */
 int memi_kernel_grad_weights(const float* D, int n, const double* eps, int n_eps,
                              const double* S, const double* g, float* Wt, void* stream);
+
+/* ---------------- Optical flow of NLSA movies (CC stage) --------------------------------- */
+/* Movies are float32 [nmov][F][H][W] (frame f of a movie is the reference's
+ * np.resize(Mov, (F, dim, dim))[f]).  Arithmetic follows numpy/scipy: float32 where numpy
+ * computes in float32, float64 accumulation rounded once where scipy.ndimage filters. */
+
+/* OpticalFlowMovie.py:202-341 (anisodiff3): niter - 1 Perona-Malik updates of every movie
+ * (option 1: exp(-(d/kappa)^2), option 2: 1/(1 + (d/kappa)^2); step = voxel spacing (z, y, x)).
+ * The result lands in out; tmp (same size) is needed when niter > 2.  in, out, tmp distinct. */
+int memi_flow_anisodiff(const float* in, float* out, float* tmp, int64_t nmov, int F, int H,
+                        int W, int niter, double kappa, double gamma, double step0, double step1,
+                        double step2, int option, void* stream);
+
+/* OpticalFlowMovie.py:447-457: out float64 [nmov][ceil(F/bs)][hw], the mean of each block of bs
+ * consecutive frames (the last block may be shorter). */
+int memi_flow_block_average(const float* in, int64_t nmov, int F, int64_t hw, int bs,
+                            double* out, void* stream);
+
+/* hornschunck_simple.py:35-38 (lowpassfilt = scipy gaussian_filter, truncate 4, mode
+ * 'reflect') on count images [H][W]: float64 -> float64, float64 -> float32 (the input is
+ * rounded to float32 first, as the reference's astype) or float32 -> float32.  tmp holds one
+ * pass in the output type (count*H*W elements). */
+int memi_gaussian_filter(const void* in, int in_f64, void* out, int out_f64, void* tmp,
+                         int64_t count, int H, int W, double sigma, void* stream);
+
+/* hornschunck_simple.py:24-31 (computeDerivatives) for nP frame pairs: pairs int32 [nP][2]
+ * holds the frame indices (im1, im2) into frames float32 [*][H][W]; fx, fy, ft float32
+ * [nP][H][W]. */
+int memi_hs_derivatives(const float* frames, const int32_t* pairs, int64_t nP, int H, int W,
+                        float* fx, float* fy, float* ft, void* stream);
+
+/* hornschunck_simple.py:70-82: niter Horn-Schunck iterations in place on U, V [nP][H][W]
+ * (float64 when is_f64, else float32 -- the reference's dtype follows the initial flow's);
+ * tmpU, tmpV are buffers of the same size. */
+int memi_hs_iterate(const float* fx, const float* fy, const float* ft, int64_t nP, int H, int W,
+                    double alpha, int niter, void* U, void* V, int is_f64, void* tmpU,
+                    void* tmpV, void* stream);
+
+/* OpticalFlowMovie.py:519-534 and getOrientMag :21-24: vx, vy float64 [nmov][hw] = the sum of
+ * U, V [nmov][A][hw] over A frame pairs (from 0.0 when from_zero, as the reference's
+ * VxM = zeros + Vx), orient = (atan2(vy, vx) % 2) * pi, mag = sqrt(vx^2 + vy^2), orient_rev the
+ * orientation of (-vx, -vy).  orient, mag and orient_rev may be null. */
+int memi_flow_sum(const void* U, const void* V, int is_f64, int64_t nmov, int A, int64_t hw,
+                  int from_zero, double* vx, double* vy, double* orient, double* mag,
+                  double* orient_rev, void* stream);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,15 @@ _SIGS = {
     "memi_fit1d_tau": [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_kernel_rowsums": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp],
     "memi_kernel_grad_weights": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
+    "memi_flow_anisodiff": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_dbl, c_dbl,
+                            c_dbl, c_dbl, c_dbl, c_int, c_vp],
+    "memi_flow_block_average": [c_vp, c_i64, c_int, c_i64, c_int, c_vp, c_vp],
+    "memi_gaussian_filter": [c_vp, c_int, c_vp, c_int, c_vp, c_i64, c_int, c_int, c_dbl, c_vp],
+    "memi_hs_derivatives": [c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp],
+    "memi_hs_iterate": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dbl, c_int, c_vp, c_vp, c_int,
+                        c_vp, c_vp, c_vp],
+    "memi_flow_sum": [c_vp, c_vp, c_int, c_i64, c_int, c_i64, c_int, c_vp, c_vp, c_vp, c_vp,
+                      c_vp, c_vp],
 }
 EXPORTS = ["memi_version", "memi_last_error"] + list(_SIGS)
 
@@ -784,3 +793,80 @@ def kernel_grad_weights(D, eps, S, g, Wt):
           Wt.dtype == torch.float32 and g.dtype == torch.float64, "kernel_grad_weights: shapes")
     call("memi_kernel_grad_weights", _p(D), D.shape[0], _p(eps), eps.shape[0], _p(S), _p(g),
          _p(Wt), _stream())
+
+
+# --- optical flow of NLSA movies (CC stage) ---------------------------------------------------
+
+def flow_anisodiff(inp, out, tmp, niter, kappa, gamma, step, option):
+    """memi_flow_anisodiff on float32 (nmov, F, H, W) movies; the result lands in out."""
+    _need(inp.dtype == torch.float32 and inp.dim() == 4, "flow_anisodiff: float32 (nmov,F,H,W)")
+    _need(out.dtype == torch.float32 and out.shape == inp.shape, "flow_anisodiff: out shape")
+    _need(tmp is None or (tmp.dtype == torch.float32 and tmp.numel() >= inp.numel()),
+          "flow_anisodiff: tmp too small")
+    nmov, F, H, W = inp.shape
+    call("memi_flow_anisodiff", _p(inp), _p(out), _p(tmp), nmov, F, H, W, int(niter),
+         float(kappa), float(gamma), float(step[0]), float(step[1]), float(step[2]),
+         int(option), _stream())
+
+
+def flow_block_average(inp, bs, out):
+    """memi_flow_block_average: inp float32 (nmov, F, H, W) -> out float64 (nmov, A, H, W)."""
+    _need(inp.dtype == torch.float32 and inp.dim() == 4, "flow_block_average: float32 input")
+    nmov, F, H, W = inp.shape
+    A = -(-F // bs)
+    _need(out.dtype == torch.float64 and tuple(out.shape) == (nmov, A, H, W),
+          "flow_block_average: out must be float64 (nmov, ceil(F/bs), H, W)")
+    call("memi_flow_block_average", _p(inp), nmov, F, H * W, int(bs), _p(out), _stream())
+
+
+def gaussian_filter(inp, out, tmp, sigma):
+    """memi_gaussian_filter on (count, H, W) images: f64->f64, f64->f32 or f32->f32."""
+    _need(inp.dim() == 3 and out.shape == inp.shape, "gaussian_filter: (count, H, W) in and out")
+    _need(inp.dtype in (torch.float32, torch.float64) and
+          out.dtype in (torch.float32, torch.float64), "gaussian_filter: float32/float64")
+    _need(tmp.dtype == out.dtype and tmp.numel() >= inp.numel(), "gaussian_filter: tmp")
+    count, H, W = inp.shape
+    call("memi_gaussian_filter", _p(inp), int(inp.dtype == torch.float64), _p(out),
+         int(out.dtype == torch.float64), _p(tmp), count, H, W, float(sigma), _stream())
+
+
+def hs_derivatives(frames, pairs, fx, fy, ft):
+    """memi_hs_derivatives: frames float32 (nF, H, W), pairs int32 (nP, 2) -> (nP, H, W) x 3."""
+    _need(frames.dtype == torch.float32 and frames.dim() == 3, "hs_derivatives: frames")
+    _need(pairs.dtype == torch.int32 and pairs.dim() == 2 and pairs.shape[1] == 2,
+          "hs_derivatives: pairs int32 (nP, 2)")
+    nF, H, W = frames.shape
+    nP = pairs.shape[0]
+    for t in (fx, fy, ft):
+        _need(t.dtype == torch.float32 and tuple(t.shape) == (nP, H, W),
+              "hs_derivatives: fx/fy/ft float32 (nP, H, W)")
+    if nP:
+        pr = pairs.cpu()
+        _need(int(pr.min()) >= 0 and int(pr.max()) < nF, "hs_derivatives: frame index range")
+    call("memi_hs_derivatives", _p(frames), _p(pairs), nP, H, W, _p(fx), _p(fy), _p(ft),
+         _stream())
+
+
+def hs_iterate(fx, fy, ft, alpha, niter, U, V, tmpU, tmpV):
+    """memi_hs_iterate in place on U, V (nP, H, W), float64 or float32."""
+    _need(fx.dim() == 3 and fx.dtype == torch.float32, "hs_iterate: fx float32 (nP, H, W)")
+    for t in (fy, ft):
+        _need(t.shape == fx.shape and t.dtype == torch.float32, "hs_iterate: fy/ft")
+    for t in (U, V, tmpU, tmpV):
+        _need(t.shape == fx.shape and t.dtype == U.dtype, "hs_iterate: U/V/tmp shapes")
+    _need(U.dtype in (torch.float32, torch.float64), "hs_iterate: U float32 or float64")
+    nP, H, W = fx.shape
+    call("memi_hs_iterate", _p(fx), _p(fy), _p(ft), nP, H, W, float(alpha), int(niter), _p(U),
+         _p(V), int(U.dtype == torch.float64), _p(tmpU), _p(tmpV), _stream())
+
+
+def flow_sum(U, V, from_zero, vx, vy, orient=None, mag=None, orient_rev=None):
+    """memi_flow_sum over the frame-pair axis of U, V (nmov, A, H, W) -> float64 (nmov, H, W)."""
+    _need(U.dim() == 4 and V.shape == U.shape and V.dtype == U.dtype and
+          U.dtype in (torch.float32, torch.float64), "flow_sum: U, V (nmov, A, H, W)")
+    nmov, A, H, W = U.shape
+    for t in (vx, vy, orient, mag, orient_rev):
+        _need(t is None or (t.dtype == torch.float64 and t.numel() == nmov * H * W),
+              "flow_sum: outputs float64 (nmov, H, W)")
+    call("memi_flow_sum", _p(U), _p(V), int(U.dtype == torch.float64), nmov, A, H * W,
+         int(bool(from_zero)), _p(vx), _p(vy), _p(orient), _p(mag), _p(orient_rev), _stream())

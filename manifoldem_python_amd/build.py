@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libmemi.so")
 SOURCES = ["common.cpp", "prep.hip", "gram.hip", "dm.hip", "eig.hip", "nlsa.hip", "grad.hip",
-           "ingest.hip"]
+           "ingest.hip", "optflow.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

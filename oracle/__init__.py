@@ -14,6 +14,10 @@ Modules:
 * ``nlsa`` - ``NLSA.op``, ``get_wiener.op``, ``svdRF.op``, ``L2_distance.op`` and the tau
   binning of ``psiAnalysisParS2.op``.
 * ``fit1d`` - ``fit_1D_open_manifold_3D.op`` and its helpers.
+* ``tessellation`` - ``S2tessellation.op``, ``distribute3Sphere.op``, the conjugate augmentation
+  and RELION's Euler-to-quaternion product.
+* ``optflow`` - ``CC/hornschunck_simple.op``, ``CC/OpticalFlowMovie.op`` (Horn-Schunck modes),
+  ``anisodiff3`` and ``getOrientMag``.
 
 Library versions this restatement is pinned on: numpy 2.2.6, scipy 1.15.3 (the reference pins
 numpy 1.19.4 / scipy 1.5.3 in ``env_linux.yml:77,101``).  Every function names the reference
