@@ -287,6 +287,15 @@ int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, cons
 int memi_tanh_fit(const double* x, const double* y, int m, int npd, const double* p0,
                   const int32_t* active, int max_iter, double* out, void* stream);
 
+/* The restart test of fergusonE.py:51-57 and the bandwidth of DMembeddingII.py:170 on the
+ * device: fit is memi_tanh_fit's out for R starts per PD ([npd][R][16]); for each PD the first
+ * start whose sum_i sqrt|diag(inv(J^T J))_i ss_res/(m-4)| <= max_res (100 in the reference; a
+ * singular J^T J or a non-finite value fails) gives ok[p] = 1 + its index and
+ * sigma[p] = tune sqrt(2 exp(-a1/a0)); ok[p] = 0 (sigma untouched) when none passes.
+ * res: optional float64 [npd][R], every start's value. */
+int memi_tanh_select(const double* fit, int npd, int R, int m, double tune, double max_res,
+                     int32_t* ok, double* sigma, double* res, void* stream);
+
 /* Markov normalisation, slaplacianonFly.py:42-81 with alpha = 1: W = exp(-v/sigma^2) on the
  * kernel's entries, d = column sums, W/(d_i d_j), d' = sqrt(column sums), L = |L + L^T| / 2.
  * Sums in float64.  sigma: float64 [npd]; L: [info->d_elems] float64 (l_is_f64 = 1, the

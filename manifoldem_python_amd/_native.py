@@ -98,6 +98,7 @@ _SIGS = {
     "memi_fit1d_tau": [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_kernel_rowsums": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp],
     "memi_kernel_grad_weights": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
+    "memi_tanh_select": [c_vp, c_int, c_int, c_int, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp],
     "memi_flow_anisodiff": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_dbl, c_dbl,
                             c_dbl, c_dbl, c_dbl, c_int, c_vp],
     "memi_flow_block_average": [c_vp, c_i64, c_int, c_i64, c_int, c_vp, c_vp],
@@ -609,6 +610,17 @@ def tanh_fit(x, y, p0, out, active=None, max_iter=400):
         raise RuntimeError("tanh_fit: shape mismatch")
     call("memi_tanh_fit", _p(x), _p(y), m, npd, _p(p0), _p(active), int(max_iter), _p(out),
          _stream())
+
+
+def tanh_select(fit, npd, R, m, tune, max_res, ok, sigma, res=None):
+    """memi_tanh_select on memi_tanh_fit's out of R starts per PD."""
+    _need(fit.dtype == torch.float64 and fit.numel() >= npd * R * 16, "tanh_select: fit")
+    _need(ok.dtype == torch.int32 and ok.numel() >= npd, "tanh_select: ok int32 (npd,)")
+    _need(sigma.dtype == torch.float64 and sigma.numel() >= npd, "tanh_select: sigma")
+    _need(res is None or (res.dtype == torch.float64 and res.numel() >= npd * R),
+          "tanh_select: res")
+    call("memi_tanh_select", _p(fit), int(npd), int(R), int(m), float(tune), float(max_res),
+         _p(ok), _p(sigma), _p(res), _stream())
 
 
 def ferguson_values(vals, logEps, out):

@@ -940,6 +940,84 @@ __global__ __launch_bounds__(256) void k_gather_blocks(const int32_t* __restrict
   for (int j = threadIdx.x; j < P.n; j += blockDim.x) drow[j] = srow[s[j]];
 }
 
+// The restart test of fergusonE.py:51-57 on one fit of memi_tanh_fit's output: pcov =
+// inv(J^T J) ss_res / (m - 4) as scipy's curve_fit forms it, value = sum_i sqrt|pcov_ii|
+// (+inf when J^T J is exactly singular or anything is not finite).  The diagonal of the inverse
+// comes from an LU factorisation with partial pivoting (first largest |pivot|, as LAPACK's
+// getrf) and one solve per unit vector.
+__device__ inline double tanh_fit_resnorm(const double* o, int m) {
+  double A[4][4];
+  int k = 4;
+  for (int i = 0; i < 4; ++i)
+    for (int j = i; j < 4; ++j) {
+      A[i][j] = o[k];
+      A[j][i] = o[k];
+      ++k;
+    }
+  int perm[4] = {0, 1, 2, 3};
+  for (int c = 0; c < 4; ++c) {
+    int pr = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (fabs(A[r][c]) > fabs(A[pr][c])) pr = r;
+    if (A[pr][c] == 0.0) return INFINITY;
+    if (pr != c) {
+      for (int j = 0; j < 4; ++j) {
+        const double t = A[c][j];
+        A[c][j] = A[pr][j];
+        A[pr][j] = t;
+      }
+      const int t = perm[c];
+      perm[c] = perm[pr];
+      perm[pr] = t;
+    }
+    for (int r = c + 1; r < 4; ++r) {
+      const double l = A[r][c] / A[c][c];
+      A[r][c] = l;
+      for (int j = c + 1; j < 4; ++j) A[r][j] -= l * A[c][j];
+    }
+  }
+  const double scale = o[14] / (double)(m - 4);
+  double sum = 0.0;
+  for (int e = 0; e < 4; ++e) {
+    // solve L U x = P e_e; only x_e is needed, but the back substitution needs the rest
+    double y[4];
+    for (int r = 0; r < 4; ++r) {
+      double v = perm[r] == e ? 1.0 : 0.0;
+      for (int j = 0; j < r; ++j) v -= A[r][j] * y[j];
+      y[r] = v;
+    }
+    double x[4];
+    for (int r = 3; r >= 0; --r) {
+      double v = y[r];
+      for (int j = r + 1; j < 4; ++j) v -= A[r][j] * x[j];
+      x[r] = v / A[r][r];
+    }
+    sum += sqrt(fabs(x[e] * scale));
+  }
+  return isfinite(sum) ? sum : INFINITY;
+}
+
+// One thread per PD: of its R fits, the first whose restart value is <= max_res; ok = 1 + its
+// index (0: none), sigma = tune sqrt(2 exp(-a1/a0)) (DMembeddingII.py:170) from its popt.
+__global__ void k_tanh_select(const double* __restrict__ fit, int npd, int R, int m, double tune,
+                              double max_res, int32_t* __restrict__ ok,
+                              double* __restrict__ sigma, double* __restrict__ res) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npd) return;
+  int pick = 0;
+  for (int r = 0; r < R; ++r) {
+    const double* o = fit + ((int64_t)p * R + r) * 16;
+    const double v = tanh_fit_resnorm(o, m);
+    if (res) res[(int64_t)p * R + r] = v;
+    if (!pick && v <= max_res) pick = r + 1;
+  }
+  ok[p] = pick;
+  if (pick) {
+    const double* o = fit + ((int64_t)p * R + pick - 1) * 16;
+    sigma[p] = tune * sqrt(2.0 * exp(-o[1] / o[0]));
+  }
+}
+
 }  // namespace memi
 
 using namespace memi;
@@ -1148,6 +1226,17 @@ int memi_tanh_fit(const double* x, const double* y, int m, int npd, const double
   hipLaunchKernelGGL(k_tanh_fit, dim3((unsigned)npd), dim3(TANH_NT), 0, (hipStream_t)stream, x, y, m,
                      p0, active, max_iter, out);
   MEMI_CHECK_LAUNCH("k_tanh_fit");
+  return MEMI_OK;
+}
+
+int memi_tanh_select(const double* fit, int npd, int R, int m, double tune, double max_res,
+                     int32_t* ok, double* sigma, double* res, void* stream) {
+  MEMI_CHECK_ARG(fit && ok && sigma && npd >= 0 && R >= 1 && m >= 5,
+                 "memi_tanh_select: bad arguments");
+  if (npd == 0) return MEMI_OK;
+  hipLaunchKernelGGL(k_tanh_select, dim3((unsigned)((npd + 63) / 64)), dim3(64), 0,
+                     (hipStream_t)stream, fit, npd, R, m, tune, max_res, ok, sigma, res);
+  MEMI_CHECK_LAUNCH("k_tanh_select");
   return MEMI_OK;
 }
 

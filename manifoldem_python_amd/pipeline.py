@@ -458,6 +458,35 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
     return popt, R2
 
 
+def fit_bandwidths(logEps, logS_dev, a0s, tune, rand=None):
+    """a13 for every PD: the tanh fit from each PD's start (memi_tanh_fit) and the restart
+    test with the bandwidth on the device (memi_tanh_select).  Only the acceptance flags come
+    back before the bandwidths are used; when a PD's first start fails the test, the host
+    restart loop (ferguson_fit_batch, the same starts first) takes over for the batch.
+    Returns (sigma (npd,) float64 device tensor, finish) with finish(logS host) -> (popt, R2)."""
+    dev = logS_dev.device
+    npd, m = logS_dev.shape
+    x = _device_const(logEps, dev)
+    starts = np.stack([np.ravel(a) for a in a0s]).astype(np.float64)
+    p0 = _to_device(starts, dev)
+    out = torch.empty(npd * 16, dtype=torch.float64, device=dev)
+    nat.tanh_fit(x, logS_dev, p0, out)
+    ok = torch.empty(npd, dtype=torch.int32, device=dev)
+    sigma = torch.empty(npd, dtype=torch.float64, device=dev)
+    nat.tanh_select(out, npd, 1, m, tune, 100.0, ok, sigma)
+    fit_h = PendingCopy(out)
+    if _to_host(ok).all():
+        def finish(logS):
+            o = fit_h.get().reshape(npd, 16)
+            ss_tot = np.sum((logS - logS.mean(axis=1, keepdims=True)) ** 2, axis=1)
+            return o[:, 0:4].copy(), 1 - o[:, 14] / ss_tot
+        return sigma, finish
+    logS = _to_host(logS_dev).reshape(npd, m)
+    popt, R2 = ferguson_fit_batch(np.asarray(logEps), logS, a0s, rand, logS_dev=logS_dev)
+    sig = tune * np.sqrt(2 * np.exp(-popt[:, 1] / popt[:, 0]))
+    return _to_device(sig, dev), lambda _logS: (popt, R2)
+
+
 class DMResult:
     __slots__ = ("lamb", "psi", "sigma", "mu", "logEps", "logSumWij", "popt", "R_squared",
                  "slope", "V", "lanczos_steps", "lanczos_residual")
@@ -792,11 +821,27 @@ def _to_device(a, dev):
     return buf.to(dev, non_blocking=True).view(a.shape)
 
 
-def ferguson_sums(plan, D, mask, logEps, method=None, want_log_dev=False):
+class PendingCopy:
+    """An asynchronous device-to-host copy into a pinned buffer of its own; ``get()`` waits for
+    it and returns the numpy array (the copy is ordered on the stream after the producers)."""
+
+    def __init__(self, t):
+        self.buf = torch.empty(t.numel(), dtype=t.dtype, pin_memory=True)
+        self.buf.copy_(t.reshape(-1), non_blocking=True)
+        self.ev = torch.cuda.Event()
+        self.ev.record()
+
+    def get(self):
+        self.ev.synchronize()
+        return self.buf.numpy().copy()
+
+
+def ferguson_sums(plan, D, mask, logEps, method=None, want_log_dev=False, defer=False):
     """a12 sums: (npd, n_eps) arrays of sum W, d log(sum W)/d log eps and #terms (and, with
     want_log_dev, log sum W as a device tensor).  method: "moments" (bucket moments, needs a
     log-uniform grid), "direct" (every entry at every eps) or "auto" (moments when the grid
-    allows)."""
+    allows).  defer: return a PendingCopy of the (3, npd, n_eps) host arrays instead of
+    waiting for them (with want_log_dev: (pending, logS_dev))."""
     npd = plan.npd
     le = _device_const(logEps, D.device)
     n_eps = le.shape[0]
@@ -815,7 +860,11 @@ def ferguson_sums(plan, D, mask, logEps, method=None, want_log_dev=False):
     # the slope (d sum W / d log eps over sum W) divided on the device (IEEE float64 division
     # there and in numpy give the same values), then S, slope and the counts in one copy
     f3 = fer[:npd * n_eps * 3].view(npd, n_eps, 3)
-    h = _to_host(torch.stack([f3[:, :, 0], f3[:, :, 1] / f3[:, :, 0], f3[:, :, 2]]))
+    st = torch.stack([f3[:, :, 0], f3[:, :, 1] / f3[:, :, 0], f3[:, :, 2]])
+    if defer:
+        pend = PendingCopy(st)
+        return (pend, logS_dev) if want_log_dev else pend
+    h = _to_host(st)
     h = h.reshape(3, npd, n_eps)
     if want_log_dev:
         return h[0], h[1], h[2], logS_dev
@@ -863,37 +912,29 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
         mask = knn_mask(plan, D)
     if timer:
         timer.mark("knn")
-    S, slope, _, logS_dev = ferguson_sums(plan, D, mask, logEps, want_log_dev=True)
+    # the sums and log sums come back to the host while the device goes on (they are outputs;
+    # the fit reads the device copy of log sum W)
+    fer_h, logS_dev = ferguson_sums(plan, D, mask, logEps, want_log_dev=True, defer=True)
+    logS_h = PendingCopy(logS_dev)
+    n_eps = logS_dev.shape[1]
     if timer:
         timer.mark("ferguson")
     rand = np.random.rand if rand is None else rand
-    results = []
-    sig = np.zeros(npd)
+    results = [DMResult() for _ in range(npd)]
     a0l = [((rand(4, 1) - 0.5) if a0s is None else a0s[p]) for p in range(npd)]
-    # log sum W was taken on the device for the fit; the host gets that copy (a host log of
-    # every PD's curve cost more than the transfer)
-    logS = _to_host(logS_dev).reshape(S.shape)
     logEps_a = np.asarray(logEps)
     if fit == "gpu":
-        popt_b, R2_b = ferguson_fit_batch(logEps_a, logS, a0l, rand, logS_dev=logS_dev)
-        # DMembeddingII.py:170, for every PD at once
-        sig_b = tune * np.sqrt(2 * np.exp(-popt_b[:, 1] / popt_b[:, 0]))
-    for p in range(npd):
-        r = DMResult()
-        r.logSumWij = logS[p]
-        r.slope = slope[p]
-        r.logEps = logEps_a
-        if fit == "gpu":
-            r.popt, r.R_squared = popt_b[p], R2_b[p]
-            r.sigma = sig_b[p]
-        else:
-            r.popt, _, r.R_squared = ferguson_fit(r.logEps, r.logSumWij, a0l[p], rand)
-            r.sigma = tune * np.sqrt(2 * np.exp(-r.popt[1] / r.popt[0]))
-        sig[p] = r.sigma
-        results.append(r)
+        sigma, fit_finish = fit_bandwidths(logEps_a, logS_dev, a0l, tune, rand)
+        sig_h = PendingCopy(sigma)
+    else:
+        logS = logS_h.get().reshape(npd, n_eps)
+        sig = np.zeros(npd)
+        for p, r in enumerate(results):
+            r.popt, _, r.R_squared = ferguson_fit(logEps_a, logS[p], a0l[p], rand)
+            sig[p] = tune * np.sqrt(2 * np.exp(-r.popt[1] / r.popt[0]))
+        sigma = _to_device(sig, dev)
     if timer:
         timer.mark("tanh_fit_host")
-    sigma = _to_device(sig, dev)
     # the kernel matrix buffer is kept on the plan so that the Lanczos graph stays valid
     L = getattr(plan, "_L", None)
     if (L is None or L.numel() < max(info.d_elems, 1) or L.device != dev or
@@ -913,6 +954,19 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
         with np.errstate(divide="ignore", invalid="ignore"):
             psi_all = Yall[:, 1:] / Yall[:, 0:1]
         mu_all = Yall[:, 0] ** 2
+    # the deferred outputs of a12 / a13 (their copies finished long before the eigensolve)
+    fh = fer_h.get().reshape(3, npd, n_eps)
+    logS = logS_h.get().reshape(npd, n_eps)
+    if fit == "gpu":
+        popt_b, R2_b = fit_finish(logS)
+        sig = sig_h.get()
+    for p, r in enumerate(results):
+        r.logSumWij = logS[p]
+        r.slope = fh[1, p]
+        r.logEps = logEps_a
+        r.sigma = sig[p]
+        if fit == "gpu":
+            r.popt, r.R_squared = popt_b[p], R2_b[p]
     for p in range(npd):
         n = int(plan.ns[p])
         nEigs = ks[p] - 1

@@ -175,6 +175,67 @@ def test_gpu_tanh_fit_matches_curve_fit(cuda):
         assert abs(R2[p] - R2r) < 1e-9
 
 
+def test_tanh_select_matches_host_rule(cuda):
+    """memi_tanh_select against the host's form of the restart rule (numpy inv of J^T J,
+    sum sqrt|diag pcov| <= 100, the first passing start) and sigma = tune sqrt(2 exp(-b/a)),
+    on fits from good and bad starts (several per PD), plus an exactly singular J^T J."""
+    import torch
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    from oracle import dm
+    curves = []
+    for seed in range(4):
+        D = _distance_matrix(120 + 30 * seed, 50 + seed)
+        yRow, yCol, yVal = dm.assemble(D.copy(), 50)
+        curves.append(dm.ferguson_sums(np.sqrt(yVal), dm.LOG_EPS)[0])
+    logS = np.stack(curves)
+    npd, m = logS.shape
+    R = 6
+    rng = np.random.default_rng(7)
+    starts = rng.random((npd, R, 4)) - 0.5
+    starts[:, 1] *= 40.0                     # far starts: some end with huge pcov
+    x = torch.as_tensor(dm.LOG_EPS, device=cuda)
+    y = torch.as_tensor(np.repeat(logS, R, axis=0), device=cuda)
+    out = torch.empty(npd * R * 16, dtype=torch.float64, device=cuda)
+    nat.tanh_fit(x, y, torch.as_tensor(starts.reshape(-1, 4), device=cuda), out)
+    o = out.cpu().numpy().reshape(npd, R, 16)
+    o[3, 0, 4:14] = 0.0                      # an exactly singular J^T J fails the test
+    o[3, 0, 4] = 1.0
+    out = torch.as_tensor(o.reshape(-1), device=cuda)
+    ok = torch.empty(npd, dtype=torch.int32, device=cuda)
+    sig = torch.full((npd,), -1.0, dtype=torch.float64, device=cuda)
+    res = torch.empty(npd * R, dtype=torch.float64, device=cuda)
+    nat.tanh_select(out, npd, R, m, 0.7, 100.0, ok, sig, res)
+    iu = np.triu_indices(4)
+    J = np.zeros((npd, R, 4, 4))
+    J[:, :, iu[0], iu[1]] = o[:, :, 4:14]
+    J = J + np.triu(J, 1).transpose(0, 1, 3, 2)
+    with np.errstate(all="ignore"):
+        diag = np.diagonal(pl.batched_inv(J.reshape(-1, 4, 4)), axis1=1, axis2=2)
+        want = np.sum(np.sqrt(np.fabs(diag.reshape(npd, R, 4) *
+                                      (o[:, :, 14] / (m - 4))[:, :, None])), axis=2)
+    want[~np.isfinite(want)] = np.inf
+    got = res.cpu().numpy().reshape(npd, R)
+    fin = np.isfinite(want)
+    assert np.array_equal(fin, np.isfinite(got))
+    # well-conditioned fits agree closely; from nearly singular ones (values 1e7 and up) any
+    # two inverses differ widely, and only the side of the threshold matters
+    wc = fin & (want < 1e3)
+    np.testing.assert_allclose(got[wc], want[wc], rtol=1e-7)
+    np.testing.assert_array_equal(got <= 100, want <= 100)
+    assert not fin[3, 0]
+    okw = want <= 100
+    assert okw.any() and not okw.all()       # both outcomes occur
+    pick = np.where(okw.any(axis=1), np.argmax(okw, axis=1) + 1, 0)
+    np.testing.assert_array_equal(ok.cpu().numpy(), pick)
+    sg = sig.cpu().numpy()
+    for p in range(npd):
+        if pick[p]:
+            a = o[p, pick[p] - 1]
+            assert abs(sg[p] - 0.7 * np.sqrt(2 * np.exp(-a[1] / a[0]))) <= 1e-14 * sg[p]
+        else:
+            assert sg[p] == -1.0
+
+
 def test_ferguson_moments_match_direct_sweep(cuda):
     """Bucket-moment sums (memi_ferguson_moments) against the direct sweep over every entry at
     every eps, for a batch of PDs with kNN and full kernels, duplicate points (zero entries)

@@ -101,6 +101,17 @@ def test_getdistance_op_writes_reference_pickle(cuda, tmp_path):
     np.testing.assert_allclose(got["PD"], ref["PD"], atol=1e-12)
 
 
+def _norm01(t):
+    t = np.ravel(t)
+    return (t - t.min()) / (t.max() - t.min())
+
+
+def _tau_gap(t, r):
+    """Largest difference of min-max normalised tau up to direction."""
+    t, r = _norm01(t), _norm01(r)
+    return min(np.max(np.abs(t - r)), np.max(np.abs(t - (1 - r))))
+
+
 def test_stage_drivers_end_to_end(cuda, tmp_path):
     """Tessellation pickle -> GetDistancesS2 -> manifoldAnalysis (rad = 5) -> psiAnalysis ->
     EL1D, with the resume-by-marker rule (a PD with a marker is skipped); the trimmed-psi and
@@ -142,6 +153,7 @@ def test_stage_drivers_end_to_end(cuda, tmp_path):
         assert d["D"].shape == (sizes[i], sizes[i])
     manifoldAnalysis.op()
     from oracle import dm, nlsa
+    from oracle import fit1d as ofit
     for i in range(3):
         d = myio.fin1("{}prD_{}".format(p.psi_file, i))
         assert d["psi"].shape[1] == 8 and len(d["posPath"]) <= sizes[i]
@@ -155,6 +167,7 @@ def test_stage_drivers_end_to_end(cuda, tmp_path):
                                            "eig_spec.txt"))
     psiAnalysis.op()
     first = {}
+    unstable = 0
     for i in range(3):
         img = myio.fin1("{}prD_{}".format(p.dist_file, i))
         psd = myio.fin1("{}prD_{}".format(p.psi_file, i))
@@ -178,13 +191,22 @@ def test_stage_drivers_end_to_end(cuda, tmp_path):
             for c in range(ref[3].shape[1]):
                 a, b = d["psiC1"][:, c], ref[3][:, c]
                 assert np.max(np.abs(np.sign(a @ b) * a - b)) < 1e-3 * np.max(np.abs(b))
-            # tau up to direction; the 1-D fit stops once its updates fall below 1 %
-            # (get_fit_1D_open_manifold_3D_param.py:17-21), so two runs whose inputs differ in
-            # the last digits agree to about that
-            tau_r = (ref[7] - ref[7].min()) / (ref[7].max() - ref[7].min())
+            # tau up to direction.  Where the reference's 1-D fit converges, tau is a stable
+            # function of psirec: the GPU's tau equals the oracle's fit of the same psirec and
+            # agrees with the end-to-end oracle to its 1 % stopping rule.  On some psi of PDs
+            # this small the fit does not converge within its 100 iterations (the curve
+            # coefficients drift) and its tau moves by ~0.2 under 1e-15 relative changes of
+            # psirec -- in the oracle as on the GPU; there tau is not compared.
             tau = np.ravel(d["tau"])
-            assert min(np.max(np.abs(tau - tau_r.ravel())),
-                       np.max(np.abs(tau - (1 - tau_r.ravel())))) < 2e-2
+            psirec = np.asarray(d["psirec"], dtype=np.float64)
+            pert = np.random.default_rng(5).standard_normal(psirec.shape)
+            fits = [_norm01(ofit.op(psirec * (1 + e * pert))[2]) for e in (0.0, 1e-13, -1e-13)]
+            if max(_tau_gap(fits[0], f) for f in fits[1:]) > 1e-3:
+                unstable += 1
+                continue
+            assert _tau_gap(tau, fits[0]) < 1e-6
+            assert _tau_gap(tau, _norm01(ref[7])) < 2e-2
+    assert unstable <= 2          # at least 4 of the 6 psi compared
     # second pass (EL1D, psiAnalysisParS2 isFull = 1): PD 0 keeps its sense, PD 1 flips, PD 2
     # is removed by the user
     p.trash_list = np.array([0, 0, 1])
