@@ -328,6 +328,13 @@ int memi_markov_coo(const int64_t* row, const int64_t* col, const double* val, i
                     int64_t nS, double sigma, double alpha, double* out, double* deg,
                     void* stream);
 
+/* The start of a Lanczos run (sembeddingonFly.py:17-39 starts ARPACK from a start vector) in
+ * one launch: V[idx[i]] = v0[i] for i < n_rows (column 0 of every PD's basis), ab[0..n_ab) = 0,
+ * j[0] = j[1] = 0, active[0..npd) = 1.  idx must lie inside V. */
+int memi_lanczos_init(double* V, const double* v0, const int64_t* idx, int64_t n_rows,
+                      double* ab, int64_t n_ab, int32_t* j, int32_t* active, int64_t npd,
+                      void* stream);
+
 /* nsteps Lanczos steps (full reorthogonalisation, CGS2) on every active PD's L, for the
  * largest-magnitude eigenpairs that sembeddingonFly.py:27 asks ARPACK eigsh for.
  * L: float32 (l_is_f64 = 0, as memi_markov_normalize writes it) or float64 (l_is_f64 = 1, a

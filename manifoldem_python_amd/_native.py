@@ -98,6 +98,7 @@ _SIGS = {
     "memi_fit1d_tau": [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "memi_kernel_rowsums": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp],
     "memi_kernel_grad_weights": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
+    "memi_lanczos_init": [c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp],
     "memi_tanh_select": [c_vp, c_int, c_int, c_int, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp],
     "memi_flow_anisodiff": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_dbl, c_dbl,
                             c_dbl, c_dbl, c_dbl, c_int, c_vp],
@@ -610,6 +611,17 @@ def tanh_fit(x, y, p0, out, active=None, max_iter=400):
         raise RuntimeError("tanh_fit: shape mismatch")
     call("memi_tanh_fit", _p(x), _p(y), m, npd, _p(p0), _p(active), int(max_iter), _p(out),
          _stream())
+
+
+def lanczos_init(V, v0, idx, ab, j, active):
+    """memi_lanczos_init: V[idx] = v0, ab = 0, j = 0, active = 1 (idx checked against V by the
+    caller when it is built)."""
+    _need(V.dtype == torch.float64 and v0.dtype == torch.float64 and idx.dtype == torch.int64
+          and idx.numel() == v0.numel(), "lanczos_init: V/v0 float64, idx int64 of v0's size")
+    _need(ab.dtype == torch.float64 and j.dtype == torch.int32 and j.numel() >= 2 and
+          active.dtype == torch.int32, "lanczos_init: ab float64, j / active int32")
+    call("memi_lanczos_init", _p(V), _p(v0), _p(idx), v0.numel(), _p(ab), ab.numel(), _p(j),
+         _p(active), active.numel(), _stream())
 
 
 def tanh_select(fit, npd, R, m, tune, max_res, ok, sigma, res=None):

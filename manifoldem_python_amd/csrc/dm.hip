@@ -356,9 +356,11 @@ __global__ __launch_bounds__(256) void k_fer_stats(const int32_t* __restrict__ f
   const bool full = P.k >= P.n;
   double sum = 0.0, cnt = 0.0, zer = 0.0;
   unsigned long long vmin = dbits(INFINITY), vmax = 0ull;
-  // blockIdx.y: which MARKOV_RQ-th of the tile's entries
-  const int e1 = (blockIdx.y + 1) * (TB * TB / MARKOV_RQ);
-  for (int e = blockIdx.y * (TB * TB / MARKOV_RQ) + threadIdx.x; e < e1; e += blockDim.x) {
+  // blockIdx.y: which gridDim.y-th of the tile's entries (one block per tile keeps the
+  // same-address atomics of a one-PD batch few)
+  const int part = TB * TB / gridDim.y;
+  const int e1 = (blockIdx.y + 1) * part;
+  for (int e = blockIdx.y * part + threadIdx.x; e < e1; e += blockDim.x) {
     const int i = f[1] * TB + e / TB, j = f[2] * TB + e % TB;
     if (i >= P.n || j >= P.n) continue;
     const double v = entry_value(D, mask, P, W, full, i, j);
@@ -1106,7 +1108,7 @@ int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, cons
   MEMI_CHECK_LAUNCH("k_zero_f64");
   hipLaunchKernelGGL(k_fer_init, dim3(grid_for(npd)), dim3(256), 0, s, npd, st);
   MEMI_CHECK_LAUNCH("k_fer_init");
-  hipLaunchKernelGGL(k_fer_stats, dim3((unsigned)info->n_ftiles, MARKOV_RQ), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_fer_stats, dim3((unsigned)info->n_ftiles, 1), dim3(256), 0, s,
                      v.ftiles,
                      v.pds, D, mask, logEps, n_eps, st);
   MEMI_CHECK_LAUNCH("k_fer_stats");

@@ -664,33 +664,82 @@ __global__ __launch_bounds__(RITZ_NT) void k_lanczos_ritz(const memi_pd_desc* __
 // Ritz vectors Y_p = V_p S_p of every PD in one launch, one thread per row: V_p the n x J
 // Lanczos basis (column-major at m*row0), S_p the J x k eigenvectors of T_p (row-major at
 // meta.s_off), Y [n_rows][kmax] (columns k..kmax-1 zero).  meta: int64 [npd][3] = {J, k, s_off}.
-__global__ __launch_bounds__(256) void k_ritz_vectors(const int32_t* __restrict__ rows,
-                                                      int64_t n_rows,
-                                                      const memi_pd_desc* __restrict__ pds,
-                                                      const double* __restrict__ V, int m,
-                                                      const double* __restrict__ S,
-                                                      const int64_t* __restrict__ meta, int kmax,
-                                                      double* __restrict__ Y) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_rows) return;
-  const int pd = rows[2 * g], i = rows[2 * g + 1];
-  const memi_pd_desc P = pds[pd];
-  const int n = P.n;
-  const int J = (int)meta[3 * pd], k = (int)meta[3 * pd + 1];
-  const double* Sp = S + meta[3 * pd + 2];
-  const double* v = V + (int64_t)m * P.row0 + i;
-  double acc[MEMI_RITZ_KMAX];
+// 64-thread blocks (a single PD of 2000 rows still spreads over 32 CUs); the S of the block's
+// first PD is staged in LDS (read by the whole wave at one address), a second PD's rows in the
+// same block read theirs from global memory.  Every output is summed over j in order.
+constexpr int RV_NT = 64;
+constexpr int RV_SMAX = 2048;
+
+template <bool LDS_S>
+__device__ __forceinline__ void ritz_row(const double* __restrict__ v, int n, int J, int k,
+                                         const double* __restrict__ Sp, double* acc) {
+  int j = 0;
+  for (; j + 8 <= J; j += 8) {
+    double x[8];
 #pragma unroll
-  for (int e = 0; e < MEMI_RITZ_KMAX; ++e) acc[e] = 0.0;
-  for (int j = 0; j < J; ++j) {
+    for (int u = 0; u < 8; ++u) x[u] = v[(int64_t)(j + u) * n];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int e = 0; e < MEMI_RITZ_KMAX; ++e)
+        if (e < k) acc[e] += x[u] * Sp[(int64_t)(j + u) * k + e];
+  }
+  for (; j < J; ++j) {
     const double x = v[(int64_t)j * n];
 #pragma unroll
     for (int e = 0; e < MEMI_RITZ_KMAX; ++e)
       if (e < k) acc[e] += x * Sp[(int64_t)j * k + e];
   }
+}
+
+__global__ __launch_bounds__(RV_NT) void k_ritz_vectors(const int32_t* __restrict__ rows,
+                                                        int64_t n_rows,
+                                                        const memi_pd_desc* __restrict__ pds,
+                                                        const double* __restrict__ V, int m,
+                                                        const double* __restrict__ S,
+                                                        const int64_t* __restrict__ meta,
+                                                        int kmax, double* __restrict__ Y) {
+  __shared__ double s_sh[RV_SMAX];
+  const int64_t g0 = (int64_t)blockIdx.x * RV_NT;          // < n_rows by the grid size
+  const int pd0 = rows[2 * g0];
+  const int64_t sz0 = meta[3 * pd0] * meta[3 * pd0 + 1];
+  const bool staged = sz0 <= RV_SMAX;
+  if (staged) {
+    const double* src = S + meta[3 * pd0 + 2];
+    for (int q = threadIdx.x; q < sz0; q += RV_NT) s_sh[q] = src[q];
+  }
+  __syncthreads();
+  const int64_t g = g0 + threadIdx.x;
+  if (g >= n_rows) return;
+  const int pd = rows[2 * g], i = rows[2 * g + 1];
+  const memi_pd_desc P = pds[pd];
+  const int n = P.n;
+  const int J = (int)meta[3 * pd], k = (int)meta[3 * pd + 1];
+  const double* v = V + (int64_t)m * P.row0 + i;
+  double acc[MEMI_RITZ_KMAX];
+#pragma unroll
+  for (int e = 0; e < MEMI_RITZ_KMAX; ++e) acc[e] = 0.0;
+  if (staged && pd == pd0)
+    ritz_row<true>(v, n, J, k, s_sh, acc);
+  else
+    ritz_row<false>(v, n, J, k, S + meta[3 * pd + 2], acc);
 #pragma unroll
   for (int e = 0; e < MEMI_RITZ_KMAX; ++e)
     if (e < kmax) Y[g * kmax + e] = e < k ? acc[e] : 0.0;
+}
+
+// The start of a Lanczos run in one launch: V[idx[i]] = v0[i] (column 0 of every PD's basis),
+// ab = 0, j = {0, 0}, active = 1.
+__global__ void k_lanczos_init(double* __restrict__ V, const double* __restrict__ v0,
+                               const int64_t* __restrict__ idx, int64_t n_rows,
+                               double* __restrict__ ab, int64_t n_ab, int32_t* __restrict__ j,
+                               int32_t* __restrict__ active, int64_t npd) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (int64_t t = t0; t < n_rows; t += stride) V[idx[t]] = v0[t];
+  for (int64_t t = t0; t < n_ab; t += stride) ab[t] = 0.0;
+  for (int64_t t = t0; t < npd; t += stride) active[t] = 1;
+  if (t0 < 2) j[t0] = 0;
 }
 
 }  // namespace memi
@@ -698,6 +747,18 @@ __global__ __launch_bounds__(256) void k_ritz_vectors(const int32_t* __restrict_
 using namespace memi;
 
 extern "C" {
+
+int memi_lanczos_init(double* V, const double* v0, const int64_t* idx, int64_t n_rows,
+                      double* ab, int64_t n_ab, int32_t* j, int32_t* active, int64_t npd,
+                      void* stream) {
+  MEMI_CHECK_ARG(V && v0 && idx && ab && j && active && n_rows >= 0 && n_ab >= 0 && npd >= 0,
+                 "memi_lanczos_init: bad arguments");
+  const int64_t work = std::max(std::max(n_rows, n_ab), std::max(npd, (int64_t)2));
+  hipLaunchKernelGGL(k_lanczos_init, dim3(grid_for(work)), dim3(256), 0, (hipStream_t)stream, V,
+                     v0, idx, n_rows, ab, n_ab, j, active, npd);
+  MEMI_CHECK_LAUNCH("k_lanczos_init");
+  return MEMI_OK;
+}
 
 int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const void* L,
                        int l_is_f64, double* V, int m, int32_t* j_dev, int nsteps,
@@ -771,7 +832,8 @@ int memi_ritz_vectors(const void* plan_dev, const memi_plan_info* info, const do
   const char* plan = (const char*)plan_dev;
   const memi_pd_desc* pds = (const memi_pd_desc*)(plan + info->off_pds);
   const int32_t* rows = (const int32_t*)(plan + info->off_rows);
-  hipLaunchKernelGGL(k_ritz_vectors, dim3((unsigned)((info->n_rows + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(k_ritz_vectors, dim3((unsigned)((info->n_rows + RV_NT - 1) / RV_NT)),
+                     dim3(RV_NT), 0,
                      s, rows, info->n_rows, pds, V, m, S, meta, kmax, Y);
   MEMI_CHECK_LAUNCH("k_ritz_vectors");
   return MEMI_OK;
