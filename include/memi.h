@@ -335,18 +335,20 @@ int memi_lanczos_init(double* V, const double* v0, const int64_t* idx, int64_t n
                       double* ab, int64_t n_ab, int32_t* j, int32_t* active, int64_t npd,
                       void* stream);
 
-/* nsteps Lanczos steps (full reorthogonalisation, CGS2) on every active PD's L, for the
+/* nsteps Lanczos steps (full reorthogonalisation: the three-term recurrence, then one classical
+ * Gram-Schmidt pass against the whole basis) on every active PD's L, for the
  * largest-magnitude eigenpairs that sembeddingonFly.py:27 asks ARPACK eigsh for.
  * L: float32 (l_is_f64 = 0, as memi_markov_normalize writes it) or float64 (l_is_f64 = 1, a
  * caller's own float64 kernel); products and sums in float64.  V: float64 bases, column-major per PD at m*row0 (column 0 = unit start vector, set by the
- * caller); W: float64 [n_rows] scratch; ab: float64 [npd][m][2] gets {alpha_j, beta_j};
+ * caller); W: float64 [n_rows] scratch ([2 * n_rows] with split = 1); ab: float64 [npd][m][2]
+ * gets {alpha_j, beta_j};
  * active: int32 [npd] (0 = skip).  The step index j lives in device memory: j_dev is int32
  * [2] = {j, arrival counter}, both zero before the first step.  Step j reads column j, writes
  * column j+1, and its last kernel advances j_dev[0] (the last of its workgroups to finish);
  * steps with j >= min(n, m) do nothing -- so the launches of a block of steps can be captured
  * once in a HIP graph and replayed.
  * split = 0: one workgroup per PD orthogonalises (H unused, may be NULL); split = 1: the
- * orthogonalisation is spread over the grid (for few large PDs), H: float64 [2*npd*(m+1)]. */
+ * orthogonalisation is spread over the grid (for few large PDs), H: float64 [npd*(m+2)]. */
 int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const void* L,
                        int l_is_f64, double* V, int m, int32_t* j_dev, int nsteps,
                        const int32_t* active, double* W, double* H, double* ab, int split,

@@ -652,17 +652,18 @@ def markov_coo(row, col, val, nS, sigma, alpha, out, deg):
 
 def lanczos_steps(plan, L, V, m, j_dev, nsteps, active, W, ab, H=None):
     """nsteps Lanczos steps from the step index held in j_dev (int32 [2] on the device: index,
-    arrival counter; both zero before the first step).  H:
-    float64 [2*npd*(m+1)] scratch selects the grid-parallel orthogonalisation; None selects one
-    workgroup per PD."""
+    arrival counter; both zero before the first step).  H: float64 [npd*(m+2)] scratch selects
+    the grid-parallel orthogonalisation (W then holds 2*n_rows); None selects one workgroup per
+    PD."""
     I = plan.info
     _need(L.dtype in (torch.float32, torch.float64) and L.numel() >= I.d_elems,
           "lanczos_steps: L must be float32 or float64 with d_elems elements")
-    _need(V.numel() >= m * I.n_rows and W.numel() >= I.n_rows and ab.numel() >= I.n_pd * m * 2
-          and active.numel() >= I.n_pd and active.dtype == torch.int32, "lanczos_steps: buffers")
+    _need(V.numel() >= m * I.n_rows and ab.numel() >= I.n_pd * m * 2 and
+          W.numel() >= (1 if H is None else 2) * I.n_rows and
+          active.numel() >= I.n_pd and active.dtype == torch.int32, "lanczos_steps: buffers")
     _need(j_dev.dtype == torch.int32 and j_dev.numel() >= 2, "lanczos_steps: j_dev int32 [2]")
-    _need(H is None or (H.dtype == torch.float64 and H.numel() >= 2 * I.n_pd * (m + 1)),
-          "lanczos_steps: H must be float64 with 2*npd*(m+1) elements")
+    _need(H is None or (H.dtype == torch.float64 and H.numel() >= I.n_pd * (m + 2)),
+          "lanczos_steps: H must be float64 with npd*(m+2) elements")
     call("memi_lanczos_steps", _p(plan.dev), ctypes.byref(plan.info), _p(L),
          int(L.dtype == torch.float64), _p(V), int(m),
          _p(j_dev), int(nsteps), _p(active), _p(W), None if H is None else _p(H), _p(ab),

@@ -3,10 +3,13 @@
 // Reference: sembeddingonFly.py:17-39 calls ARPACK eigsh(l, k=nEigs+1, which='LM'), an
 // implicitly restarted Lanczos method.  Here every PD of a batch runs Lanczos with full
 // reorthogonalisation in lockstep: one step is a grouped matvec w = L v_j over all PDs
-// (k_lanczos_matvec, many workgroups per PD) and one workgroup per PD that orthogonalises w
-// against the whole basis twice (classical Gram-Schmidt, CGS2), records alpha_j, beta_j and
-// appends v_{j+1} = w / beta_j (or, for batches of few large PDs, the same CGS2 spread over
-// the whole grid in three kernels).  The host diagonalises the small tridiagonal T; the Ritz
+// (k_lanczos_matvec, many workgroups per PD; it also leaves each row's w_i v_j[i]), then two
+// orthogonalisation passes: the three-term recurrence w' = w - alpha v_j - beta_{j-1} v_{j-1}
+// (alpha = v_j . w) removes the components Lanczos puts there, and one classical Gram-Schmidt
+// pass against the whole basis removes what rounding left (components of order eps), so w' is
+// orthogonalised twice as in CGS2, in one pass over the basis instead of two.  The step
+// records alpha_j, beta_j and appends v_{j+1} = w'' / beta_j.  One workgroup per PD does both
+// passes, or, for batches of few large PDs, two grid-wide kernels after the matvec.  The host diagonalises the small tridiagonal T; the Ritz
 // vectors V S of every PD come from one batched kernel (k_ritz_vectors).
 //
 // Layout: L float32 or float64, PD p's n*n block at plan d_off; V float64 column-major per PD, PD p at
@@ -40,7 +43,8 @@ __device__ inline void step_arrive(int32_t* jp, int j) {
 // holding the largest PDs' rows as a tail.
 template <typename T>
 __device__ __forceinline__ void matvec_row(const T* __restrict__ Lr, const double* __restrict__ v,
-                                           int n, int lane, double* __restrict__ out) {
+                                           int n, int lane, double* __restrict__ out,
+                                           double* __restrict__ aout, double vi) {
   double s[8];
 #pragma unroll
   for (int u = 0; u < 8; ++u) s[u] = 0.0;
@@ -53,13 +57,17 @@ __device__ __forceinline__ void matvec_row(const T* __restrict__ Lr, const doubl
   for (int u = 0; u < 7; ++u)
     if (c + 64 * u < n) s[u] += (double)Lr[c + 64 * u] * v[c + 64 * u];
   const double t = wave_total(((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])));
-  if (lane == 0) *out = t;
+  if (lane == 0) {
+    *out = t;
+    if (aout) *aout = t * vi;
+  }
 }
 
 template <typename T>
 __device__ __forceinline__ void matvec_rows4(const T* __restrict__ L0, int64_t stride,
                                              const double* __restrict__ v, int n, int lane,
-                                             double* __restrict__ out) {
+                                             double* __restrict__ out, double* __restrict__ aout,
+                                             const double* __restrict__ vrow) {
   double s[4][4];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
@@ -85,7 +93,10 @@ __device__ __forceinline__ void matvec_rows4(const T* __restrict__ L0, int64_t s
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const double t = wave_total((s[r][0] + s[r][1]) + (s[r][2] + s[r][3]));
-    if (lane == 0) out[r] = t;
+    if (lane == 0) {
+      out[r] = t;
+      if (aout) aout[r] = t * vrow[r];
+    }
   }
 }
 
@@ -97,7 +108,8 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restric
                                                         const double* __restrict__ V, int m,
                                                         const int32_t* __restrict__ jp,
                                                         const int32_t* __restrict__ active,
-                                                        double* __restrict__ W, int rpw) {
+                                                        double* __restrict__ W,
+                                                        double* __restrict__ A, int rpw) {
   const int j = *jp;
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -109,21 +121,23 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restric
     const int n = P.n;
     const bool live = active[pd] && j < min(n, m);
     // rows g..g+3 are rows i..i+3 of the same PD when the run and the PD both hold them
+    const double* vj = V + (int64_t)m * P.row0 + (int64_t)j * n;
     if (g + 3 < g1 && i + 3 < n) {
       if (live)
-        matvec_rows4<T>(L + P.d_off + (int64_t)i * n, n, V + (int64_t)m * P.row0 + (int64_t)j * n,
-                        n, lane, W + P.row0 + i);
+        matvec_rows4<T>(L + P.d_off + (int64_t)i * n, n, vj, n, lane, W + P.row0 + i,
+                        A ? A + P.row0 + i : nullptr, vj + i);
       g += 4;
     } else {
       if (live)
-        matvec_row<T>(L + P.d_off + (int64_t)i * n, V + (int64_t)m * P.row0 + (int64_t)j * n, n,
-                      lane, W + P.row0 + i);
+        matvec_row<T>(L + P.d_off + (int64_t)i * n, vj, n, lane, W + P.row0 + i,
+                      A ? A + P.row0 + i : nullptr, live ? vj[i] : 0.0);
       g += 1;
     }
   }
 }
 
-// one block per PD: CGS2 against columns 0..j, then normalise into column j+1
+// one block per PD: the three-term recurrence, one CGS pass against columns 0..j, then
+// normalise into column j+1
 __device__ inline void lanczos_orth_body(const memi_pd_desc* __restrict__ pds,
                                          double* __restrict__ V, int m, int j,
                                          const int32_t* __restrict__ active,
@@ -137,26 +151,31 @@ __device__ inline void lanczos_orth_body(const memi_pd_desc* __restrict__ pds,
   double* Vp = V + (int64_t)m * P.row0;
   double* w = W + P.row0;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  double alpha = 0.0;
-  for (int pass = 0; pass < 2; ++pass) {
-    // h = V[:, 0..j]^T w : one wave per column
-    for (int c = wid; c <= j; c += 4) {
-      const double* vc = Vp + (int64_t)c * n;
-      double s = 0.0;
-      for (int r = lane; r < n; r += 64) s += vc[r] * w[r];
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-      if (lane == 0) h[c] = s;
-    }
-    __syncthreads();
-    alpha += h[j];
-    // w -= V h
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-      double acc = 0.0;
-      for (int c = 0; c <= j; ++c) acc += Vp[(int64_t)c * n + r] * h[c];
-      w[r] -= acc;
-    }
-    __syncthreads();
+  // pass 1: w' = w - alpha v_j - beta_{j-1} v_{j-1}, alpha = v_j . w
+  const double* vj = Vp + (int64_t)j * n;
+  const double* vjm = j > 0 ? vj - n : vj;
+  const double bp = j > 0 ? ab[((int64_t)pd * m + j - 1) * 2 + 1] : 0.0;
+  double sa = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) sa += vj[r] * w[r];
+  double alpha = block_sum(sa, red);
+  for (int r = threadIdx.x; r < n; r += blockDim.x) w[r] = (w[r] - alpha * vj[r]) - bp * vjm[r];
+  __syncthreads();
+  // pass 2: h = V[:, 0..j]^T w' (one wave per column), w'' = w' - V h
+  for (int c = wid; c <= j; c += 4) {
+    const double* vc = Vp + (int64_t)c * n;
+    double s = 0.0;
+    for (int r = lane; r < n; r += 64) s += vc[r] * w[r];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) h[c] = s;
   }
+  __syncthreads();
+  alpha += h[j];
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    double acc = 0.0;
+    for (int c = 0; c <= j; ++c) acc += Vp[(int64_t)c * n + r] * h[c];
+    w[r] -= acc;
+  }
+  __syncthreads();
   double ss = 0.0;
   for (int r = threadIdx.x; r < n; r += blockDim.x) ss += w[r] * w[r];
   const double beta = sqrt(block_sum(ss, red));
@@ -185,22 +204,33 @@ __global__ __launch_bounds__(256) void k_lanczos_orth(const memi_pd_desc* __rest
 }
 
 
-// Grid-parallel CGS2 for large PDs (one workgroup per PD leaves most CUs idle when a batch
-// has few PDs): per pass, a dot kernel (k_orth_dot: one workgroup per (column, PD), or
-// k_orth_dot_pairs for many PDs) then k_orth_update_rows (64 rows x 4 column groups per
-// workgroup).  The second dot pass also forms w.w, so the second update can finish the step
-// itself: with V orthonormal, ||w - V h2||^2 = w.w - ||h2||^2, beta is known before w - V h2 is,
-// and each row of v_{j+1} = (w - V h2) / beta is written where it is computed (no separate
-// normalisation launch).  H: float64 [2][npd][m + 1]: the coefficients of the two passes, slot
-// m of the second pass holds w.w.
-constexpr int H_STRIDE_EXTRA = 1;
+// Grid-parallel orthogonalisation for large PDs (one workgroup per PD leaves most CUs idle
+// when a batch has few PDs), two kernels after the matvec:
+//   k_orth_dot (one workgroup per (column, PD)) or k_orth_dot_pairs (many PDs): alpha = the sum
+//     of the rows' w_i v_j[i] that the matvec left in A (summed identically by every block), and
+//     with w' = w - alpha v_j - beta_{j-1} v_{j-1} formed on the fly, h = V^T w' and w'.w';
+//   k_orth_update_rows: w'' = w' - V h, and since V is orthonormal and h is of order eps
+//     relative to w', beta^2 = w'.w' - ||h||^2 is known before w'' is, so each row of
+//     v_{j+1} = w'' / beta is written where it is computed; alpha_j = alpha + h_j.
+// H: float64 [npd][m + 2]: h in slots 0..j, w'.w' in slot m, alpha in slot m + 1.
+constexpr int H_STRIDE_EXTRA = 2;
+
+// w'_r of the three-term recurrence
+__device__ __forceinline__ double w_local(const double* __restrict__ w,
+                                          const double* __restrict__ vj,
+                                          const double* __restrict__ vjm, double alpha,
+                                          double bp, int r) {
+  return (w[r] - alpha * vj[r]) - bp * vjm[r];
+}
 
 __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict__ pds,
                                                   const double* __restrict__ V, int m,
                                                   const int32_t* __restrict__ jp,
                                                   const int32_t* __restrict__ active,
                                                   const double* __restrict__ W,
-                                                  double* __restrict__ H, int with_norm) {
+                                                  const double* __restrict__ A,
+                                                  const double* __restrict__ ab,
+                                                  double* __restrict__ H) {
   __shared__ double red[8];
   const int j = *jp;
   const int pd = blockIdx.y;
@@ -208,82 +238,110 @@ __global__ __launch_bounds__(256) void k_orth_dot(const memi_pd_desc* __restrict
   const memi_pd_desc P = pds[pd];
   const int n = P.n;
   if (j >= min(n, m)) return;
-  // a fixed grid of blocks walks the live columns (entry j + 1 with_norm: w itself, to slot
-  // m); a grid sized for the largest j would launch hundreds of blocks that exit at once
-  const int top = j + 1 + (with_norm ? 1 : 0);
   const double* w = W + P.row0;
-  for (int c = blockIdx.x; c < top; c += gridDim.x) {   // uniform over the block
+  const double* Vp = V + (int64_t)m * P.row0;
+  const double* vj = Vp + (int64_t)j * n;
+  const double* vjm = j > 0 ? vj - n : vj;
+  const double bp = j > 0 ? ab[((int64_t)pd * m + j - 1) * 2 + 1] : 0.0;
+  double sa = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) sa += A[P.row0 + r];
+  const double alpha = block_sum(sa, red);
+  const int64_t hs = m + H_STRIDE_EXTRA;
+  // a fixed grid of blocks walks the live columns and, as entry j + 1, w' itself (to slot m)
+  for (int c = blockIdx.x; c < j + 2; c += gridDim.x) {   // uniform over the block
     const bool norm = c == j + 1;
-    const double* vc = norm ? w : V + (int64_t)m * P.row0 + (int64_t)c * n;
+    const double* vc = Vp + (int64_t)(norm ? j : c) * n;
     double s = 0.0;
-    for (int r = threadIdx.x; r < n; r += blockDim.x) s += vc[r] * w[r];
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const double wr = w_local(w, vj, vjm, alpha, bp, r);
+      s += (norm ? wr : vc[r]) * wr;
+    }
     s = block_sum(s, red);
-    if (threadIdx.x == 0) H[(int64_t)pd * (m + H_STRIDE_EXTRA) + (norm ? m : c)] = s;
+    if (threadIdx.x == 0) H[(int64_t)pd * hs + (norm ? m : c)] = s;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) H[(int64_t)pd * hs + m + 1] = alpha;
 }
 
-// Many-PD form of the dot pass: a fixed grid whose waves walk the npd x (j + 1) live
-// (PD, column) pairs (j + 2 with the w.w entry), one wave per pair (grids sized for the
-// largest j leave most blocks empty when there are hundreds of PDs).  The pair index is
-// uniform over the wave (scalar loads of the PD descriptor and flag); four accumulators keep
-// four loads of each operand in flight, and the wave sums by DPP.
+// Many-PD form of the dot pass: a fixed grid whose waves walk the npd x (j + 2) live
+// (PD, column) pairs, one wave per pair (grids sized for the largest j leave most blocks empty
+// when there are hundreds of PDs).  The pair index is uniform over the wave (scalar loads of
+// the PD descriptor and flag); each wave sums alpha itself, in one fixed order, and the wave of
+// column 0 stores it.
 __global__ __launch_bounds__(256) void k_orth_dot_pairs(const memi_pd_desc* __restrict__ pds,
                                                         int64_t npd,
                                                         const double* __restrict__ V, int m,
                                                         const int32_t* __restrict__ jp,
                                                         const int32_t* __restrict__ active,
                                                         const double* __restrict__ W,
-                                                        double* __restrict__ H, int with_norm) {
+                                                        const double* __restrict__ A,
+                                                        const double* __restrict__ ab,
+                                                        double* __restrict__ H) {
   const int j = *jp;
   const int lane = threadIdx.x & 63;
-  const int per = j + 1 + (with_norm ? 1 : 0);
+  const int per = j + 2;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int64_t total = npd * (int64_t)per;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t hs = m + H_STRIDE_EXTRA;
   for (int64_t q = (int64_t)blockIdx.x * (blockDim.x >> 6) + wid; q < total; q += nwaves) {
     const int pd = (int)(q / per), c = (int)(q % per);
     if (!active[pd]) continue;                    // uniform over the wave
     const memi_pd_desc P = pds[pd];
     const int n = P.n;
     if (j >= min(n, m)) continue;
-    const bool norm = c == j + 1;
     const double* w = W + P.row0;
-    const double* vc = norm ? w : V + (int64_t)m * P.row0 + (int64_t)c * n;
+    const double* Vp = V + (int64_t)m * P.row0;
+    const double* vj = Vp + (int64_t)j * n;
+    const double* vjm = j > 0 ? vj - n : vj;
+    const double bp = j > 0 ? ab[((int64_t)pd * m + j - 1) * 2 + 1] : 0.0;
+    double sa = 0.0;
+    for (int r = lane; r < n; r += 64) sa += A[P.row0 + r];
+    const double alpha = wave_total(sa);
+    const bool norm = c == j + 1;
+    const double* vc = Vp + (int64_t)(norm ? j : c) * n;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int r = lane;
     for (; r + 3 * 64 < n; r += 4 * 64) {
-      s0 += vc[r] * w[r];
-      s1 += vc[r + 64] * w[r + 64];
-      s2 += vc[r + 128] * w[r + 128];
-      s3 += vc[r + 192] * w[r + 192];
+      const double w0 = w_local(w, vj, vjm, alpha, bp, r);
+      const double w1 = w_local(w, vj, vjm, alpha, bp, r + 64);
+      const double w2 = w_local(w, vj, vjm, alpha, bp, r + 128);
+      const double w3 = w_local(w, vj, vjm, alpha, bp, r + 192);
+      s0 += (norm ? w0 : vc[r]) * w0;
+      s1 += (norm ? w1 : vc[r + 64]) * w1;
+      s2 += (norm ? w2 : vc[r + 128]) * w2;
+      s3 += (norm ? w3 : vc[r + 192]) * w3;
     }
-    for (; r < n; r += 64) s0 += vc[r] * w[r];
+    for (; r < n; r += 64) {
+      const double w0 = w_local(w, vj, vjm, alpha, bp, r);
+      s0 += (norm ? w0 : vc[r]) * w0;
+    }
     const double s = wave_total((s0 + s1) + (s2 + s3));
-    if (lane == 0) H[(int64_t)pd * (m + H_STRIDE_EXTRA) + (norm ? m : c)] = s;
+    if (lane == 0) {
+      H[(int64_t)pd * hs + (norm ? m : c)] = s;
+      if (c == 0) H[(int64_t)pd * hs + m + 1] = alpha;
+    }
   }
 }
 
-// Row-list form of the update pass: 64 consecutive rows of the plan per block (rows of two
-// PDs may share a block), 4 column groups each, w_r -= sum_c V[c][r] h[c].  finish (second
-// pass): instead of storing w_r, write v_{j+1}[r] = w_r / beta with beta^2 = w.w - ||h2||^2,
-// the PD's first row records alpha_j = h1[j] + h2[j] and beta_j, and the grid advances j.
+// Row-list form of the finishing update: 64 consecutive rows of the plan per block (rows of two
+// PDs may share a block), 4 column groups each: w''_r = w'_r - sum_c V[c][r] h[c] and
+// v_{j+1}[r] = w''_r / beta with beta^2 = w'.w' - ||h||^2; the PD's first row records
+// alpha_j = alpha + h[j] and beta_j, and the grid advances j.
 __global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restrict__ rows,
                                                           int64_t n_rows,
                                                           const memi_pd_desc* __restrict__ pds,
                                                           double* __restrict__ V, int m,
                                                           int32_t* __restrict__ jp,
                                                           const int32_t* __restrict__ active,
-                                                          double* __restrict__ W,
+                                                          const double* __restrict__ W,
                                                           const double* __restrict__ H,
-                                                          int64_t npd, int finish, int arrive,
-                                                          double* __restrict__ ab) {
+                                                          int arrive, double* __restrict__ ab) {
   __shared__ double part[4][64];
   __shared__ double qpart[4][64];
   const int j = jp[0];
   const int64_t g = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int cg = threadIdx.x >> 6;
   const int64_t hs = m + H_STRIDE_EXTRA;
-  const double* Hp = H + (finish ? npd * hs : 0);
   double acc = 0.0, q = 0.0;
   bool live = false;
   int pd = 0, i = 0;
@@ -295,11 +353,11 @@ __global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restr
     if (active[pd] && j < min(P.n, m)) {
       live = true;
       const double* Vp = V + (int64_t)m * P.row0 + i;
-      const double* h = Hp + (int64_t)pd * hs;
+      const double* h = H + (int64_t)pd * hs;
       for (int c = cg; c <= j; c += 4) {
         const double hc = h[c];
         acc += Vp[(int64_t)c * P.n] * hc;
-        q += hc * hc;                  // ||h2||^2 in the same column groups, for beta
+        q += hc * hc;                  // ||h||^2 in the same column groups, for beta
       }
     }
   }
@@ -308,25 +366,27 @@ __global__ __launch_bounds__(256) void k_orth_update_rows(const int32_t* __restr
   __syncthreads();
   if (cg == 0 && live) {
     const int t = threadIdx.x;
-    const double w = W[P.row0 + i] - ((part[0][t] + part[1][t]) + (part[2][t] + part[3][t]));
-    if (!finish) {
-      W[P.row0 + i] = w;
-    } else {
-      // every row of the PD sums the same column-group partials in the same order: one beta
-      const double* h2 = Hp + (int64_t)pd * hs;
-      const double qq = (qpart[0][t] + qpart[1][t]) + (qpart[2][t] + qpart[3][t]);
-      const double beta = sqrt(fmax(h2[m] - qq, 0.0));
-      if (j + 1 < m) {
-        const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
-        V[(int64_t)m * P.row0 + (int64_t)(j + 1) * P.n + i] = w * inv;
-      }
-      if (i == 0) {
-        ab[((int64_t)pd * m + j) * 2 + 0] = H[(int64_t)pd * hs + j] + h2[j];
-        ab[((int64_t)pd * m + j) * 2 + 1] = beta;
-      }
+    const double* h = H + (int64_t)pd * hs;
+    const double alpha = h[m + 1];
+    const double* Vp = V + (int64_t)m * P.row0;
+    const double* vj = Vp + (int64_t)j * P.n;
+    const double* vjm = j > 0 ? vj - P.n : vj;
+    const double bp = j > 0 ? ab[((int64_t)pd * m + j - 1) * 2 + 1] : 0.0;
+    const double w = w_local(W + P.row0, vj, vjm, alpha, bp, i) -
+                     ((part[0][t] + part[1][t]) + (part[2][t] + part[3][t]));
+    // every row of the PD sums the same column-group partials in the same order: one beta
+    const double qq = (qpart[0][t] + qpart[1][t]) + (qpart[2][t] + qpart[3][t]);
+    const double beta = sqrt(fmax(h[m] - qq, 0.0));
+    if (j + 1 < m) {
+      const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
+      V[(int64_t)m * P.row0 + (int64_t)(j + 1) * P.n + i] = w * inv;
+    }
+    if (i == 0) {
+      ab[((int64_t)pd * m + j) * 2 + 0] = alpha + h[j];
+      ab[((int64_t)pd * m + j) * 2 + 1] = beta;
     }
   }
-  if (finish && arrive) step_arrive(jp, j);
+  if (arrive) step_arrive(jp, j);
 }
 
 // Advances the step index after a finishing update launched without arrivals (large grids,
@@ -783,39 +843,37 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const v
   const int64_t rpw = info->n_rows > 8192 ? 4 : 1;
   const int64_t n_waves = (info->n_rows + rpw - 1) / rpw;
   const dim3 gm((unsigned)((n_waves + 3) / 4));
+  // the rows' w_i v_j[i] for the grid-parallel orthogonalisation: the second half of W
+  double* A = split ? W + info->n_rows : nullptr;
   for (int st = 0; st < nsteps; ++st) {
     if (l_is_f64)
       hipLaunchKernelGGL(k_lanczos_matvec<double>, gm, dim3(256), 0, s, rows, info->n_rows, pds,
-                         (const double*)L, V, m, jp, active, W, (int)rpw);
+                         (const double*)L, V, m, jp, active, W, A, (int)rpw);
     else
       hipLaunchKernelGGL(k_lanczos_matvec<float>, gm, dim3(256), 0, s, rows, info->n_rows, pds,
-                         (const float*)L, V, m, jp, active, W, (int)rpw);
+                         (const float*)L, V, m, jp, active, W, A, (int)rpw);
     MEMI_CHECK_LAUNCH("k_lanczos_matvec");
     if (!split) {
       hipLaunchKernelGGL(k_lanczos_orth, dim3(npd), dim3(256), (size_t)m * sizeof(double), s,
                          pds, V, m, j_dev, active, W, ab);
       MEMI_CHECK_LAUNCH("k_lanczos_orth");
     } else {
-      // column blocks sized for the largest possible j (< m); blocks beyond j exit at once
-      for (int pass = 0; pass < 2; ++pass) {
-        const int nrm = pass == 1;
-        if (npd < 64) {       // few PDs: a 256-thread block per (column, PD), + w.w in pass 2
-          hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)min(m + nrm, 64), npd), dim3(256), 0, s, pds,
-                             V, m, jp, active, W, H + (int64_t)pass * info->n_pd * (m + 1), nrm);
-        } else {              // many PDs: waves walk the live (PD, column) pairs
-          hipLaunchKernelGGL(k_orth_dot_pairs, dim3(1024), dim3(256), 0, s, pds, info->n_pd, V,
-                             m, jp, active, W, H + (int64_t)pass * info->n_pd * (m + 1), nrm);
-        }
-        MEMI_CHECK_LAUNCH("k_orth_dot");
-        const unsigned ub = (unsigned)((info->n_rows + 63) / 64);
-        const int arrive = ub <= 256;   // else one thread advances the step after the update
-        hipLaunchKernelGGL(k_orth_update_rows, dim3(ub), dim3(256), 0, s, rows, info->n_rows, pds,
-                           V, m, j_dev, active, W, H, info->n_pd, nrm, arrive, ab);
-        MEMI_CHECK_LAUNCH("k_orth_update_rows");
-        if (nrm && !arrive) {
-          hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, j_dev);
-          MEMI_CHECK_LAUNCH("k_step_advance");
-        }
+      if (npd < 64) {       // few PDs: a 256-thread block per (column, PD)
+        hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)min(m + 1, 64), npd), dim3(256), 0, s,
+                           pds, V, m, jp, active, W, A, ab, H);
+      } else {              // many PDs: waves walk the live (PD, column) pairs
+        hipLaunchKernelGGL(k_orth_dot_pairs, dim3(1024), dim3(256), 0, s, pds, info->n_pd, V, m,
+                           jp, active, W, A, ab, H);
+      }
+      MEMI_CHECK_LAUNCH("k_orth_dot");
+      const unsigned ub = (unsigned)((info->n_rows + 63) / 64);
+      const int arrive = ub <= 256;   // else one thread advances the step after the update
+      hipLaunchKernelGGL(k_orth_update_rows, dim3(ub), dim3(256), 0, s, rows, info->n_rows, pds,
+                         V, m, j_dev, active, W, H, arrive, ab);
+      MEMI_CHECK_LAUNCH("k_orth_update_rows");
+      if (!arrive) {
+        hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, j_dev);
+        MEMI_CHECK_LAUNCH("k_step_advance");
       }
     }
   }

@@ -609,8 +609,8 @@ def lanczos_eigs(plan, L, ks, m=None, tol=LANCZOS_TOL, check_every=None, seed=20
     if st is None or st["key"] != key:
         st = dict(key=key)
         st["V"] = torch.zeros(m * n_rows, dtype=torch.float64, device=dev)
-        st["W"] = torch.empty(n_rows, dtype=torch.float64, device=dev)
-        st["H"] = torch.empty(2 * npd * (m + 1), dtype=torch.float64, device=dev) if split else None
+        st["W"] = torch.empty(2 * n_rows, dtype=torch.float64, device=dev)
+        st["H"] = torch.empty(npd * (m + 2), dtype=torch.float64, device=dev) if split else None
         st["ab"] = torch.zeros(npd * m * 2, dtype=torch.float64, device=dev)
         st["active"] = torch.zeros(npd, dtype=torch.int32, device=dev)
         st["j"] = torch.zeros(2, dtype=torch.int32, device=dev)
