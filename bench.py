@@ -61,8 +61,11 @@ def parse():
                     help="replay blocks of Lanczos steps from a captured HIP graph (default 1)")
     ap.add_argument("--lanczos-split", type=int, default=None,
                     help="1/0: force the grid-parallel / per-PD orthogonalisation")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="time steps one after another instead of overlapping step k+1's "
+                         "prep + Gram with step k's embedding on a second stream")
     ap.add_argument("--graph", type=int, default=None,
-                    help="1: replay prep + Gram as a captured HIP graph (default: on for c3)")
+                    help="1 (default): replay prep + Gram as a captured HIP graph; 0: eager")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 path")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
@@ -235,11 +238,18 @@ def build_workload(name, dev):
 
 
 def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False,
-                  logEps=None, gram_config=None):
+                  logEps=None, gram_config=None, pipeline=True):
     """Tune the Gram on the prepared operand, warm up, then time exactly `steps` full steps;
     returns timings and the last results.  graph: the prep + Gram part of each step is one
     replay of a captured HIP graph; the Gram kernel is then timed with HIP events in `steps`
-    eager launches right after the timed region (events cannot sit inside the graph)."""
+    eager launches right after the timed region (events cannot sit inside the graph).
+
+    pipeline (with graph): consecutive steps overlap on two streams -- step k+1's prep + Gram
+    (one graph replay into the second buffer set, on a side stream) runs while step k's
+    embedding (launch-bound small kernels that leave most CUs idle) runs on the main stream.
+    Every step still does its whole PD batch; events order the reuse of each buffer set.  The
+    same number of steps is then timed one after another (``t_serial``) and the last pipelined
+    step's eigenvalues are compared with a serial step's."""
     import torch
     from manifoldem_python_amd import batch as B
     from manifoldem_python_amd import autotune
@@ -256,24 +266,59 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False
         tuned = {"best": v, "chunks": c, "tile": t, "ms": {}, "forced": True}
     else:
         tuned = autotune.tune_gram(wsp, batch.N)
+    pipeline = bool(pipeline and graph)
     g = B.GraphedDistances(batch, wsp) if graph else None
     dist_fn = (lambda ev=None: g.run()) if graph else (
         lambda ev=None: B.distances(batch, wsp, events=ev))
-    for _ in range(warmup):
-        dist = dist_fn()
-        B.embed(wsp, dist, num_eigs=num_eigs, rand=rand, logEps=le)
-    torch.cuda.synchronize()
+
+    def seeded(seed):
+        r = np.random.default_rng(seed)
+        return lambda *sh: r.random(sh)
+
+    def serial_steps(n, record=None, rand=rand):
+        res = None
+        for k in range(n):
+            if record is not None:
+                record[k][0].record()
+            dist = dist_fn(pairs[k] if record is not None else None)
+            if record is not None:
+                record[k][1].record()
+            res = B.embed(wsp, dist, num_eigs=num_eigs, rand=rand, logEps=le)
+        return res
+
     pairs = [events.new_pair() for _ in range(steps)]
-    t_dist = 0.0
+    dev_ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
+    if pipeline:
+        wsp2 = B.Workspace(batch, precision)
+        autotune._install(wsp2, wsp.plan.chunks, wsp.plan.gram_tile)
+        B.distances(batch, wsp2)
+        ov = B.OverlappedSteps(batch, wsp, wsp2, graphs=(g, B.GraphedDistances(batch, wsp2)))
+
+        def pipelined_steps(n, record=None, rand=rand):
+            return ov.run(n, num_eigs=num_eigs, logEps=le, rand=rand, record=record)
+
+        run_steps = pipelined_steps
+    else:
+        run_steps = serial_steps
+    run_steps(max(warmup, 2))
+    torch.cuda.synchronize()
+    # the prep + Gram part of each step is timed by events on its stream, so that nothing
+    # waits between it and the embedding (the host queues the next stage meanwhile)
     t0 = time.perf_counter()
-    for k in range(steps):
-        a = time.perf_counter()
-        dist = dist_fn(pairs[k])
-        torch.cuda.synchronize()
-        t_dist += time.perf_counter() - a
-        res = B.embed(wsp, dist, num_eigs=num_eigs, rand=rand, logEps=le)
+    res = run_steps(steps, dev_ev, rand=seeded(1))
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
+    t_dist = sum(a.elapsed_time(b) for a, b in dev_ev) / 1e3
+    out_extra = {}
+    if pipeline:
+        # the same steps one after another, from the same random fit starts
+        t1 = time.perf_counter()
+        res_s = serial_steps(steps, rand=seeded(1))
+        torch.cuda.synchronize()
+        out_extra["t_serial"] = time.perf_counter() - t1
+        out_extra["pipelined_vs_serial_max_dlambda"] = float(max(
+            np.max(np.abs(np.asarray(a.lamb) - np.asarray(b.lamb))) for a, b in zip(res, res_s)))
     if graph:
         for k in range(steps):
             pl.ctf_distances(wsp.plan, wsp.operand, wsp.fy, wsp.ctf, batch.N, ws=wsp.ws,
@@ -285,9 +330,10 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False
     lanczos = dict(mean_steps=float(J.mean()), max_steps=int(J.max()),
                    p90_steps=float(np.percentile(J, 90)),
                    max_residual=float(max(getattr(r, "lanczos_residual", 0.0) for r in res)))
+    dist = g.dist if graph else B.distances(batch, wsp)
     return dict(t=t, t_dist=t_dist, gram_ms=float(np.mean(gram_ms)), wsp=wsp, dist=dist,
                 res=res, flags=int(dist.flag_count.item()), stages=stages, tuned=tuned,
-                graph=graph, lanczos=lanczos)
+                graph=graph, lanczos=lanczos, pipeline=pipeline, **out_extra)
 
 
 def issued_gram_flops(plan, Khp, products):
@@ -709,10 +755,12 @@ def main():
                      with_check=not args.no_check, with_cpu=False)
         keep = ("value", "unit", "pds_per_s", "ms_per_step", "steps", "config", "gram_kernel_ms",
                 "stage_ms", "gram_tuned", "kappa_flagged", "roofline", "d_rel_err_vs_fp64",
-                "speedup_vs_fp32", "lanczos")
+                "speedup_vs_fp32", "lanczos", "ms_per_step_serial",
+                "pipelined_vs_serial_max_dlambda")
         out["c3"] = {k: c3[k] for k in keep if k in c3}
         if "fp32" in c3:
             out["c3"]["fp32"] = {k: c3["fp32"][k] for k in ("value", "pds_per_s", "ms_per_step",
+                                                            "ms_per_step_serial",
                                                             "gram_kernel_ms", "roofline")}
     ev.close()
     print(json.dumps(out), flush=True)
@@ -728,9 +776,12 @@ def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=T
     torch.cuda.synchronize()
     logEps = np.linspace(-150, 150, 101) if workload == "c1" else None
     log("timing split-BF16", workload)
-    graph = bool(args.graph) if args.graph is not None else workload == "c3"
+    graph = bool(args.graph) if args.graph is not None else True
+    # overlapping steps pays where the embedding is launch-bound (one PD per step); on the
+    # 500-PD batch both stages are heavy and overlap gained nothing (93.5 vs 93.6 ms)
+    pipeline = not args.no_pipeline and workload in ("c1", "c2")
     bf = run_precision(batch, nat.PREC_SPLIT_BF16, steps, warmup, num_eigs, ev, graph=graph,
-                       logEps=logEps, gram_config=args.gram_config)
+                       logEps=logEps, gram_config=args.gram_config, pipeline=pipeline)
     ns = batch.ns
     pairs = float(np.sum(ns * (ns - 1) / 2))
     Kh = pl.half_len(batch.N)
@@ -749,6 +800,7 @@ def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=T
         "dtype": "bf16x3 (split) Gram, fp32 prep, fp64 kernel + eigensolve",
         "data": "synthetic",
         "config": dict(meta, precision="split-bf16", hip_graph=graph,
+                       pipelined_steps=bf["pipeline"],
                        lanczos=dict(pl.LANCZOS), kernel_dtype=str(pl.KERNEL_DTYPE)),
         "pds_per_s": batch.ns.size * steps / bf["t"],
         "distance_ms_per_step": 1e3 * bf["t_dist"] / steps,
@@ -757,6 +809,13 @@ def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=T
         "gram_tuned": bf["tuned"],
         "lanczos": bf["lanczos"],
         "kappa_flagged": bf["flags"],
+        "ms_per_step_serial": (1e3 * bf["t_serial"] / steps) if "t_serial" in bf else None,
+        "pipelined_vs_serial_max_dlambda": bf.get("pipelined_vs_serial_max_dlambda"),
+        "pipelined_note": ("steps overlap on two streams: step k+1's prep + Gram (second buffer "
+                           "set) runs during step k's embedding; every step does its PD's whole "
+                           "pipeline.  pipelined_vs_serial_max_dlambda compares every step's "
+                           "eigenvalues with the same steps run one after another from the same "
+                           "random fit starts") if bf["pipeline"] else None,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
                      "traffic_note": traffic_note,
@@ -777,10 +836,12 @@ def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=T
     if with_fp32:
         log("timing FP32", workload)
         f32 = run_precision(batch, nat.PREC_FP32, steps, warmup, num_eigs, ev, graph=graph,
-                            logEps=logEps)
+                            logEps=logEps, pipeline=pipeline)
         a32 = flops / (f32["gram_ms"] / 1e3) / 1e12
         out["fp32"] = {"value": pairs * steps / f32["t"],
                        "ms_per_step": 1e3 * f32["t"] / steps,
+                       "ms_per_step_serial": ((1e3 * f32["t_serial"] / steps)
+                                              if "t_serial" in f32 else None),
                        "pds_per_s": batch.ns.size * steps / f32["t"],
                        "gram_kernel_ms": f32["gram_ms"],
                        "stage_ms": f32["stages"],

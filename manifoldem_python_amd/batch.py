@@ -131,6 +131,57 @@ class GraphedDistances:
         return self.dist
 
 
+class OverlappedSteps:
+    """Steps of the same batch (a stream of PD batches) with step k+1's a1-a8 overlapping step
+    k's a10-a16: two buffer sets, each with its captured distance graph; the graph replays run
+    on a side stream while the embeddings (small, launch-bound kernels that leave most CUs
+    idle) run on the caller's stream.  Events order each set's reuse: step k's replay waits
+    until step k-2's embedding is done with the set, and step k's embedding waits for its
+    replay.  Every step does its whole batch."""
+
+    def __init__(self, batch, wsp_a, wsp_b, graphs=None):
+        self.batch = batch
+        ga, gb = graphs if graphs is not None else (GraphedDistances(batch, wsp_a),
+                                                     GraphedDistances(batch, wsp_b))
+        self.sets = [(wsp_a, ga), (wsp_b, gb)]
+        self.main = torch.cuda.current_stream()
+        self.side = torch.cuda.Stream()
+        self.done_dist = [torch.cuda.Event(), torch.cuda.Event()]
+        self.done_embed = [torch.cuda.Event(), torch.cuda.Event()]
+        for e in self.done_embed:
+            e.record(self.main)
+
+    def run(self, n, tune=3, num_eigs=15, logEps=pl.LOG_EPS, rand=None, record=None,
+            on_result=None):
+        """n steps; returns the last step's embedding results.  record: optional list of n
+        (start, end) torch events put around each step's replay on the side stream;
+        on_result(k, results) sees every step's results."""
+        dists = [None] * n
+
+        def launch(k):
+            wsp, g = self.sets[k % 2]
+            with torch.cuda.stream(self.side):
+                self.side.wait_event(self.done_embed[k % 2])
+                if record is not None:
+                    record[k][0].record(self.side)
+                dists[k] = g.run()
+                if record is not None:
+                    record[k][1].record(self.side)
+                self.done_dist[k % 2].record(self.side)
+
+        launch(0)
+        res = None
+        for k in range(n):
+            if k + 1 < n:
+                launch(k + 1)
+            self.main.wait_event(self.done_dist[k % 2])
+            res = embed(self.sets[k % 2][0], dists[k], tune, num_eigs, logEps, rand=rand)
+            self.done_embed[k % 2].record(self.main)
+            if on_result is not None:
+                on_result(k, res)
+        return res
+
+
 def embed(wsp, dist, tune=3, num_eigs=15, logEps=pl.LOG_EPS, rand=None, timer=None):
     """a10-a16 for every PD of the batch."""
     return pl.dm_embed(wsp.plan, dist.D, tune, num_eigs, logEps, rand=rand, timer=timer)

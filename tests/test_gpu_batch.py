@@ -86,3 +86,32 @@ def test_graph_replays_match_eager(cuda, precision):
         torch.cuda.synchronize()
         np.testing.assert_array_equal(dist.D.cpu().numpy(), D0.cpu().numpy())
         assert int(dist.flag_count.item()) == c0, r
+
+
+def test_overlapped_steps_match_serial(cuda):
+    """Steps overlapped on two streams (step k+1's distances during step k's embedding, two
+    buffer sets) give every step the eigenvalues and bandwidths of the same steps run one after
+    another from the same random fit starts."""
+    import torch
+    from manifoldem_python_amd import batch as B, _native as nat
+    batch = _batch(_pds(), cuda)
+
+    def seeded():
+        r = np.random.default_rng(4)
+        return lambda *sh: r.random(sh)
+
+    wa = B.Workspace(batch, nat.PREC_SPLIT_BF16)
+    wb = B.Workspace(batch, nat.PREC_SPLIT_BF16)
+    B.distances(batch, wa)
+    B.distances(batch, wb)
+    ov = B.OverlappedSteps(batch, wa, wb)
+    got = []
+    ov.run(5, num_eigs=8, rand=seeded(),
+           on_result=lambda k, res: got.append([(r.lamb.copy(), r.sigma) for r in res]))
+    torch.cuda.synchronize()
+    rs = seeded()
+    for k in range(5):
+        res = B.embed(wa, B.distances(batch, wa), num_eigs=8, rand=rs)
+        for (lam, sig), r in zip(got[k], res):
+            np.testing.assert_allclose(lam, r.lamb, rtol=0, atol=1e-10)
+            assert abs(sig - r.sigma) <= 1e-9 * abs(r.sigma)
