@@ -99,6 +99,9 @@ _SIGS = {
     "memi_kernel_rowsums": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp],
     "memi_kernel_grad_weights": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
     "memi_lanczos_init": [c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp],
+    "memi_kernel_ell": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp],
+    "memi_lanczos_steps_sparse": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_int,
+                                  c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_int, c_vp],
     "memi_tanh_select": [c_vp, c_int, c_int, c_int, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp],
     "memi_flow_anisodiff": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_dbl, c_dbl,
                             c_dbl, c_dbl, c_dbl, c_int, c_vp],
@@ -650,13 +653,25 @@ def markov_coo(row, col, val, nS, sigma, alpha, out, deg):
          float(alpha), _p(out), _p(deg), _stream())
 
 
+def kernel_ell(plan, L, col, val, cnt):
+    """memi_kernel_ell: the sparse rows of the float64 kernel L."""
+    I = plan.info
+    _need(L.dtype == torch.float64 and L.numel() >= I.d_elems, "kernel_ell: L float64")
+    _need(col.dtype == torch.int32 and col.numel() >= I.d_elems and val.dtype == torch.float64
+          and val.numel() >= I.d_elems and cnt.dtype == torch.int32 and cnt.numel() >= I.n_rows,
+          "kernel_ell: col int32 / val float64 [d_elems], cnt int32 [n_rows]")
+    call("memi_kernel_ell", _p(plan.dev), ctypes.byref(plan.info), _p(L), _p(col), _p(val),
+         _p(cnt), _stream())
+
+
 def lanczos_steps(plan, L, V, m, j_dev, nsteps, active, W, ab, H=None):
     """nsteps Lanczos steps from the step index held in j_dev (int32 [2] on the device: index,
     arrival counter; both zero before the first step).  H: float64 [npd*(m+2)] scratch selects
     the grid-parallel orthogonalisation (W then holds 2*n_rows); None selects one workgroup per
     PD."""
     I = plan.info
-    _need(L.dtype in (torch.float32, torch.float64) and L.numel() >= I.d_elems,
+    _need(isinstance(L, tuple) or (L.dtype in (torch.float32, torch.float64) and
+                                   L.numel() >= I.d_elems),
           "lanczos_steps: L must be float32 or float64 with d_elems elements")
     _need(V.numel() >= m * I.n_rows and ab.numel() >= I.n_pd * m * 2 and
           W.numel() >= (1 if H is None else 2) * I.n_rows and
@@ -664,6 +679,12 @@ def lanczos_steps(plan, L, V, m, j_dev, nsteps, active, W, ab, H=None):
     _need(j_dev.dtype == torch.int32 and j_dev.numel() >= 2, "lanczos_steps: j_dev int32 [2]")
     _need(H is None or (H.dtype == torch.float64 and H.numel() >= I.n_pd * (m + 2)),
           "lanczos_steps: H must be float64 with npd*(m+2) elements")
+    if isinstance(L, tuple):          # (col, val, cnt) of kernel_ell
+        col, val, cnt = L
+        call("memi_lanczos_steps_sparse", _p(plan.dev), ctypes.byref(plan.info), _p(col),
+             _p(val), _p(cnt), _p(V), int(m), _p(j_dev), int(nsteps), _p(active), _p(W),
+             None if H is None else _p(H), _p(ab), int(H is not None), _stream())
+        return
     call("memi_lanczos_steps", _p(plan.dev), ctypes.byref(plan.info), _p(L),
          int(L.dtype == torch.float64), _p(V), int(m),
          _p(j_dev), int(nsteps), _p(active), _p(W), None if H is None else _p(H), _p(ab),

@@ -136,6 +136,83 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restric
   }
 }
 
+// ---- sparse form of the kernel (sembeddingonFly.py passes eigsh a scipy sparse matrix) ----
+// Row i of PD p keeps its nonzero entries at the dense row's offset d_off + i*n: columns in
+// increasing order (col, int32) and values (val), cnt[row0 + i] of them.  A kNN kernel with
+// k << n has about 2k nonzeros per row, so the Lanczos matvec reads 12 B per nonzero instead
+// of 8 B per entry of the dense row.
+
+// one wave per row: ballot compaction of the row's nonzeros, 64 columns at a time
+__global__ __launch_bounds__(256) void k_kernel_ell(const int32_t* __restrict__ rows,
+                                                    int64_t n_rows,
+                                                    const memi_pd_desc* __restrict__ pds,
+                                                    const double* __restrict__ L,
+                                                    int32_t* __restrict__ col,
+                                                    double* __restrict__ val,
+                                                    int32_t* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= n_rows) return;                          // uniform over the wave
+  const int pd = rows[2 * g], i = rows[2 * g + 1];
+  const memi_pd_desc P = pds[pd];
+  const int n = P.n;
+  const int64_t off = P.d_off + (int64_t)i * n;
+  const double* Lr = L + off;
+  int base = 0;
+  for (int c0 = 0; c0 < n; c0 += 64) {
+    const int c = c0 + lane;
+    const double x = c < n ? Lr[c] : 0.0;
+    const uint64_t nz = __ballot(x != 0.0);
+    if (x != 0.0) {
+      const int pos = base + __popcll(nz & ((1ull << lane) - 1ull));
+      col[off + pos] = c;
+      val[off + pos] = x;
+    }
+    base += __popcll(nz);
+  }
+  if (lane == 0) cnt[P.row0 + i] = base;
+}
+
+// w = L v_j from the sparse rows, one wave per row (the row's nonzeros strided over the lanes,
+// four products in flight), and the row's w_i v_j[i] for the orthogonalisation
+__global__ __launch_bounds__(256) void k_lanczos_matvec_ell(const int32_t* __restrict__ rows,
+                                                            int64_t n_rows,
+                                                            const memi_pd_desc* __restrict__ pds,
+                                                            const int32_t* __restrict__ col,
+                                                            const double* __restrict__ val,
+                                                            const int32_t* __restrict__ cnt,
+                                                            const double* __restrict__ V, int m,
+                                                            const int32_t* __restrict__ jp,
+                                                            const int32_t* __restrict__ active,
+                                                            double* __restrict__ W,
+                                                            double* __restrict__ A) {
+  const int j = *jp;
+  const int lane = threadIdx.x & 63;
+  const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= n_rows) return;                          // uniform over the wave
+  const int pd = rows[2 * g], i = rows[2 * g + 1];
+  const memi_pd_desc P = pds[pd];
+  const int n = P.n;
+  if (!active[pd] || j >= min(n, m)) return;
+  const int64_t off = P.d_off + (int64_t)i * n;
+  const int c = cnt[P.row0 + i];
+  const double* v = V + (int64_t)m * P.row0 + (int64_t)j * n;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int t = lane;
+  for (; t + 3 * 64 < c; t += 4 * 64) {
+    s0 += val[off + t] * v[col[off + t]];
+    s1 += val[off + t + 64] * v[col[off + t + 64]];
+    s2 += val[off + t + 128] * v[col[off + t + 128]];
+    s3 += val[off + t + 192] * v[col[off + t + 192]];
+  }
+  for (; t < c; t += 64) s0 += val[off + t] * v[col[off + t]];
+  const double w = wave_total((s0 + s1) + (s2 + s3));
+  if (lane == 0) {
+    W[P.row0 + i] = w;
+    if (A) A[P.row0 + i] = w * v[i];
+  }
+}
+
 // one block per PD: the three-term recurrence, one CGS pass against columns 0..j, then
 // normalise into column j+1
 __device__ inline void lanczos_orth_body(const memi_pd_desc* __restrict__ pds,
@@ -820,11 +897,13 @@ int memi_lanczos_init(double* V, const double* v0, const int64_t* idx, int64_t n
   return MEMI_OK;
 }
 
-int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const void* L,
-                       int l_is_f64, double* V, int m, int32_t* j_dev, int nsteps,
-                       const int32_t* active, double* W, double* H, double* ab, int split,
-                       void* stream) {
-  MEMI_CHECK_ARG(plan_dev && info && L && V && j_dev && active && W && ab && (!split || H),
+static int lanczos_steps_impl(const void* plan_dev, const memi_plan_info* info, const void* L,
+                              int l_is_f64, const int32_t* ell_col, const double* ell_val,
+                              const int32_t* ell_cnt, double* V, int m, int32_t* j_dev,
+                              int nsteps, const int32_t* active, double* W, double* H,
+                              double* ab, int split, void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && (L || ell_col) && V && j_dev && active && W && ab &&
+                     (!split || H),
                  "memi_lanczos_steps: null pointer");
   MEMI_CHECK_ARG(m >= 2 && m <= 4096 && nsteps >= 0, "memi_lanczos_steps: bad m=%d nsteps=%d",
                  m, nsteps);
@@ -845,8 +924,12 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const v
   const dim3 gm((unsigned)((n_waves + 3) / 4));
   // the rows' w_i v_j[i] for the grid-parallel orthogonalisation: the second half of W
   double* A = split ? W + info->n_rows : nullptr;
+  const dim3 ge((unsigned)((info->n_rows + 3) / 4));
   for (int st = 0; st < nsteps; ++st) {
-    if (l_is_f64)
+    if (ell_col)
+      hipLaunchKernelGGL(k_lanczos_matvec_ell, ge, dim3(256), 0, s, rows, info->n_rows, pds,
+                         ell_col, ell_val, ell_cnt, V, m, jp, active, W, A);
+    else if (l_is_f64)
       hipLaunchKernelGGL(k_lanczos_matvec<double>, gm, dim3(256), 0, s, rows, info->n_rows, pds,
                          (const double*)L, V, m, jp, active, W, A, (int)rpw);
     else
@@ -877,6 +960,39 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const v
       }
     }
   }
+  return MEMI_OK;
+}
+
+int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const void* L,
+                       int l_is_f64, double* V, int m, int32_t* j_dev, int nsteps,
+                       const int32_t* active, double* W, double* H, double* ab, int split,
+                       void* stream) {
+  MEMI_CHECK_ARG(L, "memi_lanczos_steps: null pointer");
+  return lanczos_steps_impl(plan_dev, info, L, l_is_f64, nullptr, nullptr, nullptr, V, m, j_dev,
+                            nsteps, active, W, H, ab, split, stream);
+}
+
+int memi_lanczos_steps_sparse(const void* plan_dev, const memi_plan_info* info,
+                              const int32_t* ell_col, const double* ell_val,
+                              const int32_t* ell_cnt, double* V, int m, int32_t* j_dev,
+                              int nsteps, const int32_t* active, double* W, double* H,
+                              double* ab, int split, void* stream) {
+  MEMI_CHECK_ARG(ell_col && ell_val && ell_cnt, "memi_lanczos_steps_sparse: null pointer");
+  return lanczos_steps_impl(plan_dev, info, nullptr, 1, ell_col, ell_val, ell_cnt, V, m, j_dev,
+                            nsteps, active, W, H, ab, split, stream);
+}
+
+int memi_kernel_ell(const void* plan_dev, const memi_plan_info* info, const double* L,
+                    int32_t* ell_col, double* ell_val, int32_t* ell_cnt, void* stream) {
+  MEMI_CHECK_ARG(plan_dev && info && L && ell_col && ell_val && ell_cnt,
+                 "memi_kernel_ell: null pointer");
+  if (info->n_rows == 0) return MEMI_OK;
+  const char* plan = (const char*)plan_dev;
+  const memi_pd_desc* pds = (const memi_pd_desc*)(plan + info->off_pds);
+  const int32_t* rows = (const int32_t*)(plan + info->off_rows);
+  hipLaunchKernelGGL(k_kernel_ell, dim3((unsigned)((info->n_rows + 3) / 4)), dim3(256), 0,
+                     (hipStream_t)stream, rows, info->n_rows, pds, L, ell_col, ell_val, ell_cnt);
+  MEMI_CHECK_LAUNCH("k_kernel_ell");
   return MEMI_OK;
 }
 

@@ -583,7 +583,8 @@ def lanczos_eigs(plan, L, ks, m=None, tol=LANCZOS_TOL, check_every=None, seed=20
     check_every steps are captured once in a HIP graph and replayed (the step index lives on
     the device).  Returns a list of (lam, V (n x k) numpy, steps, max residual); with full,
     also the (n_rows x kmax) array the V are row blocks of."""
-    dev = L.device
+    sparse = isinstance(L, tuple)               # (col, val, cnt) of nat.kernel_ell
+    dev = (L[1] if sparse else L).device
     npd = plan.npd
     ns = plan.ns.astype(int)
     rows0 = plan.rows0
@@ -604,7 +605,8 @@ def lanczos_eigs(plan, L, ks, m=None, tol=LANCZOS_TOL, check_every=None, seed=20
     K = nat.RITZ_KMAX
     # buffers (and the captured block of steps) persist on the plan between calls with the
     # same L buffer and settings, so repeated embeddings only replay the graph
-    key = (L.data_ptr(), L.numel(), str(L.dtype), m, bool(split), check_every, bool(graph), check)
+    key = (tuple(t.data_ptr() for t in L) if sparse else (L.data_ptr(), L.numel(), str(L.dtype)),
+           m, bool(split), check_every, bool(graph), check)
     st = getattr(plan, "_lanczos", None)
     if st is None or st["key"] != key:
         st = dict(key=key)
@@ -921,6 +923,11 @@ class StageTimer:
         self._last = now
 
 
+# The Lanczos matvec reads the kernel's nonzeros only (nat.kernel_ell) when every PD of the
+# plan keeps k <= n / SPARSE_RATIO neighbours: a kNN kernel then has about 2k nonzeros per row,
+# and 12 B per nonzero beat 8 B per entry of the dense row.
+SPARSE_RATIO = 4
+
 # Storage type of the normalised kernel L that the Lanczos matvecs read.  float64 is the
 # reference's kernel (slaplacianonFly.py builds it in float64); float32 rounds each entry once
 # (~6e-8 relative), which moves the eigenvectors by about that over the eigengap, for half the
@@ -974,14 +981,26 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
     deg = torch.empty(2 * info.n_rows, dtype=torch.float64, device=dev)
     ks = [min(num_eigs, int(n) - 3) + 1 for n in plan.ns]
 
+    sparse = (eig == "lanczos" and KERNEL_DTYPE == torch.float64 and
+              all(int(k) * SPARSE_RATIO <= int(n) for k, n in zip(plan.ks, plan.ns)))
+    if sparse:
+        ell = getattr(plan, "_ell", None)
+        if ell is None or ell[0].numel() < max(info.d_elems, 1) or ell[0].device != dev:
+            ell = (torch.empty(max(info.d_elems, 1), dtype=torch.int32, device=dev),
+                   torch.empty(max(info.d_elems, 1), dtype=torch.float64, device=dev),
+                   torch.empty(max(info.n_rows, 1), dtype=torch.int32, device=dev))
+            plan._ell = ell
+
     def spectral(sigma):
         """a14-a15 from the bandwidths: (eigs, Yall) of the Lanczos path, else None."""
         nat.markov_normalize(plan, D, mask, sigma, L, deg)
+        if sparse:
+            nat.kernel_ell(plan, L, *ell)
         if timer:
             timer.mark("normalize")
         if eig != "lanczos":
             return None
-        out = lanczos_eigs(plan, L, ks, full=True, **LANCZOS)
+        out = lanczos_eigs(plan, ell if sparse else L, ks, full=True, **LANCZOS)
         if timer:
             timer.mark("lanczos")
         return out

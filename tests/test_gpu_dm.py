@@ -150,6 +150,53 @@ def test_batched_lanczos_matches_dense(cuda, split, check, ns, kk):
             assert abs(abs(np.dot(Y[:, c], U[:, sel[c]])) - 1) < 1e-6, (p, c)
 
 
+@pytest.mark.parametrize("split", [True, False])
+def test_sparse_kernel_lanczos_matches_dense(cuda, split):
+    """The sparse rows of a kNN kernel (memi_kernel_ell) rebuild the dense float64 kernel
+    exactly, and Lanczos over them (memi_lanczos_steps_sparse) gives the dense path's
+    eigenpairs, for PDs with k << n (one of 2000 rows, as the single-large-PD workload)."""
+    import torch
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    ns, ks, kk = [2000, 600, 333], [200, 60, 40], [11, 9, 7]
+    D = [_distance_matrix(n, 60 + i) for i, n in enumerate(ns)]
+    rows0 = np.concatenate(([0], np.cumsum(ns)[:-1]))
+    plan = nat.Plan(rows0, ns, ks, chunks=1)
+    flat = torch.as_tensor(np.concatenate([d.astype(np.float32).ravel() for d in D]),
+                           device=cuda)
+    mask = pl.knn_mask(plan, flat)
+    sigma = torch.as_tensor([np.sqrt(np.median(d)) for d in D], device=cuda)
+    L = torch.empty(plan.info.d_elems, dtype=torch.float64, device=cuda)
+    deg = torch.empty(2 * plan.info.n_rows, dtype=torch.float64, device=cuda)
+    nat.markov_normalize(plan, flat, mask, sigma, L, deg)
+    col = torch.empty(plan.info.d_elems, dtype=torch.int32, device=cuda)
+    val = torch.empty(plan.info.d_elems, dtype=torch.float64, device=cuda)
+    cnt = torch.empty(plan.info.n_rows, dtype=torch.int32, device=cuda)
+    nat.kernel_ell(plan, L, col, val, cnt)
+    cnt_h = cnt.cpu().numpy()
+    for p in range(3):
+        n = ns[p]
+        A = plan.block(L, p).cpu().numpy()
+        c = plan.block(col, p).cpu().numpy()
+        v = plan.block(val, p).cpu().numpy()
+        R = np.zeros((n, n))
+        for i in range(n):
+            k = cnt_h[rows0[p] + i]
+            assert k == np.count_nonzero(A[i])
+            assert np.all(np.diff(c[i, :k]) > 0)
+            R[i, c[i, :k]] = v[i, :k]
+        np.testing.assert_array_equal(R, A)
+        assert cnt_h[rows0[p]:rows0[p] + n].max() < n // 2      # the sparse case
+    dense = pl.lanczos_eigs(plan, L, kk, split=split)
+    sparse = pl.lanczos_eigs(plan, (col, val, cnt), kk, split=split)
+    for p in range(3):
+        ld, Yd, Jd, _ = dense[p]
+        ls, Ys, Js, _ = sparse[p]
+        np.testing.assert_allclose(ls, ld, rtol=0, atol=1e-12)
+        assert abs(Js - Jd) <= 8
+        for q in range(kk[p]):
+            assert abs(abs(Ys[:, q] @ Yd[:, q]) - 1) < 1e-8, (p, q)
+
+
 def test_gpu_tanh_fit_matches_curve_fit(cuda):
     """Batched LM fit of the Ferguson curve against scipy curve_fit (the reference's fitter)
     on sweeps of several realistic kernels, from the same start points."""
