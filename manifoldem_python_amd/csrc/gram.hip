@@ -186,28 +186,50 @@ __device__ __forceinline__ void gram_kloop(floatx4_t (&acc)[S::NBI][S::NBJ], cha
     gram_gload<NA, RPT>(R, gAh, gBh, gAl, gBl, oA, oB, sl_ * seg_bytes + kk_,             \
                         seg_pair(sl_) * seg_bytes + kk_);                                 \
   } while (0)
+  // The steady-state loops prefetch on every step with no condition around the loads: a load
+  // that only some paths issue makes the compiler's wait-count merge at the join assume the
+  // fewer outstanding loads, so the LDS store of the older stage would wait for the prefetch
+  // just issued and the prefetch distance would collapse to one step's MFMAs.
   GramStage<NA, RPT> X;
   if constexpr (DEPTH == 1) {
     MEMI_GLOAD(X, 0);
     gram_lstore<NA, RPT, TILE, RS>(X, lds + lds_st);
     __syncthreads();
-    for (int ks = 0; ks < nks; ++ks) {
+    int ks = 0;
+    for (; ks + 1 < nks; ++ks) {
       const int cur = ks & 1;
-      if (ks + 1 < nks) MEMI_GLOAD(X, ks + 1);
+      MEMI_GLOAD(X, ks + 1);
+      __builtin_amdgcn_sched_barrier(0);     // issue the prefetch before this step's MFMAs
       gram_compute<PREC, S, TILE, G>(acc, lds + cur * NA * TILE, frag_off, wr, wc, nbi, nbj,
                                      diag);
-      if (ks + 1 < nks) gram_lstore<NA, RPT, TILE, RS>(X, lds + (cur ^ 1) * NA * TILE + lds_st);
+      gram_lstore<NA, RPT, TILE, RS>(X, lds + (cur ^ 1) * NA * TILE + lds_st);
       __syncthreads();
     }
+    gram_compute<PREC, S, TILE, G>(acc, lds + (ks & 1) * NA * TILE, frag_off, wr, wc, nbi, nbj,
+                                   diag);
   } else {
     // prologue: step 0 into LDS buffer 0, step 1 in flight in Y; steady state unrolled by two
-    // so the register stages alternate by name
+    // so the register stages alternate by name; at every loop top LDS buffer 0 holds step ks
+    // and Y holds step ks + 1
     GramStage<NA, RPT> Y;
     MEMI_GLOAD(X, 0);
     if (nks > 1) MEMI_GLOAD(Y, 1);
     gram_lstore<NA, RPT, TILE, RS>(X, lds + lds_st);
     __syncthreads();
     int ks = 0;
+    for (; ks + 3 < nks; ks += 2) {
+      MEMI_GLOAD(X, ks + 2);
+      __builtin_amdgcn_sched_barrier(0);     // issue the prefetch before this step's MFMAs
+      gram_compute<PREC, S, TILE, G>(acc, lds, frag_off, wr, wc, nbi, nbj, diag);
+      gram_lstore<NA, RPT, TILE, RS>(Y, lds + NA * TILE + lds_st);
+      __syncthreads();
+      MEMI_GLOAD(Y, ks + 3);
+      __builtin_amdgcn_sched_barrier(0);
+      gram_compute<PREC, S, TILE, G>(acc, lds + NA * TILE, frag_off, wr, wc, nbi, nbj, diag);
+      gram_lstore<NA, RPT, TILE, RS>(X, lds + lds_st);
+      __syncthreads();
+    }
+    // the last one to three steps
     while (ks < nks) {
       if (ks + 2 < nks) MEMI_GLOAD(X, ks + 2);
       gram_compute<PREC, S, TILE, G>(acc, lds, frag_off, wr, wc, nbi, nbj, diag);
