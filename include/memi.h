@@ -357,18 +357,23 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const v
 /* The sparse form of L that sembeddingonFly.py:27 hands eigsh (a scipy sparse matrix): row i of
  * PD p keeps its nonzero entries at the dense row's offset d_off + i*n, columns in increasing
  * order in ell_col (int32 [d_elems]) and values in ell_val (float64 [d_elems]), ell_cnt[row0 + i]
- * of them (int32 [n_rows]).  L float64 as memi_markov_normalize writes it. */
+ * of them (int32 [n_rows]).  L float64 as memi_markov_normalize writes it.  sparse_pd (int32
+ * [npd], may be null = every PD): only the rows of PDs with sparse_pd[p] != 0 are written. */
 int memi_kernel_ell(const void* plan_dev, const memi_plan_info* info, const double* L,
-                    int32_t* ell_col, double* ell_val, int32_t* ell_cnt, void* stream);
+                    const int32_t* sparse_pd, int32_t* ell_col, double* ell_val,
+                    int32_t* ell_cnt, void* stream);
 
 /* memi_lanczos_steps with the matvec over memi_kernel_ell's sparse rows (the same steps and
  * outputs; the products of each row are summed over its nonzeros only).  For kNN kernels with
- * k much smaller than n. */
+ * k much smaller than n.  sparse_pd null: every PD is sparse (L may be null).  Otherwise the
+ * PDs with sparse_pd[p] != 0 (memi_kernel_ell called with the same selection) use the sparse
+ * rows and the others the dense float64 L (required). */
 int memi_lanczos_steps_sparse(const void* plan_dev, const memi_plan_info* info,
                               const int32_t* ell_col, const double* ell_val,
-                              const int32_t* ell_cnt, double* V, int m, int32_t* j_dev,
-                              int nsteps, const int32_t* active, double* W, double* H,
-                              double* ab, int split, void* stream);
+                              const int32_t* ell_cnt, const int32_t* sparse_pd, const double* L,
+                              double* V, int m, int32_t* j_dev, int nsteps,
+                              const int32_t* active, double* W, double* H, double* ab,
+                              int split, void* stream);
 
 /* Convergence check of the batched Lanczos run, the Ritz pairs eigsh(which='LM') returns
  * (sembeddingonFly.py:27-35), entirely on the GPU.  For every active PD with

@@ -99,9 +99,9 @@ _SIGS = {
     "memi_kernel_rowsums": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp],
     "memi_kernel_grad_weights": [c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
     "memi_lanczos_init": [c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp],
-    "memi_kernel_ell": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp],
-    "memi_lanczos_steps_sparse": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_int,
-                                  c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_int, c_vp],
+    "memi_kernel_ell": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "memi_lanczos_steps_sparse": [c_vp, ctypes.POINTER(PlanInfo), c_vp, c_vp, c_vp, c_vp, c_vp,
+                                  c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_int, c_vp],
     "memi_tanh_select": [c_vp, c_int, c_int, c_int, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp],
     "memi_flow_anisodiff": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_dbl, c_dbl,
                             c_dbl, c_dbl, c_dbl, c_int, c_vp],
@@ -653,15 +653,18 @@ def markov_coo(row, col, val, nS, sigma, alpha, out, deg):
          float(alpha), _p(out), _p(deg), _stream())
 
 
-def kernel_ell(plan, L, col, val, cnt):
-    """memi_kernel_ell: the sparse rows of the float64 kernel L."""
+def kernel_ell(plan, L, col, val, cnt, sel=None):
+    """memi_kernel_ell: the sparse rows of the float64 kernel L (of the PDs with sel != 0
+    when sel, int32 [npd], is given)."""
     I = plan.info
     _need(L.dtype == torch.float64 and L.numel() >= I.d_elems, "kernel_ell: L float64")
     _need(col.dtype == torch.int32 and col.numel() >= I.d_elems and val.dtype == torch.float64
           and val.numel() >= I.d_elems and cnt.dtype == torch.int32 and cnt.numel() >= I.n_rows,
           "kernel_ell: col int32 / val float64 [d_elems], cnt int32 [n_rows]")
-    call("memi_kernel_ell", _p(plan.dev), ctypes.byref(plan.info), _p(L), _p(col), _p(val),
-         _p(cnt), _stream())
+    _need(sel is None or (sel.dtype == torch.int32 and sel.numel() >= I.n_pd),
+          "kernel_ell: sel int32 [npd]")
+    call("memi_kernel_ell", _p(plan.dev), ctypes.byref(plan.info), _p(L),
+         None if sel is None else _p(sel), _p(col), _p(val), _p(cnt), _stream())
 
 
 def lanczos_steps(plan, L, V, m, j_dev, nsteps, active, W, ab, H=None):
@@ -679,11 +682,19 @@ def lanczos_steps(plan, L, V, m, j_dev, nsteps, active, W, ab, H=None):
     _need(j_dev.dtype == torch.int32 and j_dev.numel() >= 2, "lanczos_steps: j_dev int32 [2]")
     _need(H is None or (H.dtype == torch.float64 and H.numel() >= I.n_pd * (m + 2)),
           "lanczos_steps: H must be float64 with npd*(m+2) elements")
-    if isinstance(L, tuple):          # (col, val, cnt) of kernel_ell
-        col, val, cnt = L
+    if isinstance(L, tuple):
+        # (col, val, cnt) of kernel_ell for every PD, or (col, val, cnt, sel, L dense): the
+        # PDs with sel != 0 sparse, the others dense
+        col, val, cnt = L[:3]
+        sel, Ld = (L[3], L[4]) if len(L) == 5 else (None, None)
+        _need(sel is None or (sel.dtype == torch.int32 and sel.numel() >= I.n_pd and
+                              Ld.dtype == torch.float64 and Ld.numel() >= I.d_elems),
+              "lanczos_steps: sel int32 [npd] with a float64 dense L")
         call("memi_lanczos_steps_sparse", _p(plan.dev), ctypes.byref(plan.info), _p(col),
-             _p(val), _p(cnt), _p(V), int(m), _p(j_dev), int(nsteps), _p(active), _p(W),
-             None if H is None else _p(H), _p(ab), int(H is not None), _stream())
+             _p(val), _p(cnt), None if sel is None else _p(sel),
+             None if Ld is None else _p(Ld), _p(V), int(m), _p(j_dev), int(nsteps),
+             _p(active), _p(W), None if H is None else _p(H), _p(ab), int(H is not None),
+             _stream())
         return
     call("memi_lanczos_steps", _p(plan.dev), ctypes.byref(plan.info), _p(L),
          int(L.dtype == torch.float64), _p(V), int(m),

@@ -24,15 +24,16 @@ from . import pipeline as pl
 
 # (tile, variant) pairs: the 128-wide kernel prefetches one or two K steps, the 256-wide one
 TILE_VARIANTS = ((128, 1), (128, 2), (256, 1))
-# K chunks per half (two operand segments).  Two or more, so that no float32 accumulator
-# chain is longer than one segment's K range (the D error model assumes that length).
+# K chunks per half (two operand segments).  Split-BF16 takes two or more, so that no float32
+# accumulator chain is longer than one segment's K range (its D error model assumes that
+# length); FP32's products carry no split error and may run a half in one chain.
 CHUNK_CANDIDATES = {
     (nat.PREC_SPLIT_BF16, 128): (2, 4, 6, 8, 12),
     (nat.PREC_SPLIT_BF16, 256): (2, 3, 4, 6, 7, 8),
     (nat.PREC_FP32, 128): (4, 6, 8, 12, 16),
-    (nat.PREC_FP32, 256): (2, 3, 4, 6, 7, 8),
+    (nat.PREC_FP32, 256): (1, 2, 3, 4, 6, 7, 8),
 }
-WS_LIMIT_BYTES = 2 << 30          # skip splits whose partial tiles would exceed this
+WS_LIMIT_BYTES = 4 << 30          # skip splits whose partial tiles would exceed this
 _cache = {}
 
 

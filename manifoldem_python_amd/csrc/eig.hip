@@ -109,7 +109,8 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restric
                                                         const int32_t* __restrict__ jp,
                                                         const int32_t* __restrict__ active,
                                                         double* __restrict__ W,
-                                                        double* __restrict__ A, int rpw) {
+                                                        double* __restrict__ A, int rpw,
+                                                        const int32_t* __restrict__ skip) {
   const int j = *jp;
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restric
     const int pd = rows[2 * g], i = rows[2 * g + 1];
     const memi_pd_desc P = pds[pd];
     const int n = P.n;
-    const bool live = active[pd] && j < min(n, m);
+    const bool live = active[pd] && j < min(n, m) && !(skip && skip[pd]);
     // rows g..g+3 are rows i..i+3 of the same PD when the run and the PD both hold them
     const double* vj = V + (int64_t)m * P.row0 + (int64_t)j * n;
     if (g + 3 < g1 && i + 3 < n) {
@@ -140,13 +141,16 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec(const int32_t* __restric
 // Row i of PD p keeps its nonzero entries at the dense row's offset d_off + i*n: columns in
 // increasing order (col, int32) and values (val), cnt[row0 + i] of them.  A kNN kernel with
 // k << n has about 2k nonzeros per row, so the Lanczos matvec reads 12 B per nonzero instead
-// of 8 B per entry of the dense row.
+// of 8 B per entry of the dense row.  A batch may mix the two forms: with a selection sel
+// (int32 [npd]) only PDs with sel[pd] != 0 get sparse rows and the sparse matvec, the others
+// keep the dense matvec (which skips the selected PDs).
 
 // one wave per row: ballot compaction of the row's nonzeros, 64 columns at a time
 __global__ __launch_bounds__(256) void k_kernel_ell(const int32_t* __restrict__ rows,
                                                     int64_t n_rows,
                                                     const memi_pd_desc* __restrict__ pds,
                                                     const double* __restrict__ L,
+                                                    const int32_t* __restrict__ sel,
                                                     int32_t* __restrict__ col,
                                                     double* __restrict__ val,
                                                     int32_t* __restrict__ cnt) {
@@ -154,6 +158,7 @@ __global__ __launch_bounds__(256) void k_kernel_ell(const int32_t* __restrict__ 
   const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (g >= n_rows) return;                          // uniform over the wave
   const int pd = rows[2 * g], i = rows[2 * g + 1];
+  if (sel && !sel[pd]) return;
   const memi_pd_desc P = pds[pd];
   const int n = P.n;
   const int64_t off = P.d_off + (int64_t)i * n;
@@ -173,30 +178,16 @@ __global__ __launch_bounds__(256) void k_kernel_ell(const int32_t* __restrict__ 
   if (lane == 0) cnt[P.row0 + i] = base;
 }
 
-// w = L v_j from the sparse rows, one wave per row (the row's nonzeros strided over the lanes,
-// four products in flight), and the row's w_i v_j[i] for the orthogonalisation
-__global__ __launch_bounds__(256) void k_lanczos_matvec_ell(const int32_t* __restrict__ rows,
-                                                            int64_t n_rows,
-                                                            const memi_pd_desc* __restrict__ pds,
-                                                            const int32_t* __restrict__ col,
-                                                            const double* __restrict__ val,
-                                                            const int32_t* __restrict__ cnt,
-                                                            const double* __restrict__ V, int m,
-                                                            const int32_t* __restrict__ jp,
-                                                            const int32_t* __restrict__ active,
-                                                            double* __restrict__ W,
-                                                            double* __restrict__ A) {
-  const int j = *jp;
-  const int lane = threadIdx.x & 63;
-  const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (g >= n_rows) return;                          // uniform over the wave
-  const int pd = rows[2 * g], i = rows[2 * g + 1];
-  const memi_pd_desc P = pds[pd];
-  const int n = P.n;
-  if (!active[pd] || j >= min(n, m)) return;
-  const int64_t off = P.d_off + (int64_t)i * n;
-  const int c = cnt[P.row0 + i];
-  const double* v = V + (int64_t)m * P.row0 + (int64_t)j * n;
+// w = L v_j from the sparse rows: each wave takes `rpw` consecutive rows of the plan's row
+// list, as the dense matvec does; four rows of one PD go together (their nonzeros strided over
+// the lanes, two per row and lane in flight, so eight index loads and eight gathers of v_j per
+// lane), others one at a time with four in flight.  Each row's w_i v_j[i] is left for the
+// orthogonalisation.
+__device__ __forceinline__ void ell_row(const int32_t* __restrict__ col,
+                                        const double* __restrict__ val, int64_t off, int c,
+                                        const double* __restrict__ v, int lane,
+                                        double* __restrict__ out, double* __restrict__ aout,
+                                        double vi) {
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   int t = lane;
   for (; t + 3 * 64 < c; t += 4 * 64) {
@@ -208,8 +199,81 @@ __global__ __launch_bounds__(256) void k_lanczos_matvec_ell(const int32_t* __res
   for (; t < c; t += 64) s0 += val[off + t] * v[col[off + t]];
   const double w = wave_total((s0 + s1) + (s2 + s3));
   if (lane == 0) {
-    W[P.row0 + i] = w;
-    if (A) A[P.row0 + i] = w * v[i];
+    *out = w;
+    if (aout) *aout = w * vi;
+  }
+}
+
+__device__ __forceinline__ void ell_rows4(const int32_t* __restrict__ col,
+                                          const double* __restrict__ val, int64_t off0,
+                                          int64_t stride, const int32_t* __restrict__ cnt4,
+                                          const double* __restrict__ v, int lane,
+                                          double* __restrict__ out, double* __restrict__ aout,
+                                          const double* __restrict__ vrow) {
+  int c[4];
+  int cmax = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    c[r] = cnt4[r];
+    cmax = max(cmax, c[r]);
+  }
+  double s[4][2];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) s[r][0] = s[r][1] = 0.0;
+  for (int t = lane; t < cmax; t += 2 * 64) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t o = off0 + r * stride + t;
+      if (t < c[r]) s[r][0] += val[o] * v[col[o]];
+      if (t + 64 < c[r]) s[r][1] += val[o + 64] * v[col[o + 64]];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double w = wave_total(s[r][0] + s[r][1]);
+    if (lane == 0) {
+      out[r] = w;
+      if (aout) aout[r] = w * vrow[r];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lanczos_matvec_ell(const int32_t* __restrict__ rows,
+                                                            int64_t n_rows,
+                                                            const memi_pd_desc* __restrict__ pds,
+                                                            const int32_t* __restrict__ col,
+                                                            const double* __restrict__ val,
+                                                            const int32_t* __restrict__ cnt,
+                                                            const double* __restrict__ V, int m,
+                                                            const int32_t* __restrict__ jp,
+                                                            const int32_t* __restrict__ active,
+                                                            double* __restrict__ W,
+                                                            double* __restrict__ A,
+                                                            const int32_t* __restrict__ sel,
+                                                            int rpw) {
+  const int j = *jp;
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t g0 = ((int64_t)blockIdx.x * 4 + wid) * rpw;
+  const int64_t g1 = min(g0 + (int64_t)rpw, n_rows);
+  for (int64_t g = g0; g < g1;) {                     // uniform over the wave
+    const int pd = rows[2 * g], i = rows[2 * g + 1];
+    const memi_pd_desc P = pds[pd];
+    const int n = P.n;
+    const bool live = active[pd] && !(sel && !sel[pd]) && j < min(n, m);
+    const double* v = V + (int64_t)m * P.row0 + (int64_t)j * n;
+    const int64_t off = P.d_off + (int64_t)i * n;
+    if (g + 3 < g1 && i + 3 < n) {
+      if (live)
+        ell_rows4(col, val, off, n, cnt + P.row0 + i, v, lane, W + P.row0 + i,
+                  A ? A + P.row0 + i : nullptr, v + i);
+      g += 4;
+    } else {
+      if (live)
+        ell_row(col, val, off, cnt[P.row0 + i], v, lane, W + P.row0 + i,
+                A ? A + P.row0 + i : nullptr, v[i]);
+      g += 1;
+    }
   }
 }
 
@@ -897,13 +961,15 @@ int memi_lanczos_init(double* V, const double* v0, const int64_t* idx, int64_t n
   return MEMI_OK;
 }
 
+// L: the dense kernel (every PD when ell_col is null, else the PDs with sel[pd] == 0);
+// ell_*: the sparse rows (every PD when sel is null, else the PDs with sel[pd] != 0)
 static int lanczos_steps_impl(const void* plan_dev, const memi_plan_info* info, const void* L,
                               int l_is_f64, const int32_t* ell_col, const double* ell_val,
-                              const int32_t* ell_cnt, double* V, int m, int32_t* j_dev,
-                              int nsteps, const int32_t* active, double* W, double* H,
-                              double* ab, int split, void* stream) {
+                              const int32_t* ell_cnt, const int32_t* sel, double* V, int m,
+                              int32_t* j_dev, int nsteps, const int32_t* active, double* W,
+                              double* H, double* ab, int split, void* stream) {
   MEMI_CHECK_ARG(plan_dev && info && (L || ell_col) && V && j_dev && active && W && ab &&
-                     (!split || H),
+                     (!split || H) && (!sel || (ell_col && L && l_is_f64)),
                  "memi_lanczos_steps: null pointer");
   MEMI_CHECK_ARG(m >= 2 && m <= 4096 && nsteps >= 0, "memi_lanczos_steps: bad m=%d nsteps=%d",
                  m, nsteps);
@@ -924,18 +990,21 @@ static int lanczos_steps_impl(const void* plan_dev, const memi_plan_info* info, 
   const dim3 gm((unsigned)((n_waves + 3) / 4));
   // the rows' w_i v_j[i] for the grid-parallel orthogonalisation: the second half of W
   double* A = split ? W + info->n_rows : nullptr;
-  const dim3 ge((unsigned)((info->n_rows + 3) / 4));
   for (int st = 0; st < nsteps; ++st) {
-    if (ell_col)
-      hipLaunchKernelGGL(k_lanczos_matvec_ell, ge, dim3(256), 0, s, rows, info->n_rows, pds,
-                         ell_col, ell_val, ell_cnt, V, m, jp, active, W, A);
-    else if (l_is_f64)
-      hipLaunchKernelGGL(k_lanczos_matvec<double>, gm, dim3(256), 0, s, rows, info->n_rows, pds,
-                         (const double*)L, V, m, jp, active, W, A, (int)rpw);
-    else
-      hipLaunchKernelGGL(k_lanczos_matvec<float>, gm, dim3(256), 0, s, rows, info->n_rows, pds,
-                         (const float*)L, V, m, jp, active, W, A, (int)rpw);
-    MEMI_CHECK_LAUNCH("k_lanczos_matvec");
+    if (ell_col) {
+      hipLaunchKernelGGL(k_lanczos_matvec_ell, gm, dim3(256), 0, s, rows, info->n_rows, pds,
+                         ell_col, ell_val, ell_cnt, V, m, jp, active, W, A, sel, (int)rpw);
+      MEMI_CHECK_LAUNCH("k_lanczos_matvec_ell");
+    }
+    if (!ell_col || sel) {
+      if (l_is_f64)
+        hipLaunchKernelGGL(k_lanczos_matvec<double>, gm, dim3(256), 0, s, rows, info->n_rows,
+                           pds, (const double*)L, V, m, jp, active, W, A, (int)rpw, sel);
+      else
+        hipLaunchKernelGGL(k_lanczos_matvec<float>, gm, dim3(256), 0, s, rows, info->n_rows,
+                           pds, (const float*)L, V, m, jp, active, W, A, (int)rpw, sel);
+      MEMI_CHECK_LAUNCH("k_lanczos_matvec");
+    }
     if (!split) {
       hipLaunchKernelGGL(k_lanczos_orth, dim3(npd), dim3(256), (size_t)m * sizeof(double), s,
                          pds, V, m, j_dev, active, W, ab);
@@ -968,22 +1037,25 @@ int memi_lanczos_steps(const void* plan_dev, const memi_plan_info* info, const v
                        const int32_t* active, double* W, double* H, double* ab, int split,
                        void* stream) {
   MEMI_CHECK_ARG(L, "memi_lanczos_steps: null pointer");
-  return lanczos_steps_impl(plan_dev, info, L, l_is_f64, nullptr, nullptr, nullptr, V, m, j_dev,
-                            nsteps, active, W, H, ab, split, stream);
+  return lanczos_steps_impl(plan_dev, info, L, l_is_f64, nullptr, nullptr, nullptr, nullptr, V,
+                            m, j_dev, nsteps, active, W, H, ab, split, stream);
 }
 
 int memi_lanczos_steps_sparse(const void* plan_dev, const memi_plan_info* info,
                               const int32_t* ell_col, const double* ell_val,
-                              const int32_t* ell_cnt, double* V, int m, int32_t* j_dev,
-                              int nsteps, const int32_t* active, double* W, double* H,
-                              double* ab, int split, void* stream) {
-  MEMI_CHECK_ARG(ell_col && ell_val && ell_cnt, "memi_lanczos_steps_sparse: null pointer");
-  return lanczos_steps_impl(plan_dev, info, nullptr, 1, ell_col, ell_val, ell_cnt, V, m, j_dev,
-                            nsteps, active, W, H, ab, split, stream);
+                              const int32_t* ell_cnt, const int32_t* sparse_pd, const double* L,
+                              double* V, int m, int32_t* j_dev, int nsteps,
+                              const int32_t* active, double* W, double* H, double* ab,
+                              int split, void* stream) {
+  MEMI_CHECK_ARG(ell_col && ell_val && ell_cnt && (!sparse_pd || L),
+                 "memi_lanczos_steps_sparse: null pointer (a selection needs the dense L)");
+  return lanczos_steps_impl(plan_dev, info, L, 1, ell_col, ell_val, ell_cnt, sparse_pd, V, m,
+                            j_dev, nsteps, active, W, H, ab, split, stream);
 }
 
 int memi_kernel_ell(const void* plan_dev, const memi_plan_info* info, const double* L,
-                    int32_t* ell_col, double* ell_val, int32_t* ell_cnt, void* stream) {
+                    const int32_t* sparse_pd, int32_t* ell_col, double* ell_val,
+                    int32_t* ell_cnt, void* stream) {
   MEMI_CHECK_ARG(plan_dev && info && L && ell_col && ell_val && ell_cnt,
                  "memi_kernel_ell: null pointer");
   if (info->n_rows == 0) return MEMI_OK;
@@ -991,7 +1063,8 @@ int memi_kernel_ell(const void* plan_dev, const memi_plan_info* info, const doub
   const memi_pd_desc* pds = (const memi_pd_desc*)(plan + info->off_pds);
   const int32_t* rows = (const int32_t*)(plan + info->off_rows);
   hipLaunchKernelGGL(k_kernel_ell, dim3((unsigned)((info->n_rows + 3) / 4)), dim3(256), 0,
-                     (hipStream_t)stream, rows, info->n_rows, pds, L, ell_col, ell_val, ell_cnt);
+                     (hipStream_t)stream, rows, info->n_rows, pds, L, sparse_pd, ell_col, ell_val,
+                     ell_cnt);
   MEMI_CHECK_LAUNCH("k_kernel_ell");
   return MEMI_OK;
 }

@@ -923,9 +923,10 @@ class StageTimer:
         self._last = now
 
 
-# The Lanczos matvec reads the kernel's nonzeros only (nat.kernel_ell) when every PD of the
-# plan keeps k <= n / SPARSE_RATIO neighbours: a kNN kernel then has about 2k nonzeros per row,
-# and 12 B per nonzero beat 8 B per entry of the dense row.
+# The Lanczos matvec reads the kernel's nonzeros only (nat.kernel_ell) for the PDs that keep
+# k <= n / SPARSE_RATIO neighbours: a kNN kernel then has about 2k nonzeros per row, and 12 B
+# per nonzero beat 8 B per entry of the dense row.  The other PDs of the batch keep the dense
+# matvec (both run in every step, each skipping the other's PDs).
 SPARSE_RATIO = 4
 
 # Storage type of the normalised kernel L that the Lanczos matvecs read.  float64 is the
@@ -981,8 +982,9 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
     deg = torch.empty(2 * info.n_rows, dtype=torch.float64, device=dev)
     ks = [min(num_eigs, int(n) - 3) + 1 for n in plan.ns]
 
-    sparse = (eig == "lanczos" and KERNEL_DTYPE == torch.float64 and
-              all(int(k) * SPARSE_RATIO <= int(n) for k, n in zip(plan.ks, plan.ns)))
+    sel = tuple(int(k) * SPARSE_RATIO <= int(n) for k, n in zip(plan.ks, plan.ns))
+    sparse = eig == "lanczos" and KERNEL_DTYPE == torch.float64 and any(sel)
+    sel_t = None
     if sparse:
         ell = getattr(plan, "_ell", None)
         if ell is None or ell[0].numel() < max(info.d_elems, 1) or ell[0].device != dev:
@@ -990,12 +992,19 @@ def dm_embed(plan, D, tune, num_eigs=15, logEps=LOG_EPS, rand=None, a0s=None, ma
                    torch.empty(max(info.d_elems, 1), dtype=torch.float64, device=dev),
                    torch.empty(max(info.n_rows, 1), dtype=torch.int32, device=dev))
             plan._ell = ell
+        if not all(sel):                 # mixed batch: the selection rides with the buffers
+            cached = getattr(plan, "_ell_sel", None)
+            if cached is None or cached[0] != sel or cached[1].device != dev:
+                cached = (sel, torch.as_tensor(np.asarray(sel, dtype=np.int32), device=dev))
+                plan._ell_sel = cached
+            sel_t = cached[1]
+            ell = ell + (sel_t, L)
 
     def spectral(sigma):
         """a14-a15 from the bandwidths: (eigs, Yall) of the Lanczos path, else None."""
         nat.markov_normalize(plan, D, mask, sigma, L, deg)
         if sparse:
-            nat.kernel_ell(plan, L, *ell)
+            nat.kernel_ell(plan, L, *ell[:3], sel=sel_t)
         if timer:
             timer.mark("normalize")
         if eig != "lanczos":

@@ -197,6 +197,55 @@ def test_sparse_kernel_lanczos_matches_dense(cuda, split):
             assert abs(abs(Ys[:, q] @ Yd[:, q]) - 1) < 1e-8, (p, q)
 
 
+@pytest.mark.parametrize("split", [True, False])
+def test_mixed_sparse_dense_lanczos_matches_dense(cuda, split):
+    """A batch whose PDs are partly sparse (k <= n/4) and partly dense: memi_kernel_ell with a
+    selection writes the selected PDs' rows only, and Lanczos with the selection (sparse matvec
+    for those, dense for the rest) gives the all-dense eigenpairs."""
+    import torch
+    from manifoldem_python_amd import _native as nat, pipeline as pl
+    ns, ks, kk = [900, 300, 640, 150], [150, 150, 150, 120], [9, 9, 9, 9]
+    D = [_distance_matrix(n, 80 + i) for i, n in enumerate(ns)]
+    rows0 = np.concatenate(([0], np.cumsum(ns)[:-1]))
+    plan = nat.Plan(rows0, ns, ks, chunks=1)
+    flat = torch.as_tensor(np.concatenate([d.astype(np.float32).ravel() for d in D]),
+                           device=cuda)
+    mask = pl.knn_mask(plan, flat)
+    sigma = torch.as_tensor([np.sqrt(np.median(d)) for d in D], device=cuda)
+    L = torch.empty(plan.info.d_elems, dtype=torch.float64, device=cuda)
+    deg = torch.empty(2 * plan.info.n_rows, dtype=torch.float64, device=cuda)
+    nat.markov_normalize(plan, flat, mask, sigma, L, deg)
+    sel_h = np.array([k * pl.SPARSE_RATIO <= n for k, n in zip(ks, ns)], dtype=np.int32)
+    assert sel_h.tolist() == [1, 0, 1, 0]
+    sel = torch.as_tensor(sel_h, device=cuda)
+    col = torch.full((plan.info.d_elems,), -7, dtype=torch.int32, device=cuda)
+    val = torch.empty(plan.info.d_elems, dtype=torch.float64, device=cuda)
+    cnt = torch.full((plan.info.n_rows,), -7, dtype=torch.int32, device=cuda)
+    nat.kernel_ell(plan, L, col, val, cnt, sel=sel)
+    cnt_h = cnt.cpu().numpy()
+    for p in range(len(ns)):
+        n = ns[p]
+        c = plan.block(col, p).cpu().numpy()
+        if sel_h[p]:
+            A = plan.block(L, p).cpu().numpy()
+            v = plan.block(val, p).cpu().numpy()
+            for i in range(n):
+                k = cnt_h[rows0[p] + i]
+                assert k == np.count_nonzero(A[i])
+                np.testing.assert_array_equal(v[i, :k], A[i, c[i, :k]])
+        else:                                          # untouched
+            assert np.all(cnt_h[rows0[p]:rows0[p] + n] == -7) and np.all(c == -7)
+    dense = pl.lanczos_eigs(plan, L, kk, split=split)
+    mixed = pl.lanczos_eigs(plan, (col, val, cnt, sel, L), kk, split=split)
+    for p in range(len(ns)):
+        ld, Yd, Jd, _ = dense[p]
+        lm, Ym, Jm, _ = mixed[p]
+        np.testing.assert_allclose(lm, ld, rtol=0, atol=1e-12)
+        assert abs(Jm - Jd) <= 8
+        for q in range(kk[p]):
+            assert abs(abs(Ym[:, q] @ Yd[:, q]) - 1) < 1e-8, (p, q)
+
+
 def test_gpu_tanh_fit_matches_curve_fit(cuda):
     """Batched LM fit of the Ferguson curve against scipy curve_fit (the reference's fitter)
     on sweeps of several realistic kernels, from the same start points."""
