@@ -235,8 +235,10 @@ int memi_plan(const memi_pd_desc* pds_in_host, int npd, int chunks, int gram_til
  * info->chunks chunks of ceil(2 steps / chunks) steps, the last one shorter; a chunk count
  * that leaves an empty chunk is rejected.
  * variant: Gram kernel schedule, 0 = default; for 128-wide tiles (two workgroups per CU)
- * 1 = prefetch one K step, 2 = prefetch two (the default); 256-wide tiles (one workgroup per
- * CU, 128 x 128 per wave) prefetch one (variant 0 or 1).
+ * 1 = prefetch one K step through registers, 2 = prefetch two (the default), 3 = load each K
+ * step straight into LDS (global_load_lds) during the previous step's MFMAs; 256-wide tiles
+ * (one workgroup per CU, 128 x 64 per wave) take 1 (the default) or 3.  Every schedule gives
+ * the same D bit for bit.
  * ev_start / ev_stop: optional hipEvent_t recorded on `stream` right before and after the
  * MFMA Gram kernel (NULL: not recorded), for timing that kernel alone. */
 int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const void* op_hi,

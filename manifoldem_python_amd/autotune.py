@@ -4,8 +4,8 @@ The MFMA Gram (memi_ctf_distance) runs one workgroup per (tile, half, K chunk) w
 Three things decide how well a launch fills the 256 CUs: the tile shape (128 x 128 tiles at
 two workgroups per CU, or 256 x 256 tiles at one, which stage half the operand bytes per
 MFMA but leave fewer, larger work items and more padding on small PDs), the schedule variant
-of the 128-wide kernel (prefetch one or two K steps) and the number of K chunks, which sets
-how many work items there are (a launch of 1.06 rounds of resident workgroups runs almost as
+(prefetch one or two K steps through registers, or load straight into LDS) and the number of
+K chunks, which sets how many work items there are (a launch of 1.06 rounds of resident workgroups runs almost as
 long as two full rounds).  ``tune_gram`` times every (tile, variant, chunks) triple on the
 batch's own operand with HIP events around the Gram kernel, installs the fastest plan in the
 workspace and its variant in ``pipeline.GRAM_VARIANT``.  Results are cached per (precision,
@@ -22,8 +22,9 @@ import torch
 from . import _native as nat
 from . import pipeline as pl
 
-# (tile, variant) pairs: the 128-wide kernel prefetches one or two K steps, the 256-wide one
-TILE_VARIANTS = ((128, 1), (128, 2), (256, 1))
+# (tile, variant) pairs: variants 1 and 2 prefetch one or two K steps through registers (the
+# 256-wide kernel one only), variant 3 loads each K step straight into LDS
+TILE_VARIANTS = ((128, 1), (128, 2), (128, 3), (256, 1), (256, 3))
 # K chunks per half (two operand segments).  Split-BF16 takes two or more, so that no float32
 # accumulator chain is longer than one segment's K range (its D error model assumes that
 # length); FP32's products carry no split error and may run a half in one chain.

@@ -105,6 +105,34 @@ __device__ __forceinline__ void gram_lstore(const GramStage<NA, RPT>& R, char* b
       *(u32x4_t*)(b + a * TILE + q * RSTRIDE * LDS_ROW) = R.v[a][q];
 }
 
+// Direct-to-LDS staging of one K step (global_load_lds_dwordx4): each instruction of a wave
+// fills 16 rows x 64 bytes of LDS, lane l landing at byte 16 l, so lane l loads row l / 4,
+// chunk (l % 4) ^ swz(l / 4) of its 16-row group: the swizzled layout the fragment reads
+// expect.  The wave's groups are wvu + q * NW (q < RPT); the LDS destination is uniform.
+template <int NA, class S, int TILE>
+__device__ __forceinline__ void gram_dload(char* buf, const char* gAh, const char* gBh,
+                                           const char* gAl, const char* gBl,
+                                           const uint32_t (&oA)[S::RPT],
+                                           const uint32_t (&oB)[S::RPT], uint32_t ao,
+                                           uint32_t bo, int wvu) {
+  constexpr int NW = S::NT / 64;
+  typedef __attribute__((address_space(3))) void lds_void;
+#pragma unroll
+  for (int q = 0; q < S::RPT; ++q) {
+    char* d = buf + (wvu + q * NW) * 16 * LDS_ROW;
+    __builtin_amdgcn_global_load_lds((const void*)(gAh + (oA[q] + ao)), (lds_void*)(d), 16, 0,
+                                     0);
+    __builtin_amdgcn_global_load_lds((const void*)(gBh + (oB[q] + bo)),
+                                     (lds_void*)(d + 1 * TILE), 16, 0, 0);
+    if constexpr (NA == 4) {
+      __builtin_amdgcn_global_load_lds((const void*)(gAl + (oA[q] + ao)),
+                                       (lds_void*)(d + 2 * TILE), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(gBl + (oB[q] + bo)),
+                                       (lds_void*)(d + 3 * TILE), 16, 0, 0);
+    }
+  }
+}
+
 // MFMAs of one K step for this wave's WM x WN outputs.  G: guarded blocks (inside the PD:
 // nbi x nbj of the wave's blocks hold data; on a diagonal tile, not wholly below the
 // diagonal); full off-diagonal tiles run unguarded.  Split-BF16 keeps the B hi / lo fragments
@@ -164,13 +192,14 @@ __device__ __forceinline__ void gram_compute(floatx4_t (&acc)[S::NBI][S::NBJ],
 // The K loop over nks steps of 64 bytes per row: prefetch distance DEPTH (one or two register
 // stages), two LDS buffers, one barrier per step.
 template <int PREC, class S, int DEPTH, bool G>
-__device__ __forceinline__ void gram_kloop(floatx4_t (&acc)[S::NBI][S::NBJ], char* lds, int nks,
+__device__ __forceinline__ void gram_kloop(floatx4_t (&acc)[S::NBI][S::NBJ], char* lds,
+                                           char* lds_b, int nks,
                                            int s0, int steps, int half, uint32_t seg_bytes,
                                            const char* gAh, const char* gBh, const char* gAl,
                                            const char* gBl, const uint32_t (&oA)[S::RPT],
                                            const uint32_t (&oB)[S::RPT], int lds_st,
                                            int frag_off, int wr, int wc, int nbi, int nbj,
-                                           bool diag) {
+                                           bool diag, int wvu) {
   using Cfg = GramCfg<PREC>;
   constexpr int NA = Cfg::NARR;
   constexpr int RPT = S::RPT;
@@ -190,6 +219,48 @@ __device__ __forceinline__ void gram_kloop(floatx4_t (&acc)[S::NBI][S::NBJ], cha
   // that only some paths issue makes the compiler's wait-count merge at the join assume the
   // fewer outstanding loads, so the LDS store of the older stage would wait for the prefetch
   // just issued and the prefetch distance would collapse to one step's MFMAs.
+#define MEMI_DLOAD(B, ks)                                                                 \
+  do {                                                                                    \
+    const int gs_ = s0 + (ks);                                                            \
+    const int sec_ = gs_ >= steps;                                                        \
+    const int sl_ = 2 * half + sec_;                                                      \
+    const uint32_t kk_ = (uint32_t)(gs_ - sec_ * steps) * ROW_STEP;                       \
+    gram_dload<NA, S, TILE>(B, gAh, gBh, gAl, gBl, oA, oB, sl_ * seg_bytes + kk_,         \
+                            seg_pair(sl_) * seg_bytes + kk_, wvu);                        \
+  } while (0)
+  if constexpr (DEPTH == 3) {
+    // direct-to-LDS, two buffers: the loads of step ks + 1 fly during step ks's MFMAs; each
+    // wave waits for its own loads, then the barrier publishes the buffer
+    char* const b0 = lds;
+    char* const b1 = lds_b;
+    MEMI_DLOAD(b0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    int ks = 0;
+    for (; ks + 2 < nks; ks += 2) {
+      MEMI_DLOAD(b1, ks + 1);
+      __builtin_amdgcn_sched_barrier(0);     // the loads go out before the MFMAs
+      gram_compute<PREC, S, TILE, G>(acc, b0, frag_off, wr, wc, nbi, nbj, diag);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      MEMI_DLOAD(b0, ks + 2);
+      __builtin_amdgcn_sched_barrier(0);
+      gram_compute<PREC, S, TILE, G>(acc, b1, frag_off, wr, wc, nbi, nbj, diag);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
+    if (ks + 1 < nks) {
+      MEMI_DLOAD(b1, ks + 1);
+      gram_compute<PREC, S, TILE, G>(acc, b0, frag_off, wr, wc, nbi, nbj, diag);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      gram_compute<PREC, S, TILE, G>(acc, b1, frag_off, wr, wc, nbi, nbj, diag);
+    } else {
+      gram_compute<PREC, S, TILE, G>(acc, b0, frag_off, wr, wc, nbi, nbj, diag);
+    }
+    return;
+  }
+#undef MEMI_DLOAD
   GramStage<NA, RPT> X;
   if constexpr (DEPTH == 1) {
     MEMI_GLOAD(X, 0);
@@ -259,8 +330,11 @@ __global__ __launch_bounds__(S::NT, S::OCC) void k_gram_tiles(
   constexpr int NBI = S::NBI, NBJ = S::NBJ;
   constexpr int RPT = S::RPT;
   constexpr int TILE = BT * LDS_ROW;         // LDS bytes of one array's K step
-  static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth 1 or 2");
-  __shared__ __attribute__((aligned(16))) char lds[2 * NA * TILE];
+  static_assert(DEPTH >= 1 && DEPTH <= 3, "prefetch depth 1 or 2, or 3 = direct to LDS");
+  // DEPTH 3 keeps its two buffers in separate arrays, so that the compiler can tell the
+  // direct-to-LDS loads into one from the fragment reads of the other
+  __shared__ __attribute__((aligned(16))) char lds[(DEPTH == 3 ? 1 : 2) * NA * TILE];
+  __shared__ __attribute__((aligned(16))) char lds_b[DEPTH == 3 ? NA * TILE : 16];
 
   // XCD-aware order: workgroup w runs on XCD w % 8; give each XCD a contiguous range of
   // items (consecutive items share operand panels and K ranges).  gridDim.x % 8 == 0.
@@ -294,11 +368,24 @@ __global__ __launch_bounds__(S::NT, S::OCC) void k_gram_tiles(
   const char* const gAl = NA == 4 ? op_lo + tA : nullptr;
   const char* const gBl = NA == 4 ? op_lo + tB : nullptr;
   uint32_t oA[RPT], oB[RPT];
+  const int wvu = __builtin_amdgcn_readfirstlane(wv);
+  if constexpr (DEPTH == 3) {
+    // row (wvu + q NW) * 16 + lane / 4, swizzled chunk, for the direct-to-LDS loads
+    static_assert(S::BT == 16 * (S::NT / 64) * RPT, "16-row groups cover the tile");
+    const int ch = (lane & 3) ^ swz(lane >> 2);
 #pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    const int r = ld_row0 + (S::NT / 4) * q;
-    oA[q] = (uint32_t)(min(r, mi - 1) * row_bytes + ld_part * 16);
-    oB[q] = (uint32_t)(min(r, mj - 1) * row_bytes + ld_part * 16);
+    for (int q = 0; q < RPT; ++q) {
+      const int r = (wvu + q * (S::NT / 64)) * 16 + (lane >> 2);
+      oA[q] = (uint32_t)(min(r, mi - 1) * row_bytes + ch * 16);
+      oB[q] = (uint32_t)(min(r, mj - 1) * row_bytes + ch * 16);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int r = ld_row0 + (S::NT / 4) * q;
+      oA[q] = (uint32_t)(min(r, mi - 1) * row_bytes + ld_part * 16);
+      oB[q] = (uint32_t)(min(r, mj - 1) * row_bytes + ld_part * 16);
+    }
   }
   const int lds_st = ld_row0 * LDS_ROW + ((ld_part ^ swz(ld_row0)) * 16);
   const int frag_off = (lane & 15) * LDS_ROW + (((lane >> 4) ^ swz(lane & 15)) * 16);
@@ -314,11 +401,13 @@ __global__ __launch_bounds__(S::NT, S::OCC) void k_gram_tiles(
 
   const uint32_t seg_bytes = (uint32_t)(Khp * ES);
   if (mi == BT && mj == BT && !diag)
-    gram_kloop<PREC, S, DEPTH, false>(acc, lds, nks, s0, steps, half, seg_bytes, gAh, gBh, gAl,
-                                      gBl, oA, oB, lds_st, frag_off, wr, wc, nbi, nbj, diag);
+    gram_kloop<PREC, S, DEPTH, false>(acc, lds, lds_b, nks, s0, steps, half, seg_bytes, gAh,
+                                      gBh, gAl, gBl, oA, oB, lds_st, frag_off, wr, wc, nbi, nbj,
+                                      diag, wvu);
   else
-    gram_kloop<PREC, S, DEPTH, true>(acc, lds, nks, s0, steps, half, seg_bytes, gAh, gBh, gAl,
-                                     gBl, oA, oB, lds_st, frag_off, wr, wc, nbi, nbj, diag);
+    gram_kloop<PREC, S, DEPTH, true>(acc, lds, lds_b, nks, s0, steps, half, seg_bytes, gAh, gBh,
+                                     gAl, gBl, oA, oB, lds_st, frag_off, wr, wc, nbi, nbj, diag,
+                                     wvu);
 
   // 16x16 output block: lane holds rows 4*(lane/16) + r, column lane % 16
   float* out = ws + slot * BT * BT;
@@ -555,7 +644,7 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
                  "memi_ctf_distance: bad Khp %lld", (long long)Khp);
   const int BT = info->gram_tile;
   MEMI_CHECK_ARG(BT == 128 || BT == 256, "memi_ctf_distance: plan has Gram tile %d", BT);
-  MEMI_CHECK_ARG(variant >= 0 && variant <= 2 && !(BT == 256 && variant == 2),
+  MEMI_CHECK_ARG(variant >= 0 && variant <= 3 && !(BT == 256 && variant == 2),
                  "memi_ctf_distance: variant %d is not available for Gram tile %d", variant, BT);
   const int chunks = info->chunks;
   const int kstep = ROW_STEP / (precision == MEMI_PREC_FP32 ? 4 : 2);
@@ -589,12 +678,16 @@ int memi_ctf_distance(const void* plan_dev, const memi_plan_info* info, const vo
                      (const char*)op_hi, lo, Khp, (int)spc, (int)steps, items, info->n_items, \
                      pds, ws)
   if (precision == MEMI_PREC_FP32) {
-    if (BT == 256) MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, Shape256, 1);
+    if (BT == 256 && depth == 3) MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, Shape256, 3);
+    else if (BT == 256) MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, Shape256, 1);
     else if (depth == 1) MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, Shape128, 1);
+    else if (depth == 3) MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, Shape128, 3);
     else MEMI_LAUNCH_GRAM(MEMI_PREC_FP32, Shape128, 2);
   } else {
-    if (BT == 256) MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, Shape256, 1);
+    if (BT == 256 && depth == 3) MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, Shape256, 3);
+    else if (BT == 256) MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, Shape256, 1);
     else if (depth == 1) MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, Shape128, 1);
+    else if (depth == 3) MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, Shape128, 3);
     else MEMI_LAUNCH_GRAM(MEMI_PREC_SPLIT_BF16, Shape128, 2);
   }
 #undef MEMI_LAUNCH_GRAM

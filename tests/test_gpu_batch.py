@@ -68,6 +68,23 @@ def test_batched_distances_match_oracle_per_pd(cuda, oracle_D, precision, tile):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("tile", [128, 256])
+def test_gram_schedules_agree_bitwise(cuda, precision, tile):
+    """Every schedule of the Gram kernel (one- or two-step register prefetch, direct-to-LDS
+    loads) runs the same MFMAs in the same order on the same staged bytes: D is bitwise the
+    same, over several K chunks and PDs with edge tiles."""
+    from manifoldem_python_amd import batch as B, pipeline as pl
+    batch = _batch(_pds(), cuda)
+    wsp = B.Workspace(batch, precision, chunk_rows=200, gram_tile=tile)
+    B.distances(batch, wsp)
+    variants = (1, 3) if tile == 256 else (1, 2, 3)
+    Ds = [pl.ctf_distances(wsp.plan, wsp.operand, wsp.fy, wsp.ctf, N, ws=wsp.ws,
+                           variant=v).D.cpu().numpy() for v in variants]
+    for v, D in zip(variants[1:], Ds[1:]):
+        assert np.array_equal(D, Ds[0]), v
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_graph_replays_match_eager(cuda, precision):
     """Three replays of the captured prep + Gram, each after eager work on the same stream,
     give the eager run's D and flag count."""
