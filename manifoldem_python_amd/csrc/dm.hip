@@ -19,7 +19,13 @@ namespace memi {
 constexpr int TB = 128;  // tile side of the full-matrix tiles of the plan
 // blocks per 128 x 128 tile in the Ferguson and normalisation passes, each a quarter of the
 // tile (one block per tile left most of each CU idle for batches of few tiles)
-constexpr int MARKOV_RQ = 4;
+// blocks per 128 x 128 tile: the normalisation kernels (rows split MARKOV_RQ ways; 8 keeps
+// eight waves per SIMD on a one-PD batch of 2000 rows, whose float64 chains are latency-bound
+// at fewer), the moment sums (entries split FM_RQ ways; 80 KB of LDS per block allow two per
+// CU) and the statistics (FS_RQ ways; their counters are spread over FER_SLOTS slots per PD)
+constexpr int MARKOV_RQ = 8;
+constexpr int FM_RQ = 4;
+constexpr int FS_RQ = 4;
 constexpr int FM_GP = 8;  // grid parameters per PD of the Ferguson moments (k_fer_grid)
 
 __device__ inline uint32_t float_key(float v) {
@@ -44,6 +50,37 @@ __device__ inline double entry_value(const float* __restrict__ D, const uint32_t
   const float dij = D[P.d_off + (int64_t)i * P.n + j];
   const bool mij = full || mbit(mask, P.mask_off, W, i, j);
   const bool mji = full || mbit(mask, P.mask_off, W, j, i);
+  return entry_from(dij, mij, mji, i == j);
+}
+
+// The transposed list bits of one 128 x 128 tile (ti, tj) in LDS: bit il of mt[jl][il / 32] is
+// [i on j's list] for i = ti TB + il, j = tj TB + jl.  Read row by row from the mask (4 words
+// per row of the tile) once per block; read per entry it was a gather with a row stride of W
+// words across the lanes, 64 cache lines per wave-wide load.  Call before any thread leaves.
+typedef uint32_t TileMaskT[TB][TB / 32];
+__device__ inline void load_tile_mask_t(TileMaskT& mt, const uint32_t* __restrict__ mask,
+                                        const memi_pd_desc& P, int W, bool full, int ti, int tj) {
+  if (!full)
+    for (int q = threadIdx.x; q < TB * (TB / 32); q += blockDim.x) {
+      const int jl = q / (TB / 32), w = q % (TB / 32);
+      const int j = tj * TB + jl, word = ti * (TB / 32) + w;
+      mt[jl][w] = (j < P.n && word < W) ? mask[P.mask_off + (int64_t)j * W + word] : 0u;
+    }
+  __syncthreads();
+}
+
+__device__ inline bool tile_mji(const TileMaskT& mt, int il, int jl) {
+  return (mt[jl][il >> 5] >> (il & 31)) & 1u;
+}
+
+// entry_value with [i on j's list] from the block's LDS copy; il, jl: i, j within the tile
+__device__ inline double entry_value_t(const float* __restrict__ D,
+                                       const uint32_t* __restrict__ mask, const memi_pd_desc& P,
+                                       int W, bool full, int i, int j, const TileMaskT& mt,
+                                       int il, int jl) {
+  const float dij = D[P.d_off + (int64_t)i * P.n + j];
+  const bool mij = full || mbit(mask, P.mask_off, W, i, j);
+  const bool mji = full || tile_mji(mt, il, jl);
   return entry_from(dij, mij, mji, i == j);
 }
 
@@ -321,25 +358,22 @@ constexpr double FM_TMAX = 1.0;
 
 __device__ inline unsigned long long dbits(double v) { return (unsigned long long)__double_as_longlong(v); }
 
-__global__ void k_fer_init(int64_t npd, double* __restrict__ st) {
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npd;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    st[6 * p + 0] = 0.0;                                   // entries
-    st[6 * p + 1] = 0.0;                                   // sum exp(-v / (2 max eps))
-    st[6 * p + 2] = 0.0;                                   // zero entries
-    st[6 * p + 3] = __longlong_as_double(0x7FF0000000000000LL);   // min positive v (+inf)
-    st[6 * p + 4] = 0.0;                                   // max v
-    st[6 * p + 5] = 0.0;
-  }
-}
+// per PD: #entries, the threshold sum, #zero entries, min and max positive value, summed into
+// FER_SLOTS slots of 5 per PD (the block index modulo FER_SLOTS) at the start of the PD's moment table
+// (zeroed, and zeroed again by k_fer_grid once it has summed the slots): one set of five
+// counters for the hundreds of tiles of one large PD serialised their atomics (~60 us at
+// n = 2000).  The minimum is kept as the maximum of the complemented bit pattern, so that a
+// zeroed slot is neutral for both.
+constexpr int FER_SLOTS = 16;
+static_assert(FER_SLOTS * 5 <= FM_BMAX * FM_NM && FER_SLOTS <= 64 &&
+                  (FER_SLOTS & (FER_SLOTS - 1)) == 0, "slots: a power of two, in a wave");
 
-// per PD: #entries, the threshold sum, #zero entries, min and max positive value
 __global__ __launch_bounds__(256) void k_fer_stats(const int32_t* __restrict__ ftiles,
                                                    const memi_pd_desc* __restrict__ pds,
                                                    const float* __restrict__ D,
                                                    const uint32_t* __restrict__ mask,
                                                    const double* __restrict__ logEps, int n_eps,
-                                                   double* __restrict__ st) {
+                                                   double* __restrict__ slots) {
   __shared__ double red[8];
   __shared__ double mx_sh[4];
   __shared__ unsigned long long mm_sh[8];
@@ -354,24 +388,38 @@ __global__ __launch_bounds__(256) void k_fer_stats(const int32_t* __restrict__ f
   const memi_pd_desc P = pds[f[0]];
   const int W = (P.n + 31) / 32;
   const bool full = P.k >= P.n;
+  __shared__ TileMaskT mt;
+  load_tile_mask_t(mt, mask, P, W, full, f[1], f[2]);
   double sum = 0.0, cnt = 0.0, zer = 0.0;
   unsigned long long vmin = dbits(INFINITY), vmax = 0ull;
-  // blockIdx.y: which gridDim.y-th of the tile's entries (one block per tile keeps the
-  // same-address atomics of a one-PD batch few)
+  // blockIdx.y: which gridDim.y-th of the tile's entries
   const int part = TB * TB / gridDim.y;
-  const int e1 = (blockIdx.y + 1) * part;
-  for (int e = blockIdx.y * part + threadIdx.x; e < e1; e += blockDim.x) {
-    const int i = f[1] * TB + e / TB, j = f[2] * TB + e % TB;
-    if (i >= P.n || j >= P.n) continue;
-    const double v = entry_value(D, mask, P, W, full, i, j);
-    if (v < 0) continue;
-    sum += exp(-v * cmin);
-    cnt += 1.0;
-    if (v == 0.0) {
-      zer += 1.0;
-    } else {
-      vmin = min(vmin, dbits(v));
-      vmax = max(vmax, dbits(v));
+  const int e0 = blockIdx.y * part, e1 = e0 + part;
+  // EU entries per thread per round, their loads issued together: with one block per tile a
+  // one-PD batch has a block per CU, and a load-then-exp loop one entry at a time waited a
+  // memory latency per entry
+  constexpr int EU = 8;
+  for (int eb = e0; eb < e1; eb += EU * 256) {
+    double v[EU];
+#pragma unroll
+    for (int u = 0; u < EU; ++u) {
+      const int e = eb + u * 256 + threadIdx.x;
+      const int il = e / TB, jl = e % TB;
+      const int i = f[1] * TB + il, j = f[2] * TB + jl;
+      v[u] = (e < e1 && i < P.n && j < P.n)
+                 ? entry_value_t(D, mask, P, W, full, i, j, mt, il, jl) : -1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < EU; ++u) {
+      if (v[u] < 0) continue;
+      sum += exp(-v[u] * cmin);
+      cnt += 1.0;
+      if (v[u] == 0.0) {
+        zer += 1.0;
+      } else {
+        vmin = min(vmin, dbits(v[u]));
+        vmax = max(vmax, dbits(v[u]));
+      }
     }
   }
   sum = block_sum(sum, red);
@@ -391,12 +439,13 @@ __global__ __launch_bounds__(256) void k_fer_stats(const int32_t* __restrict__ f
       vmin = min(vmin, mm_sh[w]);
       vmax = max(vmax, mm_sh[4 + w]);
     }
-    double* sp = st + 6 * (int64_t)f[0];
-    atomicAdd(sp + 0, cnt);
-    atomicAdd(sp + 1, sum);
-    atomicAdd(sp + 2, zer);
-    atomicMin((unsigned long long*)(sp + 3), vmin);
-    atomicMax((unsigned long long*)(sp + 4), vmax);
+    double* sl = slots + (int64_t)f[0] * FM_BMAX * FM_NM +
+                 ((blockIdx.x * gridDim.y + blockIdx.y) % FER_SLOTS) * 5;
+    atomicAdd(sl + 0, cnt);
+    atomicAdd(sl + 1, sum);
+    atomicAdd(sl + 2, zer);
+    atomicMax((unsigned long long*)(sl + 3), ~vmin);
+    atomicMax((unsigned long long*)(sl + 4), vmax);
   }
 }
 
@@ -408,40 +457,69 @@ __device__ inline int fm_bucket(double v, double base, double w) {
 }
 
 // per PD (one thread each): thr and the bucket grid {thr, w, base, b0, B, s, zeros, -}
-__global__ void k_fer_grid(int64_t npd, const double* __restrict__ st, double le0, double dle,
-                           double* __restrict__ gp) {
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npd;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    const double* sp = st + 6 * p;
-    double* g = gp + FM_GP * p;
-    // fergusonE.py:25-31
-    const double thr = fmax(-log(0.01 * sp[1] / sp[0]), 10.0);
-    const double vmin = sp[3], vmax = sp[4];
-    int B = 0, s = 1, b0 = 0;
-    double w = dle, base = log(2.0 * thr) + le0;
-    bool direct = false;
-    if (vmax > 0.0 && vmin <= vmax) {
-      const double range = fmax(log(vmax) - log(vmin), 1e-300);
-      s = (int)fmin(8.0, fmax(1.0, floor((FM_BMAX - 32) * dle / range)));
-      w = dle / s;
-      // buckets needed, in float64 before any int conversion (the +2 covers the edge buckets)
-      const double need = range / w + 2.0;
-      direct = !(need <= (double)FM_BMAX) || !(thr * (exp(0.5 * w) - 1.0) <= FM_TMAX);
-      if (!direct) {
-        b0 = fm_bucket(vmin, base, w);
-        B = fm_bucket(vmax, base, w) - b0 + 1;
-        B = min(max(B, 0), FM_BMAX);    // bounds the tables for any input (NaN, inf)
-      }
-    }
-    g[0] = thr;
-    g[1] = w;
-    g[2] = base;
-    g[3] = b0;
-    g[4] = B;
-    g[5] = s;
-    g[6] = sp[2];
-    g[7] = direct ? 1.0 : 0.0;          // 1: summed by the direct sweep
+__global__ __launch_bounds__(256) void k_fer_grid(int64_t npd, double* __restrict__ st,
+                                                  double* __restrict__ slots, double le0,
+                                                  double dle, double* __restrict__ gp) {
+  // one wave per PD: lane q < FER_SLOTS reads (and clears, for the moment sums) slot q of
+  // k_fer_stats; the sums run in one fixed order (a shuffle tree), lane 0 does the rest
+  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= npd) return;                               // uniform over the wave
+  double* sl = slots + p * FM_BMAX * FM_NM + 5 * lane;
+  double cnt = 0.0, sum = 0.0, zer = 0.0;
+  unsigned long long cmin = 0ull, vmx = 0ull;
+  if (lane < FER_SLOTS) {
+    cnt = sl[0];
+    sum = sl[1];
+    zer = sl[2];
+    cmin = dbits(sl[3]);
+    vmx = dbits(sl[4]);
+    for (int q = 0; q < 5; ++q) sl[q] = 0.0;
   }
+  for (int off = FER_SLOTS / 2; off > 0; off >>= 1) {
+    cnt += __shfl_xor(cnt, off, 64);
+    sum += __shfl_xor(sum, off, 64);
+    zer += __shfl_xor(zer, off, 64);
+    cmin = max(cmin, (unsigned long long)__shfl_xor((long long)cmin, off, 64));
+    vmx = max(vmx, (unsigned long long)__shfl_xor((long long)vmx, off, 64));
+  }
+  if (lane != 0) return;
+  double* sp = st + 6 * p;
+  const double vmin = __longlong_as_double(cmin ? (long long)~cmin : 0x7FF0000000000000LL);
+  const double vmax = __longlong_as_double((long long)vmx);
+  sp[0] = cnt;                                            // entries
+  sp[1] = sum;                                            // sum exp(-v / (2 max eps))
+  sp[2] = zer;                                            // zero entries
+  sp[3] = vmin;                                           // min positive v; +inf: none
+  sp[4] = vmax;                                           // max v
+  sp[5] = 0.0;
+  double* g = gp + FM_GP * p;
+  // fergusonE.py:25-31
+  const double thr = fmax(-log(0.01 * sum / cnt), 10.0);
+  int B = 0, s = 1, b0 = 0;
+  double w = dle, base = log(2.0 * thr) + le0;
+  bool direct = false;
+  if (vmax > 0.0 && vmin <= vmax) {
+    const double range = fmax(log(vmax) - log(vmin), 1e-300);
+    s = (int)fmin(8.0, fmax(1.0, floor((FM_BMAX - 32) * dle / range)));
+    w = dle / s;
+    // buckets needed, in float64 before any int conversion (the +2 covers the edge buckets)
+    const double need = range / w + 2.0;
+    direct = !(need <= (double)FM_BMAX) || !(thr * (exp(0.5 * w) - 1.0) <= FM_TMAX);
+    if (!direct) {
+      b0 = fm_bucket(vmin, base, w);
+      B = fm_bucket(vmax, base, w) - b0 + 1;
+      B = min(max(B, 0), FM_BMAX);    // bounds the tables for any input (NaN, inf)
+    }
+  }
+  g[0] = thr;
+  g[1] = w;
+  g[2] = base;
+  g[3] = b0;
+  g[4] = B;
+  g[5] = s;
+  g[6] = zer;
+  g[7] = direct ? 1.0 : 0.0;          // 1: summed by the direct sweep
 }
 
 // power sums of every positive entry into its bucket (LDS accumulation, then one atomic
@@ -461,15 +539,17 @@ __global__ __launch_bounds__(256) void k_fer_moments(const int32_t* __restrict__
   const int b0 = (int)g[3], B = (int)g[4];
   if (B <= 0) return;                                 // uniform over the block
   for (int q = threadIdx.x; q < B * FM_NM; q += blockDim.x) ms[q] = 0.0;
-  __syncthreads();
   const int W = (P.n + 31) / 32;
   const bool full = P.k >= P.n;
-  // blockIdx.y: which MARKOV_RQ-th of the tile's entries
-  const int e1 = (blockIdx.y + 1) * (TB * TB / MARKOV_RQ);
-  for (int e = blockIdx.y * (TB * TB / MARKOV_RQ) + threadIdx.x; e < e1; e += blockDim.x) {
-    const int i = f[1] * TB + e / TB, j = f[2] * TB + e % TB;
+  __shared__ TileMaskT mt;
+  load_tile_mask_t(mt, mask, P, W, full, f[1], f[2]);   // and the barrier after the zeroing
+  // blockIdx.y: which FM_RQ-th of the tile's entries
+  const int e1 = (blockIdx.y + 1) * (TB * TB / FM_RQ);
+  for (int e = blockIdx.y * (TB * TB / FM_RQ) + threadIdx.x; e < e1; e += blockDim.x) {
+    const int il = e / TB, jl = e % TB;
+    const int i = f[1] * TB + il, j = f[2] * TB + jl;
     if (i >= P.n || j >= P.n) continue;
-    const double v = entry_value(D, mask, P, W, full, i, j);
+    const double v = entry_value_t(D, mask, P, W, full, i, j, mt, il, jl);
     if (v <= 0.0) continue;
     const int k = min(max(fm_bucket(v, base, w) - b0, 0), B - 1);
     const double lo = exp(base + (b0 + k) * w), hi = exp(base + (b0 + k + 1) * w);
@@ -572,6 +652,8 @@ __global__ __launch_bounds__(256) void k_markov_colsum(const int32_t* __restrict
   const double inv_s2 = 1.0 / (sigma[pd] * sigma[pd]);
   const int cl = threadIdx.x & (TB - 1), half = threadIdx.x >> 7;
   const int j = f[2] * TB + cl;
+  __shared__ TileMaskT mt;
+  load_tile_mask_t(mt, mask, P, W, full, f[1], f[2]);
   if (j >= P.n) return;
   double s = 0.0;
   // blockIdx.y: which MARKOV_RQ-th of the tile's rows (several blocks per tile keep the CUs
@@ -580,7 +662,8 @@ __global__ __launch_bounds__(256) void k_markov_colsum(const int32_t* __restrict
   for (int r = blockIdx.y * (TB / MARKOV_RQ) + half; r < r1; r += 2) {
     const int i = f[1] * TB + r;
     if (i >= P.n) break;
-    const double w = kernel_weight(entry_value(D, mask, P, W, full, i, j), inv_s2);
+    const double w =
+        kernel_weight(entry_value_t(D, mask, P, W, full, i, j, mt, r, cl), inv_s2);
     s += (PASS == 0) ? w : w / deg[P.row0 + i];
   }
   atomicAdd(&deg[(PASS == 0 ? 0 : n_rows) + P.row0 + j], s);
@@ -602,6 +685,8 @@ __global__ __launch_bounds__(256) void k_markov_write(const int32_t* __restrict_
   const double inv_s2 = 1.0 / (sigma[pd] * sigma[pd]);
   const int cl = threadIdx.x & (TB - 1), half = threadIdx.x >> 7;
   const int j = f[2] * TB + cl;
+  __shared__ TileMaskT mt;
+  load_tile_mask_t(mt, mask, P, W, full, f[1], f[2]);
   if (j >= P.n) return;
   const double dj = deg[P.row0 + j];
   const double dpj = sqrt(deg[n_rows + P.row0 + j] / dj);
@@ -613,7 +698,7 @@ __global__ __launch_bounds__(256) void k_markov_write(const int32_t* __restrict_
     const double dpi = sqrt(deg[n_rows + P.row0 + i] / di);
     const float dij = D[P.d_off + (int64_t)i * P.n + j];
     const bool mij = full || mbit(mask, P.mask_off, W, i, j);
-    const bool mji = full || mbit(mask, P.mask_off, W, j, i);
+    const bool mji = full || tile_mji(mt, r, cl);
     const double wij = kernel_weight(entry_from(dij, mij, mji, i == j), inv_s2);
     const double wji = kernel_weight(entry_from(dij, mji, mij, i == j), inv_s2);
     // slaplacianonFly.py:63-78: W/(d_i d_j), then /(d'_i d'_j), then |l + l^T| / 2
@@ -1106,15 +1191,13 @@ int memi_ferguson_moments(const void* plan_dev, const memi_plan_info* info, cons
   const int64_t nm = npd * (int64_t)FM_BMAX * FM_NM;
   hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(nm)), dim3(256), 0, s, M, nm);
   MEMI_CHECK_LAUNCH("k_zero_f64");
-  hipLaunchKernelGGL(k_fer_init, dim3(grid_for(npd)), dim3(256), 0, s, npd, st);
-  MEMI_CHECK_LAUNCH("k_fer_init");
-  hipLaunchKernelGGL(k_fer_stats, dim3((unsigned)info->n_ftiles, 1), dim3(256), 0, s,
-                     v.ftiles,
-                     v.pds, D, mask, logEps, n_eps, st);
+  hipLaunchKernelGGL(k_fer_stats, dim3((unsigned)info->n_ftiles, FS_RQ), dim3(256), 0, s,
+                     v.ftiles, v.pds, D, mask, logEps, n_eps, M);
   MEMI_CHECK_LAUNCH("k_fer_stats");
-  hipLaunchKernelGGL(k_fer_grid, dim3(grid_for(npd)), dim3(256), 0, s, npd, st, le0, dle, gp);
+  hipLaunchKernelGGL(k_fer_grid, dim3((unsigned)((npd + 3) / 4)), dim3(256), 0, s, npd, st, M,
+                     le0, dle, gp);
   MEMI_CHECK_LAUNCH("k_fer_grid");
-  hipLaunchKernelGGL(k_fer_moments, dim3((unsigned)info->n_ftiles, MARKOV_RQ), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_fer_moments, dim3((unsigned)info->n_ftiles, FM_RQ), dim3(256), 0, s,
                      v.ftiles,
                      v.pds, D, mask, gp, M);
   MEMI_CHECK_LAUNCH("k_fer_moments");
