@@ -396,7 +396,7 @@ def batched_inv(A):
 
 
 def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_per_round=8,
-                       logS_dev=None):
+                       logS_dev=None, first=None):
     """The fit of ``fergusonE.py:45-57`` for many PDs at once on the GPU (memi_tanh_fit):
     fit from a0, pcov = inv(J^T J) ss_res / (m - 4) as scipy's curve_fit forms it, and refit
     from rand(4,1) - 0.5 while sum(sqrt|diag pcov|) > 100.  Returns (popt (npd, 4), R^2).
@@ -405,7 +405,9 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
     starts of every unfinished PD in parallel: its pending start followed by the next draws in
     the order the reference's loop makes them (one after every fit), keeping the first
     acceptable one in that order -- the same result as sequential restarts, in fewer round
-    trips.  logS_dev: the same curves already on the device (npd, m) float64, if at hand."""
+    trips.  logS_dev: the same curves already on the device (npd, m) float64, if at hand.
+    first: memi_tanh_fit's output (npd * 16 float64, device) for every PD's own start, if
+    already computed: the first round then reuses it instead of fitting again."""
     rand = np.random.rand if rand is None else rand
     dev = device()
     npd, m = logS.shape
@@ -434,9 +436,12 @@ def ferguson_fit_batch(logEps, logS, a0s, rand=None, max_restarts=200, starts_pe
             yy = y_all
         else:
             yy = y_all[torch.as_tensor(np.repeat(idx, R), device=dev)].contiguous()
-        out = torch.empty(B * 16, dtype=torch.float64, device=dev)
-        nat.tanh_fit(x, yy, _to_device(starts.reshape(B, 4), dev), out)
-        o = _to_host(out).reshape(len(idx), R, 16)
+        if fits == 0 and first is not None:
+            out = first
+        else:
+            out = torch.empty(B * 16, dtype=torch.float64, device=dev)
+            nat.tanh_fit(x, yy, _to_device(starts.reshape(B, 4), dev), out)
+        o = _to_host(out[:B * 16]).reshape(len(idx), R, 16)
         J = np.zeros((len(idx), R, 4, 4))
         J[:, :, iu[0], iu[1]] = o[:, :, 4:14]
         J = J + np.triu(J, 1).transpose(0, 1, 3, 2)
@@ -471,7 +476,7 @@ def fit_bandwidths(logEps, logS_dev, a0s, tune, rand=None):
     """a13 for every PD: the tanh fit from each PD's start (memi_tanh_fit) and the restart
     test with the bandwidth on the device (memi_tanh_select).  Only the acceptance flags come
     back before the bandwidths are used; when a PD's first start fails the test, the host
-    restart loop (ferguson_fit_batch, the same starts first) takes over for the batch.
+    restart loop (ferguson_fit_batch, from these first fits) takes over for the batch.
     Returns (sigma (npd,) float64 device tensor, finish) with finish(logS host) -> (popt, R2).
     (Reading the flags after the eigensolve instead, so that nothing waits here, measured
     slower on one PD of 2000 images: 4.14 against 4.05 ms per step.)"""
@@ -493,7 +498,8 @@ def fit_bandwidths(logEps, logS_dev, a0s, tune, rand=None):
             return o[:, 0:4].copy(), 1 - o[:, 14] / ss_tot
         return sigma, finish
     logS = _to_host(logS_dev).reshape(npd, m)
-    popt, R2 = ferguson_fit_batch(np.asarray(logEps), logS, a0s, rand, logS_dev=logS_dev)
+    popt, R2 = ferguson_fit_batch(np.asarray(logEps), logS, a0s, rand, logS_dev=logS_dev,
+                                  first=out)
     sig = tune * np.sqrt(2 * np.exp(-popt[:, 1] / popt[:, 0]))
     return _to_device(sig, dev), lambda _logS: (popt, R2)
 

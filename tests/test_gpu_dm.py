@@ -271,6 +271,43 @@ def test_gpu_tanh_fit_matches_curve_fit(cuda):
         assert abs(R2[p] - R2r) < 1e-9
 
 
+def test_fit_bandwidths_restart_path_matches_curve_fit(cuda):
+    """fit_bandwidths when a PD's own start fails the restart rule: the host restart loop takes
+    over from the device's first fit (not fitting it again) and gives the sequential
+    reference's bandwidth and R^2 from the same random draws.  One PD per call, so that the
+    reference's global draw order (a draw after every fit) and the batch's agree."""
+    import torch
+    from manifoldem_python_amd import pipeline as pl
+    from oracle import dm
+    rng = np.random.default_rng(21)
+    restarted = 0
+    for seed in range(4):
+        D = _distance_matrix(140 + 25 * seed, 70 + seed)
+        yRow, yCol, yVal = dm.assemble(D.copy(), 60)
+        logS = dm.ferguson_sums(np.sqrt(yVal), dm.LOG_EPS)[0][None, :]
+        a0 = (rng.random((4, 1)) - 0.5) * 40.0          # far start: huge pcov, the rule refits
+        class Counted:                                  # the oracle draws rng.random(shape)
+            def __init__(self, seed):
+                self.r, self.n = np.random.default_rng(seed), 0
+
+            def random(self, sh):
+                self.n += 1
+                return self.r.random(sh)
+
+        r2 = Counted(5 + seed)
+        pr, _, R2r = dm.ferguson_fit(dm.LOG_EPS, logS[0], a0, r2)
+        restarted += r2.n > 1                           # a draw after every fit: > 1 fit
+        r1 = np.random.default_rng(5 + seed)
+        sigma, finish = pl.fit_bandwidths(dm.LOG_EPS, torch.as_tensor(logS, device=cuda), [a0],
+                                          1.0, rand=lambda *sh: r1.random(sh))
+        popt, R2 = finish(logS)
+        s_ref = np.sqrt(2 * np.exp(-pr[1] / pr[0]))
+        s_gpu = float(sigma.cpu().numpy()[0])
+        assert abs(s_gpu - s_ref) / s_ref < 1e-6, (seed, s_gpu, s_ref)
+        assert abs(R2[0] - R2r) < 1e-9, seed
+    assert restarted >= 2, "too few far starts failed: the test does not reach the fallback"
+
+
 def test_tanh_select_matches_host_rule(cuda):
     """memi_tanh_select against the host's form of the restart rule (numpy inv of J^T J,
     sum sqrt|diag pcov| <= 100, the first passing start) and sigma = tune sqrt(2 exp(-b/a)),
