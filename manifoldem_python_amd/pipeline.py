@@ -369,7 +369,8 @@ def _device_const(a, dev):
     if t is None:
         if len(_const_cache) > 16:
             _const_cache.clear()
-        t = _const_cache[key] = torch.as_tensor(a, device=dev)
+        # torch.tensor copies (as_tensor would wrap a read-only host array and warn)
+        t = _const_cache[key] = torch.tensor(a, dtype=torch.float64, device=dev)
     if isinstance(src, np.ndarray) and not src.flags.writeable:
         _const_last[str(dev)] = (src, t)
     return t
