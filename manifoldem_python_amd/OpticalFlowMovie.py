@@ -28,9 +28,10 @@ def getOrientMag(X, Y):
     Y = np.asarray(Y, dtype=np.float64)
     shape = np.broadcast_shapes(X.shape, Y.shape)
     n = int(np.prod(shape)) if shape else 1
-    Xd = torch.as_tensor(np.ascontiguousarray(np.broadcast_to(X, shape)).reshape(1, 1, 1, n),
+    # np.array copies the read-only broadcast view into a writable host array
+    Xd = torch.as_tensor(np.array(np.broadcast_to(X, shape), order="C").reshape(1, 1, 1, n),
                          device="cuda")
-    Yd = torch.as_tensor(np.ascontiguousarray(np.broadcast_to(Y, shape)).reshape(1, 1, 1, n),
+    Yd = torch.as_tensor(np.array(np.broadcast_to(Y, shape), order="C").reshape(1, 1, 1, n),
                          device="cuda")
     vx, vy, o, m = (torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(4))
     from . import _native as nat
