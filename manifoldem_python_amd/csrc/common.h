@@ -109,9 +109,11 @@ __device__ __forceinline__ double wave_total(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Block-wide sum (256 threads) of a double; result valid in all threads.
+// Block-wide sum (whole waves, up to 512 threads) of a double; result valid in all threads.
+// The wave part by DPP (wave_total): a shuffle-down tree goes through LDS permutes, six
+// dependent round trips per sum.
 __device__ inline double block_sum(double v, double* sh) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  v = wave_total(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) sh[wid] = v;
