@@ -1013,8 +1013,10 @@ static int lanczos_steps_impl(const void* plan_dev, const memi_plan_info* info, 
       if (npd < 64) {       // few PDs: a 256-thread block per (column, PD)
         hipLaunchKernelGGL(k_orth_dot, dim3((unsigned)min(m + 1, 64), npd), dim3(256), 0, s,
                            pds, V, m, jp, active, W, A, ab, H);
-      } else {              // many PDs: waves walk the live (PD, column) pairs
-        hipLaunchKernelGGL(k_orth_dot_pairs, dim3(1024), dim3(256), 0, s, pds, info->n_pd, V, m,
+      } else {              // many PDs: waves walk the live (PD, column) pairs; 2048
+                            // blocks fill the CUs (1024 left the short, latency-bound waves
+                            // half the occupancy: 3 % slower on a 500-PD batch)
+        hipLaunchKernelGGL(k_orth_dot_pairs, dim3(2048), dim3(256), 0, s, pds, info->n_pd, V, m,
                            jp, active, W, A, ab, H);
       }
       MEMI_CHECK_LAUNCH("k_orth_dot");
