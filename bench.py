@@ -47,7 +47,7 @@ PEAK_HBM_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--no-c3", action="store_true",
@@ -61,6 +61,9 @@ def parse():
                     help="replay blocks of Lanczos steps from a captured HIP graph (default 1)")
     ap.add_argument("--lanczos-split", type=int, default=None,
                     help="1/0: force the grid-parallel / per-PD orthogonalisation")
+    ap.add_argument("--cu-split", type=int, default=None,
+                    help="overlapped steps: CUs of every 32 kept for the embedding alone, the "
+                         "rest for the prep + Gram (0: shared CUs; default: the measured best)")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="time steps one after another instead of overlapping step k+1's "
                          "prep + Gram with step k's embedding on a second stream")
@@ -237,8 +240,14 @@ def build_workload(name, dev):
     return batch, meta, num_eigs
 
 
+# overlapped steps: CUs of every 32 kept for the embedding alone, the rest for the prep + Gram
+# (0: shared CUs).  Measured on one PD of 2000 images: 0 -> 3.13-3.15 ms, 8 -> 3.18,
+# 12 -> 3.29, 16 -> 2.90-2.92, 20 -> 3.98, 24 -> 3.92 ms per step
+CU_SPLIT = 16
+
+
 def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False,
-                  logEps=None, gram_config=None, pipeline=True):
+                  logEps=None, gram_config=None, pipeline=True, cu_split=0):
     """Tune the Gram on the prepared operand, warm up, then time exactly `steps` full steps;
     returns timings and the last results.  graph: the prep + Gram part of each step is one
     replay of a captured HIP graph; the Gram kernel is then timed with HIP events in `steps`
@@ -293,7 +302,8 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False
         wsp2 = B.Workspace(batch, precision)
         autotune._install(wsp2, wsp.plan.chunks, wsp.plan.gram_tile)
         B.distances(batch, wsp2)
-        ov = B.OverlappedSteps(batch, wsp, wsp2, graphs=(g, B.GraphedDistances(batch, wsp2)))
+        ov = B.OverlappedSteps(batch, wsp, wsp2, graphs=(g, B.GraphedDistances(batch, wsp2)),
+                               split=cu_split)
 
         def pipelined_steps(n, record=None, rand=rand):
             return ov.run(n, num_eigs=num_eigs, logEps=le, rand=rand, record=record)
@@ -780,8 +790,10 @@ def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=T
     # overlapping steps pays where the embedding is launch-bound (one PD per step); on the
     # 500-PD batch both stages are heavy and overlap gained nothing (93.5 vs 93.6 ms)
     pipeline = not args.no_pipeline and workload in ("c1", "c2")
+    cu_split = CU_SPLIT if args.cu_split is None else args.cu_split
     bf = run_precision(batch, nat.PREC_SPLIT_BF16, steps, warmup, num_eigs, ev, graph=graph,
-                       logEps=logEps, gram_config=args.gram_config, pipeline=pipeline)
+                       logEps=logEps, gram_config=args.gram_config, pipeline=pipeline,
+                       cu_split=cu_split)
     ns = batch.ns
     pairs = float(np.sum(ns * (ns - 1) / 2))
     Kh = pl.half_len(batch.N)
@@ -801,6 +813,7 @@ def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=T
         "data": "synthetic",
         "config": dict(meta, precision="split-bf16", hip_graph=graph,
                        pipelined_steps=bf["pipeline"],
+                       cu_split_per_32=cu_split if bf["pipeline"] else None,
                        lanczos=dict(pl.LANCZOS), kernel_dtype=str(pl.KERNEL_DTYPE)),
         "pds_per_s": batch.ns.size * steps / bf["t"],
         "distance_ms_per_step": 1e3 * bf["t_dist"] / steps,
@@ -836,7 +849,7 @@ def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=T
     if with_fp32:
         log("timing FP32", workload)
         f32 = run_precision(batch, nat.PREC_FP32, steps, warmup, num_eigs, ev, graph=graph,
-                            logEps=logEps, pipeline=pipeline)
+                            logEps=logEps, pipeline=pipeline, cu_split=cu_split)
         a32 = flops / (f32["gram_ms"] / 1e3) / 1e12
         out["fp32"] = {"value": pairs * steps / f32["t"],
                        "ms_per_step": 1e3 * f32["t"] / steps,

@@ -490,6 +490,40 @@ def ctf_distance(plan, op_hi, op_lo, Khp, precision, kappa, fy, ctf_half, N, D, 
          _p(flag_count), int(variant), ev0, ev1, _stream())
 
 
+def cu_masked_stream(keep, device=None):
+    """A HIP stream (torch ExternalStream) whose kernels run only on the CUs whose mask bit i
+    satisfies keep(i % 32) (hipExtStreamCreateWithCUMask).  Choosing the bits by i % 32 gives
+    every XCD its share whether the mask enumerates CUs XCD by XCD or interleaved.  The stream
+    is blocking with respect to the null stream; one per (device, bit pattern) lives until the
+    process ends."""
+    dev = torch.cuda.current_device() if device is None else device
+    pattern = tuple(bool(keep(b)) for b in range(32))
+    if not any(pattern):
+        raise ValueError("cu_masked_stream: the mask keeps no CU")
+    key = (dev, pattern)
+    if key in _MASKED_STREAMS:
+        return _MASKED_STREAMS[key]
+    n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
+    nw = (n_cu + 31) // 32
+    mask = (ctypes.c_uint32 * nw)()
+    for i in range(n_cu):
+        if pattern[i % 32]:
+            mask[i // 32] |= 1 << (i % 32)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipExtStreamCreateWithCUMask.argtypes = [ctypes.POINTER(c_vp), ctypes.c_uint32,
+                                                 ctypes.POINTER(ctypes.c_uint32)]
+    st = c_vp()
+    with torch.cuda.device(dev):
+        err = hip.hipExtStreamCreateWithCUMask(ctypes.byref(st), nw, mask)
+    if err or not st.value:
+        raise RuntimeError(f"hipExtStreamCreateWithCUMask failed ({err})")
+    _MASKED_STREAMS[key] = torch.cuda.ExternalStream(st.value, device=dev)
+    return _MASKED_STREAMS[key]
+
+
+_MASKED_STREAMS = {}
+
+
 class HipEvents:
     """Raw hipEvent_t pairs (libamdhip64) for timing one kernel inside a library call on the
     stream it runs on."""

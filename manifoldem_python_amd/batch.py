@@ -139,13 +139,30 @@ class OverlappedSteps:
     until step k-2's embedding is done with the set, and step k's embedding waits for its
     replay.  Every step does its whole batch."""
 
-    def __init__(self, batch, wsp_a, wsp_b, graphs=None):
+    def __init__(self, batch, wsp_a, wsp_b, graphs=None, split=0):
+        """split: CUs of every 32 given to the embeddings alone, the rest to the distances
+        alone (0: both streams on every CU).  Latency-bound embedding kernels sharing SIMDs
+        with the prep and Gram kernels ran 5-10 times slower than alone.  The embeddings keep
+        mask bits 32 - split .. 31 of every 32.  Each stream needs CUs on every XCD, and the
+        mask enumerates CUs interleaved over the eight XCDs (a mask of bits 28-31 of every 32
+        was refused), so 8 <= split <= 24."""
         self.batch = batch
         ga, gb = graphs if graphs is not None else (GraphedDistances(batch, wsp_a),
                                                      GraphedDistances(batch, wsp_b))
         self.sets = [(wsp_a, ga), (wsp_b, gb)]
-        self.main = torch.cuda.current_stream()
-        self.side = torch.cuda.Stream()
+        self.split = int(split)
+        if self.split:
+            if not 8 <= self.split <= 24:
+                raise ValueError("OverlappedSteps: split must be in 8..24 (each stream keeps "
+                                 "CUs on all eight XCDs)")
+            # disjoint CU masks; both masked streams are blocking, so the embeddings do not
+            # run on a caller's null stream
+            cut = 32 - self.split
+            self.main = nat.cu_masked_stream(lambda b: b >= cut)
+            self.side = nat.cu_masked_stream(lambda b: b < cut)
+        else:
+            self.main = torch.cuda.current_stream()
+            self.side = torch.cuda.Stream()
         self.done_dist = [torch.cuda.Event(), torch.cuda.Event()]
         self.done_embed = [torch.cuda.Event(), torch.cuda.Event()]
         for e in self.done_embed:
@@ -157,6 +174,9 @@ class OverlappedSteps:
         (start, end) torch events put around each step's replay on the side stream;
         on_result(k, results) sees every step's results."""
         dists = [None] * n
+        caller = torch.cuda.current_stream()
+        self.main.wait_stream(caller)
+        self.side.wait_stream(caller)
 
         def launch(k):
             wsp, g = self.sets[k % 2]
@@ -171,14 +191,17 @@ class OverlappedSteps:
 
         launch(0)
         res = None
-        for k in range(n):
-            if k + 1 < n:
-                launch(k + 1)
-            self.main.wait_event(self.done_dist[k % 2])
-            res = embed(self.sets[k % 2][0], dists[k], tune, num_eigs, logEps, rand=rand)
-            self.done_embed[k % 2].record(self.main)
-            if on_result is not None:
-                on_result(k, res)
+        with torch.cuda.stream(self.main):
+            for k in range(n):
+                if k + 1 < n:
+                    launch(k + 1)
+                self.main.wait_event(self.done_dist[k % 2])
+                res = embed(self.sets[k % 2][0], dists[k], tune, num_eigs, logEps, rand=rand)
+                self.done_embed[k % 2].record(self.main)
+                if on_result is not None:
+                    on_result(k, res)
+        caller.wait_stream(self.main)
+        caller.wait_stream(self.side)
         return res
 
 

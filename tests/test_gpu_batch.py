@@ -105,10 +105,12 @@ def test_graph_replays_match_eager(cuda, precision):
         assert int(dist.flag_count.item()) == c0, r
 
 
-def test_overlapped_steps_match_serial(cuda):
+@pytest.mark.parametrize("split", [0, 16])
+def test_overlapped_steps_match_serial(cuda, split):
     """Steps overlapped on two streams (step k+1's distances during step k's embedding, two
     buffer sets) give every step the eigenvalues and bandwidths of the same steps run one after
-    another from the same random fit starts."""
+    another from the same random fit starts; also with the two streams on disjoint CU sets
+    (16 of every 32 CUs for the embeddings, CU-masked streams)."""
     import torch
     from manifoldem_python_amd import batch as B, _native as nat
     batch = _batch(_pds(), cuda)
@@ -121,7 +123,7 @@ def test_overlapped_steps_match_serial(cuda):
     wb = B.Workspace(batch, nat.PREC_SPLIT_BF16)
     B.distances(batch, wa)
     B.distances(batch, wb)
-    ov = B.OverlappedSteps(batch, wa, wb)
+    ov = B.OverlappedSteps(batch, wa, wb, split=split)
     got = []
     ov.run(5, num_eigs=8, rand=seeded(),
            on_result=lambda k, res: got.append([(r.lamb.copy(), r.sigma) for r in res]))
