@@ -304,6 +304,7 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False
         B.distances(batch, wsp2)
         ov = B.OverlappedSteps(batch, wsp, wsp2, graphs=(g, B.GraphedDistances(batch, wsp2)),
                                split=cu_split)
+        cu_split = ov.split                     # 0 when the device refused the CU masks
 
         def pipelined_steps(n, record=None, rand=rand):
             return ov.run(n, num_eigs=num_eigs, logEps=le, rand=rand, record=record)
@@ -343,7 +344,8 @@ def run_precision(batch, precision, steps, warmup, num_eigs, events, graph=False
     dist = g.dist if graph else B.distances(batch, wsp)
     return dict(t=t, t_dist=t_dist, gram_ms=float(np.mean(gram_ms)), wsp=wsp, dist=dist,
                 res=res, flags=int(dist.flag_count.item()), stages=stages, tuned=tuned,
-                graph=graph, lanczos=lanczos, pipeline=pipeline, **out_extra)
+                graph=graph, lanczos=lanczos, pipeline=pipeline,
+                cu_split=cu_split if pipeline else None, **out_extra)
 
 
 def issued_gram_flops(plan, Khp, products):
@@ -813,7 +815,7 @@ def measure(workload, args, dev, ev, steps, warmup, with_fp32=True, with_check=T
         "data": "synthetic",
         "config": dict(meta, precision="split-bf16", hip_graph=graph,
                        pipelined_steps=bf["pipeline"],
-                       cu_split_per_32=cu_split if bf["pipeline"] else None,
+                       cu_split_per_32=bf["cu_split"],
                        lanczos=dict(pl.LANCZOS), kernel_dtype=str(pl.KERNEL_DTYPE)),
         "pds_per_s": batch.ns.size * steps / bf["t"],
         "distance_ms_per_step": 1e3 * bf["t_dist"] / steps,

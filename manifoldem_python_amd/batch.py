@@ -8,6 +8,7 @@ and the kNN / Ferguson / normalisation kernels each run once over the whole batc
 from __future__ import annotations
 
 import math
+import warnings
 from dataclasses import dataclass
 
 import numpy as np
@@ -151,16 +152,22 @@ class OverlappedSteps:
                                                      GraphedDistances(batch, wsp_b))
         self.sets = [(wsp_a, ga), (wsp_b, gb)]
         self.split = int(split)
+        if self.split and not 8 <= self.split <= 24:
+            raise ValueError("OverlappedSteps: split must be in 8..24 (each stream keeps CUs "
+                             "on all eight XCDs)")
         if self.split:
-            if not 8 <= self.split <= 24:
-                raise ValueError("OverlappedSteps: split must be in 8..24 (each stream keeps "
-                                 "CUs on all eight XCDs)")
             # disjoint CU masks; both masked streams are blocking, so the embeddings do not
-            # run on a caller's null stream
+            # run on a caller's null stream.  A device that refuses the masks gets the shared
+            # streams (split 0), with a warning
             cut = 32 - self.split
-            self.main = nat.cu_masked_stream(lambda b: b >= cut)
-            self.side = nat.cu_masked_stream(lambda b: b < cut)
-        else:
+            try:
+                self.main = nat.cu_masked_stream(lambda b: b >= cut)
+                self.side = nat.cu_masked_stream(lambda b: b < cut)
+            except RuntimeError as e:
+                warnings.warn(f"OverlappedSteps: CU-masked streams unavailable ({e}); the two "
+                              "streams share all CUs", RuntimeWarning, stacklevel=2)
+                self.split = 0
+        if not self.split:
             self.main = torch.cuda.current_stream()
             self.side = torch.cuda.Stream()
         self.done_dist = [torch.cuda.Event(), torch.cuda.Event()]
