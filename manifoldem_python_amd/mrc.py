@@ -8,7 +8,7 @@ Layout (MRC2014): a 1024-byte main header of 32-bit words -- nx, ny, nz (words 0
 ny rows of nx values, x fastest.  ``read_images`` returns images in mrcfile's ``data[z]``
 orientation, (ny, nx) with row y.  Only mode 2 (float32), the mode RELION writes particles in,
 is read: the reference feeds the native dtype to ``ndimage.shift``, which for integer modes
-rounds its output to that dtype, a path this build does not reproduce.
+rounds its output to that dtype, a path this reader does not reproduce (it raises).
 """
 from __future__ import annotations
 
